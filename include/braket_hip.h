@@ -1,0 +1,225 @@
+/*
+ * braket_hip.h -- C ABI of the MI355X (gfx950) state-vector engine.
+ *
+ * Every entry point takes plain pointers and sizes.  A state vector is a
+ * device array of 2^n amplitudes ("first" + "size", exactly like the
+ * [first, last) iterator pair of the reference's ket gate functions), in one
+ * of two element types:
+ *
+ *   BRK_C128  std::complex<double>  (16 B: re, im)   -- the plain path
+ *   BRK_C64   std::complex<float>   ( 8 B: re, im)   -- precision / batch forms
+ *
+ * Basis index bit q is qubit q (qubit 0 = least significant bit).  All gate
+ * functions update the state in place, allocate nothing, and are enqueued on
+ * the HIP stream passed as `stream` (NULL = the null stream); they return
+ * before the kernel finishes.  Reductions synchronise the stream and return
+ * their result on the host.  Functions never throw; each returns a brk_status
+ * and brk_last_error() gives a message for the calling thread.
+ *
+ * Two calls on the same state must not run concurrently on different streams
+ * (the same rule the reference states for its parallel policies).
+ */
+#ifndef BRAKET_HIP_H
+#define BRAKET_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Status codes and element types                                            */
+/* ------------------------------------------------------------------------ */
+
+typedef enum brk_status {
+  BRK_OK = 0,
+  BRK_ERR_INVALID_ARGUMENT = 1,   /* null pointer, size not a power of two, bad count */
+  BRK_ERR_QUBIT_OUT_OF_RANGE = 2, /* 2^qubit >= size                                  */
+  BRK_ERR_DUPLICATE_QUBIT = 3,    /* the same qubit given twice                       */
+  BRK_ERR_TOO_MANY_QUBITS = 4,    /* more operated qubits than the kernel supports    */
+  BRK_ERR_UNSUPPORTED_DTYPE = 5,
+  BRK_ERR_OUT_OF_MEMORY = 6,
+  BRK_ERR_HIP = 7,                /* HIP runtime or launch failure                     */
+  BRK_ERR_UNSUPPORTED = 8         /* valid request that this build does not implement */
+} brk_status;
+
+typedef enum brk_dtype { BRK_C128 = 0, BRK_C64 = 1 } brk_dtype;
+
+/* Maximum number of operated qubits (targets + controls) of one plain gate. */
+#define BRK_MAX_OPERATED_QUBITS 16
+
+const char* brk_status_string(int status);
+/* Message describing the last failure on the calling thread ("" if none). */
+const char* brk_last_error(void);
+/* 1 when the library was built for gfx950 and a device is visible. */
+int brk_device_count(int* count);
+const char* brk_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Device memory and state preparation (no reference counterpart: the       */
+/* reference owns a std::vector; bra::nompi_state's constructor builds the  */
+/* initial basis state, bra/include/bra/nompi_state.hpp:155-165).            */
+/* ------------------------------------------------------------------------ */
+
+int brk_malloc(void** device_ptr, size_t bytes);
+int brk_free(void* device_ptr);
+int brk_memcpy_htod(void* device_dst, const void* host_src, size_t bytes, void* stream);
+int brk_memcpy_dtoh(void* host_dst, const void* device_src, size_t bytes, void* stream);
+int brk_memcpy_dtod(void* device_dst, const void* device_src, size_t bytes, void* stream);
+int brk_stream_synchronize(void* stream);
+
+/* state := |basis_index> */
+int brk_set_basis_state(void* first, uint64_t size, int dtype, uint64_t basis_index, void* stream);
+/* Deterministic, index-only fills used to build states of any size on the
+   device (test states of the reference's ket/test/ *.cpp are of this form):
+     re[i] = scales[0] * ((params[0]*i + params[1]) mod params[2])
+     im[i] = scales[1] * ((params[3]*i + params[4]) mod params[5])
+   with uint64 wrap-around arithmetic and "mod 0" meaning no reduction. */
+int brk_fill_modular(void* first, uint64_t size, int dtype, const double scales[2],
+                     const uint64_t params[6], void* stream);
+/* re[i] = scale * u(seed, 2i), im[i] = scale * u(seed, 2i+1) where u is the
+   splitmix64 finaliser of seed + (counter+1)*0x9E3779B97F4A7C15 mapped to
+   [-1, 1) with 53 bits: (z >> 11) * 2^-53 * 2 - 1. */
+int brk_fill_hash(void* first, uint64_t size, int dtype, uint64_t seed, double scale, void* stream);
+/* dst (dtype_dst) := src (dtype_src), elementwise conversion c128 <-> c64. */
+int brk_convert(void* dst, int dtype_dst, const void* src, int dtype_src, uint64_t size, void* stream);
+/* Streaming copy kernel (bandwidth calibration and state copies). */
+int brk_copy(void* dst, const void* src, uint64_t size, int dtype, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Plain per-gate form.  Argument order follows the reference's iterator    */
+/* form: (first, last) -> (first, size, dtype), then real parameters, then  */
+/* the target qubit(s), then the control qubits.                            */
+/* ------------------------------------------------------------------------ */
+
+/* H on `target`, applied where every control bit is 1.
+   Replaces ket::gate::hadamard (ket/include/ket/gate/hadamard.hpp:34-72 H,
+   :77-130 CH, :133-223 CnH). */
+int brk_hadamard(void* first, uint64_t size, int dtype, unsigned target,
+                 const unsigned* controls, unsigned num_controls, void* stream);
+
+/* X on every qubit of `targets` at once (X, XX, Xn), under controls.
+   Replaces ket::gate::pauli_x (ket/include/ket/gate/pauli_x.hpp:31-61 X,
+   :116-165 CX, multi-target/multi-control forms after :166). */
+int brk_pauli_x(void* first, uint64_t size, int dtype, const unsigned* targets, unsigned num_targets,
+                const unsigned* controls, unsigned num_controls, void* stream);
+
+/* NOT / CNOT / CnNOT on one target.  Replaces ket::gate::not_
+   (ket/include/ket/gate/not_.hpp:33-51) and ket::gate::controlled_not
+   (ket/include/ket/gate/controlled_not.hpp:30-37). */
+int brk_controlled_not(void* first, uint64_t size, int dtype, unsigned target,
+                       const unsigned* controls, unsigned num_controls, void* stream);
+
+/* Y on every qubit of `targets` at once (Y, YY, Yn), under controls.
+   Replaces ket::gate::pauli_y (ket/include/ket/gate/pauli_y.hpp). */
+int brk_pauli_y(void* first, uint64_t size, int dtype, const unsigned* targets, unsigned num_targets,
+                const unsigned* controls, unsigned num_controls, void* stream);
+
+/* Z acting on "control" qubits: the amplitude is negated where every listed
+   bit is 1 (Z, CZ, CnZ).  Replaces ket::gate::pauli_z, case 1
+   (ket/include/ket/gate/pauli_z.hpp:55-88 Z, :90-137 CZ, :139-210 CnZ). */
+int brk_pauli_z(void* first, uint64_t size, int dtype, const unsigned* qubits, unsigned num_qubits,
+                void* stream);
+
+/* Multiply the amplitudes whose listed bits are all 1 by `coeff`
+   (U1 / CU1 / CnU1 given exp(i theta)).  Replaces ket::gate::phase_shift_coeff
+   (ket/include/ket/gate/phase_shift.hpp:37-93 and the controlled forms). */
+int brk_phase_shift_coeff(void* first, uint64_t size, int dtype, const double coeff[2],
+                          const unsigned* qubits, unsigned num_qubits, void* stream);
+/* Same with coeff = exp(i phase).  Replaces ket::gate::phase_shift
+   (ket/include/ket/gate/phase_shift.hpp:510-521). */
+int brk_phase_shift(void* first, uint64_t size, int dtype, double phase,
+                    const unsigned* qubits, unsigned num_qubits, void* stream);
+
+/* U2(phase1, phase2) on `target` under controls.  Replaces
+   ket::gate::phase_shift2 (ket/include/ket/gate/phase_shift.hpp:653-709). */
+int brk_phase_shift2(void* first, uint64_t size, int dtype, double phase1, double phase2,
+                     unsigned target, const unsigned* controls, unsigned num_controls, void* stream);
+
+/* U3(phase1, phase2, phase3) on `target` under controls.  Replaces
+   ket::gate::phase_shift3 (ket/include/ket/gate/phase_shift.hpp:1195-1254). */
+int brk_phase_shift3(void* first, uint64_t size, int dtype, double phase1, double phase2, double phase3,
+                     unsigned target, const unsigned* controls, unsigned num_controls, void* stream);
+
+/* exp(i phase X...X) on `targets` under controls.  Replaces
+   ket::gate::exponential_pauli_x (ket/include/ket/gate/exponential_pauli_x.hpp). */
+int brk_exponential_pauli_x(void* first, uint64_t size, int dtype, double phase,
+                            const unsigned* targets, unsigned num_targets,
+                            const unsigned* controls, unsigned num_controls, void* stream);
+
+/* exp(i phase Y...Y) on `targets` under controls.  Replaces
+   ket::gate::exponential_pauli_y (ket/include/ket/gate/exponential_pauli_y.hpp:37-90,
+   wrapper :473-495).  RY(theta) == exponential_pauli_y(-theta/2). */
+int brk_exponential_pauli_y(void* first, uint64_t size, int dtype, double phase,
+                            const unsigned* targets, unsigned num_targets,
+                            const unsigned* controls, unsigned num_controls, void* stream);
+
+/* exp(i phase Z...Z) on `targets` under controls.  Replaces
+   ket::gate::exponential_pauli_z (ket/include/ket/gate/exponential_pauli_z.hpp:36-78,
+   wrapper :367-390).  RZ(theta) == exponential_pauli_z(-theta/2). */
+int brk_exponential_pauli_z(void* first, uint64_t size, int dtype, double phase,
+                            const unsigned* targets, unsigned num_targets,
+                            const unsigned* controls, unsigned num_controls, void* stream);
+
+/* SWAP of qubit1 and qubit2 under controls.  Replaces ket::gate::swap
+   (ket/include/ket/gate/swap.hpp:30-75 and the controlled form after :76). */
+int brk_swap(void* first, uint64_t size, int dtype, unsigned qubit1, unsigned qubit2,
+             const unsigned* controls, unsigned num_controls, void* stream);
+
+/* General single-qubit gate: matrix = {m00.re, m00.im, m01.re, m01.im,
+   m10.re, m10.im, m11.re, m11.im}, (a0', a1') = M (a0, a1), under controls.
+   Replaces the generic driver ket::gate::gate applied with a 2x2 functor
+   (ket/include/ket/gate/detail/gate_bitmasks.hpp:470-586). */
+int brk_gate_1q(void* first, uint64_t size, int dtype, unsigned target, const double matrix[8],
+                const unsigned* controls, unsigned num_controls, void* stream);
+
+/* Data movement of a local qubit swap: exchanges the amplitudes whose
+   physical bits (qubit1, qubit2) are (1,0) and (0,1).  The logical->physical
+   map is kept on the host (ket::mpi::qubit_permutation::permutate).
+   Replaces ket::mpi::utility::detail::swap_permutated_local_qubits
+   (ket/include/ket/mpi/utility/detail/swap_permutated_local_qubits.hpp:26-81)
+   for the single-device case (one data block). */
+int brk_swap_permutated_local_qubits(void* first, uint64_t size, int dtype,
+                                     unsigned permutated_qubit1, unsigned permutated_qubit2, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* FP64 reductions (partial sums in double, deterministic two-pass tree).   */
+/* ------------------------------------------------------------------------ */
+
+/* result = <psi2|psi1> = sum conj(psi2[i]) psi1[i].  Replaces ket::inner_product
+   (ket/include/ket/inner_product.hpp:32-56).  dtype1/dtype2 may differ
+   (c64 vs c128 fidelity). */
+int brk_inner_product(const void* first1, int dtype1, const void* first2, int dtype2, uint64_t size,
+                      double result[2], void* stream);
+/* result = |<psi2|psi1>|^2.  Replaces ket::fidelity (ket/include/ket/fidelity.hpp:18-25). */
+int brk_fidelity(const void* first1, int dtype1, const void* first2, int dtype2, uint64_t size,
+                 double* result, void* stream);
+/* result = <psi|psi>. */
+int brk_norm2(const void* first, uint64_t size, int dtype, double* result, void* stream);
+/* result = {sum Re(conj(a0) a1), sum Im(conj(a0) a1), (sum |a0|^2 - |a1|^2) / 2}
+   over the pairs of `qubit`.  Replaces ket::spin_expectation_value
+   (ket/include/ket/spin_expectation_value.hpp:20-79). */
+int brk_spin_expectation_value(const void* first, uint64_t size, int dtype, unsigned qubit,
+                               double result[3], void* stream);
+/* <psi| P |psi> for a Pauli string P (characters I, X, Y, Z; character k acts on
+   qubits[k]).  Replaces ket::expectation_value with the Pauli functor of
+   bra::nompi_state::do_expectation_value (bra/src/nompi_state.cpp:1155-1202,
+   ket/include/ket/gate/utility/pauli_index_coeff.hpp:32-59). */
+int brk_expectation_pauli_string(const void* first, uint64_t size, int dtype, const char* pauli_string,
+                                 const unsigned* qubits, unsigned num_qubits, double result[2], void* stream);
+/* <psi| H |psi> for the open-chain Ising Hamiltonian
+   H = sum_{i<n-1} J[i] Z_i Z_{i+1} + sum_i h[i] X_i, n = log2(size).
+   (Composition of spin_expectation_value terms; no single reference function.) */
+int brk_expectation_ising(const void* first, uint64_t size, int dtype, const double* J, const double* h,
+                          double* result, void* stream);
+/* out := H |psi> for the same Hamiltonian (the adjoint method's starting vector). */
+int brk_apply_ising(void* out, const void* psi, uint64_t size, int dtype, const double* J, const double* h,
+                    void* stream);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* BRAKET_HIP_H */

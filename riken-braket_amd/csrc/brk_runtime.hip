@@ -1,0 +1,323 @@
+// Runtime pieces of the C ABI: status strings, per-thread error messages,
+// device memory, state preparation (basis states, deterministic fills),
+// dtype conversion and the streaming copy kernel.
+#include "brk_internal.hpp"
+
+#include <mutex>
+#include <vector>
+
+namespace brk {
+
+static thread_local std::string g_last_error;
+
+int fail(int status, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return status;
+}
+
+void clear_error() { g_last_error.clear(); }
+
+int check_state(const void* first, uint64_t size, int dtype, unsigned* num_qubits) {
+  if (first == nullptr) return fail(BRK_ERR_INVALID_ARGUMENT, "state pointer is null");
+  if (size == 0 || (size & (size - 1)) != 0)
+    return fail(BRK_ERR_INVALID_ARGUMENT, "state size %llu is not a power of two",
+                (unsigned long long)size);
+  if (dtype != BRK_C128 && dtype != BRK_C64)
+    return fail(BRK_ERR_UNSUPPORTED_DTYPE, "unknown dtype %d", dtype);
+  *num_qubits = (unsigned)__builtin_ctzll(size);
+  return BRK_OK;
+}
+
+int collect_qubits(const unsigned* qubits, unsigned n, unsigned num_qubits, unsigned* out, unsigned* count,
+                   const char* role) {
+  if (n > 0 && qubits == nullptr) return fail(BRK_ERR_INVALID_ARGUMENT, "%s list is null", role);
+  for (unsigned i = 0; i < n; ++i) {
+    unsigned q = qubits[i];
+    if (q >= num_qubits)
+      return fail(BRK_ERR_QUBIT_OUT_OF_RANGE, "%s qubit %u out of range for %u qubits", role, q, num_qubits);
+    for (unsigned j = 0; j < *count; ++j)
+      if (out[j] == q) return fail(BRK_ERR_DUPLICATE_QUBIT, "qubit %u given twice", q);
+    if (*count >= BRK_MAX_OPERATED_QUBITS)
+      return fail(BRK_ERR_TOO_MANY_QUBITS, "more than %d operated qubits", BRK_MAX_OPERATED_QUBITS);
+    out[(*count)++] = q;
+  }
+  return BRK_OK;
+}
+
+static std::mutex g_scratch_mutex;
+static std::vector<reduce_scratch> g_scratch;
+
+int get_reduce_scratch(reduce_scratch** out) {
+  int dev = 0;
+  BRK_HIP_CHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if ((int)g_scratch.size() <= dev) g_scratch.resize(dev + 1);
+  reduce_scratch& s = g_scratch[dev];
+  if (s.device == nullptr) {
+    size_t bytes = sizeof(double) * RED_BLOCKS * RED_MAX_VALUES;
+    if (hipMalloc(&s.device, bytes) != hipSuccess) {
+      s.device = nullptr;
+      return fail(BRK_ERR_OUT_OF_MEMORY, "cannot allocate %zu bytes of reduction scratch", bytes);
+    }
+    if (hipHostMalloc(&s.host, bytes, hipHostMallocDefault) != hipSuccess) {
+      (void)hipFree(s.device);
+      s.device = nullptr;
+      return fail(BRK_ERR_OUT_OF_MEMORY, "cannot allocate pinned reduction scratch");
+    }
+  }
+  *out = &s;
+  return BRK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+
+template <typename R>
+__global__ void k_set_basis(cpx<R>* __restrict__ s, uint64_t size, uint64_t basis) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < size; i += stride) s[i] = cpx<R>{i == basis ? R(1) : R(0), R(0)};
+}
+
+struct modular_params {
+  double scale_re, scale_im;
+  uint64_t mul_re, add_re, mod_re, mul_im, add_im, mod_im;
+};
+
+__device__ __forceinline__ double modular_term(uint64_t i, uint64_t mul, uint64_t add, uint64_t mod) {
+  uint64_t v = i * mul + add;
+  if (mod != 0) v %= mod;
+  return (double)v;
+}
+
+template <typename R>
+__global__ void k_fill_modular(cpx<R>* __restrict__ s, uint64_t size, modular_params p) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < size; i += stride) {
+    double re = p.scale_re * modular_term(i, p.mul_re, p.add_re, p.mod_re);
+    double im = p.scale_im * modular_term(i, p.mul_im, p.add_im, p.mod_im);
+    s[i] = cpx<R>{(R)re, (R)im};
+  }
+}
+
+// splitmix64 finaliser; value in [-1, 1) with 53 random bits.
+__device__ __forceinline__ double hash_unit(uint64_t seed, uint64_t counter) {
+  uint64_t z = seed + (counter + 1ull) * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z = z ^ (z >> 31);
+  return (double)(z >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
+}
+
+template <typename R>
+__global__ void k_fill_hash(cpx<R>* __restrict__ s, uint64_t size, uint64_t seed, double scale) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < size; i += stride)
+    s[i] = cpx<R>{(R)(scale * hash_unit(seed, 2 * i)), (R)(scale * hash_unit(seed, 2 * i + 1))};
+}
+
+template <typename RD, typename RS>
+__global__ void k_convert(cpx<RD>* __restrict__ d, const cpx<RS>* __restrict__ s, uint64_t size) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < size; i += stride) {
+    cpx<RS> v = s[i];
+    d[i] = cpx<RD>{(RD)v.re, (RD)v.im};
+  }
+}
+
+// Streaming copy: 4 elements per thread, block-contiguous.
+template <typename R>
+__global__ __launch_bounds__(256) void k_copy(cpx<R>* __restrict__ d, const cpx<R>* __restrict__ s, uint64_t size) {
+  constexpr int ITEMS = 4;
+  uint64_t base = (uint64_t)blockIdx.x * (256 * ITEMS) + threadIdx.x;
+  cpx<R> v[ITEMS];
+#pragma unroll
+  for (int u = 0; u < ITEMS; ++u) {
+    uint64_t i = base + (uint64_t)u * 256;
+    if (i < size) v[u] = s[i];
+  }
+#pragma unroll
+  for (int u = 0; u < ITEMS; ++u) {
+    uint64_t i = base + (uint64_t)u * 256;
+    if (i < size) d[i] = v[u];
+  }
+}
+
+static unsigned grid_for(uint64_t n, unsigned threads, unsigned cap = 65536u) {
+  uint64_t b = (n + threads - 1) / threads;
+  if (b < 1) b = 1;
+  if (b > cap) b = cap;
+  return (unsigned)b;
+}
+
+}  // namespace brk
+
+using namespace brk;
+
+extern "C" {
+
+const char* brk_status_string(int status) {
+  switch (status) {
+    case BRK_OK: return "ok";
+    case BRK_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case BRK_ERR_QUBIT_OUT_OF_RANGE: return "qubit out of range";
+    case BRK_ERR_DUPLICATE_QUBIT: return "duplicate qubit";
+    case BRK_ERR_TOO_MANY_QUBITS: return "too many operated qubits";
+    case BRK_ERR_UNSUPPORTED_DTYPE: return "unsupported dtype";
+    case BRK_ERR_OUT_OF_MEMORY: return "out of memory";
+    case BRK_ERR_HIP: return "HIP error";
+    case BRK_ERR_UNSUPPORTED: return "unsupported";
+    default: return "unknown status";
+  }
+}
+
+const char* brk_last_error(void) { return g_last_error.c_str(); }
+
+const char* brk_version(void) { return "braket-mi355x 0.1 (gfx950)"; }
+
+int brk_device_count(int* count) {
+  if (!count) return fail(BRK_ERR_INVALID_ARGUMENT, "count is null");
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  if (e != hipSuccess) {
+    *count = 0;
+    return fail(BRK_ERR_HIP, "hipGetDeviceCount: %s", hipGetErrorString(e));
+  }
+  *count = c;
+  return BRK_OK;
+}
+
+int brk_malloc(void** device_ptr, size_t bytes) {
+  if (!device_ptr) return fail(BRK_ERR_INVALID_ARGUMENT, "device_ptr is null");
+  *device_ptr = nullptr;
+  hipError_t e = hipMalloc(device_ptr, bytes);
+  if (e != hipSuccess) {
+    *device_ptr = nullptr;
+    (void)hipGetLastError();
+    return fail(e == hipErrorOutOfMemory ? BRK_ERR_OUT_OF_MEMORY : BRK_ERR_HIP, "hipMalloc(%zu): %s", bytes,
+                hipGetErrorString(e));
+  }
+  return BRK_OK;
+}
+
+int brk_free(void* device_ptr) {
+  BRK_HIP_CHECK(hipFree(device_ptr));
+  return BRK_OK;
+}
+
+int brk_memcpy_htod(void* dst, const void* src, size_t bytes, void* stream) {
+  if ((!dst || !src) && bytes) return fail(BRK_ERR_INVALID_ARGUMENT, "null pointer");
+  BRK_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, as_stream(stream)));
+  BRK_HIP_CHECK(hipStreamSynchronize(as_stream(stream)));
+  return BRK_OK;
+}
+
+int brk_memcpy_dtoh(void* dst, const void* src, size_t bytes, void* stream) {
+  if ((!dst || !src) && bytes) return fail(BRK_ERR_INVALID_ARGUMENT, "null pointer");
+  BRK_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, as_stream(stream)));
+  BRK_HIP_CHECK(hipStreamSynchronize(as_stream(stream)));
+  return BRK_OK;
+}
+
+int brk_memcpy_dtod(void* dst, const void* src, size_t bytes, void* stream) {
+  if ((!dst || !src) && bytes) return fail(BRK_ERR_INVALID_ARGUMENT, "null pointer");
+  BRK_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, as_stream(stream)));
+  return BRK_OK;
+}
+
+int brk_stream_synchronize(void* stream) {
+  BRK_HIP_CHECK(hipStreamSynchronize(as_stream(stream)));
+  return BRK_OK;
+}
+
+int brk_set_basis_state(void* first, uint64_t size, int dtype, uint64_t basis_index, void* stream) {
+  unsigned nq;
+  if (int st = check_state(first, size, dtype, &nq)) return st;
+  if (basis_index >= size)
+    return fail(BRK_ERR_INVALID_ARGUMENT, "basis index %llu >= size %llu", (unsigned long long)basis_index,
+                (unsigned long long)size);
+  unsigned grid = grid_for(size, 256);
+  if (dtype == BRK_C128)
+    hipLaunchKernelGGL(k_set_basis<double>, dim3(grid), dim3(256), 0, as_stream(stream), (cpx<double>*)first,
+                       size, basis_index);
+  else
+    hipLaunchKernelGGL(k_set_basis<float>, dim3(grid), dim3(256), 0, as_stream(stream), (cpx<float>*)first,
+                       size, basis_index);
+  return check_launch("set_basis_state");
+}
+
+int brk_fill_modular(void* first, uint64_t size, int dtype, const double scales[2], const uint64_t params[6],
+                     void* stream) {
+  unsigned nq;
+  if (int st = check_state(first, size, dtype, &nq)) return st;
+  if (!scales || !params) return fail(BRK_ERR_INVALID_ARGUMENT, "null scales/params");
+  modular_params p{scales[0], scales[1], params[0], params[1], params[2], params[3], params[4], params[5]};
+  unsigned grid = grid_for(size, 256);
+  if (dtype == BRK_C128)
+    hipLaunchKernelGGL(k_fill_modular<double>, dim3(grid), dim3(256), 0, as_stream(stream),
+                       (cpx<double>*)first, size, p);
+  else
+    hipLaunchKernelGGL(k_fill_modular<float>, dim3(grid), dim3(256), 0, as_stream(stream), (cpx<float>*)first,
+                       size, p);
+  return check_launch("fill_modular");
+}
+
+int brk_fill_hash(void* first, uint64_t size, int dtype, uint64_t seed, double scale, void* stream) {
+  unsigned nq;
+  if (int st = check_state(first, size, dtype, &nq)) return st;
+  unsigned grid = grid_for(size, 256);
+  if (dtype == BRK_C128)
+    hipLaunchKernelGGL(k_fill_hash<double>, dim3(grid), dim3(256), 0, as_stream(stream), (cpx<double>*)first,
+                       size, seed, scale);
+  else
+    hipLaunchKernelGGL(k_fill_hash<float>, dim3(grid), dim3(256), 0, as_stream(stream), (cpx<float>*)first,
+                       size, seed, scale);
+  return check_launch("fill_hash");
+}
+
+int brk_convert(void* dst, int dtype_dst, const void* src, int dtype_src, uint64_t size, void* stream) {
+  unsigned nq;
+  if (int st = check_state(dst, size, dtype_dst, &nq)) return st;
+  if (int st = check_state(src, size, dtype_src, &nq)) return st;
+  unsigned grid = grid_for(size, 256);
+  hipStream_t s = as_stream(stream);
+  if (dtype_dst == BRK_C64 && dtype_src == BRK_C128)
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(grid), dim3(256), 0, s, (cpx<float>*)dst,
+                       (const cpx<double>*)src, size);
+  else if (dtype_dst == BRK_C128 && dtype_src == BRK_C64)
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(grid), dim3(256), 0, s, (cpx<double>*)dst,
+                       (const cpx<float>*)src, size);
+  else if (dtype_dst == BRK_C128)
+    hipLaunchKernelGGL((k_convert<double, double>), dim3(grid), dim3(256), 0, s, (cpx<double>*)dst,
+                       (const cpx<double>*)src, size);
+  else
+    hipLaunchKernelGGL((k_convert<float, float>), dim3(grid), dim3(256), 0, s, (cpx<float>*)dst,
+                       (const cpx<float>*)src, size);
+  return check_launch("convert");
+}
+
+int brk_copy(void* dst, const void* src, uint64_t size, int dtype, void* stream) {
+  unsigned nq;
+  if (int st = check_state(dst, size, dtype, &nq)) return st;
+  if (int st = check_state(src, size, dtype, &nq)) return st;
+  uint64_t blocks = (size + 1023) / 1024;
+  if (blocks > 0x7fffffffull) return fail(BRK_ERR_INVALID_ARGUMENT, "state too large for copy grid");
+  if (dtype == BRK_C128)
+    hipLaunchKernelGGL(k_copy<double>, dim3((unsigned)blocks), dim3(256), 0, as_stream(stream),
+                       (cpx<double>*)dst, (const cpx<double>*)src, size);
+  else
+    hipLaunchKernelGGL(k_copy<float>, dim3((unsigned)blocks), dim3(256), 0, as_stream(stream), (cpx<float>*)dst,
+                       (const cpx<float>*)src, size);
+  return check_launch("copy");
+}
+
+}  // extern "C"

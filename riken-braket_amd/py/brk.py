@@ -1,0 +1,273 @@
+"""ctypes binding of libbraket_hip.so (the C ABI declared in include/braket_hip.h).
+
+Used by the test and benchmark harness.  Loading fails loudly when the HIP
+library has not been built: there is no CPU fallback.
+
+Pointers are plain integers (device addresses, e.g. ``tensor.data_ptr()``);
+``stream`` is a ``hipStream_t`` as an integer (``torch.cuda.current_stream().cuda_stream``)
+or 0 for the null stream.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+ROOT = os.path.dirname(PKG)
+LIB_PATH = os.path.join(PKG, "libbraket_hip.so")
+HEADER = os.path.join(ROOT, "include", "braket_hip.h")
+
+C128, C64 = 0, 1
+OK = 0
+
+_u = C.c_uint
+_u64 = C.c_uint64
+_d = C.c_double
+_p = C.c_void_p
+_i = C.c_int
+_up = C.POINTER(C.c_uint)
+_dp = C.POINTER(C.c_double)
+
+# name -> argtypes (all return int status unless listed in _RESTYPES)
+_SIGS = {
+    "brk_status_string": [_i],
+    "brk_last_error": [],
+    "brk_version": [],
+    "brk_device_count": [C.POINTER(C.c_int)],
+    "brk_malloc": [C.POINTER(C.c_void_p), C.c_size_t],
+    "brk_free": [_p],
+    "brk_memcpy_htod": [_p, _p, C.c_size_t, _p],
+    "brk_memcpy_dtoh": [_p, _p, C.c_size_t, _p],
+    "brk_memcpy_dtod": [_p, _p, C.c_size_t, _p],
+    "brk_stream_synchronize": [_p],
+    "brk_set_basis_state": [_p, _u64, _i, _u64, _p],
+    "brk_fill_modular": [_p, _u64, _i, _dp, C.POINTER(C.c_uint64), _p],
+    "brk_fill_hash": [_p, _u64, _i, _u64, _d, _p],
+    "brk_convert": [_p, _i, _p, _i, _u64, _p],
+    "brk_copy": [_p, _p, _u64, _i, _p],
+    "brk_hadamard": [_p, _u64, _i, _u, _up, _u, _p],
+    "brk_pauli_x": [_p, _u64, _i, _up, _u, _up, _u, _p],
+    "brk_controlled_not": [_p, _u64, _i, _u, _up, _u, _p],
+    "brk_pauli_y": [_p, _u64, _i, _up, _u, _up, _u, _p],
+    "brk_pauli_z": [_p, _u64, _i, _up, _u, _p],
+    "brk_phase_shift_coeff": [_p, _u64, _i, _dp, _up, _u, _p],
+    "brk_phase_shift": [_p, _u64, _i, _d, _up, _u, _p],
+    "brk_phase_shift2": [_p, _u64, _i, _d, _d, _u, _up, _u, _p],
+    "brk_phase_shift3": [_p, _u64, _i, _d, _d, _d, _u, _up, _u, _p],
+    "brk_exponential_pauli_x": [_p, _u64, _i, _d, _up, _u, _up, _u, _p],
+    "brk_exponential_pauli_y": [_p, _u64, _i, _d, _up, _u, _up, _u, _p],
+    "brk_exponential_pauli_z": [_p, _u64, _i, _d, _up, _u, _up, _u, _p],
+    "brk_swap": [_p, _u64, _i, _u, _u, _up, _u, _p],
+    "brk_gate_1q": [_p, _u64, _i, _u, _dp, _up, _u, _p],
+    "brk_swap_permutated_local_qubits": [_p, _u64, _i, _u, _u, _p],
+    "brk_inner_product": [_p, _i, _p, _i, _u64, _dp, _p],
+    "brk_fidelity": [_p, _i, _p, _i, _u64, _dp, _p],
+    "brk_norm2": [_p, _u64, _i, _dp, _p],
+    "brk_spin_expectation_value": [_p, _u64, _i, _u, _dp, _p],
+    "brk_expectation_pauli_string": [_p, _u64, _i, C.c_char_p, _up, _u, _dp, _p],
+    "brk_expectation_ising": [_p, _u64, _i, _dp, _dp, _dp, _p],
+    "brk_apply_ising": [_p, _p, _u64, _i, _dp, _dp, _p],
+}
+_RESTYPES = {"brk_status_string": C.c_char_p, "brk_last_error": C.c_char_p, "brk_version": C.c_char_p}
+
+_lib = None
+
+
+class BrkError(RuntimeError):
+    def __init__(self, fn, status, msg):
+        super().__init__(f"{fn} failed with status {status}: {msg}")
+        self.status = status
+
+
+def header_symbols():
+    """Entry points declared in include/braket_hip.h."""
+    with open(HEADER) as f:
+        text = f.read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(brk_\w+)\s*\(", text)))
+
+
+def load(path: str = LIB_PATH):
+    """Load the engine library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"{path} not found: build the HIP library first (python -c 'import __graft_entry__ as g; g.build()')")
+    lib = C.CDLL(path)
+    for name, args in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = _RESTYPES.get(name, C.c_int)
+    _lib = lib
+    return lib
+
+
+def _check(name, status):
+    if status != OK:
+        msg = _lib.brk_last_error().decode()
+        raise BrkError(name, status, msg)
+
+
+def call(name, *args):
+    """Call brk_<name> and raise BrkError on a non-zero status."""
+    lib = load()
+    fn = getattr(lib, name)
+    st = fn(*args)
+    _check(name, st)
+    return st
+
+
+def uarr(values):
+    values = list(values)
+    if not values:
+        return None, 0
+    arr = (C.c_uint * len(values))(*values)
+    return arr, len(values)
+
+
+def darr(values):
+    values = list(values)
+    return (C.c_double * len(values))(*values)
+
+
+# ---------------------------------------------------------------------------
+# Convenience wrappers (stream defaults to 0 = null stream)
+# ---------------------------------------------------------------------------
+
+def hadamard(ptr, size, dtype, target, controls=(), stream=0):
+    c, nc = uarr(controls)
+    call("brk_hadamard", ptr, size, dtype, target, c, nc, stream)
+
+
+def pauli_x(ptr, size, dtype, targets, controls=(), stream=0):
+    t, nt = uarr(targets)
+    c, nc = uarr(controls)
+    call("brk_pauli_x", ptr, size, dtype, t, nt, c, nc, stream)
+
+
+def controlled_not(ptr, size, dtype, target, controls, stream=0):
+    c, nc = uarr(controls)
+    call("brk_controlled_not", ptr, size, dtype, target, c, nc, stream)
+
+
+def pauli_y(ptr, size, dtype, targets, controls=(), stream=0):
+    t, nt = uarr(targets)
+    c, nc = uarr(controls)
+    call("brk_pauli_y", ptr, size, dtype, t, nt, c, nc, stream)
+
+
+def pauli_z(ptr, size, dtype, qubits, stream=0):
+    q, nq = uarr(qubits)
+    call("brk_pauli_z", ptr, size, dtype, q, nq, stream)
+
+
+def phase_shift_coeff(ptr, size, dtype, coeff, qubits, stream=0):
+    q, nq = uarr(qubits)
+    call("brk_phase_shift_coeff", ptr, size, dtype, darr([coeff.real, coeff.imag]), q, nq, stream)
+
+
+def phase_shift(ptr, size, dtype, phase, qubits, stream=0):
+    q, nq = uarr(qubits)
+    call("brk_phase_shift", ptr, size, dtype, phase, q, nq, stream)
+
+
+def phase_shift2(ptr, size, dtype, p1, p2, target, controls=(), stream=0):
+    c, nc = uarr(controls)
+    call("brk_phase_shift2", ptr, size, dtype, p1, p2, target, c, nc, stream)
+
+
+def phase_shift3(ptr, size, dtype, p1, p2, p3, target, controls=(), stream=0):
+    c, nc = uarr(controls)
+    call("brk_phase_shift3", ptr, size, dtype, p1, p2, p3, target, c, nc, stream)
+
+
+def exponential_pauli_x(ptr, size, dtype, phase, targets, controls=(), stream=0):
+    t, nt = uarr(targets)
+    c, nc = uarr(controls)
+    call("brk_exponential_pauli_x", ptr, size, dtype, phase, t, nt, c, nc, stream)
+
+
+def exponential_pauli_y(ptr, size, dtype, phase, targets, controls=(), stream=0):
+    t, nt = uarr(targets)
+    c, nc = uarr(controls)
+    call("brk_exponential_pauli_y", ptr, size, dtype, phase, t, nt, c, nc, stream)
+
+
+def exponential_pauli_z(ptr, size, dtype, phase, targets, controls=(), stream=0):
+    t, nt = uarr(targets)
+    c, nc = uarr(controls)
+    call("brk_exponential_pauli_z", ptr, size, dtype, phase, t, nt, c, nc, stream)
+
+
+def swap(ptr, size, dtype, q1, q2, controls=(), stream=0):
+    c, nc = uarr(controls)
+    call("brk_swap", ptr, size, dtype, q1, q2, c, nc, stream)
+
+
+def gate_1q(ptr, size, dtype, target, matrix, controls=(), stream=0):
+    m = darr([v for z in matrix for v in (z.real, z.imag)])
+    c, nc = uarr(controls)
+    call("brk_gate_1q", ptr, size, dtype, target, m, c, nc, stream)
+
+
+def swap_permutated_local_qubits(ptr, size, dtype, p1, p2, stream=0):
+    call("brk_swap_permutated_local_qubits", ptr, size, dtype, p1, p2, stream)
+
+
+def set_basis_state(ptr, size, dtype, index, stream=0):
+    call("brk_set_basis_state", ptr, size, dtype, index, stream)
+
+
+def fill_modular(ptr, size, dtype, scales, params, stream=0):
+    call("brk_fill_modular", ptr, size, dtype, darr(scales), (C.c_uint64 * 6)(*params), stream)
+
+
+def fill_hash(ptr, size, dtype, seed, scale=1.0, stream=0):
+    call("brk_fill_hash", ptr, size, dtype, seed, scale, stream)
+
+
+def inner_product(p1, dt1, p2, dt2, size, stream=0):
+    out = (C.c_double * 2)()
+    call("brk_inner_product", p1, dt1, p2, dt2, size, out, stream)
+    return complex(out[0], out[1])
+
+
+def norm2(ptr, size, dtype, stream=0):
+    out = C.c_double()
+    call("brk_norm2", ptr, size, dtype, C.byref(out), stream)
+    return out.value
+
+
+def spin_expectation_value(ptr, size, dtype, qubit, stream=0):
+    out = (C.c_double * 3)()
+    call("brk_spin_expectation_value", ptr, size, dtype, qubit, out, stream)
+    return tuple(out)
+
+
+def expectation_pauli_string(ptr, size, dtype, pauli, qubits, stream=0):
+    q, nq = uarr(qubits)
+    out = (C.c_double * 2)()
+    call("brk_expectation_pauli_string", ptr, size, dtype, pauli.encode(), q, nq, out, stream)
+    return complex(out[0], out[1])
+
+
+def expectation_ising(ptr, size, dtype, J, h, stream=0):
+    out = C.c_double()
+    call("brk_expectation_ising", ptr, size, dtype, darr(J), darr(h), C.byref(out), stream)
+    return out.value
+
+
+def apply_ising(out_ptr, psi_ptr, size, dtype, J, h, stream=0):
+    call("brk_apply_ising", out_ptr, psi_ptr, size, dtype, darr(J), darr(h), stream)
+
+
+def copy(dst, src, size, dtype, stream=0):
+    call("brk_copy", dst, src, size, dtype, stream)
+
+
+def convert(dst, dt_dst, src, dt_src, size, stream=0):
+    call("brk_convert", dst, dt_dst, src, dt_src, size, stream)

@@ -1,0 +1,58 @@
+"""CPU checks of the C ABI boundary: the HIP library loads and exports every
+entry point include/braket_hip.h declares, the binding covers them, and
+argument errors are reported as status codes without touching a GPU."""
+import ctypes as C
+import subprocess
+
+import pytest
+
+import brk
+
+
+def test_library_exports_every_declared_symbol():
+    declared = brk.header_symbols()
+    assert len(declared) > 30
+    out = subprocess.run(["nm", "-D", "--defined-only", brk.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    missing = [s for s in declared if s not in exported]
+    assert not missing, missing
+
+
+def test_binding_covers_header():
+    lib = brk.load()
+    for name in brk.header_symbols():
+        assert hasattr(lib, name)
+        assert name in brk._SIGS, f"{name} has no ctypes signature"
+
+
+def test_status_strings():
+    lib = brk.load()
+    assert lib.brk_status_string(0) == b"ok"
+    assert lib.brk_status_string(2) == b"qubit out of range"
+
+
+# Validation happens on the host before any HIP call, so these run without a GPU.
+# The pointer is a dummy non-null value that is never dereferenced.
+DUMMY = 0x1000
+
+
+def _status(fn, *args):
+    lib = brk.load()
+    return getattr(lib, fn)(*args)
+
+
+def test_argument_errors_without_gpu():
+    u = brk.uarr
+    t, nt = u([1])
+    c, nc = u([1])
+    assert _status("brk_hadamard", None, 16, 0, 0, None, 0, None) == 1  # null state
+    assert _status("brk_hadamard", DUMMY, 12, 0, 0, None, 0, None) == 1  # not a power of two
+    assert _status("brk_hadamard", DUMMY, 16, 7, 0, None, 0, None) == 5  # bad dtype
+    assert _status("brk_hadamard", DUMMY, 16, 0, 4, None, 0, None) == 2  # qubit 4 of 4 qubits
+    assert _status("brk_hadamard", DUMMY, 16, 0, 1, c, nc, None) == 3  # target == control
+    assert _status("brk_swap", DUMMY, 16, 0, 2, 2, None, 0, None) == 3
+    many, nm = u(range(20))
+    assert _status("brk_pauli_x", DUMMY, 1 << 30, 0, many, nm, None, 0, None) == 4
+    assert _status("brk_pauli_x", DUMMY, 16, 0, None, 0, None, 0, None) == 1  # no targets
+    assert brk.load().brk_last_error() != b""

@@ -1,0 +1,123 @@
+"""GPU parity of the plain per-gate form against the CPU oracle, through the C ABI.
+
+Bar: complex128 within 1e-12 max-abs per amplitude (the reference tests'
+tolerance, ket/test/runtime_gate_numerical.cpp:80-85); data-movement gates
+(X, CNOT, SWAP, local swap) and Z/CZ negations bit-exact; complex64 within
+2e-6 of the complex128 oracle on normalised states.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import brk
+import dense as D
+from devstate import DeviceState
+from gatecases import REFERENCE_CASES, random_cases
+
+pytestmark = pytest.mark.gpu
+
+TOL128 = 1e-12
+TOL64 = 2e-6
+
+
+def note(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def run_gpu(case, host, dtype=brk.C128):
+    st = DeviceState.from_host(host, dtype)
+    try:
+        case.gpu(st.ptr, st.size, dtype)
+        return st.download()
+    finally:
+        st.free()
+
+
+@pytest.mark.parametrize("case", REFERENCE_CASES, ids=lambda c: c.name)
+def test_reference_cases(oracle, case):
+    note(f"gpu gate {case}")
+    s = D.ket_fixture_state(4)
+    expected = s.copy()
+    case.oracle(oracle, expected)
+    got = run_gpu(case, s)
+    assert np.max(np.abs(got - expected)) < TOL128
+
+
+@pytest.mark.parametrize("n", [2, 5, 9, 13, 17])
+def test_random_cases_c128(oracle, n):
+    rng = np.random.default_rng(1000 + n)
+    for case in random_cases(n, 48, seed=2000 + n, max_targets=min(3, n), max_controls=min(3, n - 1)):
+        note(f"gpu n={n} {case}")
+        s = D.random_state(n, rng)
+        expected = s.copy()
+        case.oracle(oracle, expected)
+        got = run_gpu(case, s)
+        err = np.max(np.abs(got - expected))
+        if case.kind in ("X", "SWAP", "Z"):
+            assert err == 0.0, str(case)
+        else:
+            assert err < TOL128, (str(case), err)
+
+
+@pytest.mark.parametrize("n", [6, 14])
+def test_random_cases_c64(oracle, n):
+    rng = np.random.default_rng(3000 + n)
+    for case in random_cases(n, 26, seed=4000 + n):
+        if case.kind == "M1":
+            continue  # random non-unitary matrices amplify float rounding
+        note(f"gpu c64 n={n} {case}")
+        s = D.random_state(n, rng)
+        expected = s.copy()
+        case.oracle(oracle, expected)
+        got = run_gpu(case, s.astype(np.complex64), brk.C64).astype(np.complex128)
+        assert np.max(np.abs(got - expected)) < TOL64, str(case)
+
+
+@pytest.mark.parametrize("n,targets", [(20, (0, 19)), (20, (19, 0)), (20, (3, 4)), (20, (0, 1)), (21, (7, 15))])
+def test_local_swap_exact(oracle, n, targets):
+    note(f"gpu local swap n={n} {targets}")
+    rng = np.random.default_rng(n + targets[0])
+    s = D.random_state(n, rng)
+    expected = s.copy()
+    oracle.swap_permutated_local_qubits(expected, *targets)
+    st = DeviceState.from_host(s)
+    brk.swap_permutated_local_qubits(st.ptr, st.size, brk.C128, *targets)
+    got = st.download()
+    st.free()
+    assert np.array_equal(got, expected)
+
+
+def test_every_target_position_20q(oracle):
+    # each qubit position as target, incl. the low ones sharing cache lines
+    n = 20
+    rng = np.random.default_rng(77)
+    s = D.random_state(n, rng)
+    st = DeviceState.from_host(s)
+    expected = s.copy()
+    for q in range(n):
+        note(f"gpu 20q chain q={q}")
+        brk.hadamard(st.ptr, st.size, brk.C128, q)
+        oracle.hadamard(expected, q)
+        brk.phase_shift3(st.ptr, st.size, brk.C128, 0.3 + q, 1.1, -0.7, q)
+        oracle.phase_shift3(expected, 0.3 + q, 1.1, -0.7, q)
+        brk.exponential_pauli_z(st.ptr, st.size, brk.C128, -0.1 * q, [q])
+        oracle.exponential_pauli_z(expected, -0.1 * q, [q])
+        brk.controlled_not(st.ptr, st.size, brk.C128, (q + 7) % n, [q])
+        oracle.controlled_not(expected, (q + 7) % n, [q])
+        brk.pauli_z(st.ptr, st.size, brk.C128, [q, (q + 3) % n])
+        oracle.pauli_z(expected, [q, (q + 3) % n])
+    got = st.download()
+    st.free()
+    assert np.max(np.abs(got - expected)) < TOL128
+
+
+def test_errors_are_reported(oracle):
+    st = DeviceState(4)
+    with pytest.raises(brk.BrkError) as e:
+        brk.hadamard(st.ptr, st.size, brk.C128, 9)
+    assert e.value.status == 2
+    with pytest.raises(brk.BrkError):
+        brk.swap(st.ptr, st.size, brk.C128, 1, 1)
+    st.free()
