@@ -1,0 +1,220 @@
+#!/usr/bin/env python3
+"""Benchmark: 32-qubit plain per-gate complex128 circuit on one MI355X.
+
+Workload: the seeded random circuit (200 gates, 60 % single-qubit from
+{H, X, RZ, U3}, 40 % CNOT/CZ, plus 8 local qubit swaps) on 2^32 complex128
+amplitudes (64 GiB) resident in HBM.  A step is one pass of the circuit.
+
+  value      = sum over the circuit's ops of the algorithmic bytes
+               2 * 16 B * (amplitudes the op can change), divided by the step time
+  roofline   = the kernel instantiation with the most device time, its
+               algorithmic bytes per launch over its mean launch time (HIP events
+               recorded on the stream the engine launches on)
+  cpu_baseline = the CPU oracle (ket's algorithm, -Ofast -fopenmp, all host
+               cores) on the same circuit generator at 28 qubits for ~15 s
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--qubits 32] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "riken-braket_amd", "py"))
+
+METRIC = "32q per-gate HBM GB/s (% peak); fused-c64 vs c128 circuit time at 1-F≤1e-6"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def kernel_name(op):
+    """Kernel instantiation an op launches (family, mode, inserted bits)."""
+    name, t, c, _ = op
+    if name in ("RZ", "EZ"):
+        return f"k_diag<c128,K={len(c)},parity>"
+    if name in ("Z", "U1"):
+        return f"k_diag<c128,K={len(t) + len(c)}>"
+    if name == "H":
+        return f"k_pair<c128,H,K={1 + len(c)}>"
+    if name in ("X", "LOCAL_SWAP", "SWAP"):
+        k = 1 + len(c) if name == "X" else 2 + len(c)
+        return f"k_pair<c128,swap,K={k}>"
+    return f"k_pair<c128,matrix,K={1 + len(c)}>"
+
+
+def cpu_baseline(n_cpu=28, budget_s=15.0, seed=2):
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import circuits as W
+    from oracle import Oracle
+    from oracle_circuits import run_ops
+
+    o = Oracle(fast=True)
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    o.set_num_threads(cores)
+    ops = W.random_circuit(n_cpu, 200, 8, seed=seed)
+    s = np.zeros(1 << n_cpu, dtype=np.complex128)
+    s[0] = 1
+    # warm-up: first op (page faults of the fresh array)
+    run_ops(o, s, ops[:1])
+    s[:] = 0
+    s[0] = 1
+    t0 = time.perf_counter()
+    _, done = run_ops(o, s, ops, time_budget=budget_s)
+    dt = time.perf_counter() - t0
+    nbytes = sum(W.algorithmic_bytes(op, n_cpu) for op in ops[:done])
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {
+        "value": round(nbytes / dt / 1e9, 2),
+        "unit": "GB/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"first {done} of {len(ops)} ops of the same generator at {n_cpu} qubits "
+                  f"(2^{n_cpu} c128 = {2 ** n_cpu * 16 / 2 ** 30:.0f} GiB), {dt:.1f} s, "
+                  f"oracle -Ofast -fopenmp, {cores} threads, {model}",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--qubits", type=int, default=32)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
+              f"(the state-vector path is single-device; see DESIGN.md for how it would shard)", file=sys.stderr)
+        return 2
+
+    import brk
+    import circuits as W
+    import hip
+    from devstate import DeviceState
+
+    n = args.qubits
+    ops = W.random_circuit(n, 200, 8, seed=args.seed)
+    circuit = brk.Circuit(n, ops)
+    op_bytes = [W.algorithmic_bytes(op, n) for op in ops]
+    total_bytes = sum(op_bytes)
+    log(f"device {hip.device_name()}; {n} qubits c128 ({2 ** n * 16 / 2 ** 30:.0f} GiB), {len(ops)} ops, "
+        f"{total_bytes / 1e12:.3f} TB algorithmic per step")
+
+    st = DeviceState(n)
+    stream = hip.create_stream()
+    brk.set_basis_state(st.ptr, st.size, brk.C128, 0, stream)
+    hip.synchronize(stream)
+
+    for w in range(args.warmup):
+        brk.set_basis_state(st.ptr, st.size, brk.C128, 0, stream)
+        circuit.run(st.ptr, stream=stream)
+        hip.synchronize(stream)
+        log(f"warmup {w + 1}/{args.warmup} done")
+
+    step_ms, per_op = [], [[] for _ in ops]
+    for k in range(args.steps):
+        brk.set_basis_state(st.ptr, st.size, brk.C128, 0, stream)
+        hip.synchronize(stream)
+        e0, e1 = hip.Event(), hip.Event()
+        e0.record(stream)
+        _, ms = circuit.run_timed(st.ptr, stream=stream)
+        e1.record(stream)
+        e1.synchronize()
+        step_ms.append(e0.elapsed_ms(e1))
+        for i, m in enumerate(ms):
+            per_op[i].append(m)
+        log(f"step {k + 1}/{args.steps}: {step_ms[-1]:.1f} ms, {total_bytes / (step_ms[-1] * 1e-3) / 1e9:.0f} GB/s")
+    hip.synchronize(stream)
+    norm = brk.norm2(st.ptr, st.size, brk.C128, stream)
+    st.free()
+
+    ms_per_step = sum(step_ms) / len(step_ms)
+    value = total_bytes / (ms_per_step * 1e-3) / 1e9
+
+    # per kernel instantiation: launches, bytes per launch, mean launch time
+    kernels = {}
+    for op, b, ms in zip(ops, op_bytes, per_op):
+        kn = kernel_name(op)
+        d = kernels.setdefault(kn, {"launches": 0, "bytes": 0.0, "ms": 0.0, "bytes_per_launch": set()})
+        d["launches"] += len(ms)
+        d["bytes"] += b * len(ms)
+        d["ms"] += sum(ms)
+        d["bytes_per_launch"].add(b)
+    table = {}
+    for kn, d in kernels.items():
+        table[kn] = {
+            "launches_per_step": d["launches"] // args.steps,
+            "mean_ms": round(d["ms"] / d["launches"], 4),
+            "GB/s": round(d["bytes"] / (d["ms"] * 1e-3) / 1e9, 1),
+            "share_of_time": round(d["ms"] / (ms_per_step * args.steps), 4),
+        }
+    uniform = {kn: d for kn, d in kernels.items() if len(d["bytes_per_launch"]) == 1}
+    dom_name = max(uniform, key=lambda kn: uniform[kn]["ms"])
+    dom = uniform[dom_name]
+    per_launch = next(iter(dom["bytes_per_launch"]))
+    mean_ms = dom["ms"] / dom["launches"]
+    achieved = per_launch / (mean_ms * 1e-3) / 1e9
+
+    result = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128",
+        "data": "synthetic",
+        "config": {
+            "workload": f"random circuit, {n} qubits complex128 (2^{n} amplitudes), 200 gates "
+                        f"(120 from H/X/RZ/U3, 80 CNOT/CZ) + 8 local qubit swaps, seed {args.seed}, from |0..0>",
+            "qubits": n,
+            "ops_per_step": len(ops),
+            "algorithmic_TB_per_step": round(total_bytes / 1e12, 4),
+            "parallelism": "single GPU",
+        },
+        "hbm_fraction": round(value / HBM_PEAK_GBS, 4),
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom_name,
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "bytes_per_launch": per_launch,
+            "mean_launch_ms": round(mean_ms, 4),
+            "traffic": None,
+        },
+        "kernels": table,
+        "norm_error": abs(norm - 1.0),
+    }
+    if not args.no_cpu:
+        log("cpu baseline (oracle, 28 qubits, ~15 s)")
+        result["cpu_baseline"] = cpu_baseline(seed=args.seed)
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -57,6 +57,16 @@ const char* brk_last_error(void);
 int brk_device_count(int* count);
 const char* brk_version(void);
 
+/* Launch tuning of the streaming gate kernels (no reference counterpart; the
+   reference's analogue is its compile-time cache blocking,
+   KET_DEFAULT_NUM_ON_CACHE_QUBITS, gate_bitmasks.hpp:614-616).  Keys:
+   "pair_lds", "diag_lds" (bytes of LDS reserved per block, 0..65536: caps
+   resident blocks per CU), "pair_remap", "diag_remap" (0/1: XCD-contiguous
+   tile order), "nt_min_bytes" (states at least this large are streamed with
+   non-temporal loads/stores).  Process-wide; set before launching. */
+int brk_tune(const char* key, int64_t value);
+int brk_get_tune(const char* key, int64_t* value);
+
 /* ------------------------------------------------------------------------ */
 /* Device memory and state preparation (no reference counterpart: the       */
 /* reference owns a std::vector; bra::nompi_state's constructor builds the  */
@@ -217,6 +227,53 @@ int brk_expectation_ising(const void* first, uint64_t size, int dtype, const dou
 /* out := H |psi> for the same Hamiltonian (the adjoint method's starting vector). */
 int brk_apply_ising(void* out, const void* psi, uint64_t size, int dtype, const double* J, const double* h,
                     void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Circuits: an ordered list of operations on logical qubits, run with the  */
+/* plain per-gate kernels.  Stands in for the gate list bra's interpreter   */
+/* builds (bra/src/interpreter.cpp:371-825) and executes through            */
+/* bra::nompi_state (interpreter::apply_circuit, interpreter.cpp:851-872).  */
+/* ------------------------------------------------------------------------ */
+
+typedef struct brk_circuit brk_circuit;
+
+typedef enum brk_op {
+  BRK_OP_H = 0,          /* targets[0], controls                                   */
+  BRK_OP_X = 1,          /* targets (X..X), controls                               */
+  BRK_OP_Y = 2,          /* targets (Y..Y), controls                               */
+  BRK_OP_Z = 3,          /* -1 where every bit of targets and controls is 1         */
+  BRK_OP_U1 = 4,         /* exp(i p0) where every bit of targets and controls is 1  */
+  BRK_OP_U2 = 5,         /* U2(p0, p1) on targets[0], controls                     */
+  BRK_OP_U3 = 6,         /* U3(p0, p1, p2) on targets[0], controls                 */
+  BRK_OP_EX = 7,         /* exp(i p0 X..X) on targets, controls                    */
+  BRK_OP_EY = 8,         /* exp(i p0 Y..Y)                                         */
+  BRK_OP_EZ = 9,         /* exp(i p0 Z..Z)                                         */
+  BRK_OP_RX = 10,        /* RX(p0) = EX(-p0/2)                                     */
+  BRK_OP_RY = 11,        /* RY(p0) = EY(-p0/2)                                     */
+  BRK_OP_RZ = 12,        /* RZ(p0) = EZ(-p0/2)                                     */
+  BRK_OP_SWAP = 13,      /* SWAP(targets[0], targets[1]), controls                 */
+  BRK_OP_LOCAL_SWAP = 14,/* data swap of PHYSICAL qubits targets[0], targets[1] and
+                            permutate(inverse[t0], inverse[t1]) of the permutation  */
+  BRK_OP_MATRIX = 15     /* 2x2 matrix p0..p7 (as brk_gate_1q) on targets[0]       */
+} brk_op;
+
+int brk_circuit_create(unsigned num_qubits, brk_circuit** out);
+int brk_circuit_destroy(brk_circuit* circuit);
+/* Appends one operation; qubits are validated against num_qubits here. */
+int brk_circuit_add(brk_circuit* circuit, int op, const unsigned* targets, unsigned num_targets,
+                    const unsigned* controls, unsigned num_controls, const double* params, unsigned num_params);
+int brk_circuit_num_ops(const brk_circuit* circuit, unsigned* num_ops);
+int brk_circuit_num_qubits(const brk_circuit* circuit, unsigned* num_qubits);
+/* Runs the circuit on a state of 2^num_qubits amplitudes.  `permutation`
+   (num_qubits entries, logical -> physical, in/out) maps each logical qubit
+   to its bit in the state, as ket::mpi::qubit_permutation does
+   (ket/include/ket/mpi/qubit_permutation.hpp:40-302); NULL = identity,
+   local swaps then update an internal copy only. */
+int brk_circuit_run(const brk_circuit* circuit, void* first, int dtype, unsigned* permutation, void* stream);
+/* Same, recording HIP events around every operation; op_ms[i] receives the
+   device time of operation i (synchronises the stream). */
+int brk_circuit_run_timed(const brk_circuit* circuit, void* first, int dtype, unsigned* permutation,
+                          float* op_ms, void* stream);
 
 #ifdef __cplusplus
 } /* extern "C" */

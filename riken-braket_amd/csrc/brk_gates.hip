@@ -35,56 +35,53 @@ struct pair_args {
   double u[8];        // MODE_MATRIX: m00, m01, m10, m11 as (re, im)
 };
 
-constexpr int PAIR_THREADS = 256;
-constexpr int PAIR_ITEMS = 4;
+constexpr int GATE_THREADS = 256;
 
-template <typename R, int MODE, int K>
-__global__ __launch_bounds__(PAIR_THREADS) void k_pair(cpx<R>* __restrict__ s, uint64_t ngroups, pair_args a) {
-  const uint64_t base = (uint64_t)blockIdx.x * (PAIR_THREADS * PAIR_ITEMS) + threadIdx.x;
-  uint64_t i0[PAIR_ITEMS];
-  cpx<R> x0[PAIR_ITEMS], x1[PAIR_ITEMS];
-#pragma unroll
-  for (int u = 0; u < PAIR_ITEMS; ++u) {
-    const uint64_t g = base + (uint64_t)u * PAIR_THREADS;
-    if (g < ngroups) {
-      i0[u] = expand_index<K>(g, a.m, a.k) | a.setmask;
-      x0[u] = s[i0[u]];
-      x1[u] = s[i0[u] ^ a.flipmask];
+// Dynamic LDS is reserved only to cap the number of resident blocks per CU
+// (see tuning::pair_lds): on MI355X a far-pair butterfly streams best with
+// about 16-32 KiB of loads in flight per CU; more concurrent waves cost
+// bandwidth.  The array is never read or written.
+extern __shared__ char occupancy_lds[];
+
+// One amplitude pair: (a[i0], a[i0 ^ flipmask]) := M (a[i0], a[i0 ^ flipmask]).
+template <typename R, int MODE, int K, bool NT>
+__device__ __forceinline__ void pair_update(cpx<R>* __restrict__ s, uint64_t g, const pair_args& a) {
+  const uint64_t i0 = expand_index<K>(g, a.m, a.k) | a.setmask;
+  const uint64_t i1 = i0 ^ a.flipmask;
+  const cpx<R> x0 = load_amp<NT>(s + i0);
+  const cpx<R> x1 = load_amp<NT>(s + i1);
+  cpx<R> y0, y1;
+  if constexpr (MODE == MODE_H) {
+    // (a0 + a1)/sqrt2, (a0 - a1)/sqrt2 -- ket/include/ket/gate/hadamard.hpp:66-70
+    const R r = (R)0.707106781186547524400844362104849039;
+    y0 = cscale(cadd(x0, x1), r);
+    y1 = cscale(csub(x0, x1), r);
+  } else if constexpr (MODE == MODE_SWAP) {
+    y0 = x1;
+    y1 = x0;
+  } else {
+    const cpx<R> m00 = {(R)a.u[0], (R)a.u[1]};
+    cpx<R> m01 = {(R)a.u[2], (R)a.u[3]};
+    cpx<R> m10 = {(R)a.u[4], (R)a.u[5]};
+    const cpx<R> m11 = {(R)a.u[6], (R)a.u[7]};
+    if (__popcll(i0 & a.signmask) & 1) {
+      m01 = {-m01.re, -m01.im};
+      m10 = {-m10.re, -m10.im};
     }
+    y0 = cadd(cmul(m00, x0), cmul(m01, x1));
+    y1 = cadd(cmul(m10, x0), cmul(m11, x1));
   }
-  cpx<R> m00, m01, m10, m11;
-  if constexpr (MODE == MODE_MATRIX) {
-    m00 = {(R)a.u[0], (R)a.u[1]};
-    m01 = {(R)a.u[2], (R)a.u[3]};
-    m10 = {(R)a.u[4], (R)a.u[5]};
-    m11 = {(R)a.u[6], (R)a.u[7]};
-  }
-#pragma unroll
-  for (int u = 0; u < PAIR_ITEMS; ++u) {
-    const uint64_t g = base + (uint64_t)u * PAIR_THREADS;
-    if (g < ngroups) {
-      cpx<R> y0, y1;
-      if constexpr (MODE == MODE_H) {
-        // (a0 + a1)/sqrt2, (a0 - a1)/sqrt2 -- ket/include/ket/gate/hadamard.hpp:66-70
-        const R r = (R)0.707106781186547524400844362104849039;
-        y0 = cscale(cadd(x0[u], x1[u]), r);
-        y1 = cscale(csub(x0[u], x1[u]), r);
-      } else if constexpr (MODE == MODE_SWAP) {
-        y0 = x1[u];
-        y1 = x0[u];
-      } else {
-        cpx<R> o01 = m01, o10 = m10;
-        if (__popcll(i0[u] & a.signmask) & 1) {
-          o01 = {-m01.re, -m01.im};
-          o10 = {-m10.re, -m10.im};
-        }
-        y0 = cadd(cmul(m00, x0[u]), cmul(o01, x1[u]));
-        y1 = cadd(cmul(o10, x0[u]), cmul(m11, x1[u]));
-      }
-      s[i0[u]] = y0;
-      s[i0[u] ^ a.flipmask] = y1;
-    }
-  }
+  store_amp<NT>(s + i0, y0);
+  store_amp<NT>(s + i1, y1);
+}
+
+template <typename R, int MODE, int K, bool NT>
+__global__ __launch_bounds__(GATE_THREADS) void k_pair(cpx<R>* __restrict__ s, uint64_t ngroups, pair_args a,
+                                                      int remap) {
+  if (a.k < 0) occupancy_lds[threadIdx.x] = 0;  // never taken; keeps the reservation
+  const uint64_t stride = (uint64_t)gridDim.x * GATE_THREADS;
+  for (uint64_t g = xcd_tile(blockIdx.x, gridDim.x, remap) * GATE_THREADS + threadIdx.x; g < ngroups; g += stride)
+    pair_update<R, MODE, K, NT>(s, g, a);
 }
 
 struct diag_args {
@@ -95,34 +92,20 @@ struct diag_args {
   double d0[2], d1[2];
 };
 
-constexpr int DIAG_THREADS = 256;
-constexpr int DIAG_ITEMS = 4;
-
-template <typename R, int K, bool PARITY>
-__global__ __launch_bounds__(DIAG_THREADS) void k_diag(cpx<R>* __restrict__ s, uint64_t nitems, diag_args a) {
-  const uint64_t base = (uint64_t)blockIdx.x * (DIAG_THREADS * DIAG_ITEMS) + threadIdx.x;
+template <typename R, int K, bool PARITY, bool NT>
+__global__ __launch_bounds__(GATE_THREADS) void k_diag(cpx<R>* __restrict__ s, uint64_t nitems, diag_args a,
+                                                      int remap) {
+  if (a.k < 0) occupancy_lds[threadIdx.x] = 0;  // never taken; keeps the reservation
   const cpx<R> c0 = {(R)a.d0[0], (R)a.d0[1]};
   const cpx<R> c1 = {(R)a.d1[0], (R)a.d1[1]};
-  uint64_t idx[DIAG_ITEMS];
-  cpx<R> x[DIAG_ITEMS];
-#pragma unroll
-  for (int u = 0; u < DIAG_ITEMS; ++u) {
-    const uint64_t g = base + (uint64_t)u * DIAG_THREADS;
-    if (g < nitems) {
-      idx[u] = expand_index<K>(g, a.m, a.k) | a.setmask;
-      x[u] = s[idx[u]];
+  for (uint64_t g = xcd_tile(blockIdx.x, gridDim.x, remap) * GATE_THREADS + threadIdx.x; g < nitems;
+       g += (uint64_t)gridDim.x * GATE_THREADS) {
+    const uint64_t i = expand_index<K>(g, a.m, a.k) | a.setmask;
+    cpx<R> c = c0;
+    if constexpr (PARITY) {
+      if (__popcll(i & a.zmask) & 1) c = c1;
     }
-  }
-#pragma unroll
-  for (int u = 0; u < DIAG_ITEMS; ++u) {
-    const uint64_t g = base + (uint64_t)u * DIAG_THREADS;
-    if (g < nitems) {
-      cpx<R> c = c0;
-      if constexpr (PARITY) {
-        if (__popcll(idx[u] & a.zmask) & 1) c = c1;
-      }
-      s[idx[u]] = cmul(x[u], c);
-    }
+    store_amp<NT>(s + i, cmul(load_amp<NT>(s + i), c));
   }
 }
 
@@ -130,64 +113,91 @@ __global__ __launch_bounds__(DIAG_THREADS) void k_diag(cpx<R>* __restrict__ s, u
 // Launch helpers
 // ---------------------------------------------------------------------------
 
-static int grid_blocks(uint64_t items, int per_block, unsigned* blocks) {
-  uint64_t b = (items + per_block - 1) / per_block;
+// One tile of GATE_THREADS items per block; above MAX_GATE_BLOCKS tiles the
+// blocks loop (grid stride), keeping gridDim.x * blockDim.x < 2^32.
+constexpr uint64_t MAX_GATE_BLOCKS = 1ull << 22;
+
+static int grid_blocks(uint64_t items, unsigned* blocks) {
+  uint64_t b = (items + GATE_THREADS - 1) / GATE_THREADS;
   if (b == 0) b = 1;
-  if (b > 0x7fffffffull) return fail(BRK_ERR_INVALID_ARGUMENT, "state too large");
+  if (b > MAX_GATE_BLOCKS) b = MAX_GATE_BLOCKS;
   *blocks = (unsigned)b;
   return BRK_OK;
 }
 
-template <typename R, int MODE>
+template <typename R, int MODE, bool NT>
 static void launch_pair_k(cpx<R>* s, uint64_t ngroups, const pair_args& a, unsigned blocks, hipStream_t st) {
+  const tuning& t = get_tuning();
+  const dim3 g(blocks), b(GATE_THREADS);
+  const size_t lds = (size_t)t.pair_lds;
   switch (a.k) {
-    case 1: hipLaunchKernelGGL((k_pair<R, MODE, 1>), dim3(blocks), dim3(PAIR_THREADS), 0, st, s, ngroups, a); break;
-    case 2: hipLaunchKernelGGL((k_pair<R, MODE, 2>), dim3(blocks), dim3(PAIR_THREADS), 0, st, s, ngroups, a); break;
-    case 3: hipLaunchKernelGGL((k_pair<R, MODE, 3>), dim3(blocks), dim3(PAIR_THREADS), 0, st, s, ngroups, a); break;
-    default: hipLaunchKernelGGL((k_pair<R, MODE, -1>), dim3(blocks), dim3(PAIR_THREADS), 0, st, s, ngroups, a); break;
+    case 1: hipLaunchKernelGGL((k_pair<R, MODE, 1, NT>), g, b, lds, st, s, ngroups, a, t.pair_remap); break;
+    case 2: hipLaunchKernelGGL((k_pair<R, MODE, 2, NT>), g, b, lds, st, s, ngroups, a, t.pair_remap); break;
+    case 3: hipLaunchKernelGGL((k_pair<R, MODE, 3, NT>), g, b, lds, st, s, ngroups, a, t.pair_remap); break;
+    default: hipLaunchKernelGGL((k_pair<R, MODE, -1, NT>), g, b, lds, st, s, ngroups, a, t.pair_remap); break;
   }
+}
+
+template <typename R, bool NT>
+static void launch_pair_mode(int mode, cpx<R>* s, uint64_t ngroups, const pair_args& a, unsigned blocks,
+                             hipStream_t st) {
+  if (mode == MODE_H) launch_pair_k<R, MODE_H, NT>(s, ngroups, a, blocks, st);
+  else if (mode == MODE_SWAP) launch_pair_k<R, MODE_SWAP, NT>(s, ngroups, a, blocks, st);
+  else launch_pair_k<R, MODE_MATRIX, NT>(s, ngroups, a, blocks, st);
 }
 
 static int launch_pair(int dtype, int mode, void* first, uint64_t ngroups, const pair_args& a, void* stream) {
   unsigned blocks;
-  if (int st = grid_blocks(ngroups, PAIR_THREADS * PAIR_ITEMS, &blocks)) return st;
+  if (int st = grid_blocks(ngroups, &blocks)) return st;
   hipStream_t st = as_stream(stream);
+  const uint64_t state_bytes = (ngroups << a.k) * amp_bytes(dtype);
+  const bool nt = state_bytes >= get_tuning().nt_min_bytes;
   if (dtype == BRK_C128) {
     auto* s = (cpx<double>*)first;
-    if (mode == MODE_H) launch_pair_k<double, MODE_H>(s, ngroups, a, blocks, st);
-    else if (mode == MODE_SWAP) launch_pair_k<double, MODE_SWAP>(s, ngroups, a, blocks, st);
-    else launch_pair_k<double, MODE_MATRIX>(s, ngroups, a, blocks, st);
+    if (nt) launch_pair_mode<double, true>(mode, s, ngroups, a, blocks, st);
+    else launch_pair_mode<double, false>(mode, s, ngroups, a, blocks, st);
   } else {
     auto* s = (cpx<float>*)first;
-    if (mode == MODE_H) launch_pair_k<float, MODE_H>(s, ngroups, a, blocks, st);
-    else if (mode == MODE_SWAP) launch_pair_k<float, MODE_SWAP>(s, ngroups, a, blocks, st);
-    else launch_pair_k<float, MODE_MATRIX>(s, ngroups, a, blocks, st);
+    if (nt) launch_pair_mode<float, true>(mode, s, ngroups, a, blocks, st);
+    else launch_pair_mode<float, false>(mode, s, ngroups, a, blocks, st);
   }
   return check_launch("pair gate");
 }
 
-template <typename R, bool PARITY>
+template <typename R, bool PARITY, bool NT>
 static void launch_diag_k(cpx<R>* s, uint64_t n, const diag_args& a, unsigned blocks, hipStream_t st) {
+  const tuning& t = get_tuning();
+  const dim3 g(blocks), b(GATE_THREADS);
+  const size_t lds = (size_t)t.diag_lds;
   switch (a.k) {
-    case 0: hipLaunchKernelGGL((k_diag<R, 0, PARITY>), dim3(blocks), dim3(DIAG_THREADS), 0, st, s, n, a); break;
-    case 1: hipLaunchKernelGGL((k_diag<R, 1, PARITY>), dim3(blocks), dim3(DIAG_THREADS), 0, st, s, n, a); break;
-    case 2: hipLaunchKernelGGL((k_diag<R, 2, PARITY>), dim3(blocks), dim3(DIAG_THREADS), 0, st, s, n, a); break;
-    case 3: hipLaunchKernelGGL((k_diag<R, 3, PARITY>), dim3(blocks), dim3(DIAG_THREADS), 0, st, s, n, a); break;
-    default: hipLaunchKernelGGL((k_diag<R, -1, PARITY>), dim3(blocks), dim3(DIAG_THREADS), 0, st, s, n, a); break;
+    case 0: hipLaunchKernelGGL((k_diag<R, 0, PARITY, NT>), g, b, lds, st, s, n, a, t.diag_remap); break;
+    case 1: hipLaunchKernelGGL((k_diag<R, 1, PARITY, NT>), g, b, lds, st, s, n, a, t.diag_remap); break;
+    case 2: hipLaunchKernelGGL((k_diag<R, 2, PARITY, NT>), g, b, lds, st, s, n, a, t.diag_remap); break;
+    case 3: hipLaunchKernelGGL((k_diag<R, 3, PARITY, NT>), g, b, lds, st, s, n, a, t.diag_remap); break;
+    default: hipLaunchKernelGGL((k_diag<R, -1, PARITY, NT>), g, b, lds, st, s, n, a, t.diag_remap); break;
   }
+}
+
+template <typename R, bool NT>
+static void launch_diag_p(bool parity, cpx<R>* s, uint64_t n, const diag_args& a, unsigned blocks,
+                          hipStream_t st) {
+  if (parity) launch_diag_k<R, true, NT>(s, n, a, blocks, st);
+  else launch_diag_k<R, false, NT>(s, n, a, blocks, st);
 }
 
 static int launch_diag(int dtype, void* first, uint64_t nitems, const diag_args& a, void* stream) {
   unsigned blocks;
-  if (int st = grid_blocks(nitems, DIAG_THREADS * DIAG_ITEMS, &blocks)) return st;
+  if (int st = grid_blocks(nitems, &blocks)) return st;
   hipStream_t st = as_stream(stream);
   const bool parity = a.zmask != 0;
+  const uint64_t state_bytes = (nitems << a.k) * amp_bytes(dtype);
+  const bool nt = state_bytes >= get_tuning().nt_min_bytes;
   if (dtype == BRK_C128) {
-    if (parity) launch_diag_k<double, true>((cpx<double>*)first, nitems, a, blocks, st);
-    else launch_diag_k<double, false>((cpx<double>*)first, nitems, a, blocks, st);
+    if (nt) launch_diag_p<double, true>(parity, (cpx<double>*)first, nitems, a, blocks, st);
+    else launch_diag_p<double, false>(parity, (cpx<double>*)first, nitems, a, blocks, st);
   } else {
-    if (parity) launch_diag_k<float, true>((cpx<float>*)first, nitems, a, blocks, st);
-    else launch_diag_k<float, false>((cpx<float>*)first, nitems, a, blocks, st);
+    if (nt) launch_diag_p<float, true>(parity, (cpx<float>*)first, nitems, a, blocks, st);
+    else launch_diag_p<float, false>(parity, (cpx<float>*)first, nitems, a, blocks, st);
   }
   return check_launch("diagonal gate");
 }

@@ -61,6 +61,52 @@ __device__ __forceinline__ cpx<R> cscale(cpx<R> a, R s) { return {a.re * s, a.im
 
 inline size_t amp_bytes(int dtype) { return dtype == BRK_C128 ? 16 : 8; }
 
+// One amplitude as a native 2-vector, so that it can carry the non-temporal
+// (streaming) hint: a c128 element is one 16-byte access, a c64 one 8-byte.
+template <typename R> struct vec2;
+template <> struct vec2<double> { typedef double type __attribute__((ext_vector_type(2))); };
+template <> struct vec2<float> { typedef float type __attribute__((ext_vector_type(2))); };
+
+template <bool NT, typename R>
+__device__ __forceinline__ cpx<R> load_amp(const cpx<R>* p) {
+  using V = typename vec2<R>::type;
+  V v;
+  if constexpr (NT) v = __builtin_nontemporal_load(reinterpret_cast<const V*>(p));
+  else v = *reinterpret_cast<const V*>(p);
+  return cpx<R>{v.x, v.y};
+}
+
+template <bool NT, typename R>
+__device__ __forceinline__ void store_amp(cpx<R>* p, cpx<R> c) {
+  using V = typename vec2<R>::type;
+  V v = {c.re, c.im};
+  if constexpr (NT) __builtin_nontemporal_store(v, reinterpret_cast<V*>(p));
+  else *reinterpret_cast<V*>(p) = v;
+}
+
+// Block number -> tile number.  The dispatcher hands blocks to the 8 XCDs
+// round robin (block b -> XCD b % 8); with remap on, the blocks of one XCD
+// walk one contiguous eighth of the tiles instead of every eighth tile.  The
+// map is a bijection on [0, nblocks) for any nblocks.
+__device__ __forceinline__ uint64_t xcd_tile(uint32_t b, uint32_t nblocks, int remap) {
+  if (!remap) return b;
+  const uint32_t x = b & 7u, k = b >> 3;
+  const uint32_t q = nblocks >> 3, r = nblocks & 7u;
+  return (uint64_t)(x * q + (x < r ? x : r) + k);
+}
+
+// Launch tuning of the streaming gate kernels.  Defaults come from sweeps on
+// MI355X; every field can be overridden with brk_tune() or the environment
+// variable BRK_TUNE_<KEY> (upper case) read at first use.
+struct tuning {
+  int pair_lds;        // bytes of LDS reserved per 256-thread block of the pair kernel (occupancy cap)
+  int diag_lds;        // same for the diagonal kernel
+  int pair_remap;      // XCD-contiguous tile order for the pair kernel
+  int diag_remap;      // same for the diagonal kernel
+  uint64_t nt_min_bytes;  // states at least this large use non-temporal loads/stores
+};
+tuning& get_tuning();
+
 // ---------------------------------------------------------------------------
 // Bit insertion.  A gate on k operated qubits visits 2^(n-k) groups; group
 // number g is turned into the base index of its group by inserting a zero at

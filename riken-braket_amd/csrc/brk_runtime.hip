@@ -4,6 +4,8 @@
 #include "brk_internal.hpp"
 
 #include <mutex>
+#include <cstdlib>
+#include <type_traits>
 #include <vector>
 
 namespace brk {
@@ -47,6 +49,29 @@ int collect_qubits(const unsigned* qubits, unsigned n, unsigned num_qubits, unsi
     out[(*count)++] = q;
   }
   return BRK_OK;
+}
+
+// Defaults measured on MI355X with 2^30..2^32-amplitude c128 states.
+static tuning g_tuning = {
+    /*pair_lds=*/40960, /*diag_lds=*/0, /*pair_remap=*/1, /*diag_remap=*/1,
+    /*nt_min_bytes=*/256ull << 20,
+};
+static std::once_flag g_tuning_once;
+
+static void read_tuning_env() {
+  auto get = [](const char* key, auto* field) {
+    if (const char* v = std::getenv(key)) *field = (std::remove_pointer_t<decltype(field)>)std::strtoll(v, nullptr, 0);
+  };
+  get("BRK_TUNE_PAIR_LDS", &g_tuning.pair_lds);
+  get("BRK_TUNE_DIAG_LDS", &g_tuning.diag_lds);
+  get("BRK_TUNE_PAIR_REMAP", &g_tuning.pair_remap);
+  get("BRK_TUNE_DIAG_REMAP", &g_tuning.diag_remap);
+  get("BRK_TUNE_NT_MIN_BYTES", &g_tuning.nt_min_bytes);
+}
+
+tuning& get_tuning() {
+  std::call_once(g_tuning_once, read_tuning_env);
+  return g_tuning;
 }
 
 static std::mutex g_scratch_mutex;
@@ -183,6 +208,35 @@ const char* brk_status_string(int status) {
 const char* brk_last_error(void) { return g_last_error.c_str(); }
 
 const char* brk_version(void) { return "braket-mi355x 0.1 (gfx950)"; }
+
+int brk_tune(const char* key, int64_t value) {
+  if (!key) return fail(BRK_ERR_INVALID_ARGUMENT, "key is null");
+  tuning& t = get_tuning();
+  const std::string k = key;
+  if (k == "pair_lds" || k == "diag_lds") {
+    if (value < 0 || value > 65536) return fail(BRK_ERR_INVALID_ARGUMENT, "%s must be in [0, 65536]", key);
+    (k == "pair_lds" ? t.pair_lds : t.diag_lds) = (int)value;
+  } else if (k == "pair_remap") t.pair_remap = value != 0;
+  else if (k == "diag_remap") t.diag_remap = value != 0;
+  else if (k == "nt_min_bytes") {
+    if (value < 0) return fail(BRK_ERR_INVALID_ARGUMENT, "nt_min_bytes must be >= 0");
+    t.nt_min_bytes = (uint64_t)value;
+  } else return fail(BRK_ERR_INVALID_ARGUMENT, "unknown tuning key '%s'", key);
+  return BRK_OK;
+}
+
+int brk_get_tune(const char* key, int64_t* value) {
+  if (!key || !value) return fail(BRK_ERR_INVALID_ARGUMENT, "null argument");
+  const tuning& t = get_tuning();
+  const std::string k = key;
+  if (k == "pair_lds") *value = t.pair_lds;
+  else if (k == "diag_lds") *value = t.diag_lds;
+  else if (k == "pair_remap") *value = t.pair_remap;
+  else if (k == "diag_remap") *value = t.diag_remap;
+  else if (k == "nt_min_bytes") *value = (int64_t)t.nt_min_bytes;
+  else return fail(BRK_ERR_INVALID_ARGUMENT, "unknown tuning key '%s'", key);
+  return BRK_OK;
+}
 
 int brk_device_count(int* count) {
   if (!count) return fail(BRK_ERR_INVALID_ARGUMENT, "count is null");

@@ -36,6 +36,8 @@ _SIGS = {
     "brk_last_error": [],
     "brk_version": [],
     "brk_device_count": [C.POINTER(C.c_int)],
+    "brk_tune": [C.c_char_p, C.c_int64],
+    "brk_get_tune": [C.c_char_p, C.POINTER(C.c_int64)],
     "brk_malloc": [C.POINTER(C.c_void_p), C.c_size_t],
     "brk_free": [_p],
     "brk_memcpy_htod": [_p, _p, C.c_size_t, _p],
@@ -69,6 +71,13 @@ _SIGS = {
     "brk_expectation_pauli_string": [_p, _u64, _i, C.c_char_p, _up, _u, _dp, _p],
     "brk_expectation_ising": [_p, _u64, _i, _dp, _dp, _dp, _p],
     "brk_apply_ising": [_p, _p, _u64, _i, _dp, _dp, _p],
+    "brk_circuit_create": [_u, C.POINTER(C.c_void_p)],
+    "brk_circuit_destroy": [_p],
+    "brk_circuit_add": [_p, _i, _up, _u, _up, _u, _dp, _u],
+    "brk_circuit_num_ops": [_p, _up],
+    "brk_circuit_num_qubits": [_p, _up],
+    "brk_circuit_run": [_p, _p, _i, _up, _p],
+    "brk_circuit_run_timed": [_p, _p, _i, _up, C.POINTER(C.c_float), _p],
 }
 _RESTYPES = {"brk_status_string": C.c_char_p, "brk_last_error": C.c_char_p, "brk_version": C.c_char_p}
 
@@ -218,6 +227,16 @@ def swap_permutated_local_qubits(ptr, size, dtype, p1, p2, stream=0):
     call("brk_swap_permutated_local_qubits", ptr, size, dtype, p1, p2, stream)
 
 
+def tune(key, value):
+    call("brk_tune", key.encode(), int(value))
+
+
+def get_tune(key):
+    v = C.c_int64()
+    call("brk_get_tune", key.encode(), C.byref(v))
+    return v.value
+
+
 def set_basis_state(ptr, size, dtype, index, stream=0):
     call("brk_set_basis_state", ptr, size, dtype, index, stream)
 
@@ -271,3 +290,56 @@ def copy(dst, src, size, dtype, stream=0):
 
 def convert(dst, dt_dst, src, dt_src, size, stream=0):
     call("brk_convert", dst, dt_dst, src, dt_src, size, stream)
+
+
+# ---------------------------------------------------------------------------
+# Circuits
+# ---------------------------------------------------------------------------
+
+OP = dict(H=0, X=1, Y=2, Z=3, U1=4, U2=5, U3=6, EX=7, EY=8, EZ=9, RX=10, RY=11, RZ=12, SWAP=13,
+          LOCAL_SWAP=14, MATRIX=15)
+
+
+class Circuit:
+    """Owns a brk_circuit built from (name, targets, controls, params) tuples."""
+
+    def __init__(self, num_qubits, ops=()):
+        h = C.c_void_p()
+        call("brk_circuit_create", num_qubits, C.byref(h))
+        self.h = h.value
+        self.num_qubits = num_qubits
+        self.ops = []
+        for op in ops:
+            self.add(*op)
+
+    def add(self, name, targets, controls=(), params=()):
+        t, nt = uarr(targets)
+        c, nc = uarr(controls)
+        p = darr(params) if len(params) else None
+        call("brk_circuit_add", self.h, OP[name], t, nt, c, nc, p, len(params))
+        self.ops.append((name, tuple(targets), tuple(controls), tuple(params)))
+
+    def __len__(self):
+        return len(self.ops)
+
+    def run(self, ptr, dtype=C128, permutation=None, stream=0):
+        """Runs on the device state; returns the final logical->physical permutation."""
+        perm = list(range(self.num_qubits)) if permutation is None else list(permutation)
+        pa = (C.c_uint * self.num_qubits)(*perm)
+        call("brk_circuit_run", self.h, ptr, dtype, pa, stream)
+        return list(pa)
+
+    def run_timed(self, ptr, dtype=C128, permutation=None, stream=0):
+        perm = list(range(self.num_qubits)) if permutation is None else list(permutation)
+        pa = (C.c_uint * self.num_qubits)(*perm)
+        ms = (C.c_float * max(1, len(self.ops)))()
+        call("brk_circuit_run_timed", self.h, ptr, dtype, pa, ms, stream)
+        return list(pa), list(ms)[: len(self.ops)]
+
+    def __del__(self):
+        try:
+            if self.h:
+                load().brk_circuit_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
