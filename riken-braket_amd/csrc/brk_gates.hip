@@ -1,20 +1,30 @@
 // Plain per-gate form: in-place gate kernels on a 2^n-amplitude state.
 //
-// Two kernel families cover every gate of the reference's gate set that the
-// engine exposes:
+// Kernel families (all one work item per thread, 256-thread blocks, every
+// state access a whole amplitude -- one 16-byte access for c128):
 //
-//  * k_pair  -- visits the 2^(n-k) groups of a gate with k inserted bit
-//    positions; each group yields one amplitude pair (i0, i1 = i0 ^ flip),
-//    read once and written once.  Modes: Hadamard, swap (X/CNOT/SWAP/local
-//    swap), general 2x2 matrix (U2, U3, EX, EY, Y, user matrices).
-//  * k_diag  -- visits only the amplitudes whose control bits are all 1 and
-//    multiplies each by one of two coefficients chosen by the parity of its
-//    target bits (EZ...Z, U1, Z, CZ, CnZ).  Untouched amplitudes are never
-//    read, so CZ moves a quarter of the state.
+//  * k_pair  -- the 2^(n-k) groups of a gate with k inserted bit positions
+//    each yield one amplitude pair (i0, i1 = i0 ^ flip), read once and written
+//    once.  Modes: Hadamard, swap (X/CNOT/SWAP/local swap), 2x2 matrix (U2,
+//    U3, EX, EY, Y, user matrices).  For targets >= 6 a wavefront touches 64
+//    consecutive amplitudes on each side of the pair.
+//  * k_lane  -- single-target gates on a low qubit t < 6: lane l of a
+//    wavefront owns amplitude base + l and gets its partner base + (l ^ 2^t)
+//    with a cross-lane shuffle, so every load and store of a wavefront is one
+//    contiguous KiB (no partial cache lines).
+//  * k_lane_swap -- SWAP / local swap of a low qubit (< 6) with a high one:
+//    the two halves of the high qubit stream contiguously and the low-qubit
+//    exchange happens across lanes.
+//  * k_diag  -- multiplies the amplitudes of a subspace by one of two
+//    coefficients chosen by the parity of the target bits (EZ...Z, U1, Z, CZ).
 //
-// Thread t of block b handles groups b*1024 + t + 256*u, u = 0..3: for every
-// u a wavefront touches 64 consecutive groups, i.e. contiguous amplitudes in
-// each half of the pair whenever the lowest operated qubit is >= 6.
+// Controls: a control bit at or above the line boundary (the bits addressing
+// amplitudes inside one 64-byte line) is removed from the iteration space, so
+// untouched amplitudes are never read.  A control bit below it stays in the
+// iteration and is tested per amplitude ("cond"), so that every line the
+// kernel writes is written whole -- a partial-line write costs a
+// read-modify-write in the memory system.
+//
 // Group -> index mapping: brk::expand_index (bit insertion, the same
 // decomposition as ket's index masks, gate_bitmasks.hpp:50-130).
 #include "brk_internal.hpp"
@@ -26,31 +36,51 @@ namespace brk {
 
 enum pair_mode { MODE_H = 0, MODE_SWAP = 1, MODE_MATRIX = 2 };
 
-struct pair_args {
-  uint64_t m[BRK_MAX_OPERATED_QUBITS + 1];
-  int k;
-  uint64_t setmask;   // OR'ed into the expanded index (controls, SWAP's first qubit)
-  uint64_t flipmask;  // i1 = i0 ^ flipmask
-  uint64_t signmask;  // MODE_MATRIX: off-diagonal entries change sign when parity(i0 & signmask) is odd
-  double u[8];        // MODE_MATRIX: m00, m01, m10, m11 as (re, im)
+struct gate_args {
+  uint64_t m[BRK_MAX_OPERATED_QUBITS + 1];  // expand masks
+  int k;                                    // inserted bit positions
+  uint64_t setmask;   // OR'ed into the expanded index (controls above the line boundary, SWAP's first qubit)
+  uint64_t flipmask;  // pair kernels: i1 = i0 ^ flipmask
+  uint64_t signmask;  // matrix mode: off-diagonal entries change sign when parity(i0 & signmask) is odd
+  uint64_t condmask;  // the op acts only where all these bits are 1 (controls kept in the iteration)
+  uint64_t zmask;     // diagonal kernel: parity mask selecting the second coefficient
+  uint64_t swapmask;  // k_lane swap mode on two low qubits: acts where parity(i & swapmask) is odd
+  int lane_bit;       // lane kernels: the low qubit
+  double u[8];        // matrix (m00, m01, m10, m11) or diagonal coefficients (d0, d1) as (re, im)
 };
 
 constexpr int GATE_THREADS = 256;
+constexpr unsigned LANE_BITS = 6;  // log2(wavefront size)
 
 // Dynamic LDS is reserved only to cap the number of resident blocks per CU
-// (see tuning::pair_lds): on MI355X a far-pair butterfly streams best with
-// about 16-32 KiB of loads in flight per CU; more concurrent waves cost
-// bandwidth.  The array is never read or written.
+// (tuning::pair_lds): on MI355X a far-pair butterfly streams best with about
+// 16-32 KiB of loads in flight per CU; more concurrent waves cost bandwidth.
+// The array is never read or written.
 extern __shared__ char occupancy_lds[];
 
-// One amplitude pair: (a[i0], a[i0 ^ flipmask]) := M (a[i0], a[i0 ^ flipmask]).
-template <typename R, int MODE, int K, bool NT>
-__device__ __forceinline__ void pair_update(cpx<R>* __restrict__ s, uint64_t g, const pair_args& a) {
-  const uint64_t i0 = expand_index<K>(g, a.m, a.k) | a.setmask;
-  const uint64_t i1 = i0 ^ a.flipmask;
-  const cpx<R> x0 = load_amp<NT>(s + i0);
-  const cpx<R> x1 = load_amp<NT>(s + i1);
-  cpx<R> y0, y1;
+template <typename R>
+__device__ __forceinline__ cpx<R> shfl_xor_amp(cpx<R> v, int mask) {
+  return cpx<R>{__shfl_xor(v.re, mask, 64), __shfl_xor(v.im, mask, 64)};
+}
+
+// Component-wise select (a struct-valued ?: can make the compiler go through scratch memory).
+template <typename R>
+__device__ __forceinline__ cpx<R> csel(bool c, cpx<R> a, cpx<R> b) {
+  return cpx<R>{c ? a.re : b.re, c ? a.im : b.im};
+}
+
+template <typename R>
+__device__ __forceinline__ void load_matrix(const gate_args& a, cpx<R>& m00, cpx<R>& m01, cpx<R>& m10,
+                                            cpx<R>& m11) {
+  m00 = {(R)a.u[0], (R)a.u[1]};
+  m01 = {(R)a.u[2], (R)a.u[3]};
+  m10 = {(R)a.u[4], (R)a.u[5]};
+  m11 = {(R)a.u[6], (R)a.u[7]};
+}
+
+template <typename R, int MODE>
+__device__ __forceinline__ void pair_math(const gate_args& a, uint64_t i0, cpx<R> x0, cpx<R> x1, cpx<R>& y0,
+                                          cpx<R>& y1) {
   if constexpr (MODE == MODE_H) {
     // (a0 + a1)/sqrt2, (a0 - a1)/sqrt2 -- ket/include/ket/gate/hadamard.hpp:66-70
     const R r = (R)0.707106781186547524400844362104849039;
@@ -60,52 +90,113 @@ __device__ __forceinline__ void pair_update(cpx<R>* __restrict__ s, uint64_t g, 
     y0 = x1;
     y1 = x0;
   } else {
-    const cpx<R> m00 = {(R)a.u[0], (R)a.u[1]};
-    cpx<R> m01 = {(R)a.u[2], (R)a.u[3]};
-    cpx<R> m10 = {(R)a.u[4], (R)a.u[5]};
-    const cpx<R> m11 = {(R)a.u[6], (R)a.u[7]};
-    if (__popcll(i0 & a.signmask) & 1) {
-      m01 = {-m01.re, -m01.im};
-      m10 = {-m10.re, -m10.im};
-    }
+    cpx<R> m00, m01, m10, m11;
+    load_matrix(a, m00, m01, m10, m11);
+    const bool flip = (__popcll(i0 & a.signmask) & 1) != 0;
+    m01 = csel(flip, cpx<R>{-m01.re, -m01.im}, m01);
+    m10 = csel(flip, cpx<R>{-m10.re, -m10.im}, m10);
     y0 = cadd(cmul(m00, x0), cmul(m01, x1));
     y1 = cadd(cmul(m10, x0), cmul(m11, x1));
   }
-  store_amp<NT>(s + i0, y0);
-  store_amp<NT>(s + i1, y1);
 }
 
-template <typename R, int MODE, int K, bool NT>
-__global__ __launch_bounds__(GATE_THREADS) void k_pair(cpx<R>* __restrict__ s, uint64_t ngroups, pair_args a,
+template <typename R, int MODE, int K, bool NT, bool COND>
+__global__ __launch_bounds__(GATE_THREADS) void k_pair(cpx<R>* __restrict__ s, uint64_t ngroups, gate_args a,
                                                       int remap) {
   if (a.k < 0) occupancy_lds[threadIdx.x] = 0;  // never taken; keeps the reservation
   const uint64_t stride = (uint64_t)gridDim.x * GATE_THREADS;
-  for (uint64_t g = xcd_tile(blockIdx.x, gridDim.x, remap) * GATE_THREADS + threadIdx.x; g < ngroups; g += stride)
-    pair_update<R, MODE, K, NT>(s, g, a);
+  for (uint64_t g = xcd_tile(blockIdx.x, gridDim.x, remap) * GATE_THREADS + threadIdx.x; g < ngroups;
+       g += stride) {
+    const uint64_t i0 = expand_index<K>(g, a.m, a.k) | a.setmask;
+    const uint64_t i1 = i0 ^ a.flipmask;
+    const cpx<R> x0 = load_amp<NT>(s + i0);
+    const cpx<R> x1 = load_amp<NT>(s + i1);
+    cpx<R> y0, y1;
+    pair_math<R, MODE>(a, i0, x0, x1, y0, y1);
+    if (COND) {
+      const bool on = (i0 & a.condmask) == a.condmask;
+      y0 = csel(on, y0, x0);
+      y1 = csel(on, y1, x1);
+    }
+    store_amp<NT>(s + i0, y0);
+    store_amp<NT>(s + i1, y1);
+  }
 }
 
-struct diag_args {
-  uint64_t m[BRK_MAX_OPERATED_QUBITS + 1];
-  int k;
-  uint64_t setmask;  // bits that must be 1 (controls), OR'ed into the expanded index
-  uint64_t zmask;    // parity mask selecting d1
-  double d0[2], d1[2];
-};
-
-template <typename R, int K, bool PARITY, bool NT>
-__global__ __launch_bounds__(GATE_THREADS) void k_diag(cpx<R>* __restrict__ s, uint64_t nitems, diag_args a,
+// Single target t = a.lane_bit < 6.  Item g -> amplitude i; whole wavefronts
+// are in or out of range (the item count is a multiple of 64), so every lane
+// takes part in the shuffle.
+template <typename R, int MODE, int K, bool NT, bool COND, bool SWAP2>
+__global__ __launch_bounds__(GATE_THREADS) void k_lane(cpx<R>* __restrict__ s, uint64_t nitems, gate_args a,
                                                       int remap) {
-  if (a.k < 0) occupancy_lds[threadIdx.x] = 0;  // never taken; keeps the reservation
-  const cpx<R> c0 = {(R)a.d0[0], (R)a.d0[1]};
-  const cpx<R> c1 = {(R)a.d1[0], (R)a.d1[1]};
+  if (a.k < 0) occupancy_lds[threadIdx.x] = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * GATE_THREADS;
+  const int lane_mask = SWAP2 ? (int)a.swapmask : (1 << a.lane_bit);
   for (uint64_t g = xcd_tile(blockIdx.x, gridDim.x, remap) * GATE_THREADS + threadIdx.x; g < nitems;
-       g += (uint64_t)gridDim.x * GATE_THREADS) {
+       g += stride) {
     const uint64_t i = expand_index<K>(g, a.m, a.k) | a.setmask;
-    cpx<R> c = c0;
-    if constexpr (PARITY) {
-      if (__popcll(i & a.zmask) & 1) c = c1;
+    const cpx<R> x = load_amp<NT>(s + i);
+    const cpx<R> p = shfl_xor_amp(x, lane_mask);
+    cpx<R> y;
+    if constexpr (SWAP2) {
+      y = csel((__popcll(i & a.swapmask) & 1) != 0, p, x);  // SWAP of two low qubits
+    } else {
+      const bool one = (i >> a.lane_bit) & 1ull;
+      cpx<R> y0, y1;
+      // pair (a0, a1) = (this, partner) on the 0 side, (partner, this) on the 1 side
+      pair_math<R, MODE>(a, i & ~(1ull << a.lane_bit), csel(one, p, x), csel(one, x, p), y0, y1);
+      y = csel(one, y1, y0);
     }
-    store_amp<NT>(s + i, cmul(load_amp<NT>(s + i), c));
+    if (COND) y = csel((i & a.condmask) == a.condmask, y, x);
+    store_amp<NT>(s + i, y);
+  }
+}
+
+// SWAP of the low qubit a.lane_bit (< 6) with a high qubit h (>= 6): pairs
+// i0 (h = 0) and i1 = i0 | 2^h, exchange across lanes l <-> l ^ 2^lane_bit.
+template <typename R, int K, bool NT, bool COND>
+__global__ __launch_bounds__(GATE_THREADS) void k_lane_swap(cpx<R>* __restrict__ s, uint64_t ngroups, gate_args a,
+                                                           int remap) {
+  if (a.k < 0) occupancy_lds[threadIdx.x] = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * GATE_THREADS;
+  const int lane_mask = 1 << a.lane_bit;
+  for (uint64_t g = xcd_tile(blockIdx.x, gridDim.x, remap) * GATE_THREADS + threadIdx.x; g < ngroups;
+       g += stride) {
+    const uint64_t i0 = expand_index<K>(g, a.m, a.k) | a.setmask;
+    const uint64_t i1 = i0 ^ a.flipmask;
+    const cpx<R> x0 = load_amp<NT>(s + i0);
+    const cpx<R> x1 = load_amp<NT>(s + i1);
+    const cpx<R> p1 = shfl_xor_amp(x1, lane_mask);  // value at i1 ^ 2^lane_bit
+    const cpx<R> p0 = shfl_xor_amp(x0, lane_mask);  // value at i0 ^ 2^lane_bit
+    const bool low_one = (i0 >> a.lane_bit) & 1ull;
+    cpx<R> y0 = csel(low_one, p1, x0);  // (h=0, l=1) <- (h=1, l=0)
+    cpx<R> y1 = csel(low_one, x1, p0);  // (h=1, l=0) <- (h=0, l=1)
+    if (COND) {
+      const bool on = (i0 & a.condmask) == a.condmask;
+      y0 = csel(on, y0, x0);
+      y1 = csel(on, y1, x1);
+    }
+    store_amp<NT>(s + i0, y0);
+    store_amp<NT>(s + i1, y1);
+  }
+}
+
+template <typename R, int K, bool PARITY, bool NT, bool COND>
+__global__ __launch_bounds__(GATE_THREADS) void k_diag(cpx<R>* __restrict__ s, uint64_t nitems, gate_args a,
+                                                      int remap) {
+  if (a.k < 0) occupancy_lds[threadIdx.x] = 0;
+  const cpx<R> c0 = {(R)a.u[0], (R)a.u[1]};
+  const cpx<R> c1 = {(R)a.u[2], (R)a.u[3]};
+  const uint64_t stride = (uint64_t)gridDim.x * GATE_THREADS;
+  for (uint64_t g = xcd_tile(blockIdx.x, gridDim.x, remap) * GATE_THREADS + threadIdx.x; g < nitems;
+       g += stride) {
+    const uint64_t i = expand_index<K>(g, a.m, a.k) | a.setmask;
+    const cpx<R> x = load_amp<NT>(s + i);
+    cpx<R> c = c0;
+    if constexpr (PARITY) c = csel((__popcll(i & a.zmask) & 1) != 0, c1, c0);
+    cpx<R> y = cmul(x, c);
+    if (COND) y = csel((i & a.condmask) == a.condmask, y, x);
+    store_amp<NT>(s + i, y);
   }
 }
 
@@ -113,158 +204,214 @@ __global__ __launch_bounds__(GATE_THREADS) void k_diag(cpx<R>* __restrict__ s, u
 // Launch helpers
 // ---------------------------------------------------------------------------
 
-// One tile of GATE_THREADS items per block; above MAX_GATE_BLOCKS tiles the
-// blocks loop (grid stride), keeping gridDim.x * blockDim.x < 2^32.
-constexpr uint64_t MAX_GATE_BLOCKS = 1ull << 22;
-
-static int grid_blocks(uint64_t items, unsigned* blocks) {
+static unsigned grid_blocks(uint64_t items) {
   uint64_t b = (items + GATE_THREADS - 1) / GATE_THREADS;
   if (b == 0) b = 1;
-  if (b > MAX_GATE_BLOCKS) b = MAX_GATE_BLOCKS;
-  *blocks = (unsigned)b;
+  const uint64_t cap = std::min<uint64_t>(get_tuning().max_blocks, (1ull << 24) - 1);
+  return (unsigned)std::min(b, cap);
+}
+
+// Expands a runtime K into the template instantiations K = 0..3 or generic.
+#define BRK_DISPATCH_K(k, KERNEL, ...)                               \
+  switch (k) {                                                       \
+    case 0: KERNEL(0, __VA_ARGS__); break;                           \
+    case 1: KERNEL(1, __VA_ARGS__); break;                           \
+    case 2: KERNEL(2, __VA_ARGS__); break;                           \
+    case 3: KERNEL(3, __VA_ARGS__); break;                           \
+    default: KERNEL(-1, __VA_ARGS__); break;                         \
+  }
+
+struct launch_ctx {
+  dim3 grid, block;
+  size_t lds;
+  hipStream_t st;
+  int remap;
+};
+
+template <typename R, int MODE, bool NT, bool COND>
+static void pair_k(const launch_ctx& c, cpx<R>* s, uint64_t n, const gate_args& a) {
+#define L(K, ...) hipLaunchKernelGGL((k_pair<R, MODE, K, NT, COND>), c.grid, c.block, c.lds, c.st, s, n, a, c.remap)
+  BRK_DISPATCH_K(a.k, L, 0)
+#undef L
+}
+
+template <typename R, int MODE, bool NT, bool COND, bool SWAP2>
+static void lane_k(const launch_ctx& c, cpx<R>* s, uint64_t n, const gate_args& a) {
+#define L(K, ...) \
+  hipLaunchKernelGGL((k_lane<R, MODE, K, NT, COND, SWAP2>), c.grid, c.block, c.lds, c.st, s, n, a, c.remap)
+  BRK_DISPATCH_K(a.k, L, 0)
+#undef L
+}
+
+template <typename R, bool NT, bool COND>
+static void lane_swap_k(const launch_ctx& c, cpx<R>* s, uint64_t n, const gate_args& a) {
+#define L(K, ...) hipLaunchKernelGGL((k_lane_swap<R, K, NT, COND>), c.grid, c.block, c.lds, c.st, s, n, a, c.remap)
+  BRK_DISPATCH_K(a.k, L, 0)
+#undef L
+}
+
+template <typename R, bool PARITY, bool NT, bool COND>
+static void diag_k(const launch_ctx& c, cpx<R>* s, uint64_t n, const gate_args& a) {
+#define L(K, ...) hipLaunchKernelGGL((k_diag<R, K, PARITY, NT, COND>), c.grid, c.block, c.lds, c.st, s, n, a, c.remap)
+  BRK_DISPATCH_K(a.k, L, 0)
+#undef L
+}
+
+enum kernel_family { FAM_PAIR, FAM_LANE, FAM_LANE_SWAP2, FAM_LANE_SWAP, FAM_DIAG };
+
+// Picks the instantiation from (dtype, family, mode, nt, cond, parity) and launches.
+template <typename R, bool NT, bool COND>
+static void launch_family(int family, int mode, const launch_ctx& c, cpx<R>* s, uint64_t n, const gate_args& a) {
+  switch (family) {
+    case FAM_PAIR:
+      if (mode == MODE_H) pair_k<R, MODE_H, NT, COND>(c, s, n, a);
+      else if (mode == MODE_SWAP) pair_k<R, MODE_SWAP, NT, COND>(c, s, n, a);
+      else pair_k<R, MODE_MATRIX, NT, COND>(c, s, n, a);
+      break;
+    case FAM_LANE:
+      if (mode == MODE_H) lane_k<R, MODE_H, NT, COND, false>(c, s, n, a);
+      else if (mode == MODE_SWAP) lane_k<R, MODE_SWAP, NT, COND, false>(c, s, n, a);
+      else lane_k<R, MODE_MATRIX, NT, COND, false>(c, s, n, a);
+      break;
+    case FAM_LANE_SWAP2: lane_k<R, MODE_SWAP, NT, COND, true>(c, s, n, a); break;
+    case FAM_LANE_SWAP: lane_swap_k<R, NT, COND>(c, s, n, a); break;
+    default:
+      if (a.zmask) diag_k<R, true, NT, COND>(c, s, n, a);
+      else diag_k<R, false, NT, COND>(c, s, n, a);
+      break;
+  }
+}
+
+static int launch(int dtype, int family, int mode, void* first, uint64_t size, uint64_t items, const gate_args& a,
+                  void* stream) {
+  const tuning& t = get_tuning();
+  // one load per thread (diagonal, lane kernels) vs two (pair kernels): different occupancy caps
+  const bool single = family == FAM_DIAG || family == FAM_LANE || family == FAM_LANE_SWAP2;
+  launch_ctx c{dim3(grid_blocks(items)), dim3(GATE_THREADS), (size_t)(single ? t.diag_lds : t.pair_lds),
+               as_stream(stream), single ? t.diag_remap : t.pair_remap};
+  const bool nt = size * amp_bytes(dtype) >= t.nt_min_bytes;
+  const bool cond = a.condmask != 0;
+  if (dtype == BRK_C128) {
+    auto* s = (cpx<double>*)first;
+    if (nt) cond ? launch_family<double, true, true>(family, mode, c, s, items, a)
+               : launch_family<double, true, false>(family, mode, c, s, items, a);
+    else cond ? launch_family<double, false, true>(family, mode, c, s, items, a)
+              : launch_family<double, false, false>(family, mode, c, s, items, a);
+  } else {
+    auto* s = (cpx<float>*)first;
+    if (nt) cond ? launch_family<float, true, true>(family, mode, c, s, items, a)
+               : launch_family<float, true, false>(family, mode, c, s, items, a);
+    else cond ? launch_family<float, false, true>(family, mode, c, s, items, a)
+              : launch_family<float, false, false>(family, mode, c, s, items, a);
+  }
+  return check_launch("gate");
+}
+
+// ---------------------------------------------------------------------------
+// Host-side planning: which kernel, which bits are inserted, which tested
+// ---------------------------------------------------------------------------
+
+// Number of low bits addressing amplitudes inside one line (tuning::line_bytes).
+static unsigned line_bits(int dtype) {
+  const uint64_t per_line = std::max<uint64_t>(1, get_tuning().line_bytes / amp_bytes(dtype));
+  return (unsigned)__builtin_ctzll(per_line);
+}
+
+struct plan {
+  unsigned pos[BRK_MAX_OPERATED_QUBITS];  // inserted positions
+  unsigned np = 0;
+  uint64_t setmask = 0, condmask = 0;
+  void insert(unsigned q) { pos[np++] = q; }
+  void control(unsigned q, unsigned threshold) {
+    if (q < threshold) condmask |= 1ull << q;
+    else {
+      insert(q);
+      setmask |= 1ull << q;
+    }
+  }
+};
+
+static gate_args make_args(const plan& p) {
+  gate_args a{};
+  expand_masks e = make_expand_masks(p.pos, (int)p.np);
+  std::copy(e.m, e.m + BRK_MAX_OPERATED_QUBITS + 1, a.m);
+  a.k = (int)p.np;
+  a.setmask = p.setmask;
+  a.condmask = p.condmask;
+  return a;
+}
+
+struct validated {
+  unsigned nq;
+};
+
+static int validate(void* first, uint64_t size, int dtype, const unsigned* t, unsigned nt, const unsigned* c,
+                    unsigned nc, validated* v) {
+  if (int st = check_state(first, size, dtype, &v->nq)) return st;
+  unsigned ops[BRK_MAX_OPERATED_QUBITS];
+  unsigned cnt = 0;
+  if (int st = collect_qubits(t, nt, v->nq, ops, &cnt, "target")) return st;
+  if (int st = collect_qubits(c, nc, v->nq, ops, &cnt, "control")) return st;
   return BRK_OK;
 }
 
-template <typename R, int MODE, bool NT>
-static void launch_pair_k(cpx<R>* s, uint64_t ngroups, const pair_args& a, unsigned blocks, hipStream_t st) {
-  const tuning& t = get_tuning();
-  const dim3 g(blocks), b(GATE_THREADS);
-  const size_t lds = (size_t)t.pair_lds;
-  switch (a.k) {
-    case 1: hipLaunchKernelGGL((k_pair<R, MODE, 1, NT>), g, b, lds, st, s, ngroups, a, t.pair_remap); break;
-    case 2: hipLaunchKernelGGL((k_pair<R, MODE, 2, NT>), g, b, lds, st, s, ngroups, a, t.pair_remap); break;
-    case 3: hipLaunchKernelGGL((k_pair<R, MODE, 3, NT>), g, b, lds, st, s, ngroups, a, t.pair_remap); break;
-    default: hipLaunchKernelGGL((k_pair<R, MODE, -1, NT>), g, b, lds, st, s, ngroups, a, t.pair_remap); break;
-  }
-}
-
-template <typename R, bool NT>
-static void launch_pair_mode(int mode, cpx<R>* s, uint64_t ngroups, const pair_args& a, unsigned blocks,
-                             hipStream_t st) {
-  if (mode == MODE_H) launch_pair_k<R, MODE_H, NT>(s, ngroups, a, blocks, st);
-  else if (mode == MODE_SWAP) launch_pair_k<R, MODE_SWAP, NT>(s, ngroups, a, blocks, st);
-  else launch_pair_k<R, MODE_MATRIX, NT>(s, ngroups, a, blocks, st);
-}
-
-static int launch_pair(int dtype, int mode, void* first, uint64_t ngroups, const pair_args& a, void* stream) {
-  unsigned blocks;
-  if (int st = grid_blocks(ngroups, &blocks)) return st;
-  hipStream_t st = as_stream(stream);
-  const uint64_t state_bytes = (ngroups << a.k) * amp_bytes(dtype);
-  const bool nt = state_bytes >= get_tuning().nt_min_bytes;
-  if (dtype == BRK_C128) {
-    auto* s = (cpx<double>*)first;
-    if (nt) launch_pair_mode<double, true>(mode, s, ngroups, a, blocks, st);
-    else launch_pair_mode<double, false>(mode, s, ngroups, a, blocks, st);
-  } else {
-    auto* s = (cpx<float>*)first;
-    if (nt) launch_pair_mode<float, true>(mode, s, ngroups, a, blocks, st);
-    else launch_pair_mode<float, false>(mode, s, ngroups, a, blocks, st);
-  }
-  return check_launch("pair gate");
-}
-
-template <typename R, bool PARITY, bool NT>
-static void launch_diag_k(cpx<R>* s, uint64_t n, const diag_args& a, unsigned blocks, hipStream_t st) {
-  const tuning& t = get_tuning();
-  const dim3 g(blocks), b(GATE_THREADS);
-  const size_t lds = (size_t)t.diag_lds;
-  switch (a.k) {
-    case 0: hipLaunchKernelGGL((k_diag<R, 0, PARITY, NT>), g, b, lds, st, s, n, a, t.diag_remap); break;
-    case 1: hipLaunchKernelGGL((k_diag<R, 1, PARITY, NT>), g, b, lds, st, s, n, a, t.diag_remap); break;
-    case 2: hipLaunchKernelGGL((k_diag<R, 2, PARITY, NT>), g, b, lds, st, s, n, a, t.diag_remap); break;
-    case 3: hipLaunchKernelGGL((k_diag<R, 3, PARITY, NT>), g, b, lds, st, s, n, a, t.diag_remap); break;
-    default: hipLaunchKernelGGL((k_diag<R, -1, PARITY, NT>), g, b, lds, st, s, n, a, t.diag_remap); break;
-  }
-}
-
-template <typename R, bool NT>
-static void launch_diag_p(bool parity, cpx<R>* s, uint64_t n, const diag_args& a, unsigned blocks,
-                          hipStream_t st) {
-  if (parity) launch_diag_k<R, true, NT>(s, n, a, blocks, st);
-  else launch_diag_k<R, false, NT>(s, n, a, blocks, st);
-}
-
-static int launch_diag(int dtype, void* first, uint64_t nitems, const diag_args& a, void* stream) {
-  unsigned blocks;
-  if (int st = grid_blocks(nitems, &blocks)) return st;
-  hipStream_t st = as_stream(stream);
-  const bool parity = a.zmask != 0;
-  const uint64_t state_bytes = (nitems << a.k) * amp_bytes(dtype);
-  const bool nt = state_bytes >= get_tuning().nt_min_bytes;
-  if (dtype == BRK_C128) {
-    if (nt) launch_diag_p<double, true>(parity, (cpx<double>*)first, nitems, a, blocks, st);
-    else launch_diag_p<double, false>(parity, (cpx<double>*)first, nitems, a, blocks, st);
-  } else {
-    if (nt) launch_diag_p<float, true>(parity, (cpx<float>*)first, nitems, a, blocks, st);
-    else launch_diag_p<float, false>(parity, (cpx<float>*)first, nitems, a, blocks, st);
-  }
-  return check_launch("diagonal gate");
-}
-
-static pair_args make_pair_args(const unsigned* positions, int k, uint64_t setmask, uint64_t flipmask) {
-  pair_args a{};
-  expand_masks e = make_expand_masks(positions, k);
-  std::copy(e.m, e.m + BRK_MAX_OPERATED_QUBITS + 1, a.m);
-  a.k = k;
-  a.setmask = setmask;
-  a.flipmask = flipmask;
-  a.signmask = 0;
-  return a;
-}
-
-static diag_args make_diag_args(const unsigned* positions, int k, uint64_t setmask, uint64_t zmask,
-                                std::complex<double> d0, std::complex<double> d1) {
-  diag_args a{};
-  expand_masks e = make_expand_masks(positions, k);
-  std::copy(e.m, e.m + BRK_MAX_OPERATED_QUBITS + 1, a.m);
-  a.k = k;
-  a.setmask = setmask;
-  a.zmask = zmask;
-  a.d0[0] = d0.real();
-  a.d0[1] = d0.imag();
-  a.d1[0] = d1.real();
-  a.d1[1] = d1.imag();
-  return a;
-}
-
-// Single target + controls, general 2x2 matrix or Hadamard mode.
+// Single target under controls (H, U2, U3, EX/EY/Y on one qubit, X/CNOT, matrices).
 static int single_target(void* first, uint64_t size, int dtype, unsigned target, const unsigned* controls,
                          unsigned nc, int mode, const double* u, void* stream) {
-  unsigned nq;
-  if (int st = check_state(first, size, dtype, &nq)) return st;
-  unsigned ops[BRK_MAX_OPERATED_QUBITS];
-  unsigned cnt = 0;
-  if (int st = collect_qubits(&target, 1, nq, ops, &cnt, "target")) return st;
-  if (int st = collect_qubits(controls, nc, nq, ops, &cnt, "control")) return st;
-  pair_args a = make_pair_args(ops, (int)cnt, mask_of(controls, nc), 1ull << target);
+  validated v;
+  if (int st = validate(first, size, dtype, &target, 1, controls, nc, &v)) return st;
+  plan p;
+  gate_args a;
+  // Below the line boundary the two amplitudes of a pair share a line: the lane
+  // kernel keeps every access a whole contiguous wavefront.
+  if (v.nq >= LANE_BITS && target < line_bits(dtype)) {
+    for (unsigned i = 0; i < nc; ++i) p.control(controls[i], LANE_BITS);
+    a = make_args(p);
+    a.lane_bit = (int)target;
+    if (u) std::copy(u, u + 8, a.u);
+    return launch(dtype, FAM_LANE, mode, first, size, size >> p.np, a, stream);
+  }
+  p.insert(target);
+  for (unsigned i = 0; i < nc; ++i) p.control(controls[i], line_bits(dtype));
+  a = make_args(p);
+  a.flipmask = 1ull << target;
   if (u) std::copy(u, u + 8, a.u);
-  return launch_pair(dtype, mode, first, size >> cnt, a, stream);
+  return launch(dtype, FAM_PAIR, mode, first, size, size >> p.np, a, stream);
 }
 
-// Targets flipped together (X...X strings) + controls; pairs (i0, i0 ^ T) with the
-// lowest target bit of i0 equal to 0.
+// Targets flipped together (X...X, Y...Y strings) under controls; pairs (i0, i0 ^ T)
+// with the lowest target bit of i0 equal to 0.
 static int multi_target_flip(void* first, uint64_t size, int dtype, const unsigned* targets, unsigned nt,
-                             const unsigned* controls, unsigned nc, int mode, const double* u,
-                             uint64_t signmask_is_targets, void* stream) {
-  unsigned nq;
-  if (int st = check_state(first, size, dtype, &nq)) return st;
+                             const unsigned* controls, unsigned nc, int mode, const double* u, bool sign_by_targets,
+                             void* stream) {
   if (nt == 0) return fail(BRK_ERR_INVALID_ARGUMENT, "no target qubits");
-  unsigned ops[BRK_MAX_OPERATED_QUBITS];
-  unsigned cnt = 0;
-  if (int st = collect_qubits(targets, nt, nq, ops, &cnt, "target")) return st;
-  if (int st = collect_qubits(controls, nc, nq, ops, &cnt, "control")) return st;
-  const uint64_t tmask = mask_of(targets, nt);
+  if (!targets) return fail(BRK_ERR_INVALID_ARGUMENT, "target list is null");
+  if (nt == 1) return single_target(first, size, dtype, targets[0], controls, nc, mode, u, stream);
+  validated v;
+  if (int st = validate(first, size, dtype, targets, nt, controls, nc, &v)) return st;
   unsigned lowest = targets[0];
   for (unsigned i = 1; i < nt; ++i) lowest = std::min(lowest, targets[i]);
-  unsigned pos[BRK_MAX_OPERATED_QUBITS];
-  unsigned np = 0;
-  pos[np++] = lowest;
-  for (unsigned i = 0; i < nc; ++i) pos[np++] = controls[i];
-  pair_args a = make_pair_args(pos, (int)np, mask_of(controls, nc), tmask);
+  plan p;
+  p.insert(lowest);
+  for (unsigned i = 0; i < nc; ++i) p.control(controls[i], line_bits(dtype));
+  gate_args a = make_args(p);
+  a.flipmask = mask_of(targets, nt);
+  if (sign_by_targets) a.signmask = a.flipmask;
   if (u) std::copy(u, u + 8, a.u);
-  if (signmask_is_targets) a.signmask = tmask;
-  return launch_pair(dtype, mode, first, size >> np, a, stream);
+  return launch(dtype, FAM_PAIR, mode, first, size, size >> p.np, a, stream);
+}
+
+static int diagonal(void* first, uint64_t size, int dtype, const unsigned* controls, unsigned nc, uint64_t zmask,
+                    std::complex<double> d0, std::complex<double> d1, void* stream) {
+  plan p;
+  for (unsigned i = 0; i < nc; ++i) p.control(controls[i], line_bits(dtype));
+  gate_args a = make_args(p);
+  a.zmask = zmask;
+  a.u[0] = d0.real();
+  a.u[1] = d0.imag();
+  a.u[2] = d1.real();
+  a.u[3] = d1.imag();
+  return launch(dtype, FAM_DIAG, 0, first, size, size >> p.np, a, stream);
 }
 
 // (-i)^m as a complex number.
@@ -302,12 +449,12 @@ int brk_gate_1q(void* first, uint64_t size, int dtype, unsigned target, const do
 int brk_pauli_x(void* first, uint64_t size, int dtype, const unsigned* targets, unsigned num_targets,
                 const unsigned* controls, unsigned num_controls, void* stream) {
   return multi_target_flip(first, size, dtype, targets, num_targets, controls, num_controls, MODE_SWAP, nullptr,
-                           0, stream);
+                           false, stream);
 }
 
 int brk_controlled_not(void* first, uint64_t size, int dtype, unsigned target, const unsigned* controls,
                        unsigned num_controls, void* stream) {
-  return multi_target_flip(first, size, dtype, &target, 1, controls, num_controls, MODE_SWAP, nullptr, 0, stream);
+  return single_target(first, size, dtype, target, controls, num_controls, MODE_SWAP, nullptr, stream);
 }
 
 int brk_pauli_y(void* first, uint64_t size, int dtype, const unsigned* targets, unsigned num_targets,
@@ -321,40 +468,32 @@ int brk_pauli_y(void* first, uint64_t size, int dtype, const unsigned* targets, 
   put(u, 1, ph);
   put(u, 2, ph * sgn_m);
   put(u, 3, {0.0, 0.0});
-  return multi_target_flip(first, size, dtype, targets, num_targets, controls, num_controls, MODE_MATRIX, u, 1,
+  return multi_target_flip(first, size, dtype, targets, num_targets, controls, num_controls, MODE_MATRIX, u, true,
                            stream);
 }
 
 int brk_pauli_z(void* first, uint64_t size, int dtype, const unsigned* qubits, unsigned num_qubits, void* stream) {
-  unsigned nq;
-  if (int st = check_state(first, size, dtype, &nq)) return st;
   if (num_qubits == 0) return fail(BRK_ERR_INVALID_ARGUMENT, "no qubits");
-  unsigned ops[BRK_MAX_OPERATED_QUBITS];
-  unsigned cnt = 0;
-  if (int st = collect_qubits(qubits, num_qubits, nq, ops, &cnt, "control")) return st;
-  diag_args a = make_diag_args(ops, (int)cnt, mask_of(qubits, num_qubits), 0, {-1.0, 0.0}, {-1.0, 0.0});
-  return launch_diag(dtype, first, size >> cnt, a, stream);
+  validated v;
+  if (int st = validate(first, size, dtype, nullptr, 0, qubits, num_qubits, &v)) return st;
+  return diagonal(first, size, dtype, qubits, num_qubits, 0, {-1.0, 0.0}, {-1.0, 0.0}, stream);
 }
 
 int brk_phase_shift_coeff(void* first, uint64_t size, int dtype, const double coeff[2], const unsigned* qubits,
                           unsigned num_qubits, void* stream) {
-  unsigned nq;
-  if (int st = check_state(first, size, dtype, &nq)) return st;
   if (!coeff) return fail(BRK_ERR_INVALID_ARGUMENT, "coeff is null");
-  unsigned ops[BRK_MAX_OPERATED_QUBITS];
-  unsigned cnt = 0;
-  if (int st = collect_qubits(qubits, num_qubits, nq, ops, &cnt, "control")) return st;
-  std::complex<double> c{coeff[0], coeff[1]};
-  diag_args a = make_diag_args(ops, (int)cnt, mask_of(qubits, num_qubits), 0, c, c);
-  return launch_diag(dtype, first, size >> cnt, a, stream);
+  validated v;
+  if (int st = validate(first, size, dtype, nullptr, 0, qubits, num_qubits, &v)) return st;
+  const std::complex<double> c{coeff[0], coeff[1]};
+  return diagonal(first, size, dtype, qubits, num_qubits, 0, c, c, stream);
 }
 
 int brk_phase_shift(void* first, uint64_t size, int dtype, double phase, const unsigned* qubits,
                     unsigned num_qubits, void* stream) {
   // exp(i phase) computed once on the host with std::exp, as ket::utility::exp_i
   // does (ket/include/ket/utility/exp_i.hpp:24-29).
-  std::complex<double> c = std::exp(std::complex<double>{0.0, phase});
-  double cc[2] = {c.real(), c.imag()};
+  const std::complex<double> c = std::exp(std::complex<double>{0.0, phase});
+  const double cc[2] = {c.real(), c.imag()};
   return brk_phase_shift_coeff(first, size, dtype, cc, qubits, num_qubits, stream);
 }
 
@@ -363,8 +502,8 @@ int brk_phase_shift2(void* first, uint64_t size, int dtype, double phase1, doubl
   // U2(p1, p2) = 1/sqrt2 [[1, -e^{i p2}], [e^{i p1}, e^{i(p1+p2)}]]
   // (ket/include/ket/gate/phase_shift.hpp:649-652)
   const double r = 0.707106781186547524400844362104849039;
-  std::complex<double> e1 = std::exp(std::complex<double>{0.0, phase1});
-  std::complex<double> e2 = std::exp(std::complex<double>{0.0, phase2});
+  const std::complex<double> e1 = std::exp(std::complex<double>{0.0, phase1});
+  const std::complex<double> e2 = std::exp(std::complex<double>{0.0, phase2});
   double u[8];
   put(u, 0, {r, 0.0});
   put(u, 1, -e2 * r);
@@ -379,8 +518,8 @@ int brk_phase_shift3(void* first, uint64_t size, int dtype, double phase1, doubl
   // (ket/include/ket/gate/phase_shift.hpp:1195-1198, coefficients :1219-1229)
   const double sine = std::sin(0.5 * phase1);
   const double cosine = std::cos(0.5 * phase1);
-  std::complex<double> e2 = std::exp(std::complex<double>{0.0, phase2});
-  std::complex<double> e3 = std::exp(std::complex<double>{0.0, phase3});
+  const std::complex<double> e2 = std::exp(std::complex<double>{0.0, phase2});
+  const std::complex<double> e3 = std::exp(std::complex<double>{0.0, phase3});
   double u[8];
   put(u, 0, {cosine, 0.0});
   put(u, 1, -(sine * e3));
@@ -392,14 +531,14 @@ int brk_phase_shift3(void* first, uint64_t size, int dtype, double phase1, doubl
 int brk_exponential_pauli_x(void* first, uint64_t size, int dtype, double phase, const unsigned* targets,
                             unsigned num_targets, const unsigned* controls, unsigned num_controls, void* stream) {
   // exp(i phase X..X) = cos I + i sin X..X; pairs (i, i ^ T).
-  std::complex<double> e = std::exp(std::complex<double>{0.0, phase});
+  const std::complex<double> e = std::exp(std::complex<double>{0.0, phase});
   const double c = e.real(), s = e.imag();
   double u[8];
   put(u, 0, {c, 0.0});
   put(u, 1, {0.0, s});
   put(u, 2, {0.0, s});
   put(u, 3, {c, 0.0});
-  return multi_target_flip(first, size, dtype, targets, num_targets, controls, num_controls, MODE_MATRIX, u, 0,
+  return multi_target_flip(first, size, dtype, targets, num_targets, controls, num_controls, MODE_MATRIX, u, false,
                            stream);
 }
 
@@ -408,8 +547,7 @@ int brk_exponential_pauli_y(void* first, uint64_t size, int dtype, double phase,
   // exp(i phase Y..Y) = cos I + i sin Y..Y, with (Y^m a)[i] = (-i)^m (-1)^{popcount(i & T)} a[i ^ T].
   // For one target: a0' = c a0 + s a1, a1' = -s a0 + c a1
   // (ket/include/ket/gate/exponential_pauli_y.hpp:36-38, :84-87).
-  if (!targets && num_targets) return fail(BRK_ERR_INVALID_ARGUMENT, "targets is null");
-  std::complex<double> e = std::exp(std::complex<double>{0.0, phase});
+  const std::complex<double> e = std::exp(std::complex<double>{0.0, phase});
   const double c = e.real(), s = e.imag();
   const std::complex<double> ph = minus_i_pow(num_targets);
   const double sgn_m = (num_targets & 1u) ? -1.0 : 1.0;
@@ -418,7 +556,7 @@ int brk_exponential_pauli_y(void* first, uint64_t size, int dtype, double phase,
   put(u, 1, std::complex<double>{0.0, s} * ph);
   put(u, 2, std::complex<double>{0.0, s} * ph * sgn_m);
   put(u, 3, {c, 0.0});
-  return multi_target_flip(first, size, dtype, targets, num_targets, controls, num_controls, MODE_MATRIX, u, 1,
+  return multi_target_flip(first, size, dtype, targets, num_targets, controls, num_controls, MODE_MATRIX, u, true,
                            stream);
 }
 
@@ -426,33 +564,44 @@ int brk_exponential_pauli_z(void* first, uint64_t size, int dtype, double phase,
                             unsigned num_targets, const unsigned* controls, unsigned num_controls, void* stream) {
   // exp(i phase Z..Z): e^{i phase} on even parity of the target bits, e^{-i phase} on odd
   // (ket/include/ket/gate/exponential_pauli_z.hpp:36-38, :73-74).
-  unsigned nq;
-  if (int st = check_state(first, size, dtype, &nq)) return st;
   if (num_targets == 0) return fail(BRK_ERR_INVALID_ARGUMENT, "no target qubits");
-  unsigned ops[BRK_MAX_OPERATED_QUBITS];
-  unsigned cnt = 0;
-  if (int st = collect_qubits(targets, num_targets, nq, ops, &cnt, "target")) return st;
-  unsigned nt = cnt;
-  if (int st = collect_qubits(controls, num_controls, nq, ops, &cnt, "control")) return st;
-  std::complex<double> e = std::exp(std::complex<double>{0.0, phase});
-  diag_args a = make_diag_args(ops + nt, (int)(cnt - nt), mask_of(controls, num_controls),
-                               mask_of(targets, num_targets), e, std::conj(e));
-  return launch_diag(dtype, first, size >> (cnt - nt), a, stream);
+  validated v;
+  if (int st = validate(first, size, dtype, targets, num_targets, controls, num_controls, &v)) return st;
+  const std::complex<double> e = std::exp(std::complex<double>{0.0, phase});
+  return diagonal(first, size, dtype, controls, num_controls, mask_of(targets, num_targets), e, std::conj(e),
+                  stream);
 }
 
 int brk_swap(void* first, uint64_t size, int dtype, unsigned qubit1, unsigned qubit2, const unsigned* controls,
              unsigned num_controls, void* stream) {
-  unsigned nq;
-  if (int st = check_state(first, size, dtype, &nq)) return st;
-  unsigned ops[BRK_MAX_OPERATED_QUBITS];
-  unsigned cnt = 0;
-  if (int st = collect_qubits(&qubit1, 1, nq, ops, &cnt, "target")) return st;
-  if (int st = collect_qubits(&qubit2, 1, nq, ops, &cnt, "target")) return st;
-  if (int st = collect_qubits(controls, num_controls, nq, ops, &cnt, "control")) return st;
+  const unsigned t[2] = {qubit1, qubit2};
+  validated v;
+  if (int st = validate(first, size, dtype, t, 2, controls, num_controls, &v)) return st;
   // pairs ...1(q1)...0(q2)... <-> ...0(q1)...1(q2)... (ket/include/ket/gate/swap.hpp:30-32)
-  pair_args a = make_pair_args(ops, (int)cnt, mask_of(controls, num_controls) | (1ull << qubit1),
-                               (1ull << qubit1) | (1ull << qubit2));
-  return launch_pair(dtype, MODE_SWAP, first, size >> cnt, a, stream);
+  const unsigned lo = std::min(qubit1, qubit2), hi = std::max(qubit1, qubit2);
+  plan p;
+  gate_args a;
+  if (v.nq >= LANE_BITS && hi < LANE_BITS && lo < line_bits(dtype)) {
+    for (unsigned i = 0; i < num_controls; ++i) p.control(controls[i], LANE_BITS);
+    a = make_args(p);
+    a.swapmask = (1ull << lo) | (1ull << hi);
+    return launch(dtype, FAM_LANE_SWAP2, MODE_SWAP, first, size, size >> p.np, a, stream);
+  }
+  if (v.nq > LANE_BITS && lo < line_bits(dtype) && hi >= LANE_BITS) {
+    p.insert(hi);
+    for (unsigned i = 0; i < num_controls; ++i) p.control(controls[i], LANE_BITS);
+    a = make_args(p);
+    a.flipmask = 1ull << hi;
+    a.lane_bit = (int)lo;
+    return launch(dtype, FAM_LANE_SWAP, MODE_SWAP, first, size, size >> p.np, a, stream);
+  }
+  p.insert(qubit1);
+  p.insert(qubit2);
+  p.setmask |= 1ull << qubit1;
+  for (unsigned i = 0; i < num_controls; ++i) p.control(controls[i], line_bits(dtype));
+  a = make_args(p);
+  a.flipmask = (1ull << qubit1) | (1ull << qubit2);
+  return launch(dtype, FAM_PAIR, MODE_SWAP, first, size, size >> p.np, a, stream);
 }
 
 int brk_swap_permutated_local_qubits(void* first, uint64_t size, int dtype, unsigned permutated_qubit1,

@@ -104,6 +104,8 @@ struct tuning {
   int pair_remap;      // XCD-contiguous tile order for the pair kernel
   int diag_remap;      // same for the diagonal kernel
   uint64_t nt_min_bytes;  // states at least this large use non-temporal loads/stores
+  uint64_t max_blocks;    // grid cap of the gate kernels (larger states loop)
+  uint64_t line_bytes;    // write granularity: control bits addressing inside it are tested, not skipped
 };
 tuning& get_tuning();
 

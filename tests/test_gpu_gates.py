@@ -121,3 +121,55 @@ def test_errors_are_reported(oracle):
     with pytest.raises(brk.BrkError):
         brk.swap(st.ptr, st.size, brk.C128, 1, 1)
     st.free()
+
+
+def _band_cases(n):
+    """Gates placed on every band the dispatcher distinguishes: lane bits (< 6),
+    line bits (< 2 for c128), and high bits, for targets and controls."""
+    from gatecases import Case
+    out = []
+    qs = sorted({0, 1, 2, 3, 5, min(6, n - 1), n - 1})
+    for t in qs:
+        for ctrl in ([], [c for c in qs if c != t][:1], [c for c in qs if c != t][-1:],
+                     [c for c in qs if c != t][:2], [c for c in qs if c != t][-2:]):
+            out.append(Case(f"H t{t} c{ctrl}", "H", [t], list(ctrl)))
+            out.append(Case(f"U3 t{t} c{ctrl}", "U3", [t], list(ctrl), (0.3, 1.2, -0.8)))
+            out.append(Case(f"X t{t} c{ctrl}", "X", [t], list(ctrl)))
+            out.append(Case(f"Z t{t} c{ctrl}", "Z", [t], list(ctrl)))
+            out.append(Case(f"EZ t{t} c{ctrl}", "EZ", [t], list(ctrl), (0.4,)))
+    for a in qs:
+        for b in qs:
+            if a < b:
+                out.append(Case(f"SWAP {a},{b}", "SWAP", [a, b]))
+                out.append(Case(f"SWAP {b},{a}", "SWAP", [b, a]))
+                c = [q for q in qs if q not in (a, b)][:1]
+                out.append(Case(f"SWAP {a},{b} c{c}", "SWAP", [a, b], c))
+    return out
+
+
+@pytest.mark.parametrize("n", [6, 7, 10])
+def test_kernel_bands_c128(oracle, n):
+    rng = np.random.default_rng(500 + n)
+    for case in _band_cases(n):
+        note(f"gpu band n={n} {case}")
+        s = D.random_state(n, rng)
+        expected = s.copy()
+        case.oracle(oracle, expected)
+        got = run_gpu(case, s)
+        err = np.max(np.abs(got - expected))
+        if case.kind in ("X", "SWAP", "Z"):
+            assert err == 0.0, str(case)
+        else:
+            assert err < TOL128, (str(case), err)
+
+
+@pytest.mark.parametrize("n", [7])
+def test_kernel_bands_c64(oracle, n):
+    rng = np.random.default_rng(600 + n)
+    for case in _band_cases(n):
+        note(f"gpu band c64 n={n} {case}")
+        s = D.random_state(n, rng)
+        expected = s.copy()
+        case.oracle(oracle, expected)
+        got = run_gpu(case, s.astype(np.complex64), brk.C64).astype(np.complex128)
+        assert np.max(np.abs(got - expected)) < TOL64, str(case)
