@@ -11,7 +11,7 @@ amplitudes (64 GiB) resident in HBM.  A step is one pass of the circuit.
                algorithmic bytes per launch over its mean launch time (HIP events
                recorded on the stream the engine launches on)
   cpu_baseline = the CPU oracle (ket's algorithm, -Ofast -fopenmp, all host
-               cores) on the same circuit generator at 28 qubits for ~15 s
+               cores) on the same circuit generator at 30 qubits (<= 25 s)
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--qubits 32] [--no-cpu]
 """
@@ -34,22 +34,55 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def kernel_name(op):
-    """Kernel instantiation an op launches (family, mode, inserted bits)."""
+LINE_BITS = 2  # 64-byte lines of complex128 amplitudes (the engine's default line_bytes)
+LANE_BITS = 6  # wavefront of 64 lanes
+
+
+def kernel_name(op, perm, n):
+    """The kernel template instantiation the engine launches for `op` (c128 state of n qubits,
+    non-temporal access), named as rocprofv3 demangles it without the argument list.  Mirrors
+    the dispatch in riken-braket_amd/csrc/brk_gates.hip."""
     name, t, c, _ = op
-    if name in ("RZ", "EZ"):
-        return f"k_diag<c128,K={len(c)},parity>"
-    if name in ("Z", "U1"):
-        return f"k_diag<c128,K={len(t) + len(c)}>"
-    if name == "H":
-        return f"k_pair<c128,H,K={1 + len(c)}>"
-    if name in ("X", "LOCAL_SWAP", "SWAP"):
-        k = 1 + len(c) if name == "X" else 2 + len(c)
-        return f"k_pair<c128,swap,K={k}>"
-    return f"k_pair<c128,matrix,K={1 + len(c)}>"
+    if name == "LOCAL_SWAP":
+        lo, hi = sorted(t)
+        if hi < LANE_BITS and lo < LINE_BITS:
+            return "k_lane<double, 1, 0, true, false, true>"
+        if lo < LINE_BITS and hi >= LANE_BITS:
+            return "k_lane_swap<double, 1, true, false>"
+        return "k_pair<double, 1, 2, true, false>"
+    pt = [perm[q] for q in t]
+    pc = [perm[q] for q in c]
+    K = lambda k: str(k if k <= 3 else -1)
+    if name in ("Z", "U1", "RZ", "EZ"):
+        ctrl = pt + pc if name in ("Z", "U1") else pc
+        high = [q for q in ctrl if q >= LINE_BITS]
+        parity = "true" if name in ("RZ", "EZ") else "false"
+        cond = "true" if len(high) < len(ctrl) else "false"
+        return f"k_diag<double, {K(len(high))}, {parity}, true, {cond}>"
+    mode = {"H": 0, "X": 1}.get(name, 2)
+    if len(pt) == 1 and pt[0] < LINE_BITS and n >= LANE_BITS:
+        high = [q for q in pc if q >= LANE_BITS]
+        cond = "true" if len(high) < len(pc) else "false"
+        return f"k_lane<double, {mode}, {K(len(high))}, true, {cond}, false>"
+    high = [q for q in pc if q >= LINE_BITS]
+    cond = "true" if len(high) < len(pc) else "false"
+    return f"k_pair<double, {mode}, {K(1 + len(high))}, true, {cond}>"
 
 
-def cpu_baseline(n_cpu=28, budget_s=15.0, seed=2):
+def physical_kernel_names(ops, n):
+    """kernel_name for every op, replaying the logical->physical permutation."""
+    perm, inv = list(range(n)), list(range(n))
+    names = []
+    for op in ops:
+        names.append(kernel_name(op, perm, n))
+        if op[0] == "LOCAL_SWAP":
+            a, b = inv[op[1][0]], inv[op[1][1]]
+            inv[perm[a]], inv[perm[b]] = inv[perm[b]], inv[perm[a]]
+            perm[a], perm[b] = perm[b], perm[a]
+    return names
+
+
+def cpu_baseline(n_cpu=30, budget_s=25.0, seed=2):
     import numpy as np
 
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -91,6 +124,21 @@ def cpu_baseline(n_cpu=28, budget_s=15.0, seed=2):
     }
 
 
+def load_traffic(kernel, n):
+    """HBM bytes per launch of `kernel` measured by the rocprofv3 FETCH_SIZE / WRITE_SIZE
+    runs recorded in profiles/traffic.json (null when that file does not cover it)."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            data = json.load(f)
+    except (OSError, ValueError):
+        return None
+    entry = data.get("kernels", {}).get(kernel)
+    if not entry or data.get("qubits") != n:
+        return None
+    return {"bytes_per_launch": entry["bytes_per_launch"], "source": "profiles/traffic.json"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,6 +147,9 @@ def main():
     ap.add_argument("--qubits", type=int, default=32)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run the streaming copy kernel once on 2^30 c128 amplitudes (16 GiB read + "
+                         "16 GiB written) so a counter run can scale FETCH_SIZE/WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
@@ -118,8 +169,15 @@ def main():
     log(f"device {hip.device_name()}; {n} qubits c128 ({2 ** n * 16 / 2 ** 30:.0f} GiB), {len(ops)} ops, "
         f"{total_bytes / 1e12:.3f} TB algorithmic per step")
 
-    st = DeviceState(n)
     stream = hip.create_stream()
+    if args.copy_calibration:
+        a, b = DeviceState(30), DeviceState(30)
+        brk.copy(b.ptr, a.ptr, a.size, brk.C128, stream)
+        hip.synchronize(stream)
+        a.free()
+        b.free()
+        log("copy calibration: 17179869184 B read, 17179869184 B written")
+    st = DeviceState(n)
     brk.set_basis_state(st.ptr, st.size, brk.C128, 0, stream)
     hip.synchronize(stream)
 
@@ -151,8 +209,7 @@ def main():
 
     # per kernel instantiation: launches, bytes per launch, mean launch time
     kernels = {}
-    for op, b, ms in zip(ops, op_bytes, per_op):
-        kn = kernel_name(op)
+    for kn, b, ms in zip(physical_kernel_names(ops, n), op_bytes, per_op):
         d = kernels.setdefault(kn, {"launches": 0, "bytes": 0.0, "ms": 0.0, "bytes_per_launch": set()})
         d["launches"] += len(ms)
         d["bytes"] += b * len(ms)
@@ -209,8 +266,12 @@ def main():
         "kernels": table,
         "norm_error": abs(norm - 1.0),
     }
+    traffic = load_traffic(dom_name, n)
+    if traffic is not None:
+        result["roofline"]["traffic"] = traffic["bytes_per_launch"]
+        result["roofline"]["traffic_source"] = traffic["source"]
     if not args.no_cpu:
-        log("cpu baseline (oracle, 28 qubits, ~15 s)")
+        log("cpu baseline (oracle, 30 qubits, <= 25 s)")
         result["cpu_baseline"] = cpu_baseline(seed=args.seed)
     print(json.dumps(result), flush=True)
     return 0
