@@ -12,8 +12,12 @@ amplitudes (64 GiB) resident in HBM.  A step is one pass of the circuit.
                recorded on the stream the engine launches on)
   cpu_baseline = the CPU oracle (ket's algorithm, -Ofast -fopenmp, all host
                cores) on the same circuit generator at 30 qubits (<= 25 s)
+  fused_c64  = the precision form on the same circuit: greedy fusion into dense
+               4/5/6-qubit blocks (width autotuned by timing each once), complex64
+               state, FP32 MFMA GEMMs; circuit time, speedup over the complex128
+               per-gate path and infidelity 1 - |<psi_c128|psi_c64>|^2 (FP64)
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--qubits 32] [--no-cpu]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--qubits 32] [--no-cpu] [--no-fused]
 """
 from __future__ import annotations
 
@@ -139,6 +143,62 @@ def load_traffic(kernel, n):
     return {"bytes_per_launch": entry["bytes_per_launch"], "source": "profiles/traffic.json"}
 
 
+def fused_leg(circuit, plain_state, args, stream):
+    """Precision form on the same circuit: plans of width 4, 5, 6 are each timed once on a
+    complex64 copy of |0..0> (the autotuner), the fastest is timed over `steps` runs, and
+    its final state is compared with the complex128 plain-path state `plain_state`
+    (same physical layout) through an FP64 inner product."""
+    import time as _time
+
+    import brk
+    import hip
+    from devstate import DeviceState
+
+    n = circuit.num_qubits
+    st = DeviceState(n, brk.C64)
+    widths = {}
+    plans = {}
+    for w in (4, 5, 6):
+        t0 = _time.perf_counter()
+        plans[w] = brk.FusedPlan(circuit, w)
+        plan_ms = (_time.perf_counter() - t0) * 1e3
+        brk.set_basis_state(st.ptr, st.size, brk.C64, 0, stream)
+        ms = plans[w].apply_timed(st.ptr, stream=stream)
+        widths[w] = {"blocks": plans[w].num_blocks, "ms": round(sum(ms), 2), "plan_ms": round(plan_ms, 1)}
+        log(f"fused width {w}: {plans[w].num_blocks} blocks, {sum(ms):.1f} ms")
+    best = min(widths, key=lambda w: widths[w]["ms"])
+    plan = plans[best]
+    run_ms = []
+    for _ in range(max(1, args.steps)):
+        brk.set_basis_state(st.ptr, st.size, brk.C64, 0, stream)
+        hip.synchronize(stream)
+        e0, e1 = hip.Event(), hip.Event()
+        e0.record(stream)
+        plan.apply(st.ptr, stream=stream)
+        e1.record(stream)
+        e1.synchronize()
+        run_ms.append(e0.elapsed_ms(e1))
+    ip = brk.inner_product(st.ptr, brk.C64, plain_state.ptr, brk.C128, st.size, stream)
+    n64 = brk.norm2(st.ptr, st.size, brk.C64, stream)
+    n128 = brk.norm2(plain_state.ptr, st.size, brk.C128, stream)
+    st.free()
+    ms = sum(run_ms) / len(run_ms)
+    blocks = plan.num_blocks
+    flops = 8 * (1 << best) * (1 << n) * blocks
+    hbm = 2 * 8 * (1 << n) * blocks
+    return {
+        "width": best,
+        "blocks": blocks,
+        "ms_per_circuit": round(ms, 2),
+        "infidelity": 1.0 - abs(ip) ** 2 / (n64 * n128),
+        "widths": widths,
+        "mfma_tflops": round(flops / (ms * 1e-3) / 1e12, 1),
+        "mfma_fraction_fp32": round(flops / (ms * 1e-3) / 157.3e12, 4),
+        "hbm_GBps": round(hbm / (ms * 1e-3) / 1e9, 1),
+        "dtype": "c64 state, FP32 MFMA (v_mfma_f32_32x32x2_f32), FP32 accumulate; FP64 fidelity",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -147,6 +207,7 @@ def main():
     ap.add_argument("--qubits", type=int, default=32)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fused", action="store_true", help="skip the fused complex64 (precision form) leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run the streaming copy kernel once on 2^30 c128 amplitudes (16 GiB read + "
                          "16 GiB written) so a counter run can scale FETCH_SIZE/WRITE_SIZE")
@@ -202,6 +263,10 @@ def main():
         log(f"step {k + 1}/{args.steps}: {step_ms[-1]:.1f} ms, {total_bytes / (step_ms[-1] * 1e-3) / 1e9:.0f} GB/s")
     hip.synchronize(stream)
     norm = brk.norm2(st.ptr, st.size, brk.C128, stream)
+
+    fused = None
+    if not args.no_fused:
+        fused = fused_leg(circuit, st, args, stream)
     st.free()
 
     ms_per_step = sum(step_ms) / len(step_ms)
@@ -266,6 +331,9 @@ def main():
         "kernels": table,
         "norm_error": abs(norm - 1.0),
     }
+    if fused is not None:
+        fused["speedup_vs_c128"] = round(ms_per_step / fused["ms_per_circuit"], 2)
+        result["fused_c64"] = fused
     traffic = load_traffic(dom_name, n)
     if traffic is not None:
         result["roofline"]["traffic"] = traffic["bytes_per_launch"]

@@ -275,6 +275,40 @@ int brk_circuit_run(const brk_circuit* circuit, void* first, int dtype, unsigned
 int brk_circuit_run_timed(const brk_circuit* circuit, void* first, int dtype, unsigned* permutation,
                           float* op_ms, void* stream);
 
+/* ------------------------------------------------------------------------ */
+/* Precision form: greedy gate fusion into dense 4..6-qubit unitaries,      */
+/* applied to a complex64 state as batched complex GEMMs on MFMA (FP32      */
+/* multiply, FP32 accumulate).  Semantics follow bra's BEGIN/END FUSION     */
+/* (bra/src/state.cpp:2193-2217, bra/src/nompi_state.cpp:1285-1357): a      */
+/* fused block is the in-order product of its gates; the block matrix is    */
+/* formed in FP64 on the host and rounded to FP32 once.                     */
+/* ------------------------------------------------------------------------ */
+
+typedef struct brk_fused_plan brk_fused_plan;
+
+/* Plans `circuit` with blocks of exactly `width` qubits (4, 5 or 6; smaller
+   blocks are widened with identity qubits).  Gates move into an earlier block
+   only past gates on disjoint qubits (they commute).  Local swaps become
+   SWAP gates on physical qubits, so the planned state has the same physical
+   layout as brk_circuit_run's.  `permutation` (may be NULL = identity) is the
+   logical->physical map at the start.  Needs num_qubits >= width + 5.
+   The block matrices are uploaded to the current device. */
+int brk_fused_plan_create(const brk_circuit* circuit, unsigned width, const unsigned* permutation,
+                          brk_fused_plan** out);
+int brk_fused_plan_destroy(brk_fused_plan* plan);
+int brk_fused_plan_num_blocks(const brk_fused_plan* plan, unsigned* num_blocks);
+/* Block i: its qubits (sorted, `width` entries), the number of gates it holds,
+   and (if matrix != NULL) its 2^width x 2^width matrix as (re, im) pairs,
+   row-major, local index bit j <-> qubits[j]. */
+int brk_fused_plan_block(const brk_fused_plan* plan, unsigned i, unsigned* qubits, unsigned* num_gates,
+                         double* matrix);
+/* Logical->physical permutation after the circuit (num_qubits entries). */
+int brk_fused_plan_final_permutation(const brk_fused_plan* plan, unsigned* permutation);
+/* Applies every block in order to a BRK_C64 state of 2^num_qubits amplitudes. */
+int brk_fused_apply(const brk_fused_plan* plan, void* first, int dtype, void* stream);
+/* Same, with the device time of block i in block_ms[i] (synchronises). */
+int brk_fused_apply_timed(const brk_fused_plan* plan, void* first, int dtype, float* block_ms, void* stream);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

@@ -12,6 +12,7 @@
 #include <algorithm>
 
 #include "braket_hip.h"
+#include "brk_bits.hpp"
 
 namespace brk {
 
@@ -106,39 +107,12 @@ struct tuning {
   uint64_t nt_min_bytes;  // states at least this large use non-temporal loads/stores
   uint64_t max_blocks;    // grid cap of the gate kernels (larger states loop)
   uint64_t line_bytes;    // write granularity: control bits addressing inside it are tested, not skipped
+  uint64_t fused_groups;  // grid cap of the fused-block kernel (workgroups loop over the tiles)
+  int fused_chunked;      // fused tiles: 0 = interleaved over all waves, 1 = contiguous run per wave
 };
 tuning& get_tuning();
 
-// ---------------------------------------------------------------------------
-// Bit insertion.  A gate on k operated qubits visits 2^(n-k) groups; group
-// number g is turned into the base index of its group by inserting a zero at
-// every operated bit position.  With the positions sorted p_0 < ... < p_{k-1}
-// the index splits into k+1 bit fields; field j is shifted left by j:
-//     base = sum_j (g & field_mask[j]) << j.
-// ---------------------------------------------------------------------------
-
-struct expand_masks {
-  uint64_t m[BRK_MAX_OPERATED_QUBITS + 1];
-  int k;  // number of inserted bits
-};
-
-// `positions` need not be sorted; they must be distinct and < 64.
-inline expand_masks make_expand_masks(const unsigned* positions, int k) {
-  unsigned sorted[BRK_MAX_OPERATED_QUBITS];
-  std::copy(positions, positions + k, sorted);
-  std::sort(sorted, sorted + k);
-  expand_masks e{};
-  e.k = k;
-  uint64_t prev = 0;  // mask of all bits below the current field
-  for (int j = 0; j < k; ++j) {
-    // field j holds bits [sorted[j-1]-(j-1), sorted[j]-j) of g
-    uint64_t upto = (sorted[j] - j) >= 64 ? ~0ull : ((1ull << (sorted[j] - j)) - 1ull);
-    e.m[j] = upto ^ prev;
-    prev = upto;
-  }
-  e.m[k] = ~0ull ^ prev;
-  return e;
-}
+// Bit insertion: see brk_bits.hpp.
 
 template <int K>
 __device__ __forceinline__ uint64_t expand_index(uint64_t g, const uint64_t* m, int k_runtime) {

@@ -54,7 +54,7 @@ int collect_qubits(const unsigned* qubits, unsigned n, unsigned num_qubits, unsi
 // Defaults measured on MI355X with 2^30..2^32-amplitude c128 states.
 static tuning g_tuning = {
     /*pair_lds=*/40960, /*diag_lds=*/0, /*pair_remap=*/1, /*diag_remap=*/1,
-    /*nt_min_bytes=*/256ull << 20, /*max_blocks=*/(1ull << 24) - 1, /*line_bytes=*/64,
+    /*nt_min_bytes=*/256ull << 20, /*max_blocks=*/(1ull << 24) - 1, /*line_bytes=*/64, /*fused_groups=*/8192, /*fused_chunked=*/0,
 };
 static std::once_flag g_tuning_once;
 
@@ -69,6 +69,7 @@ static void read_tuning_env() {
   get("BRK_TUNE_NT_MIN_BYTES", &g_tuning.nt_min_bytes);
   get("BRK_TUNE_MAX_BLOCKS", &g_tuning.max_blocks);
   get("BRK_TUNE_LINE_BYTES", &g_tuning.line_bytes);
+  get("BRK_TUNE_FUSED_GROUPS", &g_tuning.fused_groups);
 }
 
 tuning& get_tuning() {
@@ -226,6 +227,11 @@ int brk_tune(const char* key, int64_t value) {
   } else if (k == "line_bytes") {
     if (value < 1 || value > 4096 || (value & (value - 1))) return fail(BRK_ERR_INVALID_ARGUMENT, "line_bytes must be a power of two <= 4096");
     t.line_bytes = (uint64_t)value;
+  } else if (k == "fused_chunked") {
+    t.fused_chunked = value != 0;
+  } else if (k == "fused_groups") {
+    if (value < 1 || value > (1 << 22)) return fail(BRK_ERR_INVALID_ARGUMENT, "fused_groups must be in [1, 2^22]");
+    t.fused_groups = (uint64_t)value;
   } else if (k == "max_blocks") {
     if (value < 1) return fail(BRK_ERR_INVALID_ARGUMENT, "max_blocks must be >= 1");
     t.max_blocks = (uint64_t)value;
@@ -244,6 +250,8 @@ int brk_get_tune(const char* key, int64_t* value) {
   else if (k == "nt_min_bytes") *value = (int64_t)t.nt_min_bytes;
   else if (k == "max_blocks") *value = (int64_t)t.max_blocks;
   else if (k == "line_bytes") *value = (int64_t)t.line_bytes;
+  else if (k == "fused_groups") *value = (int64_t)t.fused_groups;
+  else if (k == "fused_chunked") *value = t.fused_chunked;
   else return fail(BRK_ERR_INVALID_ARGUMENT, "unknown tuning key '%s'", key);
   return BRK_OK;
 }

@@ -7,7 +7,7 @@
 // (ket/include/ket/mpi/utility/detail/make_local_swap_qubit.hpp:62-66).
 #include "circuit.hpp"
 
-#include <hip/hip_runtime.h>
+#include <hip/hip_runtime_api.h>
 
 #include <cmath>
 #include <complex>

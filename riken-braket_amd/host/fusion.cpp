@@ -1,0 +1,318 @@
+// Precision form, host side: greedy gate fusion and the fused-plan objects of
+// the C ABI.
+//
+// Planning (FP64 on the host):
+//  1. The circuit's ops are mapped to physical qubits by replaying the
+//     logical -> physical permutation; a local swap becomes a SWAP gate on its
+//     two physical qubits (same physical result as the plain runner).
+//  2. Greedy fusion: a block starts at the first unplaced gate and scans the
+//     rest in order.  A gate joins the block when none of its qubits is
+//     "blocked" and the union of qubits stays within `width`; otherwise its
+//     qubits become blocked (a later gate on them must stay after it).  A gate
+//     that joins therefore only moves past gates on disjoint qubits, with
+//     which it commutes.
+//  3. The block matrix is the in-order product of its gates (each gate
+//     applied to every column), widened with identity qubits to exactly
+//     `width` qubits, then rounded once to FP32 for the device.
+//
+// The reference's fusion (bra BEGIN/END FUSION, nompi_state.cpp:1285-1357)
+// applies the recorded gates one by one per 2^k-amplitude group; a dense
+// block matrix gives the same result and runs as a GEMM.
+#include "circuit.hpp"
+#include "brk_fused.hpp"
+#include "brk_bits.hpp"
+
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <complex>
+#include <new>
+#include <set>
+
+namespace brk {
+int fail(int status, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+}
+
+using cd = std::complex<double>;
+
+struct brk_fused_plan {
+  struct block {
+    std::vector<unsigned> qubits;  // sorted physical positions
+    std::vector<cd> u;             // row-major 2^w x 2^w
+    unsigned num_gates = 0;
+    brk::fused_launch launch{};
+  };
+  unsigned num_qubits = 0, width = 0;
+  mutable std::vector<block> blocks;  // launch.matrix is filled in on the first apply
+  std::vector<unsigned> final_perm;
+  std::vector<float> host_matrices;   // FP32 embeddings, fused_matrix_floats(width) per block
+  mutable float* device_matrices = nullptr;
+  ~brk_fused_plan() {
+    if (device_matrices) (void)hipFree(device_matrices);
+  }
+};
+
+namespace brk {
+
+struct physical_gate {
+  std::vector<unsigned> qubits;  // gate-local bit j <-> qubits[j]
+  std::vector<std::array<double, 2>> u;
+};
+
+// y := G x on the gate's local positions `pos` inside a 2^w vector (applied in place).
+static void apply_small(std::vector<cd>& x, unsigned w, const std::vector<std::array<double, 2>>& g,
+                        const std::vector<unsigned>& pos) {
+  const unsigned m = (unsigned)pos.size(), d = 1u << m;
+  unsigned gmask = 0;
+  for (unsigned p : pos) gmask |= 1u << p;
+  std::vector<cd> in(d), out(d);
+  std::vector<unsigned> idx(d);
+  for (unsigned base = 0; base < (1u << w); ++base) {
+    if (base & gmask) continue;
+    for (unsigned v = 0; v < d; ++v) {
+      unsigned i = base;
+      for (unsigned j = 0; j < m; ++j)
+        if ((v >> j) & 1u) i |= 1u << pos[j];
+      idx[v] = i;
+      in[v] = x[i];
+    }
+    for (unsigned r = 0; r < d; ++r) {
+      cd acc{0, 0};
+      for (unsigned c = 0; c < d; ++c) acc += cd(g[(size_t)r * d + c][0], g[(size_t)r * d + c][1]) * in[c];
+      out[r] = acc;
+    }
+    for (unsigned v = 0; v < d; ++v) x[idx[v]] = out[v];
+  }
+}
+
+static uint64_t deposit(uint64_t v, const unsigned* bits, unsigned nbits) {
+  uint64_t r = 0;
+  for (unsigned b = 0; b < nbits; ++b)
+    if ((v >> b) & 1ull) r |= 1ull << bits[b];
+  return r;
+}
+
+// Tile geometry of a block: rows = block qubits, columns = the 5 lowest
+// qubits outside the block, tiles = every other qubit.
+static void build_launch(brk_fused_plan::block& blk, unsigned n, unsigned w) {
+  fused_launch& L = blk.launch;
+  L.width = (int)w;
+  std::vector<unsigned> cols;
+  for (unsigned q = 0; cols.size() < 5 && q < n; ++q)
+    if (!std::binary_search(blk.qubits.begin(), blk.qubits.end(), q)) cols.push_back(q);
+  std::vector<unsigned> tile(blk.qubits);
+  tile.insert(tile.end(), cols.begin(), cols.end());
+  expand_masks e = make_expand_masks(tile.data(), (int)tile.size());
+  std::copy(e.m, e.m + BRK_MAX_OPERATED_QUBITS + 1, L.outer_masks);
+  L.outer_k = (int)tile.size();
+  L.ntiles = 1ull << (n - tile.size());
+  for (unsigned r = 0; r < FUSED_MAX_ROWS; ++r) L.row_bits[r] = r < (1u << w) ? deposit(r, blk.qubits.data(), w) : 0;
+  for (unsigned c = 0; c < FUSED_COLUMNS; ++c) L.col_bits[c] = deposit(c, cols.data(), 5);
+}
+
+// Real embedding [[Ur, -Ui], [Ui, Ur]] of a 2^w block, stored transposed
+// (element (r, c) at c*W + r) with odd columns XOR-swizzled by 32 when W >= 64.
+static void embed(const std::vector<cd>& u, unsigned w, float* out) {
+  const unsigned R = 1u << w, W = 2 * R;
+  for (unsigned r = 0; r < W; ++r)
+    for (unsigned c = 0; c < W; ++c) {
+      const cd z = u[(size_t)(r % R) * R + (c % R)];
+      double v;
+      if ((r < R) == (c < R)) v = z.real();
+      else if (r < R) v = -z.imag();  // top-right: -Ui
+      else v = z.imag();              // bottom-left: Ui
+      const unsigned swz = (W >= 64 && (c & 1u)) ? 32u : 0u;
+      out[(size_t)c * W + (r ^ swz)] = (float)v;
+    }
+}
+
+}  // namespace brk
+
+using namespace brk;
+
+extern "C" {
+
+int brk_fused_plan_create(const brk_circuit* circuit, unsigned width, const unsigned* permutation,
+                          brk_fused_plan** out) {
+  if (!circuit || !out) return fail(BRK_ERR_INVALID_ARGUMENT, "null argument");
+  if (width < 4 || width > 6) return fail(BRK_ERR_UNSUPPORTED, "fusion width %u not in 4..6", width);
+  const unsigned n = circuit->num_qubits;
+  if (n < width + 5) return fail(BRK_ERR_UNSUPPORTED, "fusion width %u needs at least %u qubits", width, width + 5);
+  brk::permutation perm(n);
+  if (permutation && !perm.load(permutation, n)) return fail(BRK_ERR_INVALID_ARGUMENT, "bad permutation");
+
+  // 1. physical gate list
+  std::vector<physical_gate> gates;
+  gates.reserve(circuit->ops.size());
+  for (const auto& o : circuit->ops) {
+    physical_gate g;
+    if (o.code == BRK_OP_LOCAL_SWAP) {
+      brk_circuit::op sw;
+      sw.code = BRK_OP_SWAP;
+      sw.targets = o.targets;
+      std::vector<unsigned> q;
+      op_unitary(sw, q, g.u);
+      g.qubits = o.targets;
+      perm.permutate(perm.inverse[o.targets[0]], perm.inverse[o.targets[1]]);
+    } else {
+      std::vector<unsigned> logical;
+      if (!op_unitary(o, logical, g.u)) return fail(BRK_ERR_UNSUPPORTED, "op %d has no dense form", o.code);
+      if (logical.size() > width)
+        return fail(BRK_ERR_UNSUPPORTED, "op on %zu qubits does not fit fusion width %u", logical.size(), width);
+      for (unsigned q : logical) g.qubits.push_back(perm.perm[q]);
+    }
+    gates.push_back(std::move(g));
+  }
+
+  auto* plan = new (std::nothrow) brk_fused_plan;
+  if (!plan) return fail(BRK_ERR_OUT_OF_MEMORY, "cannot allocate plan");
+  plan->num_qubits = n;
+  plan->width = width;
+  plan->final_perm = perm.perm;
+
+  // 2. greedy fusion
+  std::vector<char> placed(gates.size(), 0);
+  size_t first = 0;
+  while (first < gates.size()) {
+    std::set<unsigned> S, blocked;
+    std::vector<size_t> members;
+    for (size_t gi = first; gi < gates.size(); ++gi) {
+      if (placed[gi]) continue;
+      const auto& q = gates[gi].qubits;
+      bool free_of_blocked = true;
+      std::set<unsigned> uni = S;
+      for (unsigned x : q) {
+        if (blocked.count(x)) free_of_blocked = false;
+        uni.insert(x);
+      }
+      if (free_of_blocked && uni.size() <= width) {
+        S = std::move(uni);
+        members.push_back(gi);
+        placed[gi] = 1;
+      } else {
+        blocked.insert(q.begin(), q.end());
+        if (S.size() == width && std::includes(blocked.begin(), blocked.end(), S.begin(), S.end())) break;
+      }
+    }
+    while (first < gates.size() && placed[first]) ++first;
+    // widen to exactly `width` qubits with the lowest unused ones
+    for (unsigned q = 0; S.size() < width; ++q) S.insert(q);
+    brk_fused_plan::block blk;
+    blk.qubits.assign(S.begin(), S.end());
+    blk.num_gates = (unsigned)members.size();
+    // 3. block matrix: columns of the identity pushed through the gates
+    const unsigned R = 1u << width;
+    std::vector<cd> cols((size_t)R * R);  // column-major scratch
+    std::vector<cd> col(R);
+    for (unsigned c = 0; c < R; ++c) {
+      std::fill(col.begin(), col.end(), cd{0, 0});
+      col[c] = 1.0;
+      for (size_t gi : members) {
+        std::vector<unsigned> pos;
+        for (unsigned x : gates[gi].qubits)
+          pos.push_back((unsigned)(std::lower_bound(blk.qubits.begin(), blk.qubits.end(), x) - blk.qubits.begin()));
+        apply_small(col, width, gates[gi].u, pos);
+      }
+      for (unsigned r = 0; r < R; ++r) cols[(size_t)c * R + r] = col[r];
+    }
+    blk.u.resize((size_t)R * R);
+    for (unsigned r = 0; r < R; ++r)
+      for (unsigned c = 0; c < R; ++c) blk.u[(size_t)r * R + c] = cols[(size_t)c * R + r];
+    build_launch(blk, n, width);
+    plan->blocks.push_back(std::move(blk));
+  }
+
+  // 4. FP32 real embeddings for the device (uploaded on the first apply)
+  const uint64_t per = fused_matrix_floats((int)width);
+  plan->host_matrices.resize(per * plan->blocks.size());
+  for (size_t b = 0; b < plan->blocks.size(); ++b)
+    embed(plan->blocks[b].u, width, plan->host_matrices.data() + b * per);
+  *out = plan;
+  return BRK_OK;
+}
+
+// Uploads the block matrices to the current device once.
+static int ensure_device_matrices(const brk_fused_plan* plan) {
+  if (plan->device_matrices || plan->host_matrices.empty()) return BRK_OK;
+  const size_t bytes = plan->host_matrices.size() * sizeof(float);
+  float* d = nullptr;
+  if (hipMalloc(&d, bytes) != hipSuccess) return fail(BRK_ERR_OUT_OF_MEMORY, "cannot allocate fused matrices");
+  if (hipMemcpy(d, plan->host_matrices.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(d);
+    return fail(BRK_ERR_HIP, "cannot upload fused matrices");
+  }
+  const uint64_t per = fused_matrix_floats((int)plan->width);
+  for (size_t b = 0; b < plan->blocks.size(); ++b) plan->blocks[b].launch.matrix = d + b * per;
+  plan->device_matrices = d;
+  return BRK_OK;
+}
+
+int brk_fused_plan_destroy(brk_fused_plan* plan) {
+  delete plan;
+  return BRK_OK;
+}
+
+int brk_fused_plan_num_blocks(const brk_fused_plan* plan, unsigned* num_blocks) {
+  if (!plan || !num_blocks) return fail(BRK_ERR_INVALID_ARGUMENT, "null argument");
+  *num_blocks = (unsigned)plan->blocks.size();
+  return BRK_OK;
+}
+
+int brk_fused_plan_block(const brk_fused_plan* plan, unsigned i, unsigned* qubits, unsigned* num_gates,
+                         double* matrix) {
+  if (!plan) return fail(BRK_ERR_INVALID_ARGUMENT, "plan is null");
+  if (i >= plan->blocks.size()) return fail(BRK_ERR_INVALID_ARGUMENT, "block %u out of range", i);
+  const auto& b = plan->blocks[i];
+  if (qubits) std::copy(b.qubits.begin(), b.qubits.end(), qubits);
+  if (num_gates) *num_gates = b.num_gates;
+  if (matrix)
+    for (size_t k = 0; k < b.u.size(); ++k) {
+      matrix[2 * k] = b.u[k].real();
+      matrix[2 * k + 1] = b.u[k].imag();
+    }
+  return BRK_OK;
+}
+
+int brk_fused_plan_final_permutation(const brk_fused_plan* plan, unsigned* permutation) {
+  if (!plan || !permutation) return fail(BRK_ERR_INVALID_ARGUMENT, "null argument");
+  std::copy(plan->final_perm.begin(), plan->final_perm.end(), permutation);
+  return BRK_OK;
+}
+
+static int apply_impl(const brk_fused_plan* plan, void* first, int dtype, float* block_ms, void* stream) {
+  if (!plan || !first) return fail(BRK_ERR_INVALID_ARGUMENT, "null argument");
+  if (dtype != BRK_C64) return fail(BRK_ERR_UNSUPPORTED_DTYPE, "the fused form runs on complex64 states");
+  if (int s = ensure_device_matrices(plan)) return s;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  std::vector<hipEvent_t> ev;
+  if (block_ms) {
+    ev.resize(plan->blocks.size() + 1);
+    for (auto& e : ev)
+      if (hipEventCreate(&e) != hipSuccess) return fail(BRK_ERR_HIP, "hipEventCreate failed");
+    (void)hipEventRecord(ev[0], st);
+  }
+  int status = BRK_OK;
+  for (size_t b = 0; b < plan->blocks.size() && status == BRK_OK; ++b) {
+    status = launch_fused_block(plan->blocks[b].launch, first, stream);
+    if (block_ms && status == BRK_OK) (void)hipEventRecord(ev[b + 1], st);
+  }
+  if (block_ms) {
+    if (status == BRK_OK && hipEventSynchronize(ev.back()) != hipSuccess)
+      status = fail(BRK_ERR_HIP, "hipEventSynchronize failed");
+    for (size_t b = 0; status == BRK_OK && b < plan->blocks.size(); ++b)
+      (void)hipEventElapsedTime(&block_ms[b], ev[b], ev[b + 1]);
+    for (auto& e : ev) (void)hipEventDestroy(e);
+  }
+  return status;
+}
+
+int brk_fused_apply(const brk_fused_plan* plan, void* first, int dtype, void* stream) {
+  return apply_impl(plan, first, dtype, nullptr, stream);
+}
+
+int brk_fused_apply_timed(const brk_fused_plan* plan, void* first, int dtype, float* block_ms, void* stream) {
+  if (!block_ms) return fail(BRK_ERR_INVALID_ARGUMENT, "block_ms is null");
+  return apply_impl(plan, first, dtype, block_ms, stream);
+}
+
+}  // extern "C"

@@ -78,6 +78,13 @@ _SIGS = {
     "brk_circuit_num_qubits": [_p, _up],
     "brk_circuit_run": [_p, _p, _i, _up, _p],
     "brk_circuit_run_timed": [_p, _p, _i, _up, C.POINTER(C.c_float), _p],
+    "brk_fused_plan_create": [_p, _u, _up, C.POINTER(C.c_void_p)],
+    "brk_fused_plan_destroy": [_p],
+    "brk_fused_plan_num_blocks": [_p, _up],
+    "brk_fused_plan_block": [_p, _u, _up, _up, _dp],
+    "brk_fused_plan_final_permutation": [_p, _up],
+    "brk_fused_apply": [_p, _p, _i, _p],
+    "brk_fused_apply_timed": [_p, _p, _i, C.POINTER(C.c_float), _p],
 }
 _RESTYPES = {"brk_status_string": C.c_char_p, "brk_last_error": C.c_char_p, "brk_version": C.c_char_p}
 
@@ -340,6 +347,55 @@ class Circuit:
         try:
             if self.h:
                 load().brk_circuit_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+class FusedPlan:
+    """brk_fused_plan of a Circuit (precision form)."""
+
+    def __init__(self, circuit: "Circuit", width: int, permutation=None):
+        self.num_qubits = circuit.num_qubits
+        self.width = width
+        pa = None if permutation is None else (C.c_uint * self.num_qubits)(*permutation)
+        h = C.c_void_p()
+        call("brk_fused_plan_create", circuit.h, width, pa, C.byref(h))
+        self.h = h.value
+        n = C.c_uint()
+        call("brk_fused_plan_num_blocks", self.h, C.byref(n))
+        self.num_blocks = n.value
+
+    def block(self, i, with_matrix=True):
+        import numpy as np
+        q = (C.c_uint * self.width)()
+        g = C.c_uint()
+        d = 1 << self.width
+        m = (C.c_double * (2 * d * d))() if with_matrix else None
+        call("brk_fused_plan_block", self.h, i, q, C.byref(g), m)
+        mat = None
+        if with_matrix:
+            a = np.frombuffer(m, dtype=np.float64)
+            mat = (a[0::2] + 1j * a[1::2]).reshape(d, d)
+        return list(q), g.value, mat
+
+    def final_permutation(self):
+        p = (C.c_uint * self.num_qubits)()
+        call("brk_fused_plan_final_permutation", self.h, p)
+        return list(p)
+
+    def apply(self, ptr, dtype=C64, stream=0):
+        call("brk_fused_apply", self.h, ptr, dtype, stream)
+
+    def apply_timed(self, ptr, dtype=C64, stream=0):
+        ms = (C.c_float * max(1, self.num_blocks))()
+        call("brk_fused_apply_timed", self.h, ptr, dtype, ms, stream)
+        return list(ms)[: self.num_blocks]
+
+    def __del__(self):
+        try:
+            if self.h:
+                load().brk_fused_plan_destroy(self.h)
                 self.h = None
         except Exception:
             pass
