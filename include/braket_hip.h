@@ -309,6 +309,23 @@ int brk_fused_apply(const brk_fused_plan* plan, void* first, int dtype, void* st
 /* Same, with the device time of block i in block_ms[i] (synchronises). */
 int brk_fused_apply_timed(const brk_fused_plan* plan, void* first, int dtype, float* block_ms, void* stream);
 
+/* ------------------------------------------------------------------------ */
+/* Differentiation form (no reference counterpart): adjoint (reverse-sweep) */
+/* gradient of E(theta) = <psi(theta)| H |psi(theta)>, psi = U(theta)|psi0>, */
+/* H the open-chain Ising Hamiltonian of brk_expectation_ising.             */
+/* ------------------------------------------------------------------------ */
+
+/* Number of parameters of a circuit: one per RX/RY/RZ/EX/EY/EZ/U1 op, in op order. */
+int brk_circuit_num_parameters(const brk_circuit* circuit, unsigned* num_parameters);
+/* `psi` holds |psi0> on entry; it is overwritten (on return it holds |psi0>
+   again up to rounding).  `work` is a second state of the same size and dtype.
+   energy receives E; gradient[k] = dE/dp_k for the k-th parameter
+   (num_gradient must equal brk_circuit_num_parameters).  FP64 accumulation.
+   Circuits with BRK_OP_LOCAL_SWAP are not supported. */
+int brk_adjoint_ising_gradient(const brk_circuit* circuit, void* psi, void* work, int dtype, const double* J,
+                               const double* h, double* energy, double* gradient, unsigned num_gradient,
+                               void* stream);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

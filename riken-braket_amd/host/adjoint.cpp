@@ -1,0 +1,230 @@
+// Differentiation form, host side: the adjoint method for
+// E(theta) = <psi(theta)| H |psi(theta)> with the open-chain Ising H.
+//
+//   forward:  psi = U_N ... U_1 |psi0>        (plain per-gate kernels)
+//             lam = H psi, E = <psi|lam>
+//   reverse:  for i = N..1:  dE/dtheta_i = 2 Re <lam| D_i |psi>,
+//             psi := U_i^dagger psi, lam := U_i^dagger lam,
+// with D_i = (dU_i/dtheta) U_i^dagger (so D_i |psi_i> = dU_i/dtheta |psi_(i-1)>):
+//   RX/RY/RZ(theta) = exp(-i theta G / 2):  D = -(i/2) G
+//   EX/EY/EZ(phi)   = exp(i phi G):         D = i G
+//   U1(lambda)      = diag(1, e^{i lambda}):  D = diag(0, i)
+// Runs of consecutive single-qubit ops on the same qubit are swept by one
+// fused kernel (csrc/brk_adjoint.hip); other ops are un-applied to both
+// states with the plain kernels.
+#include "circuit.hpp"
+#include "brk_adjoint.hpp"
+
+#include <hip/hip_runtime_api.h>
+
+#include <complex>
+#include <mutex>
+#include <vector>
+
+namespace brk {
+int fail(int status, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+}
+
+using cd = std::complex<double>;
+
+namespace brk {
+
+static bool is_parametric(int code) {
+  return code == BRK_OP_RX || code == BRK_OP_RY || code == BRK_OP_RZ || code == BRK_OP_EX || code == BRK_OP_EY ||
+         code == BRK_OP_EZ || code == BRK_OP_U1;
+}
+
+// Single-qubit op without controls: the adjoint kernel can sweep it.
+static bool chainable(const brk_circuit::op& o) {
+  if (!o.controls.empty() || o.targets.size() != 1) return false;
+  return o.code != BRK_OP_SWAP && o.code != BRK_OP_LOCAL_SWAP;
+}
+
+static void put(double* d, int slot, cd v) {
+  d[2 * slot] = v.real();
+  d[2 * slot + 1] = v.imag();
+}
+
+// M = U^dagger and D for a chainable op.
+static bool make_adj_op(const brk_circuit::op& o, adj_op& a) {
+  std::vector<unsigned> q;
+  std::vector<std::array<double, 2>> u;
+  if (!op_unitary(o, q, u) || u.size() != 4) return false;
+  for (int r = 0; r < 2; ++r)
+    for (int c = 0; c < 2; ++c) put(a.m, 2 * r + c, std::conj(cd(u[2 * c + r][0], u[2 * c + r][1])));
+  a.has_grad = is_parametric(o.code);
+  for (double& x : a.d) x = 0.0;
+  const cd I(0, 1);
+  cd g00 = 0, g01 = 0, g10 = 0, g11 = 0;  // generator G
+  cd scale = 0;
+  switch (o.code) {
+    case BRK_OP_RX: case BRK_OP_EX: g01 = g10 = 1; break;
+    case BRK_OP_RY: case BRK_OP_EY: g01 = -I; g10 = I; break;
+    case BRK_OP_RZ: case BRK_OP_EZ: g00 = 1; g11 = -1; break;
+    case BRK_OP_U1: g11 = 1; break;
+    default: break;
+  }
+  if (o.code == BRK_OP_RX || o.code == BRK_OP_RY || o.code == BRK_OP_RZ) scale = -0.5 * I;
+  else if (a.has_grad) scale = I;
+  put(a.d, 0, scale * g00);
+  put(a.d, 1, scale * g01);
+  put(a.d, 2, scale * g10);
+  put(a.d, 3, scale * g11);
+  return true;
+}
+
+static brk_circuit::op inverse_op(const brk_circuit::op& o) {
+  brk_circuit::op r = o;
+  switch (o.code) {
+    case BRK_OP_U1: case BRK_OP_EX: case BRK_OP_EY: case BRK_OP_EZ:
+    case BRK_OP_RX: case BRK_OP_RY: case BRK_OP_RZ:
+      r.params[0] = -o.params[0];
+      break;
+    case BRK_OP_U3:
+      r.params[0] = -o.params[0];
+      r.params[1] = -o.params[2];
+      r.params[2] = -o.params[1];
+      break;
+    case BRK_OP_U2:  // U2(phi, lam) = U3(pi/2, phi, lam)
+      r.code = BRK_OP_U3;
+      r.num_params = 3;
+      r.params[0] = -1.57079632679489661923132169163975144;
+      r.params[1] = -o.params[1];
+      r.params[2] = -o.params[0];
+      break;
+    case BRK_OP_MATRIX:  // conjugate transpose
+      for (int rr = 0; rr < 2; ++rr)
+        for (int c = 0; c < 2; ++c) {
+          r.params[2 * (2 * rr + c)] = o.params[2 * (2 * c + rr)];
+          r.params[2 * (2 * rr + c) + 1] = -o.params[2 * (2 * c + rr) + 1];
+        }
+      break;
+    default: break;  // H, X, Y, Z, SWAP are their own inverses
+  }
+  return r;
+}
+
+// Per-device FP64 partial-sum buffer, grown as needed and kept until exit.
+static std::mutex g_partials_mutex;
+static std::vector<std::pair<double*, size_t>> g_partials;
+
+static int get_partials(size_t doubles, double** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return fail(BRK_ERR_HIP, "hipGetDevice failed");
+  std::lock_guard<std::mutex> lock(g_partials_mutex);
+  if ((int)g_partials.size() <= dev) g_partials.resize(dev + 1, {nullptr, 0});
+  auto& slot = g_partials[dev];
+  if (slot.second < doubles) {
+    if (slot.first) (void)hipFree(slot.first);
+    slot = {nullptr, 0};
+    if (hipMalloc(&slot.first, doubles * sizeof(double)) != hipSuccess) {
+      slot.first = nullptr;
+      return fail(BRK_ERR_OUT_OF_MEMORY, "cannot allocate %zu gradient partials", doubles);
+    }
+    slot.second = doubles;
+  }
+  *out = slot.first;
+  return BRK_OK;
+}
+
+}  // namespace brk
+
+using namespace brk;
+
+extern "C" {
+
+int brk_circuit_num_parameters(const brk_circuit* circuit, unsigned* num_parameters) {
+  if (!circuit || !num_parameters) return fail(BRK_ERR_INVALID_ARGUMENT, "null argument");
+  unsigned n = 0;
+  for (const auto& o : circuit->ops) n += is_parametric(o.code) ? 1u : 0u;
+  *num_parameters = n;
+  return BRK_OK;
+}
+
+int brk_adjoint_ising_gradient(const brk_circuit* circuit, void* psi, void* work, int dtype, const double* J,
+                               const double* h, double* energy, double* gradient, unsigned num_gradient,
+                               void* stream) {
+  if (!circuit || !psi || !work || !energy || (num_gradient && !gradient))
+    return fail(BRK_ERR_INVALID_ARGUMENT, "null argument");
+  if (psi == work) return fail(BRK_ERR_INVALID_ARGUMENT, "psi and work must be different states");
+  unsigned nparams = 0;
+  (void)brk_circuit_num_parameters(circuit, &nparams);
+  if (nparams != num_gradient)
+    return fail(BRK_ERR_INVALID_ARGUMENT, "circuit has %u parameters, gradient has %u", nparams, num_gradient);
+  const auto& ops = circuit->ops;
+  // parameter index of every op (-1 if none); reject ops the sweep cannot handle
+  std::vector<int> pidx(ops.size(), -1);
+  int next = 0;
+  for (size_t i = 0; i < ops.size(); ++i) {
+    if (ops[i].code == BRK_OP_LOCAL_SWAP) return fail(BRK_ERR_UNSUPPORTED, "local swaps are not supported here");
+    if (is_parametric(ops[i].code)) {
+      if (!chainable(ops[i]))
+        return fail(BRK_ERR_UNSUPPORTED, "parametric op %zu must act on one qubit without controls", i);
+      pidx[i] = next++;
+    }
+  }
+  const uint64_t size = 1ull << circuit->num_qubits;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+
+  // forward pass, lam = H psi, E = <psi|lam>
+  if (int s = brk_circuit_run(circuit, psi, dtype, nullptr, stream)) return s;
+  if (int s = brk_apply_ising(work, psi, size, dtype, J, h, stream)) return s;
+  double ip[2];
+  if (int s = brk_inner_product(work, dtype, psi, dtype, size, ip, stream)) return s;
+  *energy = ip[0];
+
+  // reverse sweep; one partial region of ADJ_MAX_CHAIN * ADJ_BLOCKS doubles per chain
+  size_t nchains = 0;
+  for (size_t i = 0; i < ops.size(); ++i) nchains += chainable(ops[i]) ? 1 : 0;  // upper bound
+  double* partials = nullptr;
+  if (int s = get_partials(std::max<size_t>(1, nchains) * ADJ_MAX_CHAIN * ADJ_BLOCKS, &partials)) return s;
+  struct slot {
+    int param;
+    size_t offset;
+  };
+  std::vector<slot> slots;
+  size_t region = 0;
+  long i = (long)ops.size() - 1;
+  while (i >= 0) {
+    const auto& o = ops[(size_t)i];
+    if (chainable(o)) {
+      long j = i;
+      while (j - 1 >= 0 && (i - (j - 1)) < ADJ_MAX_CHAIN && chainable(ops[(size_t)(j - 1)]) &&
+             ops[(size_t)(j - 1)].targets[0] == o.targets[0])
+        --j;
+      adj_chain c{};
+      c.target = o.targets[0];
+      c.nops = (int)(i - j + 1);
+      for (long k = i; k >= j; --k) {
+        const int pos = (int)(i - k);
+        if (!make_adj_op(ops[(size_t)k], c.ops[pos])) return fail(BRK_ERR_UNSUPPORTED, "op %ld has no 2x2 form", k);
+        if (pidx[(size_t)k] >= 0) slots.push_back({pidx[(size_t)k], region + (size_t)pos * ADJ_BLOCKS});
+      }
+      if (int s = launch_adjoint_chain(c, psi, work, size, dtype, partials + region, stream)) return s;
+      region += (size_t)ADJ_MAX_CHAIN * ADJ_BLOCKS;
+      i = j - 1;
+    } else {
+      const brk_circuit::op inv = inverse_op(o);
+      if (int s = apply_physical_op(inv, psi, size, dtype, stream)) return s;
+      if (int s = apply_physical_op(inv, work, size, dtype, stream)) return s;
+      --i;
+    }
+  }
+
+  // gradient: sum each parameter's block partials in block order
+  std::vector<double> host(region);
+  if (region) {
+    if (hipMemcpyAsync(host.data(), partials, region * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess)
+      return fail(BRK_ERR_HIP, "copying gradient partials failed");
+  }
+  if (hipStreamSynchronize(st) != hipSuccess) return fail(BRK_ERR_HIP, "hipStreamSynchronize failed");
+  for (unsigned k = 0; k < num_gradient; ++k) gradient[k] = 0.0;
+  for (const auto& s : slots) {
+    double t = 0.0;
+    for (int b = 0; b < ADJ_BLOCKS; ++b) t += host[s.offset + (size_t)b];
+    gradient[s.param] = 2.0 * t;
+  }
+  return BRK_OK;
+}
+
+}  // extern "C"

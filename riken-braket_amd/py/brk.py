@@ -85,6 +85,8 @@ _SIGS = {
     "brk_fused_plan_final_permutation": [_p, _up],
     "brk_fused_apply": [_p, _p, _i, _p],
     "brk_fused_apply_timed": [_p, _p, _i, C.POINTER(C.c_float), _p],
+    "brk_circuit_num_parameters": [_p, _up],
+    "brk_adjoint_ising_gradient": [_p, _p, _p, _i, _dp, _dp, _dp, _dp, _u, _p],
 }
 _RESTYPES = {"brk_status_string": C.c_char_p, "brk_last_error": C.c_char_p, "brk_version": C.c_char_p}
 
@@ -399,3 +401,20 @@ class FusedPlan:
                 self.h = None
         except Exception:
             pass
+
+
+def circuit_num_parameters(circuit):
+    n = C.c_uint()
+    call("brk_circuit_num_parameters", circuit.h, C.byref(n))
+    return n.value
+
+
+def adjoint_ising_gradient(circuit, psi_ptr, work_ptr, dtype, J, h, stream=0):
+    """(energy, gradient list) of <psi|H|psi>; psi_ptr holds |psi0> on entry."""
+    import numpy as np
+    npar = circuit_num_parameters(circuit)
+    e = C.c_double()
+    g = (C.c_double * max(1, npar))()
+    call("brk_adjoint_ising_gradient", circuit.h, psi_ptr, work_ptr, dtype, darr(J), darr(h), C.byref(e), g, npar,
+         stream)
+    return e.value, np.array(list(g)[:npar])

@@ -110,6 +110,19 @@ def tfim(n, J=1.0, h=0.7):
     return np.full(n - 1, J), np.full(n, h)
 
 
+PARAMETRIC = ("RX", "RY", "RZ", "EX", "EY", "EZ", "U1")
+
+
+def parameters(ops):
+    return [op[3][0] for op in ops if op[0] in PARAMETRIC]
+
+
+def with_parameters(ops, params):
+    """The same circuit with the angles of its parametric ops replaced, in op order."""
+    it = iter(params)
+    return [(name, t, c, (float(next(it)),)) if name in PARAMETRIC else (name, t, c, p) for name, t, c, p in ops]
+
+
 def logical_ops(ops):
     """Drops local swaps: the logical circuit a permuted run must agree with."""
     return [op for op in ops if op[0] != "LOCAL_SWAP"]

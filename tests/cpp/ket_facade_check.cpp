@@ -1,0 +1,108 @@
+// Drives the ket-shaped C++ facade (riken-braket_amd/include/ket_hip.hpp) with
+// ket-style call sites on an 8-qubit device state, in FP64 and FP32.
+//
+//   ket_facade_check <out-prefix>
+// writes <out-prefix>_c128.bin and <out-prefix>_c64.bin (the final state as
+// complex128, physical order) and prints the permutation and the reductions as
+// "key value..." lines.  tests/test_gpu_ket_facade.py replays the same
+// sequence on the CPU oracle and compares.
+#ifdef KET_FACADE_WITH_REFERENCE_TYPES
+#include <ket/control.hpp>
+#include <ket/qubit.hpp>
+#endif
+#include "ket_hip.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <string>
+
+namespace {
+
+template <typename Complex>
+int run(const std::string& prefix, const char* tag) {
+  using namespace ket::literals;
+  using ket::hip::state;
+  auto p = ket::hip::make_policy();
+  state<Complex> s(8);
+  namespace g = ket::gate;
+  namespace r = ket::gate::ranges;
+
+  r::hadamard(p, s, 0_q);
+  r::hadamard(p, s, 5_q);
+  g::hadamard(p, s.begin(), s.end(), 3_q, 0_cq);
+  r::pauli_x(p, s, 7_q);
+  r::controlled_not(p, s, 2_q, 5_cq);
+  r::pauli_x(p, s, 1_q, 4_q, 3_cq);
+  r::not_(p, s, 6_q);
+  r::pauli_y(p, s, 6_q);
+  r::pauli_y(p, s, 1_q, 2_q, 0_cq);
+  r::pauli_z(p, s, 4_q, 6_cq);
+  r::pauli_z(p, s, 2_q);
+  r::phase_shift(p, s, 0.3, 1_q);
+  r::phase_shift(p, s, 0.7, 3_q, 0_cq);
+  r::phase_shift_coeff(p, s, std::complex<double>(std::cos(0.5), std::sin(0.5)), 5_q);
+  r::phase_shift2(p, s, 0.2, 0.9, 6_q);
+  r::phase_shift3(p, s, 0.1, 0.4, 1.3, 7_q, 2_cq);
+  r::exponential_pauli_x(p, s, 0.35, 0_q);
+  r::exponential_pauli_y(p, s, 0.25, 3_q, 6_q);
+  r::exponential_pauli_z(p, s, 0.45, 1_q, 5_q, 7_cq);
+  r::swap(p, s, 2_q, 6_q);
+  r::swap(p, s, 0_q, 7_q, 4_cq);
+
+  // local swap of physical qubits 1 and 6, then gates addressed through the permutation
+  ket::mpi::qubit_permutation<> perm(8);
+  ket::mpi::utility::swap_permutated_local_qubits(p, s, perm, 1u, 6u);
+  r::hadamard(p, s, ket::qubit<>{perm[1_q]});
+  r::exponential_pauli_x(p, s, 0.6, ket::qubit<>{perm[6_q]}, ket::make_control(ket::qubit<>{perm[3_q]}));
+
+  const auto host = s.download();
+  const std::string path = prefix + "_" + tag + ".bin";
+  std::FILE* f = std::fopen(path.c_str(), "wb");
+  if (!f) return 1;
+  for (const auto& a : host) {
+    const double v[2] = {static_cast<double>(a.real()), static_cast<double>(a.imag())};
+    std::fwrite(v, sizeof(double), 2, f);
+  }
+  std::fclose(f);
+
+  std::printf("%s perm", tag);
+  for (unsigned q = 0; q < 8; ++q) std::printf(" %u", perm.data()[q]);
+  std::printf("\n");
+  // permutate_bits agrees with the permutation itself
+  std::printf("%s permutate_bits %llu %llu\n", tag,
+              static_cast<unsigned long long>(ket::mpi::permutate_bits(perm, std::uint64_t{0x82})),
+              static_cast<unsigned long long>(ket::mpi::inverse_permutate_bits(perm, std::uint64_t{0x82})));
+  const auto spin = ket::spin_expectation_value(p, s.begin(), s.end(), 4_q);
+  std::printf("%s spin4 %.17g %.17g %.17g\n", tag, spin[0], spin[1], spin[2]);
+  state<Complex> t(8);
+  r::hadamard(p, t, 2_q);
+  const auto ip = ket::inner_product(p, s.begin(), s.end(), t.begin());
+  std::printf("%s inner %.17g %.17g\n", tag, ip.real(), ip.imag());
+  std::printf("%s fidelity %.17g\n", tag, ket::fidelity(p, s.begin(), s.end(), t.begin()));
+
+  // an invalid request throws ket::hip::error
+  try {
+    r::hadamard(p, s, 9_q);
+    std::printf("%s error none\n", tag);
+  } catch (const ket::hip::error& e) {
+    std::printf("%s error %d\n", tag, e.status());
+  }
+  return 0;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  if (argc != 2) {
+    std::fprintf(stderr, "usage: %s <out-prefix>\n", argv[0]);
+    return 2;
+  }
+  try {
+    if (run<std::complex<double>>(argv[1], "c128")) return 1;
+    if (run<std::complex<float>>(argv[1], "c64")) return 1;
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "ket_facade_check: %s\n", e.what());
+    return 1;
+  }
+  return 0;
+}
