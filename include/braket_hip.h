@@ -326,6 +326,57 @@ int brk_adjoint_ising_gradient(const brk_circuit* circuit, void* psi, void* work
                                const double* h, double* energy, double* gradient, unsigned num_gradient,
                                void* stream);
 
+/* ------------------------------------------------------------------------ */
+/* Batching form: many small independent circuits in one persistent launch, */
+/* captured once as a HIP graph and replayed.  Each circuit runs on its own */
+/* complex64 state from |0...0> and yields its FP64 open-chain Ising energy */
+/* sum_i J_i <Z_i Z_i+1> + sum_i h_i <X_i>.  Reference ancestor: bra's      */
+/* CIRCUITS mode, which advances several states in turn                     */
+/* (bra/src/bra.cpp:315-336, bra/src/interpreter.cpp:411-419).              */
+/* ------------------------------------------------------------------------ */
+
+typedef struct brk_batch brk_batch;
+
+#define BRK_BATCH_MAX_QUBITS 24
+#define BRK_BATCH_MAX_LOCAL 14
+
+/* planner op kinds (brk_batch_stage_op) */
+enum {
+  BRK_BATCH_GATE = 0,       /* 2x2 matrix on one qubit                             */
+  BRK_BATCH_SIGN = 1,       /* negate amplitudes whose index has every mask bit set */
+  BRK_BATCH_MEASURE_X = 2,  /* add h_q <X_q>                                        */
+  BRK_BATCH_MEASURE_ZZ = 3  /* add sum_i J_i <Z_i Z_i+1>                            */
+};
+
+/* Compiles `num_circuits` circuits (1..BRK_BATCH_MAX_QUBITS qubits each) with
+   their Hamiltonians: J[i] has num_qubits-1 entries, h[i] num_qubits (J, h,
+   J[i] or h[i] may be NULL = zeros).  Circuits may hold uncontrolled
+   single-qubit ops (consecutive ones on one qubit are multiplied into one
+   2x2 matrix in FP64) and BRK_OP_Z on any qubits (Z, CZ, CnZ); anything else
+   is BRK_ERR_UNSUPPORTED.  Planning needs no device; the first run uploads
+   the program to the current device and captures the graph. */
+int brk_batch_create(const brk_circuit* const* circuits, unsigned num_circuits, const double* const* J,
+                     const double* const* h, brk_batch** out);
+int brk_batch_destroy(brk_batch* batch);
+/* Plan introspection: a circuit runs as stages; a stage has up to
+   BRK_BATCH_MAX_LOCAL local qubits (ascending) and applies its ops, in the
+   order listed, to the chunks of the state over those qubits. */
+int brk_batch_num_stages(const brk_batch* batch, unsigned circuit, unsigned* num_stages);
+int brk_batch_stage(const brk_batch* batch, unsigned circuit, unsigned stage, unsigned* local_qubits,
+                    unsigned* num_local, unsigned* num_ops);
+/* Op k of a stage: its kind, its qubits as a mask and, for BRK_BATCH_GATE,
+   the FP64 matrix (row-major (re, im) pairs; the device uses it rounded to FP32). */
+int brk_batch_stage_op(const brk_batch* batch, unsigned circuit, unsigned stage, unsigned k, int* kind,
+                       uint64_t* qubit_mask, double* matrix);
+/* Runs every circuit; energies[i] receives circuit i's energy (host array of
+   num_circuits).  Ordered after the work already on `stream`; returns when done. */
+int brk_batch_run(brk_batch* batch, double* energies, void* stream);
+/* Same, with the device time of the graph launch and, per circuit (may be
+   NULL), the wall-clock times in ms when a workgroup took it from the queue
+   and when its energy was written, both relative to the first take. */
+int brk_batch_run_timed(brk_batch* batch, double* energies, double* elapsed_ms, double* circuit_start_ms,
+                        double* circuit_end_ms, void* stream);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

@@ -1,0 +1,519 @@
+// Batching form, host side: compiles a list of small circuits into the stage
+// programs of csrc/brk_batch.hpp, uploads them once, and runs the persistent
+// kernel as a captured HIP graph (queue reset + one persistent launch).
+//
+// Per circuit:
+//  1. ops -> planner ops: uncontrolled single-qubit ops become 2x2 gates
+//     (consecutive gates on one qubit, with nothing else on that qubit in
+//     between, are multiplied into one in FP64); Z-type ops (Z, CZ, CnZ on
+//     qubits and controls alike, ket/include/ket/gate/pauli_z.hpp:90-137)
+//     become sign masks; the Hamiltonian adds "measure h_q X_q" per qubit and
+//     one "measure sum J_i Z_i Z_i+1" after all gates.
+//  2. stages: the local set is qubits 0..3 plus the qubits of the first
+//     pending gates, in order of appearance; the stage then takes every
+//     pending op that can run now: a gate needs its qubit local and no earlier
+//     pending op on that qubit; a sign mask (diagonal) needs no earlier pending
+//     gate on its qubits, since diagonal ops commute with each other.  Ops on
+//     disjoint qubits commute, so taking ops out of order never changes the
+//     state.
+//  3. instructions: gates are grouped (<= 4 local positions, <= 8 gates),
+//     consecutive sign masks are merged (<= 32 masks).
+#include "circuit.hpp"
+#include "brk_batch.hpp"
+
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <complex>
+#include <cstring>
+#include <vector>
+
+namespace brk {
+int fail(int status, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+}
+
+using cd = std::complex<double>;
+
+namespace {
+
+struct pop {
+  int kind;       // BRK_BATCH_GATE, _SIGN, _MEASURE_X, _MEASURE_ZZ
+  uint32_t mask;  // qubits
+  unsigned q;     // gate / measure_x qubit
+  cd m[4];        // gate matrix, row-major
+  bool diagonal() const { return kind == BRK_BATCH_SIGN || kind == BRK_BATCH_MEASURE_ZZ; }
+};
+
+struct planned_stage {
+  std::vector<unsigned> local;  // ascending
+  std::vector<pop> ops;         // taken ops in program order
+};
+
+struct planned_circuit {
+  unsigned n = 0;
+  std::vector<planned_stage> stages;
+  std::vector<double> ham;  // J (n-1), h (n)
+};
+
+uint32_t all_qubits(unsigned n) { return n >= 32 ? 0xffffffffu : ((1u << n) - 1u); }
+
+int to_planner_ops(const brk_circuit& c, const double* J, const double* h, std::vector<pop>& out) {
+  const unsigned n = c.num_qubits;
+  std::vector<long> last_gate(n, -1);  // index in `out` of a gate on q with nothing on q after it
+  for (size_t i = 0; i < c.ops.size(); ++i) {
+    const auto& o = c.ops[i];
+    if (o.code == BRK_OP_Z) {
+      uint32_t mask = 0;
+      for (unsigned q : o.targets) mask |= 1u << q;
+      for (unsigned q : o.controls) mask |= 1u << q;
+      out.push_back(pop{BRK_BATCH_SIGN, mask, 0, {}});
+      for (unsigned q = 0; q < n; ++q)
+        if (mask & (1u << q)) last_gate[q] = -1;
+      continue;
+    }
+    const bool single = o.targets.size() == 1 && o.controls.empty() && o.code != BRK_OP_SWAP &&
+                        o.code != BRK_OP_LOCAL_SWAP;
+    if (!single)
+      return brk::fail(BRK_ERR_UNSUPPORTED,
+                       "batched circuits take uncontrolled single-qubit ops and Z/CZ/CnZ (op %zu is not one)", i);
+    std::vector<unsigned> qs;
+    std::vector<std::array<double, 2>> u;
+    if (!brk::op_unitary(o, qs, u) || u.size() != 4)
+      return brk::fail(BRK_ERR_UNSUPPORTED, "op %zu has no 2x2 form", i);
+    const unsigned q = o.targets[0];
+    cd m[4];
+    for (int k = 0; k < 4; ++k) m[k] = cd(u[k][0], u[k][1]);
+    if (last_gate[q] >= 0) {
+      pop& g = out[(size_t)last_gate[q]];
+      const cd a0 = g.m[0], a1 = g.m[1], a2 = g.m[2], a3 = g.m[3];
+      g.m[0] = m[0] * a0 + m[1] * a2;  // new * old: the old gate acts first
+      g.m[1] = m[0] * a1 + m[1] * a3;
+      g.m[2] = m[2] * a0 + m[3] * a2;
+      g.m[3] = m[2] * a1 + m[3] * a3;
+      continue;
+    }
+    pop g{BRK_BATCH_GATE, 1u << q, q, {m[0], m[1], m[2], m[3]}};
+    out.push_back(g);
+    last_gate[q] = (long)out.size() - 1;
+  }
+  for (unsigned q = 0; q < n; ++q)
+    if (h && h[q] != 0.0) out.push_back(pop{BRK_BATCH_MEASURE_X, 1u << q, q, {}});
+  bool any_j = false;
+  for (unsigned i = 0; J && i + 1 < n; ++i) any_j |= J[i] != 0.0;
+  if (any_j) out.push_back(pop{BRK_BATCH_MEASURE_ZZ, all_qubits(n), 0, {}});
+  return BRK_OK;
+}
+
+void plan_stages(unsigned n, std::vector<pop> pending, std::vector<planned_stage>& stages) {
+  const unsigned L = std::min<unsigned>(n, brk::BATCH_MAX_LOCAL);
+  const unsigned P = std::min<unsigned>(n, brk::BATCH_PASSENGERS);
+  do {  // a circuit with no ops still gets one stage (it initialises and measures nothing)
+    uint32_t local = all_qubits(P);
+    unsigned count = P;
+    for (const pop& o : pending) {
+      if (count == L) break;
+      if (o.diagonal()) continue;
+      if (!(local & o.mask)) {
+        local |= o.mask;
+        ++count;
+      }
+    }
+    for (unsigned q = 0; q < n && count < L; ++q)
+      if (!(local & (1u << q))) {
+        local |= 1u << q;
+        ++count;
+      }
+    planned_stage st;
+    for (unsigned q = 0; q < n; ++q)
+      if (local & (1u << q)) st.local.push_back(q);
+    uint32_t blocked_any = 0, blocked_gate = 0;
+    std::vector<pop> rest;
+    for (const pop& o : pending) {
+      const bool ok = o.diagonal() ? !(o.mask & blocked_gate) : (!(o.mask & ~local) && !(o.mask & blocked_any));
+      if (ok) {
+        st.ops.push_back(o);
+      } else {
+        rest.push_back(o);
+        blocked_any |= o.mask;
+        if (!o.diagonal()) blocked_gate |= o.mask;
+      }
+    }
+    stages.push_back(std::move(st));
+    pending.swap(rest);
+  } while (!pending.empty());
+}
+
+uint32_t float_bits(double v) {
+  const float f = (float)v;
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  return u;
+}
+
+// Appends the instruction stream of one stage; returns the instruction count.
+uint32_t emit_stage(const planned_stage& st, unsigned n, std::vector<uint32_t>& words, uint32_t& measure_x,
+                    uint32_t& flags) {
+  int pos_of[32];
+  for (int& p : pos_of) p = -1;
+  for (size_t p = 0; p < st.local.size(); ++p) pos_of[st.local[p]] = (int)p;
+  uint32_t count = 0;
+  std::vector<const pop*> group;  // pending gate group
+  std::vector<uint32_t> masks;    // pending sign masks
+  auto flush_group = [&]() {
+    if (group.empty()) return;
+    std::vector<unsigned> pos;
+    for (const pop* g : group)
+      if (std::find(pos.begin(), pos.end(), (unsigned)pos_of[g->q]) == pos.end()) pos.push_back(pos_of[g->q]);
+    std::sort(pos.begin(), pos.end());
+    uint32_t pw = 0;
+    for (size_t k = 0; k < pos.size(); ++k) pw |= (uint32_t)pos[k] << (8 * k);
+    words.push_back(brk::BI_GATES | (uint32_t)pos.size() << 8 | (uint32_t)group.size() << 16);
+    words.push_back(pw);
+    for (const pop* g : group) {
+      const unsigned slot = (unsigned)(std::find(pos.begin(), pos.end(), (unsigned)pos_of[g->q]) - pos.begin());
+      words.push_back(slot);
+      for (int k = 0; k < 4; ++k) {
+        words.push_back(float_bits(g->m[k].real()));
+        words.push_back(float_bits(g->m[k].imag()));
+      }
+    }
+    ++count;
+    group.clear();
+  };
+  auto flush_masks = [&]() {
+    if (masks.empty()) return;
+    words.push_back(brk::BI_SIGN | (uint32_t)masks.size() << 16);
+    for (uint32_t m : masks) words.push_back(m);
+    ++count;
+    masks.clear();
+  };
+  auto group_positions_with = [&](unsigned q) {
+    std::vector<unsigned> pos{(unsigned)pos_of[q]};
+    for (const pop* g : group)
+      if (std::find(pos.begin(), pos.end(), (unsigned)pos_of[g->q]) == pos.end()) pos.push_back(pos_of[g->q]);
+    return pos.size();
+  };
+  for (const pop& o : st.ops) {
+    switch (o.kind) {
+      case BRK_BATCH_GATE:
+        flush_masks();
+        if (group.size() == (size_t)brk::BATCH_MAX_GROUP_OPS || group_positions_with(o.q) > (size_t)brk::BATCH_MAX_GROUP)
+          flush_group();
+        group.push_back(&o);
+        break;
+      case BRK_BATCH_SIGN:
+        flush_group();
+        if (masks.size() == (size_t)brk::BATCH_MAX_SIGN_MASKS) flush_masks();
+        masks.push_back(o.mask);
+        break;
+      case BRK_BATCH_MEASURE_X:
+        measure_x |= 1u << pos_of[o.q];
+        break;
+      case BRK_BATCH_MEASURE_ZZ:
+        flags |= brk::BS_ZZ;
+        break;
+    }
+  }
+  flush_group();
+  flush_masks();
+  (void)n;
+  return count;
+}
+
+}  // namespace
+
+struct brk_batch {
+  std::vector<planned_circuit> plans;
+  std::vector<brk::batch_circuit> circuits;
+  std::vector<brk::batch_stage> stages;
+  std::vector<uint32_t> words;
+  std::vector<double> ham;
+  std::vector<uint32_t> order;
+  unsigned max_multi_n = 0;  // largest n with more than one stage (0: none)
+
+  // device side, created on the first run
+  int device = -1;
+  unsigned grid = 0;
+  void* d_program = nullptr;  // circuits, stages, words, ham, order, counter, energies, t_start, t_end
+  void* d_slots = nullptr;
+  brk::batch_launch launch{};
+  hipStream_t stream = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t ev_in = nullptr, ev_begin = nullptr, ev_end = nullptr;
+  double tick_ms = 0.0;  // wall-clock tick in ms
+
+  ~brk_batch() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (ev_in) (void)hipEventDestroy(ev_in);
+    if (ev_begin) (void)hipEventDestroy(ev_begin);
+    if (ev_end) (void)hipEventDestroy(ev_end);
+    if (stream) (void)hipStreamDestroy(stream);
+    if (d_slots) (void)hipFree(d_slots);
+    if (d_program) (void)hipFree(d_program);
+  }
+};
+
+namespace {
+
+template <typename T>
+size_t aligned_bytes(size_t count) {
+  return (count * sizeof(T) + 255) & ~(size_t)255;
+}
+
+int ensure_device(brk_batch* b) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return brk::fail(BRK_ERR_HIP, "hipGetDevice failed");
+  if (b->device >= 0) {
+    if (dev != b->device) return brk::fail(BRK_ERR_INVALID_ARGUMENT, "batch was set up on device %d", b->device);
+    return BRK_OK;
+  }
+  const size_t nc = b->circuits.size();
+  const size_t sizes[] = {aligned_bytes<brk::batch_circuit>(nc), aligned_bytes<brk::batch_stage>(b->stages.size()),
+                          aligned_bytes<uint32_t>(std::max<size_t>(1, b->words.size())),
+                          aligned_bytes<double>(std::max<size_t>(1, b->ham.size())), aligned_bytes<uint32_t>(nc),
+                          aligned_bytes<uint32_t>(1), aligned_bytes<double>(nc), aligned_bytes<uint64_t>(nc),
+                          aligned_bytes<uint64_t>(nc)};
+  size_t total = 0;
+  for (size_t s : sizes) total += s;
+  if (hipMalloc(&b->d_program, total) != hipSuccess) {
+    b->d_program = nullptr;
+    return brk::fail(BRK_ERR_OUT_OF_MEMORY, "cannot allocate %zu bytes for the batch program", total);
+  }
+  char* base = static_cast<char*>(b->d_program);
+  char* ptr[9];
+  for (int i = 0; i < 9; ++i) {
+    ptr[i] = base;
+    base += sizes[i];
+  }
+  const void* host[] = {b->circuits.data(), b->stages.data(), b->words.data(), b->ham.data(), b->order.data()};
+  const size_t bytes[] = {nc * sizeof(brk::batch_circuit), b->stages.size() * sizeof(brk::batch_stage),
+                          b->words.size() * sizeof(uint32_t), b->ham.size() * sizeof(double), nc * sizeof(uint32_t)};
+  for (int i = 0; i < 5; ++i)
+    if (bytes[i] && hipMemcpy(ptr[i], host[i], bytes[i], hipMemcpyHostToDevice) != hipSuccess)
+      return brk::fail(BRK_ERR_HIP, "uploading the batch program failed");
+
+  int cus = 0, rate_khz = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+    return brk::fail(BRK_ERR_HIP, "cannot read the CU count");
+  if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || rate_khz <= 0)
+    return brk::fail(BRK_ERR_HIP, "cannot read the wall-clock rate");
+  b->tick_ms = 1.0 / (double)rate_khz;
+  b->grid = (unsigned)std::min<size_t>((size_t)cus, nc);
+  uint64_t slot_elems = 0;
+  if (b->max_multi_n) {
+    slot_elems = 1ull << b->max_multi_n;
+    const size_t slot_bytes = (size_t)b->grid * slot_elems * 8;
+    if (hipMalloc(&b->d_slots, slot_bytes) != hipSuccess) {
+      b->d_slots = nullptr;
+      return brk::fail(BRK_ERR_OUT_OF_MEMORY, "cannot allocate %zu bytes of batch state slots", slot_bytes);
+    }
+  }
+  brk::batch_launch& L = b->launch;
+  L.circuits = reinterpret_cast<const brk::batch_circuit*>(ptr[0]);
+  L.stages = reinterpret_cast<const brk::batch_stage*>(ptr[1]);
+  L.words = reinterpret_cast<const uint32_t*>(ptr[2]);
+  L.ham = reinterpret_cast<const double*>(ptr[3]);
+  L.order = reinterpret_cast<const uint32_t*>(ptr[4]);
+  L.num_circuits = (uint32_t)nc;
+  L.counter = reinterpret_cast<uint32_t*>(ptr[5]);
+  L.slots = b->d_slots;
+  L.slot_elems = slot_elems;
+  L.energies = reinterpret_cast<double*>(ptr[6]);
+  L.t_start = reinterpret_cast<uint64_t*>(ptr[7]);
+  L.t_end = reinterpret_cast<uint64_t*>(ptr[8]);
+
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+    return brk::fail(BRK_ERR_HIP, "cannot create the batch stream");
+  if (hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreate(&b->ev_begin) != hipSuccess || hipEventCreate(&b->ev_end) != hipSuccess)
+    return brk::fail(BRK_ERR_HIP, "cannot create the batch events");
+  if (int s = brk::configure_batch()) return s;
+  // graph: queue reset + persistent kernel, captured once and replayed
+  if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+    return brk::fail(BRK_ERR_HIP, "hipStreamBeginCapture failed");
+  const int launch_status = brk::launch_batch(L, b->grid, b->stream);
+  hipGraph_t graph = nullptr;
+  const hipError_t end = hipStreamEndCapture(b->stream, &graph);
+  if (launch_status) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return launch_status;
+  }
+  if (end != hipSuccess || !graph) return brk::fail(BRK_ERR_HIP, "hipStreamEndCapture failed");
+  const hipError_t inst = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (inst != hipSuccess) {
+    b->exec = nullptr;
+    return brk::fail(BRK_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(inst));
+  }
+  b->device = dev;
+  return BRK_OK;
+}
+
+int run(brk_batch* b, double* energies, double* elapsed_ms, double* start_ms, double* end_ms, void* stream) {
+  if (int s = ensure_device(b)) return s;
+  hipStream_t caller = reinterpret_cast<hipStream_t>(stream);
+  if (hipEventRecord(b->ev_in, caller) != hipSuccess || hipStreamWaitEvent(b->stream, b->ev_in, 0) != hipSuccess)
+    return brk::fail(BRK_ERR_HIP, "ordering the batch after the caller's stream failed");
+  if (hipEventRecord(b->ev_begin, b->stream) != hipSuccess) return brk::fail(BRK_ERR_HIP, "hipEventRecord failed");
+  const hipError_t e = hipGraphLaunch(b->exec, b->stream);
+  if (e != hipSuccess) return brk::fail(BRK_ERR_HIP, "hipGraphLaunch failed: %s", hipGetErrorString(e));
+  if (hipEventRecord(b->ev_end, b->stream) != hipSuccess) return brk::fail(BRK_ERR_HIP, "hipEventRecord failed");
+  const size_t nc = b->circuits.size();
+  if (energies &&
+      hipMemcpyAsync(energies, b->launch.energies, nc * sizeof(double), hipMemcpyDeviceToHost, b->stream) != hipSuccess)
+    return brk::fail(BRK_ERR_HIP, "copying the energies failed");
+  std::vector<uint64_t> ts, te;
+  if (start_ms || end_ms) {
+    ts.resize(nc);
+    te.resize(nc);
+    if (hipMemcpyAsync(ts.data(), b->launch.t_start, nc * 8, hipMemcpyDeviceToHost, b->stream) != hipSuccess ||
+        hipMemcpyAsync(te.data(), b->launch.t_end, nc * 8, hipMemcpyDeviceToHost, b->stream) != hipSuccess)
+      return brk::fail(BRK_ERR_HIP, "copying the circuit timestamps failed");
+  }
+  if (hipStreamSynchronize(b->stream) != hipSuccess) return brk::fail(BRK_ERR_HIP, "batch run failed");
+  if (elapsed_ms) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, b->ev_begin, b->ev_end) != hipSuccess)
+      return brk::fail(BRK_ERR_HIP, "hipEventElapsedTime failed");
+    *elapsed_ms = ms;
+  }
+  if (start_ms || end_ms) {
+    const uint64_t t0 = *std::min_element(ts.begin(), ts.end());
+    for (size_t i = 0; i < nc; ++i) {
+      if (start_ms) start_ms[i] = (double)(ts[i] - t0) * b->tick_ms;
+      if (end_ms) end_ms[i] = (double)(te[i] - t0) * b->tick_ms;
+    }
+  }
+  return BRK_OK;
+}
+
+}  // namespace
+
+using brk::fail;
+
+extern "C" {
+
+int brk_batch_create(const brk_circuit* const* circuits, unsigned num_circuits, const double* const* J,
+                     const double* const* h, brk_batch** out) {
+  if (!out) return fail(BRK_ERR_INVALID_ARGUMENT, "null output");
+  *out = nullptr;
+  if (!circuits || num_circuits == 0) return fail(BRK_ERR_INVALID_ARGUMENT, "empty batch");
+  auto* b = new brk_batch();
+  std::vector<double> cost(num_circuits);
+  for (unsigned i = 0; i < num_circuits; ++i) {
+    const brk_circuit* c = circuits[i];
+    if (!c) {
+      delete b;
+      return fail(BRK_ERR_INVALID_ARGUMENT, "circuit %u is null", i);
+    }
+    const unsigned n = c->num_qubits;
+    if (n < 1 || n > (unsigned)brk::BATCH_MAX_QUBITS) {
+      delete b;
+      return fail(BRK_ERR_INVALID_ARGUMENT, "circuit %u has %u qubits (batched circuits take 1..%d)", i, n,
+                  brk::BATCH_MAX_QUBITS);
+    }
+    planned_circuit pc;
+    pc.n = n;
+    std::vector<pop> ops;
+    if (int s = to_planner_ops(*c, J ? J[i] : nullptr, h ? h[i] : nullptr, ops)) {
+      delete b;
+      return s;
+    }
+    plan_stages(n, std::move(ops), pc.stages);
+    for (unsigned k = 0; k + 1 < n; ++k) pc.ham.push_back(J && J[i] ? J[i][k] : 0.0);
+    for (unsigned k = 0; k < n; ++k) pc.ham.push_back(h && h[i] ? h[i][k] : 0.0);
+
+    brk::batch_circuit bc{};
+    bc.n = n;
+    bc.num_stages = (uint32_t)pc.stages.size();
+    bc.first_stage = (uint32_t)b->stages.size();
+    bc.ham_offset = (uint32_t)b->ham.size();
+    b->ham.insert(b->ham.end(), pc.ham.begin(), pc.ham.end());
+    double work = 0.0;
+    for (size_t s = 0; s < pc.stages.size(); ++s) {
+      const planned_stage& st = pc.stages[s];
+      brk::batch_stage bs{};
+      bs.L = (uint32_t)st.local.size();
+      uint32_t local_mask = 0;
+      for (size_t p = 0; p < st.local.size(); ++p) {
+        bs.local[p] = (uint8_t)st.local[p];
+        local_mask |= 1u << st.local[p];
+      }
+      bs.nonlocal_mask = all_qubits(n) & ~local_mask;
+      bs.first_word = (uint32_t)b->words.size();
+      bool changes = false;
+      for (const pop& o : st.ops) changes |= o.kind == BRK_BATCH_GATE || o.kind == BRK_BATCH_SIGN;
+      bs.num_instr = emit_stage(st, n, b->words, bs.measure_x, bs.flags);
+      if (s == 0) bs.flags |= brk::BS_INIT;
+      if (s + 1 < pc.stages.size() && (changes || s == 0)) bs.flags |= brk::BS_STORE;
+      b->stages.push_back(bs);
+      work += (double)(1ull << n) * (bs.num_instr + 3);
+    }
+    if (pc.stages.size() > 1) b->max_multi_n = std::max(b->max_multi_n, n);
+    cost[i] = work;
+    b->circuits.push_back(bc);
+    b->plans.push_back(std::move(pc));
+  }
+  b->order.resize(num_circuits);
+  for (unsigned i = 0; i < num_circuits; ++i) b->order[i] = i;
+  std::stable_sort(b->order.begin(), b->order.end(), [&](uint32_t x, uint32_t y) { return cost[x] > cost[y]; });
+  *out = b;
+  return BRK_OK;
+}
+
+int brk_batch_destroy(brk_batch* batch) {
+  delete batch;
+  return BRK_OK;
+}
+
+int brk_batch_num_stages(const brk_batch* batch, unsigned circuit, unsigned* num_stages) {
+  if (!batch || !num_stages) return fail(BRK_ERR_INVALID_ARGUMENT, "null argument");
+  if (circuit >= batch->plans.size()) return fail(BRK_ERR_INVALID_ARGUMENT, "circuit %u out of range", circuit);
+  *num_stages = (unsigned)batch->plans[circuit].stages.size();
+  return BRK_OK;
+}
+
+int brk_batch_stage(const brk_batch* batch, unsigned circuit, unsigned stage, unsigned* local_qubits,
+                    unsigned* num_local, unsigned* num_ops) {
+  if (!batch || !num_local || !num_ops) return fail(BRK_ERR_INVALID_ARGUMENT, "null argument");
+  if (circuit >= batch->plans.size()) return fail(BRK_ERR_INVALID_ARGUMENT, "circuit %u out of range", circuit);
+  const auto& stages = batch->plans[circuit].stages;
+  if (stage >= stages.size()) return fail(BRK_ERR_INVALID_ARGUMENT, "stage %u out of range", stage);
+  const planned_stage& st = stages[stage];
+  *num_local = (unsigned)st.local.size();
+  *num_ops = (unsigned)st.ops.size();
+  if (local_qubits)
+    for (size_t p = 0; p < st.local.size(); ++p) local_qubits[p] = st.local[p];
+  return BRK_OK;
+}
+
+int brk_batch_stage_op(const brk_batch* batch, unsigned circuit, unsigned stage, unsigned k, int* kind,
+                       uint64_t* qubit_mask, double* matrix) {
+  if (!batch || !kind || !qubit_mask) return fail(BRK_ERR_INVALID_ARGUMENT, "null argument");
+  if (circuit >= batch->plans.size()) return fail(BRK_ERR_INVALID_ARGUMENT, "circuit %u out of range", circuit);
+  const auto& stages = batch->plans[circuit].stages;
+  if (stage >= stages.size() || k >= stages[stage].ops.size())
+    return fail(BRK_ERR_INVALID_ARGUMENT, "stage %u / op %u out of range", stage, k);
+  const pop& o = stages[stage].ops[k];
+  *kind = o.kind;
+  *qubit_mask = o.mask;
+  if (matrix)
+    for (int i = 0; i < 4; ++i) {
+      matrix[2 * i] = o.kind == BRK_BATCH_GATE ? o.m[i].real() : 0.0;
+      matrix[2 * i + 1] = o.kind == BRK_BATCH_GATE ? o.m[i].imag() : 0.0;
+    }
+  return BRK_OK;
+}
+
+int brk_batch_run(brk_batch* batch, double* energies, void* stream) {
+  if (!batch || !energies) return fail(BRK_ERR_INVALID_ARGUMENT, "null argument");
+  return run(batch, energies, nullptr, nullptr, nullptr, stream);
+}
+
+int brk_batch_run_timed(brk_batch* batch, double* energies, double* elapsed_ms, double* circuit_start_ms,
+                        double* circuit_end_ms, void* stream) {
+  if (!batch || !energies || !elapsed_ms) return fail(BRK_ERR_INVALID_ARGUMENT, "null argument");
+  return run(batch, energies, elapsed_ms, circuit_start_ms, circuit_end_ms, stream);
+}
+
+}  // extern "C"

@@ -87,6 +87,13 @@ _SIGS = {
     "brk_fused_apply_timed": [_p, _p, _i, C.POINTER(C.c_float), _p],
     "brk_circuit_num_parameters": [_p, _up],
     "brk_adjoint_ising_gradient": [_p, _p, _p, _i, _dp, _dp, _dp, _dp, _u, _p],
+    "brk_batch_create": [C.POINTER(C.c_void_p), _u, C.POINTER(_dp), C.POINTER(_dp), C.POINTER(C.c_void_p)],
+    "brk_batch_destroy": [_p],
+    "brk_batch_num_stages": [_p, _u, _up],
+    "brk_batch_stage": [_p, _u, _u, _up, _up, _up],
+    "brk_batch_stage_op": [_p, _u, _u, _u, C.POINTER(C.c_int), C.POINTER(_u64), _dp],
+    "brk_batch_run": [_p, _dp, _p],
+    "brk_batch_run_timed": [_p, _dp, _dp, _dp, _dp, _p],
 }
 _RESTYPES = {"brk_status_string": C.c_char_p, "brk_last_error": C.c_char_p, "brk_version": C.c_char_p}
 
@@ -418,3 +425,66 @@ def adjoint_ising_gradient(circuit, psi_ptr, work_ptr, dtype, J, h, stream=0):
     call("brk_adjoint_ising_gradient", circuit.h, psi_ptr, work_ptr, dtype, darr(J), darr(h), C.byref(e), g, npar,
          stream)
     return e.value, np.array(list(g)[:npar])
+
+
+BATCH_GATE, BATCH_SIGN, BATCH_MEASURE_X, BATCH_MEASURE_ZZ = 0, 1, 2, 3
+
+
+class Batch:
+    """brk_batch over [(num_qubits, ops, J, h)] (batching form)."""
+
+    def __init__(self, items):
+        self.circuits = [Circuit(n, ops) for n, ops, _, _ in items]
+        self.num_circuits = len(items)
+        self._J = [darr(J) if J is not None else None for _, _, J, _ in items]
+        self._h = [darr(h) if h is not None else None for _, _, _, h in items]
+        handles = (C.c_void_p * self.num_circuits)(*[c.h for c in self.circuits])
+        Ja = (_dp * self.num_circuits)(*[C.cast(a, _dp) if a is not None else None for a in self._J])
+        ha = (_dp * self.num_circuits)(*[C.cast(a, _dp) if a is not None else None for a in self._h])
+        out = C.c_void_p()
+        call("brk_batch_create", handles, self.num_circuits, Ja, ha, C.byref(out))
+        self.h = out.value
+
+    def num_stages(self, circuit):
+        n = C.c_uint()
+        call("brk_batch_num_stages", self.h, circuit, C.byref(n))
+        return n.value
+
+    def stage(self, circuit, stage):
+        """(local qubits, [(kind, qubit mask, 2x2 matrix or None)])."""
+        import numpy as np
+        local = (C.c_uint * 32)()
+        nl, nops = C.c_uint(), C.c_uint()
+        call("brk_batch_stage", self.h, circuit, stage, local, C.byref(nl), C.byref(nops))
+        ops = []
+        for k in range(nops.value):
+            kind, mask = C.c_int(), C.c_uint64()
+            m = (C.c_double * 8)()
+            call("brk_batch_stage_op", self.h, circuit, stage, k, C.byref(kind), C.byref(mask), m)
+            a = np.frombuffer(m, dtype=np.float64)
+            mat = (a[0::2] + 1j * a[1::2]).reshape(2, 2) if kind.value == BATCH_GATE else None
+            ops.append((kind.value, mask.value, mat))
+        return list(local)[: nl.value], ops
+
+    def run(self, stream=0):
+        import numpy as np
+        e = (C.c_double * self.num_circuits)()
+        call("brk_batch_run", self.h, e, stream)
+        return np.array(list(e))
+
+    def run_timed(self, stream=0):
+        """(energies, graph ms, per-circuit start ms, per-circuit end ms)."""
+        import numpy as np
+        nc = self.num_circuits
+        e, t0, t1 = (C.c_double * nc)(), (C.c_double * nc)(), (C.c_double * nc)()
+        ms = C.c_double()
+        call("brk_batch_run_timed", self.h, e, C.byref(ms), t0, t1, stream)
+        return np.array(list(e)), ms.value, np.array(list(t0)), np.array(list(t1))
+
+    def __del__(self):
+        try:
+            if self.h:
+                load().brk_batch_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
