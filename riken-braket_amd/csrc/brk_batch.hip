@@ -7,8 +7,11 @@
 // expensive first).  The program layout is described in brk_batch.hpp.  Per
 // chunk: load (or initialise) into LDS, apply the stage's instructions with a
 // barrier between them, accumulate the energy terms the stage measures, store.
-// Gate instructions hold up to 4 local positions: each thread keeps the 16
-// amplitudes of one group in registers and applies up to 8 2x2 gates to them.
+// A gate group holds up to 4 local positions: each thread keeps the 16
+// amplitudes of one group in registers, flips the signs of the CZ/Z terms
+// attached before and after the gates, and applies the gates slot by slot.
+// The program is read through the constant address space: it does not change
+// during the launch, so the compiler may use scalar loads for it.
 // LDS addresses are XOR-swizzled, phys(j) = j ^ ((j >> 5) & 31), so that
 // threads working on low positions do not collide on banks; the swizzle is
 // linear over GF(2), so phys(base | off) = phys(base) ^ phys(off) for disjoint
@@ -20,6 +23,12 @@ namespace brk {
 namespace {
 
 using f2 = cpx<float>;
+typedef const __attribute__((address_space(4))) uint32_t* cu32;
+typedef const __attribute__((address_space(4))) double* cf64;
+typedef const __attribute__((address_space(4))) batch_stage* cstage;
+typedef const __attribute__((address_space(4))) batch_circuit* ccircuit;
+
+constexpr int ZZ_TABLES = 3;  // byte tables of the ZZ energy: n - 1 <= 23 bond bits
 
 __device__ __forceinline__ uint32_t lpad(uint32_t j) { return j ^ ((j >> 5) & 31u); }
 
@@ -31,6 +40,32 @@ __device__ __forceinline__ uint32_t deposit(uint32_t c, uint32_t mask) {
   return r;
 }
 
+__device__ __forceinline__ uint32_t term_bits(uint32_t x, uint32_t d, uint32_t A) {
+  if (d == 0) return __builtin_popcount(x & A);
+  if (d < BATCH_TERM_ALL) return __builtin_popcount(x & (x >> d) & A);
+  return (x & A) == A ? 1u : 0u;
+}
+
+__device__ __forceinline__ f2 flip(f2 a, uint32_t odd) {
+  const uint32_t s = (odd & 1u) << 31;
+  return f2{__uint_as_float(__float_as_uint(a.re) ^ s), __uint_as_float(__float_as_uint(a.im) ^ s)};
+}
+
+template <int NP>
+__device__ __forceinline__ void apply_terms(f2 (&a)[1 << NP], uint32_t xb, const uint32_t (&xoff)[1 << NP], cu32 t,
+                                            uint32_t nterms) {
+  uint32_t par[1 << NP];
+#pragma unroll
+  for (int e = 0; e < (1 << NP); ++e) par[e] = 0;
+  for (uint32_t k = 0; k < nterms; ++k) {
+    const uint32_t d = t[2 * k], A = t[2 * k + 1];
+#pragma unroll
+    for (int e = 0; e < (1 << NP); ++e) par[e] += term_bits(xb | xoff[e], d, A);
+  }
+#pragma unroll
+  for (int e = 0; e < (1 << NP); ++e) a[e] = flip(a[e], par[e]);
+}
+
 __device__ __forceinline__ f2 gmul_add(f2 m0, f2 a0, f2 m1, f2 a1) {
   f2 r;
   r.re = m0.re * a0.re - m0.im * a0.im + m1.re * a1.re - m1.im * a1.im;
@@ -39,11 +74,11 @@ __device__ __forceinline__ f2 gmul_add(f2 m0, f2 a0, f2 m1, f2 a1) {
 }
 
 template <int NP, int S>
-__device__ __forceinline__ void apply_slot(f2 (&a)[1 << NP], const uint32_t* op) {
-  const f2 m00 = {__uint_as_float(op[1]), __uint_as_float(op[2])};
-  const f2 m01 = {__uint_as_float(op[3]), __uint_as_float(op[4])};
-  const f2 m10 = {__uint_as_float(op[5]), __uint_as_float(op[6])};
-  const f2 m11 = {__uint_as_float(op[7]), __uint_as_float(op[8])};
+__device__ __forceinline__ void apply_slot(f2 (&a)[1 << NP], cu32 op) {
+  const f2 m00 = {__uint_as_float(op[0]), __uint_as_float(op[1])};
+  const f2 m01 = {__uint_as_float(op[2]), __uint_as_float(op[3])};
+  const f2 m10 = {__uint_as_float(op[4]), __uint_as_float(op[5])};
+  const f2 m11 = {__uint_as_float(op[6]), __uint_as_float(op[7])};
 #pragma unroll
   for (int e = 0; e < (1 << NP); ++e) {
     if (e & (1 << S)) continue;
@@ -53,13 +88,48 @@ __device__ __forceinline__ void apply_slot(f2 (&a)[1 << NP], const uint32_t* op)
   }
 }
 
+template <int NP, int S>
+__device__ __forceinline__ cu32 apply_slot_run(f2 (&a)[1 << NP], cu32 op, uint32_t counts) {
+  if constexpr (S < NP) {
+    const uint32_t c = (counts >> (8 * S)) & 0xffu;
+    for (uint32_t i = 0; i < c; ++i, op += BATCH_GATE_WORDS) apply_slot<NP, S>(a, op);
+  }
+  return op;
+}
+
+struct chunk_ctx {
+  f2* chunk;
+  const uint32_t* tlo;
+  const uint32_t* thi;
+  uint32_t cbase;  // global index bits of the chunk
+  uint32_t L;
+  cstage S;
+};
+
+__device__ __forceinline__ uint32_t global_index(const chunk_ctx& c, uint32_t j) {
+  return c.cbase | c.tlo[j & 127u] | c.thi[j >> 7];
+}
+
 template <int NP>
-__device__ void gate_pass(f2* chunk, uint32_t L, const uint32_t* w, uint32_t tid) {
-  const uint32_t nops = (w[0] >> 16) & 0xffu;
+__device__ void gate_pass(const chunk_ctx& c, cu32 w, uint32_t tid) {
+  const uint32_t w0 = w[0], pw = w[1], counts = w[2], qw = w[3];
+  const uint32_t npre = (w0 >> 16) & 0xffu, npost = (w0 >> 24) & 0xffu;
+  const cu32 pre = w + 4;
+  const cu32 post = pre + 2 * npre;
+  const cu32 ops = post + 2 * npost;
   uint32_t p[NP];
 #pragma unroll
-  for (int k = 0; k < NP; ++k) p[k] = (w[1] >> (8 * k)) & 0xffu;
-  const uint32_t ngroups = 1u << (L - NP);
+  for (int k = 0; k < NP; ++k) p[k] = (pw >> (8 * k)) & 0xffu;
+  // global index bits of each group element (uniform)
+  uint32_t xoff[1 << NP];
+#pragma unroll
+  for (int e = 0; e < (1 << NP); ++e) {
+    uint32_t x = 0;
+#pragma unroll
+    for (int k = 0; k < NP; ++k) x |= ((uint32_t)(e >> k) & 1u) << ((qw >> (8 * k)) & 0xffu);
+    xoff[e] = x;
+  }
+  const uint32_t ngroups = 1u << (c.L - NP);
   for (uint32_t g = tid; g < ngroups; g += BATCH_THREADS) {
     uint32_t base = g;
 #pragma unroll
@@ -71,33 +141,54 @@ __device__ void gate_pass(f2* chunk, uint32_t L, const uint32_t* w, uint32_t tid
       uint32_t off = 0;
 #pragma unroll
       for (int k = 0; k < NP; ++k) off |= ((uint32_t)(e >> k) & 1u) << p[k];
-      a[e] = chunk[pb ^ lpad(off)];
+      a[e] = c.chunk[pb ^ lpad(off)];
     }
+    uint32_t xb = 0;
+    if (npre | npost) xb = global_index(c, base);
+    if (npre) apply_terms<NP>(a, xb, xoff, pre, npre);
+    cu32 op = ops;
+    op = apply_slot_run<NP, 0>(a, op, counts);
+    op = apply_slot_run<NP, 1>(a, op, counts);
+    op = apply_slot_run<NP, 2>(a, op, counts);
+    op = apply_slot_run<NP, 3>(a, op, counts);
+    if (npost) apply_terms<NP>(a, xb, xoff, post, npost);
     // recompute the store addresses instead of keeping 2^NP of them live across the gates
     uint32_t pbs = pb;
     asm volatile("" : "+v"(pbs));
-    const uint32_t* op = w + 2;
-    for (uint32_t o = 0; o < nops; ++o, op += BATCH_GATE_WORDS) {
-      switch (op[0]) {
-        case 0: apply_slot<NP, 0>(a, op); break;
-        case 1: if constexpr (NP > 1) apply_slot<NP, 1>(a, op); break;
-        case 2: if constexpr (NP > 2) apply_slot<NP, 2>(a, op); break;
-        default: if constexpr (NP > 3) apply_slot<NP, 3>(a, op); break;
-      }
-    }
 #pragma unroll
     for (int e = 0; e < (1 << NP); ++e) {
       uint32_t off = 0;
 #pragma unroll
       for (int k = 0; k < NP; ++k) off |= ((uint32_t)(e >> k) & 1u) << p[k];
-      chunk[pbs ^ lpad(off)] = a[e];
+      c.chunk[pbs ^ lpad(off)] = a[e];
     }
   }
 }
 
-__device__ __forceinline__ uint32_t global_index(uint32_t cbase, const uint32_t* tlo, const uint32_t* thi,
-                                                 uint32_t j) {
-  return cbase | tlo[j & 127u] | thi[j >> 7];
+// A sign pass of its own: 8 amplitudes per thread per step, terms outermost.
+__device__ void sign_pass(const chunk_ctx& c, cu32 w, uint32_t tid) {
+  const uint32_t nterms = (w[0] >> 16) & 0xffffu;
+  const cu32 t = w + 1;
+  const uint32_t csize = 1u << c.L;
+  for (uint32_t j0 = tid; j0 < csize; j0 += 8 * BATCH_THREADS) {
+    uint32_t x[8], par[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const uint32_t j = j0 + r * BATCH_THREADS;
+      x[r] = j < csize ? global_index(c, j) : 0u;
+      par[r] = 0;
+    }
+    for (uint32_t k = 0; k < nterms; ++k) {
+      const uint32_t d = t[2 * k], A = t[2 * k + 1];
+#pragma unroll
+      for (int r = 0; r < 8; ++r) par[r] += term_bits(x[r], d, A);
+    }
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const uint32_t j = j0 + r * BATCH_THREADS;
+      if (j < csize && (par[r] & 1u)) c.chunk[lpad(j)] = flip(c.chunk[lpad(j)], 1u);
+    }
+  }
 }
 
 __global__ __launch_bounds__(BATCH_THREADS, 1) void k_batch(batch_launch P) {
@@ -105,102 +196,108 @@ __global__ __launch_bounds__(BATCH_THREADS, 1) void k_batch(batch_launch P) {
   f2* chunk = reinterpret_cast<f2*>(smem);
   uint32_t* tlo = reinterpret_cast<uint32_t*>(smem + sizeof(f2) * BATCH_CHUNK_PADDED);
   uint32_t* thi = tlo + 128;
-  double* ham = reinterpret_cast<double*>(thi + 128);  // J[0..n-2], h[0..n-1]
-  uint32_t* next = reinterpret_cast<uint32_t*>(ham + 2 * BATCH_MAX_QUBITS);
-  double* red = reinterpret_cast<double*>(smem);       // reuses the chunk after the last stage
+  double* zz = reinterpret_cast<double*>(thi + 128);  // [ZZ_TABLES][256]
+  uint32_t* next = reinterpret_cast<uint32_t*>(zz + ZZ_TABLES * 256);
+  double* red = reinterpret_cast<double*>(smem);      // reuses the chunk after the last stage
   const uint32_t tid = threadIdx.x;
   f2* slot = P.slots ? reinterpret_cast<f2*>(P.slots) + (size_t)blockIdx.x * P.slot_elems : nullptr;
+  const ccircuit circuits = (ccircuit)P.circuits;
+  const cstage stages = (cstage)P.stages;
+  const cu32 words = (cu32)P.words;
+  const cf64 ham = (cf64)P.ham;
+  const cu32 order = (cu32)P.order;
 
   for (;;) {
     if (tid == 0) *next = atomicAdd(P.counter, 1u);
     __syncthreads();
-    const uint32_t k = __builtin_amdgcn_readfirstlane(*next);  // uniform: scalar loads of the program
+    const uint32_t k = __builtin_amdgcn_readfirstlane(*next);
     if (k >= P.num_circuits) break;
-    const uint32_t ci = P.order[k];
-    const batch_circuit C = P.circuits[ci];
+    const uint32_t ci = order[k];
+    const uint32_t n = circuits[ci].n;
+    const uint32_t num_stages = circuits[ci].num_stages;
+    const uint32_t first_stage = circuits[ci].first_stage;
+    const cf64 J = ham + circuits[ci].ham_offset;
+    const cf64 h = J + (n - 1);
     if (tid == 0) P.t_start[ci] = (uint64_t)wall_clock64();
-    if (tid < 2 * C.n - 1) ham[tid] = P.ham[C.ham_offset + tid];
-    const double* J = ham;
-    const double* h = ham + (C.n - 1);
+    // ZZ energy of a bond pattern d (bit i set: Z_i Z_i+1 = -1) = sum of its three byte-table entries
+    if (tid < ZZ_TABLES * 256) {
+      const uint32_t byte = tid >> 8, b = tid & 255u;
+      double e = 0.0;
+      for (uint32_t i = 0; i < 8; ++i) {
+        const uint32_t bond = 8 * byte + i;
+        if (bond + 1 < n) e += ((b >> i) & 1u) ? -J[bond] : J[bond];
+      }
+      zz[tid] = e;
+    }
     double acc = 0.0;
 
-    for (uint32_t s = 0; s < C.num_stages; ++s) {
-      const batch_stage& S = P.stages[C.first_stage + s];
-      const uint32_t L = S.L;
+    for (uint32_t s = 0; s < num_stages; ++s) {
+      const cstage S = stages + first_stage + s;
+      const uint32_t L = S->L, flags = S->flags, nonlocal = S->nonlocal_mask, measure_x = S->measure_x;
+      const uint32_t num_instr = S->num_instr;
+      const cu32 first_word = words + S->first_word;
       if (tid < 128) {
         uint32_t lo = 0, hi = 0;
         for (uint32_t q = 0; q < 7; ++q) {
-          if (q < L && ((tid >> q) & 1u)) lo |= 1u << S.local[q];
-          if (q + 7 < L && ((tid >> q) & 1u)) hi |= 1u << S.local[q + 7];
+          if (q < L && ((tid >> q) & 1u)) lo |= 1u << S->local[q];
+          if (q + 7 < L && ((tid >> q) & 1u)) hi |= 1u << S->local[q + 7];
         }
         tlo[tid] = lo;
         thi[tid] = hi;
       }
       __syncthreads();
       const uint32_t csize = 1u << L;
-      const uint32_t nchunks = 1u << (C.n - L);
-      for (uint32_t c = 0; c < nchunks; ++c) {
-        const uint32_t cbase = deposit(c, S.nonlocal_mask);
-        if (S.flags & BS_INIT) {
-          for (uint32_t j = tid; j < csize; j += BATCH_THREADS) {
-            const uint32_t x = global_index(cbase, tlo, thi, j);
-            chunk[lpad(j)] = f2{x == 0 ? 1.0f : 0.0f, 0.0f};
-          }
-        } else {
+      const uint32_t nchunks = 1u << (n - L);
+      for (uint32_t cidx = 0; cidx < nchunks; ++cidx) {
+        const chunk_ctx c{chunk, tlo, thi, deposit(cidx, nonlocal), L, S};
+        if (flags & BS_INIT) {
           for (uint32_t j = tid; j < csize; j += BATCH_THREADS)
-            chunk[lpad(j)] = slot[global_index(cbase, tlo, thi, j)];
+            chunk[lpad(j)] = f2{global_index(c, j) == 0 ? 1.0f : 0.0f, 0.0f};
+        } else {
+          for (uint32_t j = tid; j < csize; j += BATCH_THREADS) chunk[lpad(j)] = slot[global_index(c, j)];
         }
         __syncthreads();
-        const uint32_t* w = P.words + S.first_word;
-        for (uint32_t i = 0; i < S.num_instr; ++i) {
-          const uint32_t type = w[0] & 0xffu;
-          if (type == BI_GATES) {
-            const uint32_t npos = (w[0] >> 8) & 0xffu;
-            switch (npos) {
-              case 1: gate_pass<1>(chunk, L, w, tid); break;
-              case 2: gate_pass<2>(chunk, L, w, tid); break;
-              case 3: gate_pass<3>(chunk, L, w, tid); break;
-              default: gate_pass<4>(chunk, L, w, tid); break;
+        cu32 w = first_word;
+        for (uint32_t i = 0; i < num_instr; ++i) {
+          const uint32_t w0 = w[0];
+          if ((w0 & 0xffu) == BI_GATES) {
+            switch ((w0 >> 8) & 0xffu) {
+              case 1: gate_pass<1>(c, w, tid); break;
+              case 2: gate_pass<2>(c, w, tid); break;
+              case 3: gate_pass<3>(c, w, tid); break;
+              default: gate_pass<4>(c, w, tid); break;
             }
-            w += 2 + BATCH_GATE_WORDS * ((w[0] >> 16) & 0xffu);
+            const uint32_t counts = w[2];
+            const uint32_t nops =
+                (counts & 0xffu) + ((counts >> 8) & 0xffu) + ((counts >> 16) & 0xffu) + (counts >> 24);
+            w += 4 + 2 * (((w0 >> 16) & 0xffu) + (w0 >> 24)) + BATCH_GATE_WORDS * nops;
           } else {
-            const uint32_t nm = (w[0] >> 16) & 0xffu;
-            for (uint32_t j = tid; j < csize; j += BATCH_THREADS) {
-              const uint32_t x = global_index(cbase, tlo, thi, j);
-              uint32_t par = 0;
-              for (uint32_t m = 0; m < nm; ++m) par ^= (x & w[1 + m]) == w[1 + m] ? 1u : 0u;
-              if (par) {
-                const f2 v = chunk[lpad(j)];
-                chunk[lpad(j)] = f2{-v.re, -v.im};
-              }
-            }
-            w += 1 + nm;
+            sign_pass(c, w, tid);
+            w += 1 + 2 * ((w0 >> 16) & 0xffffu);
           }
           __syncthreads();
         }
-        if (S.flags & BS_ZZ) {
+        if (flags & BS_ZZ) {
+          const uint32_t bonds = n > 1 ? (1u << (n - 1)) - 1u : 0u;
           for (uint32_t j = tid; j < csize; j += BATCH_THREADS) {
-            const uint32_t x = global_index(cbase, tlo, thi, j);
+            const uint32_t x = global_index(c, j);
             const f2 v = chunk[lpad(j)];
             const double p = (double)v.re * (double)v.re + (double)v.im * (double)v.im;
-            const uint32_t d = x ^ (x >> 1);  // bit i: Z_i Z_i+1 = -1
-            double e = 0.0;
-            for (uint32_t i = 0; i + 1 < C.n; ++i) e += ((d >> i) & 1u) ? -J[i] : J[i];
-            acc += p * e;
+            const uint32_t d = (x ^ (x >> 1)) & bonds;
+            acc += p * (zz[d & 255u] + zz[256 + ((d >> 8) & 255u)] + zz[512 + (d >> 16)]);
           }
         }
-        for (uint32_t mx = S.measure_x; mx; mx &= mx - 1) {
+        for (uint32_t mx = measure_x; mx; mx &= mx - 1) {
           const uint32_t pos = __builtin_ctz(mx);
-          const double h2 = 2.0 * h[S.local[pos]];
+          const double h2 = 2.0 * h[S->local[pos]];
           for (uint32_t g = tid; g < (csize >> 1); g += BATCH_THREADS) {
             const uint32_t j0 = ((g >> pos) << (pos + 1)) | (g & ((1u << pos) - 1u));
             const f2 a0 = chunk[lpad(j0)], a1 = chunk[lpad(j0 | (1u << pos))];
             acc += h2 * ((double)a0.re * (double)a1.re + (double)a0.im * (double)a1.im);
           }
         }
-        if (S.flags & BS_STORE) {
-          for (uint32_t j = tid; j < csize; j += BATCH_THREADS)
-            slot[global_index(cbase, tlo, thi, j)] = chunk[lpad(j)];
+        if (flags & BS_STORE) {
+          for (uint32_t j = tid; j < csize; j += BATCH_THREADS) slot[global_index(c, j)] = chunk[lpad(j)];
         }
         __syncthreads();
       }
@@ -228,8 +325,7 @@ __global__ void k_batch_reset(uint32_t* counter) {
 }  // namespace
 
 size_t batch_lds_bytes() {
-  return sizeof(f2) * BATCH_CHUNK_PADDED + 2 * 128 * sizeof(uint32_t) + 2 * BATCH_MAX_QUBITS * sizeof(double) +
-         16;
+  return sizeof(f2) * BATCH_CHUNK_PADDED + 2 * 128 * sizeof(uint32_t) + ZZ_TABLES * 256 * sizeof(double) + 16;
 }
 
 int configure_batch() {

@@ -19,18 +19,28 @@ constexpr int BATCH_PASSENGERS = 4;     // qubits 0..3 are always local: chunk r
 constexpr int BATCH_MAX_QUBITS = 24;    // per circuit (one 2^n slot per workgroup for multi-stage circuits)
 constexpr int BATCH_MAX_GROUP = 4;      // local positions per gate instruction (16 amplitudes per thread)
 constexpr int BATCH_MAX_GROUP_OPS = 8;  // 2x2 gates per gate instruction
-constexpr int BATCH_MAX_SIGN_MASKS = 32;
 constexpr int BATCH_CHUNK = 1 << BATCH_MAX_LOCAL;
 constexpr int BATCH_CHUNK_PADDED = BATCH_CHUNK;  // LDS slots of a chunk (swizzled, not padded)
 
 // Instruction stream (32-bit words):
-//   gate group : w0 = BI_GATES | npos << 8 | nops << 16; w1 = local positions (bytes, ascending);
-//                then nops x 9 words: slot, m00.re, m00.im, m01.re, m01.im, m10.re, m10.im, m11.re, m11.im
-//                (FP32 bit patterns; a0' = m00 a0 + m01 a1, a1' = m10 a0 + m11 a1 on the pairs of that slot)
-//   sign flips : w0 = BI_SIGN | nmasks << 16; then nmasks global qubit masks; an amplitude is negated
-//                when an odd number of the masks have all their bits set in its index (Z, CZ, CnZ)
+//   gate group : w0 = BI_GATES | npos << 8 | npre << 16 | npost << 24
+//                w1 = local positions (bytes, ascending), w2 = number of gates per slot (bytes),
+//                w3 = the qubits at those positions (bytes)
+//                then npre sign terms, npost sign terms (2 words each), then the gates ordered by slot
+//                (gates on different slots commute), 8 words each: m00.re, m00.im, m01.re, m01.im,
+//                m10.re, m10.im, m11.re, m11.im (FP32 bit patterns; a0' = m00 a0 + m01 a1,
+//                a1' = m10 a0 + m11 a1 on the pairs of that slot).  Pre-signs are applied to the group's
+//                amplitudes before its gates, post-signs after them.
+//   signs      : w0 = BI_SIGN | nterms << 16, then nterms sign terms (a pass of its own).
+// A sign term (d, A) negates an amplitude with index x when its bit count is odd:
+//   d = 0      : x & A                  (Z on each qubit of A)
+//   0 < d < 32 : x & (x >> d) & A       (CZ(a, a + d) for each bit a of A)
+//   d = 32     : (x & A) == A           (Z on the all-ones subspace of A: CnZ)
+// Terms are merged by XOR, so repeated masks cancel exactly as the gates do.
 enum : uint32_t { BI_GATES = 1, BI_SIGN = 2 };
-constexpr int BATCH_GATE_WORDS = 9;
+constexpr int BATCH_GATE_WORDS = 8;
+constexpr int BATCH_MAX_ATTACHED_TERMS = 15;  // per pre/post list of a gate group
+constexpr uint32_t BATCH_TERM_ALL = 32;
 
 // stage flags
 enum : uint32_t {
@@ -72,7 +82,7 @@ struct batch_launch {
   uint64_t* t_end;         // ... and when its energy was written
 };
 
-// LDS bytes the kernel needs (chunk + index tables + Hamiltonian + queue slot).
+// LDS bytes the kernel needs (chunk + index tables + ZZ energy tables + queue slot).
 size_t batch_lds_bytes();
 // Lets the persistent kernel use more than 64 KiB of dynamic LDS (call once per device, outside capture).
 int configure_batch();

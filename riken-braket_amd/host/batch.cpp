@@ -16,8 +16,9 @@
 //     gate on its qubits, since diagonal ops commute with each other.  Ops on
 //     disjoint qubits commute, so taking ops out of order never changes the
 //     state.
-//  3. instructions: gates are grouped (<= 4 local positions, <= 8 gates),
-//     consecutive sign masks are merged (<= 32 masks).
+//  3. instructions: gates are grouped (<= 4 local positions, <= 8 gates);
+//     a run of sign masks becomes popcount terms (a CZ ladder is one term)
+//     folded into the neighbouring gate group, or a pass of its own.
 #include "circuit.hpp"
 #include "brk_batch.hpp"
 
@@ -150,73 +151,123 @@ uint32_t float_bits(double v) {
   return u;
 }
 
+// Sign terms (d, A) of a list of Z-type masks (encoding in brk_batch.hpp);
+// contributions add mod 2, so masks are merged by XOR.
+std::vector<std::pair<uint32_t, uint32_t>> sign_terms(const std::vector<uint32_t>& masks) {
+  uint32_t by_distance[32] = {};
+  std::vector<std::pair<uint32_t, uint32_t>> terms, generic;
+  for (uint32_t m : masks) {
+    const int bits = __builtin_popcount(m);
+    if (bits == 1) {
+      by_distance[0] ^= m;
+    } else if (bits == 2) {
+      const int a = __builtin_ctz(m), b = 31 - __builtin_clz(m);
+      by_distance[b - a] ^= 1u << a;
+    } else {
+      generic.push_back({brk::BATCH_TERM_ALL, m});
+    }
+  }
+  for (uint32_t d = 0; d < 32; ++d)
+    if (by_distance[d]) terms.push_back({d, by_distance[d]});
+  terms.insert(terms.end(), generic.begin(), generic.end());
+  return terms;
+}
+
 // Appends the instruction stream of one stage; returns the instruction count.
-uint32_t emit_stage(const planned_stage& st, unsigned n, std::vector<uint32_t>& words, uint32_t& measure_x,
-                    uint32_t& flags) {
+uint32_t emit_stage(const planned_stage& st, std::vector<uint32_t>& words, uint32_t& measure_x, uint32_t& flags) {
   int pos_of[32];
   for (int& p : pos_of) p = -1;
   for (size_t p = 0; p < st.local.size(); ++p) pos_of[st.local[p]] = (int)p;
-  uint32_t count = 0;
-  std::vector<const pop*> group;  // pending gate group
-  std::vector<uint32_t> masks;    // pending sign masks
-  auto flush_group = [&]() {
-    if (group.empty()) return;
-    std::vector<unsigned> pos;
-    for (const pop* g : group)
-      if (std::find(pos.begin(), pos.end(), (unsigned)pos_of[g->q]) == pos.end()) pos.push_back(pos_of[g->q]);
-    std::sort(pos.begin(), pos.end());
-    uint32_t pw = 0;
-    for (size_t k = 0; k < pos.size(); ++k) pw |= (uint32_t)pos[k] << (8 * k);
-    words.push_back(brk::BI_GATES | (uint32_t)pos.size() << 8 | (uint32_t)group.size() << 16);
-    words.push_back(pw);
-    for (const pop* g : group) {
-      const unsigned slot = (unsigned)(std::find(pos.begin(), pos.end(), (unsigned)pos_of[g->q]) - pos.begin());
-      words.push_back(slot);
-      for (int k = 0; k < 4; ++k) {
-        words.push_back(float_bits(g->m[k].real()));
-        words.push_back(float_bits(g->m[k].imag()));
-      }
-    }
-    ++count;
-    group.clear();
+
+  // segments: gate groups and runs of sign masks, in program order
+  struct segment {
+    bool gates;
+    std::vector<const pop*> ops;
+    std::vector<unsigned> pos;  // distinct local positions of the group
+    std::vector<uint32_t> masks;
+    std::vector<std::pair<uint32_t, uint32_t>> pre, post;
+    bool attached = false;  // a sign run folded into a neighbouring group
   };
-  auto flush_masks = [&]() {
-    if (masks.empty()) return;
-    words.push_back(brk::BI_SIGN | (uint32_t)masks.size() << 16);
-    for (uint32_t m : masks) words.push_back(m);
-    ++count;
-    masks.clear();
-  };
-  auto group_positions_with = [&](unsigned q) {
-    std::vector<unsigned> pos{(unsigned)pos_of[q]};
-    for (const pop* g : group)
-      if (std::find(pos.begin(), pos.end(), (unsigned)pos_of[g->q]) == pos.end()) pos.push_back(pos_of[g->q]);
-    return pos.size();
-  };
+  std::vector<segment> segs;
   for (const pop& o : st.ops) {
-    switch (o.kind) {
-      case BRK_BATCH_GATE:
-        flush_masks();
-        if (group.size() == (size_t)brk::BATCH_MAX_GROUP_OPS || group_positions_with(o.q) > (size_t)brk::BATCH_MAX_GROUP)
-          flush_group();
-        group.push_back(&o);
-        break;
-      case BRK_BATCH_SIGN:
-        flush_group();
-        if (masks.size() == (size_t)brk::BATCH_MAX_SIGN_MASKS) flush_masks();
-        masks.push_back(o.mask);
-        break;
-      case BRK_BATCH_MEASURE_X:
-        measure_x |= 1u << pos_of[o.q];
-        break;
-      case BRK_BATCH_MEASURE_ZZ:
-        flags |= brk::BS_ZZ;
-        break;
+    if (o.kind == BRK_BATCH_GATE) {
+      const unsigned pos = (unsigned)pos_of[o.q];
+      segment* g = (!segs.empty() && segs.back().gates) ? &segs.back() : nullptr;
+      if (g) {
+        const bool known = std::find(g->pos.begin(), g->pos.end(), pos) != g->pos.end();
+        if (g->ops.size() == (size_t)brk::BATCH_MAX_GROUP_OPS ||
+            (!known && g->pos.size() == (size_t)brk::BATCH_MAX_GROUP))
+          g = nullptr;
+      }
+      if (!g) {
+        segs.push_back(segment{true, {}, {}, {}, {}, {}});
+        g = &segs.back();
+      }
+      g->ops.push_back(&o);
+      if (std::find(g->pos.begin(), g->pos.end(), pos) == g->pos.end()) g->pos.push_back(pos);
+    } else if (o.kind == BRK_BATCH_SIGN) {
+      if (segs.empty() || segs.back().gates) segs.push_back(segment{false, {}, {}, {}, {}, {}});
+      segs.back().masks.push_back(o.mask);
+    } else if (o.kind == BRK_BATCH_MEASURE_X) {
+      measure_x |= 1u << pos_of[o.q];
+    } else {
+      flags |= brk::BS_ZZ;
     }
   }
-  flush_group();
-  flush_masks();
-  (void)n;
+  // fold each sign run into the next group (before its gates) or the previous one (after its gates)
+  for (size_t i = 0; i < segs.size(); ++i) {
+    if (segs[i].gates) continue;
+    auto terms = sign_terms(segs[i].masks);
+    if (terms.empty()) {  // the masks cancelled
+      segs[i].attached = true;
+      continue;
+    }
+    if (terms.size() > (size_t)brk::BATCH_MAX_ATTACHED_TERMS) continue;
+    if (i + 1 < segs.size() && segs[i + 1].gates && segs[i + 1].pre.empty()) {
+      segs[i + 1].pre = terms;
+      segs[i].attached = true;
+    } else if (i > 0 && segs[i - 1].gates && segs[i - 1].post.empty()) {
+      segs[i - 1].post = terms;
+      segs[i].attached = true;
+    }
+  }
+  uint32_t count = 0;
+  for (const segment& sg : segs) {
+    if (sg.gates) {
+      std::vector<unsigned> pos = sg.pos;
+      std::sort(pos.begin(), pos.end());
+      uint32_t pw = 0, counts = 0, qw = 0;
+      for (size_t k = 0; k < pos.size(); ++k) {
+        pw |= (uint32_t)pos[k] << (8 * k);
+        qw |= (uint32_t)st.local[pos[k]] << (8 * k);
+      }
+      std::vector<std::vector<const pop*>> by_slot(pos.size());
+      for (const pop* g : sg.ops) {
+        const size_t slot = (size_t)(std::find(pos.begin(), pos.end(), (unsigned)pos_of[g->q]) - pos.begin());
+        by_slot[slot].push_back(g);  // program order within a slot; different slots commute
+      }
+      for (size_t k = 0; k < pos.size(); ++k) counts |= (uint32_t)by_slot[k].size() << (8 * k);
+      words.push_back(brk::BI_GATES | (uint32_t)pos.size() << 8 | (uint32_t)sg.pre.size() << 16 |
+                      (uint32_t)sg.post.size() << 24);
+      words.push_back(pw);
+      words.push_back(counts);
+      words.push_back(qw);
+      for (const auto& t : sg.pre) words.insert(words.end(), {t.first, t.second});
+      for (const auto& t : sg.post) words.insert(words.end(), {t.first, t.second});
+      for (const auto& slot_ops : by_slot)
+        for (const pop* g : slot_ops)
+          for (int k = 0; k < 4; ++k) {
+            words.push_back(float_bits(g->m[k].real()));
+            words.push_back(float_bits(g->m[k].imag()));
+          }
+    } else {
+      if (sg.attached) continue;
+      const auto terms = sign_terms(sg.masks);
+      words.push_back(brk::BI_SIGN | (uint32_t)terms.size() << 16);
+      for (const auto& t : terms) words.insert(words.end(), {t.first, t.second});
+    }
+    ++count;
+  }
   return count;
 }
 
@@ -443,7 +494,7 @@ int brk_batch_create(const brk_circuit* const* circuits, unsigned num_circuits, 
       bs.first_word = (uint32_t)b->words.size();
       bool changes = false;
       for (const pop& o : st.ops) changes |= o.kind == BRK_BATCH_GATE || o.kind == BRK_BATCH_SIGN;
-      bs.num_instr = emit_stage(st, n, b->words, bs.measure_x, bs.flags);
+      bs.num_instr = emit_stage(st, b->words, bs.measure_x, bs.flags);
       if (s == 0) bs.flags |= brk::BS_INIT;
       if (s + 1 < pc.stages.size() && (changes || s == 0)) bs.flags |= brk::BS_STORE;
       b->stages.push_back(bs);
