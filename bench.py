@@ -16,8 +16,13 @@ amplitudes (64 GiB) resident in HBM.  A step is one pass of the circuit.
                4/5/6-qubit blocks (width autotuned by timing each once), complex64
                state, FP32 MFMA GEMMs; circuit time, speedup over the complex128
                per-gate path and infidelity 1 - |<psi_c128|psi_c64>|^2 (FP64)
+  batch_c64  = the batching form: 4096 ansatz circuits of 10..20 qubits in one
+               graph-captured persistent launch; circuits/s, p50/p99 latency
+  adjoint_c128 = the differentiation form: adjoint gradient of a 24-qubit,
+               576-parameter ansatz; gradients/s and error vs parameter shift / FD
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--qubits 32] [--no-cpu] [--no-fused]
+                       [--no-batch] [--no-adjoint]
 """
 from __future__ import annotations
 
@@ -199,6 +204,121 @@ def fused_leg(circuit, plain_state, args, stream):
     }
 
 
+def plain_energy(n, ops, J, h, stream):
+    import brk
+    from devstate import DeviceState
+
+    st = DeviceState(n)
+    brk.set_basis_state(st.ptr, st.size, brk.C128, 0, stream)
+    brk.Circuit(n, ops).run(st.ptr, stream=stream)
+    e = brk.expectation_ising(st.ptr, st.size, brk.C128, J, h, stream)
+    st.free()
+    return e
+
+
+def batch_leg(args, stream):
+    """Batching form: 4096 hardware-efficient-ansatz circuits of 10..20 qubits (8 layers),
+    complex64 states, one graph-captured persistent launch per batch; FP64 energies."""
+    import numpy as np
+
+    import brk
+    import circuits as W
+
+    items = W.batch_ansatz()
+    t0 = time.perf_counter()
+    batch = brk.Batch(items)
+    plan_s = time.perf_counter() - t0
+    energies = batch.run(stream)  # first run uploads the program and captures the graph
+    runs = []
+    for _ in range(max(1, args.steps)):
+        e, ms, start, end = batch.run_timed(stream)
+        runs.append((ms, end - start, end))
+        assert np.array_equal(e, energies)
+    ms = sum(r[0] for r in runs) / len(runs)
+    service = np.concatenate([r[1] for r in runs])
+    completion = np.concatenate([r[2] for r in runs])
+    rng = np.random.default_rng(0)
+    sample = sorted(int(i) for i in rng.choice(len(items), 8, replace=False))
+    err = max(abs(energies[i] - plain_energy(*items[i], stream)) for i in sample)
+    log(f"batch: {len(items)} circuits in {ms:.2f} ms ({len(items) / ms * 1e3:.0f} circuits/s)")
+    return {
+        "workload": "4096 circuits, qubits uniform in [10, 20], 8 layers of RY, RZ on each qubit + CZ ladder; "
+                    "open-chain Ising energy, J, h ~ U[-1, 1], seed 4",
+        "circuits_per_s": round(len(items) / ms * 1e3, 1),
+        "ms_per_batch": round(ms, 3),
+        "latency_ms": {
+            "service_p50": round(float(np.percentile(service, 50)), 4),
+            "service_p99": round(float(np.percentile(service, 99)), 4),
+            "completion_p50": round(float(np.percentile(completion, 50)), 4),
+            "completion_p99": round(float(np.percentile(completion, 99)), 4),
+        },
+        "latency_definition": "service: from a workgroup taking the circuit off the queue to its energy "
+                              "being written; completion: from the first take in the launch to that write",
+        "plan_s": round(plan_s, 2),
+        "max_abs_energy_err_vs_c128_plain": err,
+        "energy_check_sample": len(sample),
+        "dtype": "c64 state, FP32 gate arithmetic, FP64 energy accumulation",
+    }
+
+
+def adjoint_leg(args, stream):
+    """Differentiation form: 24 qubits, 12 layers (576 parameters), TFIM J = 1.0, h = 0.7,
+    complex128; adjoint gradient timed, checked against parameter shift and central
+    differences (step 1e-5) on the FP64 plain path for a sample of parameters."""
+    import math
+
+    import numpy as np
+
+    import brk
+    import circuits as W
+    from devstate import DeviceState
+
+    n, layers = 24, 12
+    ops, theta = W.adjoint_ansatz(n, layers, seed=5)
+    theta = np.array(theta)
+    J, h = W.tfim(n)
+    circuit = brk.Circuit(n, ops)
+    psi, work = DeviceState(n), DeviceState(n)
+    times = []
+    for k in range(max(1, args.steps) + 1):
+        brk.set_basis_state(psi.ptr, psi.size, brk.C128, 0, stream)
+        t0 = time.perf_counter()
+        e, g = brk.adjoint_ising_gradient(circuit, psi.ptr, work.ptr, brk.C128, J, h, stream)
+        if k:  # the first call is the warm-up
+            times.append(time.perf_counter() - t0)
+    psi.free()
+    work.free()
+    rng = np.random.default_rng(1)
+    sample = sorted(int(i) for i in rng.choice(theta.size, 6, replace=False))
+    err_shift = err_fd = 0.0
+    for k in sample:
+        tp, tm = theta.copy(), theta.copy()
+        tp[k] += math.pi / 2
+        tm[k] -= math.pi / 2
+        shift = 0.5 * (plain_energy(n, W.with_parameters(ops, tp), J, h, stream) -
+                       plain_energy(n, W.with_parameters(ops, tm), J, h, stream))
+        tp, tm = theta.copy(), theta.copy()
+        tp[k] += 1e-5
+        tm[k] -= 1e-5
+        fd = (plain_energy(n, W.with_parameters(ops, tp), J, h, stream) -
+              plain_energy(n, W.with_parameters(ops, tm), J, h, stream)) / 2e-5
+        err_shift = max(err_shift, abs(g[k] - shift))
+        err_fd = max(err_fd, abs(g[k] - fd))
+    ms = sum(times) / len(times) * 1e3
+    log(f"adjoint: {theta.size} parameters in {ms:.1f} ms")
+    return {
+        "workload": "24 qubits complex128, 12 layers of RY, RZ on each qubit + CZ ladder (576 parameters, "
+                    "U[-pi, pi), seed 5), TFIM open chain J = 1.0, h = 0.7",
+        "gradients_per_s": round(1e3 / ms, 2),
+        "ms_per_gradient": round(ms, 2),
+        "energy": e,
+        "max_abs_err_vs_parameter_shift": err_shift,
+        "max_abs_err_vs_central_fd_1e-5": err_fd,
+        "checked_parameters": sample,
+        "dtype": "c128 state, FP64 accumulation",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -208,6 +328,8 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fused", action="store_true", help="skip the fused complex64 (precision form) leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batching-form leg")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiation-form leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run the streaming copy kernel once on 2^30 c128 amplitudes (16 GiB read + "
                          "16 GiB written) so a counter run can scale FETCH_SIZE/WRITE_SIZE")
@@ -334,6 +456,10 @@ def main():
     if fused is not None:
         fused["speedup_vs_c128"] = round(ms_per_step / fused["ms_per_circuit"], 2)
         result["fused_c64"] = fused
+    if not args.no_batch:
+        result["batch_c64"] = batch_leg(args, stream)
+    if not args.no_adjoint:
+        result["adjoint_c128"] = adjoint_leg(args, stream)
     traffic = load_traffic(dom_name, n)
     if traffic is not None:
         result["roofline"]["traffic"] = traffic["bytes_per_launch"]

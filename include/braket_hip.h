@@ -338,7 +338,7 @@ int brk_adjoint_ising_gradient(const brk_circuit* circuit, void* psi, void* work
 typedef struct brk_batch brk_batch;
 
 #define BRK_BATCH_MAX_QUBITS 24
-#define BRK_BATCH_MAX_LOCAL 14
+#define BRK_BATCH_MAX_LOCAL 13
 
 /* planner op kinds (brk_batch_stage_op) */
 enum {

@@ -3,13 +3,14 @@
 // CIRCUITS mode, which advances several states one after another,
 // bra/src/bra.cpp:315-336, bra/src/interpreter.cpp:411-419).
 //
-// One 1024-thread workgroup per CU takes circuits from a global queue (most
-// expensive first).  The program layout is described in brk_batch.hpp.  Per
+// Two 512-thread workgroups per CU take circuits from a global queue (most
+// expensive first); while one is in a barrier-separated LDS phase the other
+// computes.  The program layout is described in brk_batch.hpp.  Per
 // chunk: load (or initialise) into LDS, apply the stage's instructions with a
 // barrier between them, accumulate the energy terms the stage measures, store.
 // A gate group holds up to 4 local positions: each thread keeps the 16
 // amplitudes of one group in registers, flips the signs of the CZ/Z terms
-// attached before and after the gates, and applies the gates slot by slot.
+// attached before and after the gates, and applies one 2x2 matrix per slot.
 // The program is read through the constant address space: it does not change
 // during the launch, so the compiler may use scalar loads for it.
 // LDS addresses are XOR-swizzled, phys(j) = j ^ ((j >> 5) & 31), so that
@@ -66,10 +67,12 @@ __device__ __forceinline__ void apply_terms(f2 (&a)[1 << NP], uint32_t xb, const
   for (int e = 0; e < (1 << NP); ++e) a[e] = flip(a[e], par[e]);
 }
 
+// m0 a0 + m1 a1 as one multiply and three fused multiply-adds per component
+// (the library builds without FP contraction; the batching form fuses on purpose).
 __device__ __forceinline__ f2 gmul_add(f2 m0, f2 a0, f2 m1, f2 a1) {
   f2 r;
-  r.re = m0.re * a0.re - m0.im * a0.im + m1.re * a1.re - m1.im * a1.im;
-  r.im = m0.re * a0.im + m0.im * a0.re + m1.re * a1.im + m1.im * a1.re;
+  r.re = __builtin_fmaf(m0.re, a0.re, __builtin_fmaf(-m0.im, a0.im, __builtin_fmaf(m1.re, a1.re, -m1.im * a1.im)));
+  r.im = __builtin_fmaf(m0.re, a0.im, __builtin_fmaf(m0.im, a0.re, __builtin_fmaf(m1.re, a1.im, m1.im * a1.re)));
   return r;
 }
 
@@ -88,15 +91,6 @@ __device__ __forceinline__ void apply_slot(f2 (&a)[1 << NP], cu32 op) {
   }
 }
 
-template <int NP, int S>
-__device__ __forceinline__ cu32 apply_slot_run(f2 (&a)[1 << NP], cu32 op, uint32_t counts) {
-  if constexpr (S < NP) {
-    const uint32_t c = (counts >> (8 * S)) & 0xffu;
-    for (uint32_t i = 0; i < c; ++i, op += BATCH_GATE_WORDS) apply_slot<NP, S>(a, op);
-  }
-  return op;
-}
-
 struct chunk_ctx {
   f2* chunk;
   const uint32_t* tlo;
@@ -112,9 +106,9 @@ __device__ __forceinline__ uint32_t global_index(const chunk_ctx& c, uint32_t j)
 
 template <int NP>
 __device__ void gate_pass(const chunk_ctx& c, cu32 w, uint32_t tid) {
-  const uint32_t w0 = w[0], pw = w[1], counts = w[2], qw = w[3];
+  const uint32_t w0 = w[0], pw = w[1], qw = w[2];
   const uint32_t npre = (w0 >> 16) & 0xffu, npost = (w0 >> 24) & 0xffu;
-  const cu32 pre = w + 4;
+  const cu32 pre = w + 3;
   const cu32 post = pre + 2 * npre;
   const cu32 ops = post + 2 * npost;
   uint32_t p[NP];
@@ -146,11 +140,10 @@ __device__ void gate_pass(const chunk_ctx& c, cu32 w, uint32_t tid) {
     uint32_t xb = 0;
     if (npre | npost) xb = global_index(c, base);
     if (npre) apply_terms<NP>(a, xb, xoff, pre, npre);
-    cu32 op = ops;
-    op = apply_slot_run<NP, 0>(a, op, counts);
-    op = apply_slot_run<NP, 1>(a, op, counts);
-    op = apply_slot_run<NP, 2>(a, op, counts);
-    op = apply_slot_run<NP, 3>(a, op, counts);
+    apply_slot<NP, 0>(a, ops);
+    if constexpr (NP > 1) apply_slot<NP, 1>(a, ops + BATCH_GATE_WORDS);
+    if constexpr (NP > 2) apply_slot<NP, 2>(a, ops + 2 * BATCH_GATE_WORDS);
+    if constexpr (NP > 3) apply_slot<NP, 3>(a, ops + 3 * BATCH_GATE_WORDS);
     if (npost) apply_terms<NP>(a, xb, xoff, post, npost);
     // recompute the store addresses instead of keeping 2^NP of them live across the gates
     uint32_t pbs = pb;
@@ -191,7 +184,7 @@ __device__ void sign_pass(const chunk_ctx& c, cu32 w, uint32_t tid) {
   }
 }
 
-__global__ __launch_bounds__(BATCH_THREADS, 1) void k_batch(batch_launch P) {
+__global__ __launch_bounds__(BATCH_THREADS, BATCH_WG_PER_CU) void k_batch(batch_launch P) {
   extern __shared__ __align__(16) unsigned char smem[];
   f2* chunk = reinterpret_cast<f2*>(smem);
   uint32_t* tlo = reinterpret_cast<uint32_t*>(smem + sizeof(f2) * BATCH_CHUNK_PADDED);
@@ -220,14 +213,14 @@ __global__ __launch_bounds__(BATCH_THREADS, 1) void k_batch(batch_launch P) {
     const cf64 h = J + (n - 1);
     if (tid == 0) P.t_start[ci] = (uint64_t)wall_clock64();
     // ZZ energy of a bond pattern d (bit i set: Z_i Z_i+1 = -1) = sum of its three byte-table entries
-    if (tid < ZZ_TABLES * 256) {
-      const uint32_t byte = tid >> 8, b = tid & 255u;
+    for (uint32_t t = tid; t < ZZ_TABLES * 256; t += BATCH_THREADS) {
+      const uint32_t byte = t >> 8, b = t & 255u;
       double e = 0.0;
       for (uint32_t i = 0; i < 8; ++i) {
         const uint32_t bond = 8 * byte + i;
         if (bond + 1 < n) e += ((b >> i) & 1u) ? -J[bond] : J[bond];
       }
-      zz[tid] = e;
+      zz[t] = e;
     }
     double acc = 0.0;
 
@@ -261,16 +254,14 @@ __global__ __launch_bounds__(BATCH_THREADS, 1) void k_batch(batch_launch P) {
         for (uint32_t i = 0; i < num_instr; ++i) {
           const uint32_t w0 = w[0];
           if ((w0 & 0xffu) == BI_GATES) {
-            switch ((w0 >> 8) & 0xffu) {
+            const uint32_t npos = (w0 >> 8) & 0xffu;
+            switch (npos) {
               case 1: gate_pass<1>(c, w, tid); break;
               case 2: gate_pass<2>(c, w, tid); break;
               case 3: gate_pass<3>(c, w, tid); break;
               default: gate_pass<4>(c, w, tid); break;
             }
-            const uint32_t counts = w[2];
-            const uint32_t nops =
-                (counts & 0xffu) + ((counts >> 8) & 0xffu) + ((counts >> 16) & 0xffu) + (counts >> 24);
-            w += 4 + 2 * (((w0 >> 16) & 0xffu) + (w0 >> 24)) + BATCH_GATE_WORDS * nops;
+            w += 3 + 2 * (((w0 >> 16) & 0xffu) + (w0 >> 24)) + BATCH_GATE_WORDS * npos;
           } else {
             sign_pass(c, w, tid);
             w += 1 + 2 * ((w0 >> 16) & 0xffffu);

@@ -13,8 +13,9 @@
 
 namespace brk {
 
-constexpr int BATCH_THREADS = 1024;     // one workgroup per CU (the chunk fills most of the LDS)
-constexpr int BATCH_MAX_LOCAL = 14;     // 2^14 complex64 = 128 KiB of LDS per chunk
+constexpr int BATCH_THREADS = 512;      // two workgroups per CU overlap each other's LDS and ALU phases
+constexpr int BATCH_WG_PER_CU = 2;
+constexpr int BATCH_MAX_LOCAL = 13;     // 2^13 complex64 = 64 KiB of LDS per chunk
 constexpr int BATCH_PASSENGERS = 4;     // qubits 0..3 are always local: chunk rows are 128-byte runs
 constexpr int BATCH_MAX_QUBITS = 24;    // per circuit (one 2^n slot per workgroup for multi-stage circuits)
 constexpr int BATCH_MAX_GROUP = 4;      // local positions per gate instruction (16 amplitudes per thread)
@@ -24,12 +25,12 @@ constexpr int BATCH_CHUNK_PADDED = BATCH_CHUNK;  // LDS slots of a chunk (swizzl
 
 // Instruction stream (32-bit words):
 //   gate group : w0 = BI_GATES | npos << 8 | npre << 16 | npost << 24
-//                w1 = local positions (bytes, ascending), w2 = number of gates per slot (bytes),
-//                w3 = the qubits at those positions (bytes)
-//                then npre sign terms, npost sign terms (2 words each), then the gates ordered by slot
-//                (gates on different slots commute), 8 words each: m00.re, m00.im, m01.re, m01.im,
-//                m10.re, m10.im, m11.re, m11.im (FP32 bit patterns; a0' = m00 a0 + m01 a1,
-//                a1' = m10 a0 + m11 a1 on the pairs of that slot).  Pre-signs are applied to the group's
+//                w1 = local positions (bytes, ascending), w2 = the qubits at those positions (bytes)
+//                then npre sign terms, npost sign terms (2 words each), then one 2x2 matrix per position,
+//                8 words each: m00.re, m00.im, m01.re, m01.im, m10.re, m10.im, m11.re, m11.im (FP32 bit
+//                patterns; a0' = m00 a0 + m01 a1, a1' = m10 a0 + m11 a1 on the pairs of that position).
+//                The gates of a group on one position are multiplied into that matrix (in FP64 on the
+//                host): gates on different positions commute.  Pre-signs are applied to the group's
 //                amplitudes before its gates, post-signs after them.
 //   signs      : w0 = BI_SIGN | nterms << 16, then nterms sign terms (a pass of its own).
 // A sign term (d, A) negates an amplitude with index x when its bit count is odd:

@@ -16,7 +16,8 @@
 //     gate on its qubits, since diagonal ops commute with each other.  Ops on
 //     disjoint qubits commute, so taking ops out of order never changes the
 //     state.
-//  3. instructions: gates are grouped (<= 4 local positions, <= 8 gates);
+//  3. instructions: gates are grouped (<= 4 local positions, <= 8 gates,
+//     one product matrix per position);
 //     a run of sign masks becomes popcount terms (a CZ ladder is one term)
 //     folded into the neighbouring gate group, or a pass of its own.
 #include "circuit.hpp"
@@ -236,30 +237,30 @@ uint32_t emit_stage(const planned_stage& st, std::vector<uint32_t>& words, uint3
     if (sg.gates) {
       std::vector<unsigned> pos = sg.pos;
       std::sort(pos.begin(), pos.end());
-      uint32_t pw = 0, counts = 0, qw = 0;
+      uint32_t pw = 0, qw = 0;
       for (size_t k = 0; k < pos.size(); ++k) {
         pw |= (uint32_t)pos[k] << (8 * k);
         qw |= (uint32_t)st.local[pos[k]] << (8 * k);
       }
-      std::vector<std::vector<const pop*>> by_slot(pos.size());
+      // one matrix per position: the product of that position's gates, latest on the left
+      std::vector<std::array<cd, 4>> mats(pos.size(), std::array<cd, 4>{cd(1), cd(0), cd(0), cd(1)});
       for (const pop* g : sg.ops) {
         const size_t slot = (size_t)(std::find(pos.begin(), pos.end(), (unsigned)pos_of[g->q]) - pos.begin());
-        by_slot[slot].push_back(g);  // program order within a slot; different slots commute
+        const auto a = mats[slot];
+        mats[slot] = {g->m[0] * a[0] + g->m[1] * a[2], g->m[0] * a[1] + g->m[1] * a[3],
+                      g->m[2] * a[0] + g->m[3] * a[2], g->m[2] * a[1] + g->m[3] * a[3]};
       }
-      for (size_t k = 0; k < pos.size(); ++k) counts |= (uint32_t)by_slot[k].size() << (8 * k);
       words.push_back(brk::BI_GATES | (uint32_t)pos.size() << 8 | (uint32_t)sg.pre.size() << 16 |
                       (uint32_t)sg.post.size() << 24);
       words.push_back(pw);
-      words.push_back(counts);
       words.push_back(qw);
       for (const auto& t : sg.pre) words.insert(words.end(), {t.first, t.second});
       for (const auto& t : sg.post) words.insert(words.end(), {t.first, t.second});
-      for (const auto& slot_ops : by_slot)
-        for (const pop* g : slot_ops)
-          for (int k = 0; k < 4; ++k) {
-            words.push_back(float_bits(g->m[k].real()));
-            words.push_back(float_bits(g->m[k].imag()));
-          }
+      for (const auto& m : mats)
+        for (int k = 0; k < 4; ++k) {
+          words.push_back(float_bits(m[k].real()));
+          words.push_back(float_bits(m[k].imag()));
+        }
     } else {
       if (sg.attached) continue;
       const auto terms = sign_terms(sg.masks);
@@ -349,7 +350,7 @@ int ensure_device(brk_batch* b) {
   if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || rate_khz <= 0)
     return brk::fail(BRK_ERR_HIP, "cannot read the wall-clock rate");
   b->tick_ms = 1.0 / (double)rate_khz;
-  b->grid = (unsigned)std::min<size_t>((size_t)cus, nc);
+  b->grid = (unsigned)std::min<size_t>((size_t)cus * brk::BATCH_WG_PER_CU, nc);
   uint64_t slot_elems = 0;
   if (b->max_multi_n) {
     slot_elems = 1ull << b->max_multi_n;

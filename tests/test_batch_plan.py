@@ -24,7 +24,7 @@ def execute_plan(batch, c, n, J, h):
     energy = 0.0
     for st in range(batch.num_stages(c)):
         local, ops = batch.stage(c, st)
-        L = min(n, 14)
+        L = min(n, 13)
         assert len(local) == L and local == sorted(local)
         assert set(range(min(n, 4))) <= set(local)
         lmask = sum(1 << q for q in local)
@@ -69,10 +69,10 @@ def test_plan_reproduces_oracle_energy(oracle, n, layers):
     assert np.max(np.abs(s - s_ref)) < 1e-12
     assert abs(e - e_ref) < 1e-12
     stages = b.num_stages(0)
-    if n <= 14:
+    if n <= 13:
         assert stages == 1
     else:
-        assert stages <= layers + 3  # about one stage per layer once n > 14
+        assert stages <= layers + 3  # about one stage per layer once n > 13
 
 
 def test_plan_mixed_ops(oracle):
