@@ -42,29 +42,33 @@ __device__ __forceinline__ uint32_t deposit(uint32_t c, uint32_t mask) {
 }
 
 __device__ __forceinline__ uint32_t term_bits(uint32_t x, uint32_t d, uint32_t A) {
-  if (d == 0) return __builtin_popcount(x & A);
-  if (d < BATCH_TERM_ALL) return __builtin_popcount(x & (x >> d) & A);
+  if (d < BATCH_TERM_ALL) return __builtin_popcount(x & (x >> d) & A);  // d = 0: x & x = x
   return (x & A) == A ? 1u : 0u;
 }
 
 __device__ __forceinline__ f2 flip(f2 a, uint32_t odd) {
-  const uint32_t s = (odd & 1u) << 31;
+  const uint32_t s = odd << 31;
   return f2{__uint_as_float(__float_as_uint(a.re) ^ s), __uint_as_float(__float_as_uint(a.im) ^ s)};
 }
 
+// Sign terms attached to a gate group are of the Z / CZ kinds only (d < 32): the
+// bit count of x & (x >> d) & A is branch-free and accumulates in one instruction.
 template <int NP>
 __device__ __forceinline__ void apply_terms(f2 (&a)[1 << NP], uint32_t xb, const uint32_t (&xoff)[1 << NP], cu32 t,
                                             uint32_t nterms) {
-  uint32_t par[1 << NP];
+  uint32_t x[1 << NP], par[1 << NP];
 #pragma unroll
-  for (int e = 0; e < (1 << NP); ++e) par[e] = 0;
+  for (int e = 0; e < (1 << NP); ++e) {
+    x[e] = xb | xoff[e];
+    par[e] = 0;
+  }
   for (uint32_t k = 0; k < nterms; ++k) {
     const uint32_t d = t[2 * k], A = t[2 * k + 1];
 #pragma unroll
-    for (int e = 0; e < (1 << NP); ++e) par[e] += term_bits(xb | xoff[e], d, A);
+    for (int e = 0; e < (1 << NP); ++e) par[e] += __builtin_popcount(x[e] & (x[e] >> d) & A);
   }
 #pragma unroll
-  for (int e = 0; e < (1 << NP); ++e) a[e] = flip(a[e], par[e]);
+  for (int e = 0; e < (1 << NP); ++e) a[e] = flip(a[e], par[e] & 1u);
 }
 
 // m0 a0 + m1 a1 as one multiply and three fused multiply-adds per component
@@ -114,29 +118,30 @@ __device__ void gate_pass(const chunk_ctx& c, cu32 w, uint32_t tid) {
   uint32_t p[NP];
 #pragma unroll
   for (int k = 0; k < NP; ++k) p[k] = (pw >> (8 * k)) & 0xffu;
-  // global index bits of each group element (uniform)
-  uint32_t xoff[1 << NP];
+  // per group element (uniform): global index bits and swizzled LDS byte offset
+  uint32_t xoff[1 << NP], boff[1 << NP];
 #pragma unroll
   for (int e = 0; e < (1 << NP); ++e) {
-    uint32_t x = 0;
+    uint32_t x = 0, off = 0;
 #pragma unroll
-    for (int k = 0; k < NP; ++k) x |= ((uint32_t)(e >> k) & 1u) << ((qw >> (8 * k)) & 0xffu);
+    for (int k = 0; k < NP; ++k) {
+      x |= ((uint32_t)(e >> k) & 1u) << ((qw >> (8 * k)) & 0xffu);
+      off |= ((uint32_t)(e >> k) & 1u) << p[k];
+    }
     xoff[e] = x;
+    boff[e] = lpad(off) * 8u;
   }
   const uint32_t ngroups = 1u << (c.L - NP);
   for (uint32_t g = tid; g < ngroups; g += BATCH_THREADS) {
     uint32_t base = g;
 #pragma unroll
     for (int k = 0; k < NP; ++k) base = ((base >> p[k]) << (p[k] + 1)) | (base & ((1u << p[k]) - 1u));
-    const uint32_t pb = lpad(base);
+    // byte addresses: (phys(base) ^ phys(off)) * 8 = phys(base) * 8 ^ phys(off) * 8
+    const uint32_t pb = lpad(base) * 8u;
+    char* const bytes = reinterpret_cast<char*>(c.chunk);
     f2 a[1 << NP];
 #pragma unroll
-    for (int e = 0; e < (1 << NP); ++e) {
-      uint32_t off = 0;
-#pragma unroll
-      for (int k = 0; k < NP; ++k) off |= ((uint32_t)(e >> k) & 1u) << p[k];
-      a[e] = c.chunk[pb ^ lpad(off)];
-    }
+    for (int e = 0; e < (1 << NP); ++e) a[e] = *reinterpret_cast<const f2*>(bytes + (pb ^ boff[e]));
     uint32_t xb = 0;
     if (npre | npost) xb = global_index(c, base);
     if (npre) apply_terms<NP>(a, xb, xoff, pre, npre);
@@ -149,12 +154,7 @@ __device__ void gate_pass(const chunk_ctx& c, cu32 w, uint32_t tid) {
     uint32_t pbs = pb;
     asm volatile("" : "+v"(pbs));
 #pragma unroll
-    for (int e = 0; e < (1 << NP); ++e) {
-      uint32_t off = 0;
-#pragma unroll
-      for (int k = 0; k < NP; ++k) off |= ((uint32_t)(e >> k) & 1u) << p[k];
-      c.chunk[pbs ^ lpad(off)] = a[e];
-    }
+    for (int e = 0; e < (1 << NP); ++e) *reinterpret_cast<f2*>(bytes + (pbs ^ boff[e])) = a[e];
   }
 }
 
@@ -185,13 +185,12 @@ __device__ void sign_pass(const chunk_ctx& c, cu32 w, uint32_t tid) {
 }
 
 __global__ __launch_bounds__(BATCH_THREADS, BATCH_WG_PER_CU) void k_batch(batch_launch P) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  f2* chunk = reinterpret_cast<f2*>(smem);
-  uint32_t* tlo = reinterpret_cast<uint32_t*>(smem + sizeof(f2) * BATCH_CHUNK_PADDED);
-  uint32_t* thi = tlo + 128;
-  double* zz = reinterpret_cast<double*>(thi + 128);  // [ZZ_TABLES][256]
-  uint32_t* next = reinterpret_cast<uint32_t*>(zz + ZZ_TABLES * 256);
-  double* red = reinterpret_cast<double*>(smem);      // reuses the chunk after the last stage
+  // static LDS: addresses are compile-time constants, so group offsets fold into the instructions
+  __shared__ __align__(16) f2 chunk[BATCH_CHUNK_PADDED];
+  __shared__ uint32_t tlo[128], thi[128];
+  __shared__ double zz[ZZ_TABLES * 256];  // ZZ energy byte tables
+  __shared__ uint32_t next[1];
+  double* red = reinterpret_cast<double*>(chunk);  // reuses the chunk after the last stage
   const uint32_t tid = threadIdx.x;
   f2* slot = P.slots ? reinterpret_cast<f2*>(P.slots) + (size_t)blockIdx.x * P.slot_elems : nullptr;
   const ccircuit circuits = (ccircuit)P.circuits;
@@ -316,20 +315,15 @@ __global__ void k_batch_reset(uint32_t* counter) {
 }  // namespace
 
 size_t batch_lds_bytes() {
-  return sizeof(f2) * BATCH_CHUNK_PADDED + 2 * 128 * sizeof(uint32_t) + ZZ_TABLES * 256 * sizeof(double) + 16;
+  return sizeof(f2) * BATCH_CHUNK_PADDED + 2 * 128 * sizeof(uint32_t) + ZZ_TABLES * 256 * sizeof(double) + 4;
 }
 
-int configure_batch() {
-  BRK_HIP_CHECK(hipFuncSetAttribute((const void*)k_batch, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)batch_lds_bytes()));
-  return BRK_OK;
-}
+int configure_batch() { return BRK_OK; }  // static LDS: nothing to opt in to
 
 int launch_batch(const batch_launch& L, unsigned grid, void* stream) {
-  const size_t lds = batch_lds_bytes();
   hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, as_stream(stream), L.counter);
   if (int s = check_launch("batch queue reset")) return s;
-  hipLaunchKernelGGL(k_batch, dim3(grid), dim3(BATCH_THREADS), lds, as_stream(stream), L);
+  hipLaunchKernelGGL(k_batch, dim3(grid), dim3(BATCH_THREADS), 0, as_stream(stream), L);
   return check_launch("batch");
 }
 

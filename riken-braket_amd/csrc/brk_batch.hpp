@@ -31,7 +31,7 @@ constexpr int BATCH_CHUNK_PADDED = BATCH_CHUNK;  // LDS slots of a chunk (swizzl
 //                patterns; a0' = m00 a0 + m01 a1, a1' = m10 a0 + m11 a1 on the pairs of that position).
 //                The gates of a group on one position are multiplied into that matrix (in FP64 on the
 //                host): gates on different positions commute.  Pre-signs are applied to the group's
-//                amplitudes before its gates, post-signs after them.
+//                amplitudes before its gates, post-signs after them; attached terms have d < 32.
 //   signs      : w0 = BI_SIGN | nterms << 16, then nterms sign terms (a pass of its own).
 // A sign term (d, A) negates an amplitude with index x when its bit count is odd:
 //   d = 0      : x & A                  (Z on each qubit of A)

@@ -223,7 +223,10 @@ uint32_t emit_stage(const planned_stage& st, std::vector<uint32_t>& words, uint3
       segs[i].attached = true;
       continue;
     }
-    if (terms.size() > (size_t)brk::BATCH_MAX_ATTACHED_TERMS) continue;
+    // groups take Z / CZ terms only; CnZ (three or more qubits) keeps a pass of its own
+    bool pairs_only = terms.size() <= (size_t)brk::BATCH_MAX_ATTACHED_TERMS;
+    for (const auto& t : terms) pairs_only &= t.first < brk::BATCH_TERM_ALL;
+    if (!pairs_only) continue;
     if (i + 1 < segs.size() && segs[i + 1].gates && segs[i + 1].pre.empty()) {
       segs[i + 1].pre = terms;
       segs[i].attached = true;
