@@ -264,6 +264,10 @@ int brk_circuit_add(brk_circuit* circuit, int op, const unsigned* targets, unsig
                     const unsigned* controls, unsigned num_controls, const double* params, unsigned num_params);
 int brk_circuit_num_ops(const brk_circuit* circuit, unsigned* num_ops);
 int brk_circuit_num_qubits(const brk_circuit* circuit, unsigned* num_qubits);
+/* Reads operation i back: its code, targets, controls and parameters (arrays of
+   BRK_MAX_OPERATED_QUBITS unsigned / 8 doubles; any pointer may be NULL). */
+int brk_circuit_op(const brk_circuit* circuit, unsigned i, int* op, unsigned* targets, unsigned* num_targets,
+                   unsigned* controls, unsigned* num_controls, double* params, unsigned* num_params);
 /* Runs the circuit on a state of 2^num_qubits amplitudes.  `permutation`
    (num_qubits entries, logical -> physical, in/out) maps each logical qubit
    to its bit in the state, as ket::mpi::qubit_permutation does
@@ -376,6 +380,45 @@ int brk_batch_run(brk_batch* batch, double* energies, void* stream);
    and when its energy was written, both relative to the first take. */
 int brk_batch_run_timed(brk_batch* batch, double* energies, double* elapsed_ms, double* circuit_start_ms,
                         double* circuit_end_ms, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* bra front end (nompi mode): quantum-assembler (qcx) text -> statements.  */
+/* Replaces the parsing half of bra::interpreter::invoke                    */
+/* (bra/src/interpreter.cpp:371-872, mnemonics as in docs/bra.md:109-188);  */
+/* the gates between two directives form one brk_circuit.  Subset:          */
+/*   QUBITS, CIRCUITS 1, INITIAL STATE, BIT ASSIGNMENT (accepted: nompi     */
+/*   mode does not permute), comments after '!', case-insensitive mnemonics;*/
+/*   gates I IC In, H, NOT, X, Y, Z, Xn/XX.., Yn/YY.., Zn/ZZ.., SWAP, S S+,  */
+/*   SZ SZ+, T T+, U1 U1+, U2 U2+, U3 U3+, R R+, U U+, +X -X +Y -Y, SX SX+,  */
+/*   SY SY+, EX EY EZ (+ adjoints, EXX../EXn forms), TOFFOLI, and every one */
+/*   of them behind a C..C / Cn control prefix (controls come first);       */
+/*   DO MEASUREMENT [p], BEGIN MEASUREMENT, DO AMPLITUDES [i..],            */
+/*   BEGIN FUSION / END FUSION (gates in between run in order).             */
+/* Anything else fails with BRK_ERR_UNSUPPORTED and a message naming the    */
+/* line.                                                                    */
+/* ------------------------------------------------------------------------ */
+
+typedef struct brk_qcx brk_qcx;
+
+typedef enum brk_qcx_kind {
+  BRK_QCX_GATES = 0,        /* a brk_circuit of consecutive gates               */
+  BRK_QCX_MEASUREMENT = 1,  /* spin expectation values of every qubit (precision) */
+  BRK_QCX_AMPLITUDES = 2,   /* amplitudes (all, or the listed indices)          */
+  BRK_QCX_BEGIN_FUSION = 3,
+  BRK_QCX_END_FUSION = 4
+} brk_qcx_kind;
+
+int brk_qcx_parse(const char* text, brk_qcx** out);
+int brk_qcx_destroy(brk_qcx* program);
+int brk_qcx_num_qubits(const brk_qcx* program, unsigned* num_qubits);
+/* the INITIAL STATE value (0 if absent): the run starts from that basis state */
+int brk_qcx_initial_state(const brk_qcx* program, uint64_t* basis_index);
+int brk_qcx_num_statements(const brk_qcx* program, unsigned* num_statements);
+/* Statement i: kind; for BRK_QCX_GATES its circuit (owned by the program);
+   for BRK_QCX_MEASUREMENT the precision; for BRK_QCX_AMPLITUDES the indices
+   (num_indices 0 = all).  Output pointers may be NULL. */
+int brk_qcx_statement(const brk_qcx* program, unsigned i, int* kind, const brk_circuit** gates, int* precision,
+                      unsigned* num_indices, const uint64_t** indices);
 
 #ifdef __cplusplus
 } /* extern "C" */

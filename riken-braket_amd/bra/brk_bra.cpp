@@ -1,0 +1,151 @@
+// brk-bra: bra's nompi front end on the MI355X engine.
+//
+//   brk-bra [--file <path>] [--threads <n>] [--seed <n>]
+//
+// Reads a quantum-assembler program (from --file or standard input, as
+// bra/src/bra.cpp does), runs it on one GPU in complex<double> and prints what
+// bra prints for the supported statements:
+//   DO MEASUREMENT [p] / BEGIN MEASUREMENT  bra::state::measurement (bra/src/state.cpp:1984-2038)
+//   DO AMPLITUDES [i..]                     bra::nompi_state::do_amplitudes (bra/src/nompi_state.cpp:1108-1133)
+// each framed by the "... finished: <seconds since start> (<seconds since last>)"
+// lines.  --threads and --seed are accepted for command-line compatibility;
+// the program uses no random numbers.
+#include <chrono>
+#include <cstdio>
+#include <fstream>
+#include <iostream>
+#include <iterator>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "bra_hip_state.hpp"
+
+namespace {
+
+using clock_type = std::chrono::steady_clock;
+
+std::string bits_string(std::uint64_t value, unsigned n) {  // bra/include/bra/state.hpp:101-117
+  std::string s;
+  for (unsigned b = n; b > 0; --b) s.push_back(((value >> (b - 1)) & 1u) ? '1' : '0');
+  return s;
+}
+
+// fmt's "{:^W}" centring: the left pad is half the padding, rounded down
+std::string centre(const std::string& s, int width) {
+  const int pad = width - (int)s.size();
+  if (pad <= 0) return s;
+  return std::string(pad / 2, ' ') + s + std::string(pad - pad / 2, ' ');
+}
+
+std::string fixed_with_space_sign(double v, int precision) {  // fmt "{: .Pf}"
+  char buf[64];
+  std::snprintf(buf, sizeof buf, "% .*f", precision, v);
+  return buf;
+}
+
+struct timer {
+  clock_type::time_point start = clock_type::now(), last = start;
+  std::string lap(const char* what) {
+    const auto now = clock_type::now();
+    std::ostringstream oss;
+    oss << what << " finished: " << std::chrono::duration<double>(now - start).count() << " ("
+        << std::chrono::duration<double>(now - last).count() << ")\n";
+    last = now;
+    return oss.str();
+  }
+};
+
+void measurement(brk::bra::hip_state& st, int precision, timer& t) {
+  std::cout << t.lap("Operations") << std::flush;
+  const auto spins = st.do_expectation_values();
+  std::ostringstream oss;
+  if (precision > 0) {
+    const int width = std::max(precision + 3, 8);
+    oss << "Expectation values of spins:\n"
+        << centre("<Qx>", width) << ' ' << centre("<Qy>", width) << ' ' << centre("<Qz>", width) << '\n';
+    for (const auto& s : spins)
+      oss << centre(fixed_with_space_sign(0.5 - s[0], precision), width) << ' '
+          << centre(fixed_with_space_sign(0.5 - s[1], precision), width) << ' '
+          << centre(fixed_with_space_sign(0.5 - s[2], precision), width) << '\n';
+  } else {
+    oss << "Expectation values of spins:\n<Qx> <Qy> <Qz>\n";
+    if (precision < 0) oss.precision(-precision);
+    for (const auto& s : spins) oss << (0.5 - s[0]) << ' ' << (0.5 - s[1]) << ' ' << (0.5 - s[2]) << '\n';
+  }
+  std::cout << oss.str() << t.lap("Expectation values") << std::flush;
+}
+
+void amplitudes(brk::bra::hip_state& st, const std::vector<std::uint64_t>& indices, timer& t) {
+  std::cout << t.lap("Operations") << std::flush;
+  const auto a = st.amplitudes(indices);
+  std::ostringstream oss;
+  for (size_t i = 0; i < a.size(); ++i) {
+    const std::uint64_t index = indices.empty() ? i : indices[i];
+    oss << bits_string(index, st.num_qubits()) << " => " << a[i].real() << " + " << a[i].imag() << " i\n";
+  }
+  std::cout << oss.str() << t.lap("Amplitudes") << std::flush;
+}
+
+int usage() {
+  std::cerr << "usage: brk-bra [--file <path>] [--threads <n>] [--seed <n>]\n";
+  return 2;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  std::string file;
+  for (int i = 1; i < argc; ++i) {
+    const std::string a = argv[i];
+    if ((a == "--file" || a == "-f") && i + 1 < argc) file = argv[++i];
+    else if ((a == "--threads" || a == "--seed") && i + 1 < argc) ++i;
+    else return usage();
+  }
+  std::string text;
+  if (file.empty()) {
+    text.assign(std::istreambuf_iterator<char>(std::cin), std::istreambuf_iterator<char>());
+  } else {
+    std::ifstream in(file);
+    if (!in) {
+      std::cerr << "ERROR: cannot open an input file " << file << '\n';
+      return 1;
+    }
+    text.assign(std::istreambuf_iterator<char>(in), std::istreambuf_iterator<char>());
+  }
+
+  brk_qcx* prog = nullptr;
+  if (brk_qcx_parse(text.c_str(), &prog) != BRK_OK) {
+    std::cerr << "Error: " << brk_last_error() << '\n';
+    return 1;
+  }
+  int rc = 0;
+  try {
+    unsigned n = 0, count = 0;
+    std::uint64_t initial = 0;
+    brk::bra::check(brk_qcx_num_qubits(prog, &n));
+    brk::bra::check(brk_qcx_initial_state(prog, &initial));
+    brk::bra::check(brk_qcx_num_statements(prog, &count));
+    timer t;
+    brk::bra::hip_state st(n, initial);
+    for (unsigned i = 0; i < count; ++i) {
+      int kind = 0, precision = 3;
+      const brk_circuit* gates = nullptr;
+      unsigned nidx = 0;
+      const std::uint64_t* idx = nullptr;
+      brk::bra::check(brk_qcx_statement(prog, i, &kind, &gates, &precision, &nidx, &idx));
+      switch (kind) {
+        case BRK_QCX_GATES: st.apply(gates); break;
+        case BRK_QCX_MEASUREMENT: measurement(st, precision, t); break;
+        case BRK_QCX_AMPLITUDES: amplitudes(st, std::vector<std::uint64_t>(idx, idx + nidx), t); break;
+        default: break;  // fusion brackets: the gates inside run in order either way
+      }
+    }
+    brk::bra::check(brk_stream_synchronize(nullptr));
+  } catch (const std::exception& e) {
+    std::cerr << "Error: " << e.what() << '\n';
+    rc = 1;
+  }
+  brk_qcx_destroy(prog);
+  return rc;
+}

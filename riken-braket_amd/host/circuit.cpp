@@ -9,6 +9,7 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstdarg>
@@ -237,6 +238,21 @@ int brk_circuit_num_ops(const brk_circuit* circuit, unsigned* num_ops) {
 int brk_circuit_num_qubits(const brk_circuit* circuit, unsigned* num_qubits) {
   if (!circuit || !num_qubits) return fail(BRK_ERR_INVALID_ARGUMENT, "null argument");
   *num_qubits = circuit->num_qubits;
+  return BRK_OK;
+}
+
+int brk_circuit_op(const brk_circuit* circuit, unsigned i, int* op, unsigned* targets, unsigned* num_targets,
+                   unsigned* controls, unsigned* num_controls, double* params, unsigned* num_params) {
+  if (!circuit) return fail(BRK_ERR_INVALID_ARGUMENT, "circuit is null");
+  if (i >= circuit->ops.size()) return fail(BRK_ERR_INVALID_ARGUMENT, "op %u out of range", i);
+  const auto& o = circuit->ops[i];
+  if (op) *op = o.code;
+  if (num_targets) *num_targets = (unsigned)o.targets.size();
+  if (num_controls) *num_controls = (unsigned)o.controls.size();
+  if (num_params) *num_params = o.num_params;
+  if (targets) std::copy(o.targets.begin(), o.targets.end(), targets);
+  if (controls) std::copy(o.controls.begin(), o.controls.end(), controls);
+  if (params) std::copy(o.params.begin(), o.params.begin() + o.num_params, params);
   return BRK_OK;
 }
 

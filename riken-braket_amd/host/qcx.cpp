@@ -1,0 +1,465 @@
+// bra front end (nompi mode): quantum-assembler text -> statements.
+//
+// Follows bra::interpreter::invoke (bra/src/interpreter.cpp:371-872):
+// everything after '!' is a comment, lines are trimmed and split on white
+// space, the first column is upper-cased, and controlled forms are a run of
+// 'C's or 'C' plus a count in front of a base mnemonic
+// (interpreter::interpret_controlled_gates, interpreter.cpp:3341-3369); the
+// control qubits come first among the operands.  Qubits of Z, S, T, U1, R and
+// U are "control" qubits in bra: the phase applies where all of them are 1.
+// A Pauli string over several targets (XX, Y3, ZZ, EXX, ...) acts on all of
+// them at once.  Gates between two directives are collected into one
+// brk_circuit, in program order.
+#include "circuit.hpp"
+
+#include <cctype>
+#include <cerrno>
+#include <cmath>
+#include <cstdlib>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <vector>
+
+namespace brk {
+int fail(int status, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+}
+
+struct brk_qcx {
+  struct statement {
+    int kind = BRK_QCX_GATES;
+    brk_circuit* gates = nullptr;
+    int precision = 3;
+    std::vector<uint64_t> indices;
+  };
+  unsigned num_qubits = 0;
+  uint64_t initial_state = 0;
+  std::vector<statement> statements;
+  ~brk_qcx() {
+    for (auto& s : statements)
+      if (s.gates) brk_circuit_destroy(s.gates);
+  }
+};
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846264338327950288;
+
+struct parse_error {
+  int status;
+  std::string message;
+};
+
+std::string upper(std::string s) {
+  for (char& c : s) c = (char)std::toupper((unsigned char)c);
+  return s;
+}
+
+bool all_of(const std::string& s, size_t from, char c) {
+  if (from >= s.size()) return false;
+  for (size_t i = from; i < s.size(); ++i)
+    if (s[i] != c) return false;
+  return true;
+}
+
+bool digits_from(const std::string& s, size_t from, size_t to) {
+  if (from >= to) return false;
+  for (size_t i = from; i < to; ++i)
+    if (!std::isdigit((unsigned char)s[i])) return false;
+  return true;
+}
+
+class parser {
+ public:
+  explicit parser(brk_qcx& p) : prog_(p) {}
+
+  void line(size_t number, const std::string& raw) {
+    line_ = number;
+    std::string text = raw.substr(0, raw.find('!'));
+    std::istringstream in(text);
+    cols_.clear();
+    for (std::string w; in >> w;) cols_.push_back(w);
+    if (cols_.empty()) return;
+    const std::string m = upper(cols_[0]);
+    if (m == "QUBITS") return qubits();
+    if (m == "CIRCUITS") {
+      if (cols_.size() != 2 || integer(1) != 1) unsupported("more than one circuit (CIRCUITS n > 1)");
+      return;
+    }
+    if (m == "INITIAL") {
+      if (cols_.size() != 3 || upper(cols_[1]) != "STATE") error("expected INITIAL STATE <value>");
+      prog_.initial_state = (uint64_t)integer(2);
+      return;
+    }
+    if (m == "BIT") return bit_assignment();
+    if (m == "DO") return do_statement();
+    if (m == "BEGIN") return begin_statement();
+    if (m == "END") return end_statement();
+    need_qubits();
+    gate(m);
+  }
+
+  void finish() { flush(); }
+
+ private:
+  brk_qcx& prog_;
+  size_t line_ = 0;
+  std::vector<std::string> cols_;
+  brk_circuit* current_ = nullptr;
+
+  [[noreturn]] void error(const std::string& what) const {
+    throw parse_error{BRK_ERR_INVALID_ARGUMENT, "line " + std::to_string(line_) + ": " + what};
+  }
+  [[noreturn]] void unsupported(const std::string& what) const {
+    throw parse_error{BRK_ERR_UNSUPPORTED, "line " + std::to_string(line_) + ": " + what + " is not supported"};
+  }
+  void need_qubits() const {
+    if (!prog_.num_qubits) error("QUBITS must come before the first gate");
+  }
+
+  long long integer(size_t i) const {
+    if (i >= cols_.size()) error("missing operand");
+    errno = 0;
+    char* end = nullptr;
+    const long long v = std::strtoll(cols_[i].c_str(), &end, 10);
+    if (errno || !end || *end) error("'" + cols_[i] + "' is not an integer");
+    return v;
+  }
+  double real(size_t i) const {
+    if (i >= cols_.size()) error("missing operand");
+    errno = 0;
+    char* end = nullptr;
+    const double v = std::strtod(cols_[i].c_str(), &end);
+    if (errno == ERANGE || !end || *end) error("'" + cols_[i] + "' is not a number (classical variables are not supported)");
+    return v;
+  }
+  unsigned qubit(size_t i) const {
+    const long long q = integer(i);
+    if (q < 0 || (unsigned long long)q >= prog_.num_qubits)
+      error("qubit " + cols_[i] + " out of range for " + std::to_string(prog_.num_qubits) + " qubits");
+    return (unsigned)q;
+  }
+
+  void qubits() {
+    if (cols_.size() != 2) error("expected QUBITS <n>");
+    const long long n = integer(1);
+    if (n < 1 || n > 40) error("QUBITS must be 1..40");
+    if (prog_.num_qubits) error("QUBITS given twice");
+    prog_.num_qubits = (unsigned)n;
+  }
+
+  void bit_assignment() {
+    // nompi mode ignores the initial permutation (only the MPI states use it);
+    // it is still checked to be a permutation of the qubits
+    if (cols_.size() < 2 || upper(cols_[1]) != "ASSIGNMENT") unsupported("BIT " + (cols_.size() > 1 ? cols_[1] : ""));
+    need_qubits();
+    if (cols_.size() != 2 + (size_t)prog_.num_qubits) error("BIT ASSIGNMENT needs one entry per qubit");
+    std::vector<int> seen(prog_.num_qubits, 0);
+    for (size_t i = 2; i < cols_.size(); ++i) {
+      const unsigned q = qubit(i);
+      if (seen[q]++) error("BIT ASSIGNMENT repeats qubit " + cols_[i]);
+    }
+  }
+
+  void flush() {
+    if (current_) {
+      brk_qcx::statement s;
+      s.kind = BRK_QCX_GATES;
+      s.gates = current_;
+      prog_.statements.push_back(std::move(s));
+      current_ = nullptr;
+    }
+  }
+  void directive(int kind, int precision = 3, std::vector<uint64_t> indices = {}) {
+    flush();
+    brk_qcx::statement s;
+    s.kind = kind;
+    s.precision = precision;
+    s.indices = std::move(indices);
+    prog_.statements.push_back(std::move(s));
+  }
+
+  void do_statement() {
+    if (cols_.size() < 2) error("DO needs a statement");
+    const std::string what = upper(cols_[1]);
+    need_qubits();
+    if (what == "MEASUREMENT") {
+      if (cols_.size() > 3) error("DO MEASUREMENT takes at most one precision");
+      directive(BRK_QCX_MEASUREMENT, cols_.size() == 3 ? (int)integer(2) : 3);
+    } else if (what == "AMPLITUDES") {
+      std::vector<uint64_t> idx;
+      for (size_t i = 2; i < cols_.size(); ++i) {
+        const long long v = integer(i);
+        if (v < 0 || (unsigned long long)v >= (1ull << prog_.num_qubits)) error("amplitude index out of range");
+        idx.push_back((uint64_t)v);
+      }
+      directive(BRK_QCX_AMPLITUDES, 3, std::move(idx));
+    } else {
+      unsupported("DO " + what);
+    }
+  }
+  void begin_statement() {
+    if (cols_.size() < 2) error("BEGIN needs a statement");
+    const std::string what = upper(cols_[1]);
+    need_qubits();
+    if (what == "MEASUREMENT") directive(BRK_QCX_MEASUREMENT, 3);
+    else if (what == "FUSION") directive(BRK_QCX_BEGIN_FUSION);
+    else unsupported("BEGIN " + what);
+  }
+  void end_statement() {
+    if (cols_.size() < 2) error("END needs a statement");
+    const std::string what = upper(cols_[1]);
+    if (what == "FUSION") directive(BRK_QCX_END_FUSION);
+    else unsupported("END " + what);
+  }
+
+  void add(int op, const std::vector<unsigned>& targets, const std::vector<unsigned>& controls,
+           const std::vector<double>& params = {}) {
+    if (!current_ && brk_circuit_create(prog_.num_qubits, &current_) != BRK_OK) error("cannot create a circuit");
+    const int s = brk_circuit_add(current_, op, targets.data(), (unsigned)targets.size(),
+                                  controls.empty() ? nullptr : controls.data(), (unsigned)controls.size(),
+                                  params.empty() ? nullptr : params.data(), (unsigned)params.size());
+    if (s != BRK_OK) throw parse_error{s, "line " + std::to_string(line_) + ": " + brk_last_error()};
+  }
+  void add_matrix(unsigned t, const std::vector<unsigned>& controls, const double (&m)[8]) {
+    add(BRK_OP_MATRIX, {t}, controls, std::vector<double>(m, m + 8));
+  }
+
+  void expect_columns(size_t n) const {
+    if (cols_.size() != n)
+      error("'" + cols_[0] + "' takes " + std::to_string(n - 1) + " operands, got " + std::to_string(cols_.size() - 1));
+  }
+
+  // Reads `nc` controls then `nt` targets starting at column 1.
+  void operands(size_t nc, size_t nt, std::vector<unsigned>& controls, std::vector<unsigned>& targets) const {
+    controls.clear();
+    targets.clear();
+    for (size_t i = 0; i < nc; ++i) controls.push_back(qubit(1 + i));
+    for (size_t i = 0; i < nt; ++i) targets.push_back(qubit(1 + nc + i));
+  }
+
+  // Multiplicity of a Pauli-string mnemonic: "XXX" -> 3, "X4" -> 4, "X" -> 1; 0 if not of that form.
+  static size_t multiplicity(const std::string& base, size_t from, char letter) {
+    if (from >= base.size() || base[from] != letter) return 0;
+    if (all_of(base, from, letter)) return base.size() - from;
+    if (digits_from(base, from + 1, base.size())) return (size_t)std::atoi(base.c_str() + from + 1);
+    return 0;
+  }
+
+  void gate(const std::string& m) {
+    size_t nc = 0;
+    std::string base = m;
+    if (m.size() >= 2 && m[0] == 'C' && m != "CIRCUITS") {
+      size_t cs = 0;
+      while (cs < m.size() && m[cs] == 'C') ++cs;
+      if (cs > 1) {
+        nc = cs;
+        base = m.substr(cs);
+      } else {
+        size_t d = 1;
+        while (d < m.size() && std::isdigit((unsigned char)m[d])) ++d;
+        nc = d > 1 ? (size_t)std::atoi(m.c_str() + 1) : 1;
+        base = m.substr(d);
+      }
+      if (nc == 0 || base.empty()) unsupported("mnemonic " + m);
+    }
+    apply_base(m, base, nc);
+  }
+
+  void apply_base(const std::string& m, const std::string& base, size_t nc) {
+    std::vector<unsigned> c, t;
+    const double r2 = 0.707106781186547524400844362104849039;
+    // identity forms: checked, no operation
+    if (base == "I" || base == "IC") {
+      expect_columns(2 + nc);
+      operands(nc, 1, c, t);
+      return;
+    }
+    if (size_t k = multiplicity(base, 0, 'I'); k >= 2) {
+      expect_columns(1 + nc + k);
+      operands(nc, k, c, t);
+      return;
+    }
+    if (base == "H") {
+      expect_columns(2 + nc);
+      operands(nc, 1, c, t);
+      return add(BRK_OP_H, t, c);
+    }
+    if (base == "NOT" || base == "TOFFOLI") {
+      const size_t extra = base == "TOFFOLI" ? 2 : 0;
+      if (base == "TOFFOLI" && nc) unsupported("controlled TOFFOLI");
+      expect_columns(2 + nc + extra);
+      operands(nc + extra, 1, c, t);
+      return add(BRK_OP_X, t, c);
+    }
+    for (const char letter : {'X', 'Y'}) {
+      if (size_t k = multiplicity(base, 0, letter); k >= 1) {
+        expect_columns(1 + nc + k);
+        operands(nc, k, c, t);
+        return add(letter == 'X' ? BRK_OP_X : BRK_OP_Y, t, c);
+      }
+    }
+    if (base == "Z") {
+      // Z on a control qubit; with n >= 1 control prefixes bra reads n + 1 control
+      // qubits (CZ c1 c2, CCZ c1 c2 c3: interpreter::add_cz, interpreter.cpp:3762-3779)
+      const size_t q = nc == 0 ? 1 : nc + 1;
+      expect_columns(1 + q);
+      operands(0, q, c, t);
+      return add(BRK_OP_Z, t, {});
+    }
+    if (size_t k = multiplicity(base, 0, 'Z'); k >= 2) {
+      expect_columns(1 + nc + k);
+      operands(nc, k, c, t);
+      for (unsigned q : t) add(BRK_OP_Z, {q}, c);  // Z..Z = product of single-qubit Z's
+      return;
+    }
+    if (base == "SWAP") {
+      expect_columns(3 + nc);
+      operands(nc, 2, c, t);
+      return add(BRK_OP_SWAP, t, c);
+    }
+    // phases on control qubits: the controls of the prefix and the operand are all "control" qubits
+    struct phase_gate {
+      const char* name;
+      double phase;
+    };
+    static const phase_gate phases[] = {{"S", kPi / 2}, {"SZ", kPi / 2}, {"S+", -kPi / 2}, {"SZ+", -kPi / 2},
+                                        {"T", kPi / 4}, {"T+", -kPi / 4}};
+    for (const auto& g : phases)
+      if (base == g.name) {
+        expect_columns(2 + nc);
+        operands(nc, 1, c, t);
+        return add(BRK_OP_U1, t, c, {g.phase});
+      }
+    if (base == "U1" || base == "U1+") {
+      expect_columns(3 + nc);
+      operands(nc, 1, c, t);
+      const double lambda = real(2 + nc);
+      return add(BRK_OP_U1, t, c, {base == "U1" ? lambda : -lambda});
+    }
+    if (base == "R" || base == "R+" || base == "U" || base == "U+") {
+      // phase 2 pi / 2^k (k < 0: the adjoint), bra/src/state.cpp:2650-2669
+      const size_t q = (base[0] == 'U' ? 2 : 1) + nc;
+      expect_columns(2 + q);
+      operands(0, q, c, t);
+      const long long k = integer(1 + q);
+      if (k < -62 || k > 62) error("phase exponent out of range");
+      const unsigned long long kk = (unsigned long long)(k < 0 ? -k : k);
+      double phase = 2.0 * kPi / (double)(1ull << kk);
+      if ((k < 0) != (base.back() == '+')) phase = -phase;
+      return add(BRK_OP_U1, {t[0]}, std::vector<unsigned>(t.begin() + 1, t.end()), {phase});
+    }
+    if (base == "U2" || base == "U2+") {
+      expect_columns(4 + nc);
+      operands(nc, 1, c, t);
+      const double phi = real(2 + nc), lambda = real(3 + nc);
+      if (base == "U2") return add(BRK_OP_U2, t, c, {phi, lambda});
+      return add(BRK_OP_U3, t, c, {-kPi / 2, -lambda, -phi});  // U2(phi, lambda)^dagger
+    }
+    if (base == "U3" || base == "U3+") {
+      expect_columns(5 + nc);
+      operands(nc, 1, c, t);
+      const double theta = real(2 + nc), phi = real(3 + nc), lambda = real(4 + nc);
+      if (base == "U3") return add(BRK_OP_U3, t, c, {theta, phi, lambda});
+      return add(BRK_OP_U3, t, c, {-theta, -lambda, -phi});
+    }
+    // fixed one-qubit matrices (docs/bra.md:109-188), row-major (re, im)
+    struct fixed_gate {
+      const char* name;
+      double m[8];
+    };
+    static const fixed_gate fixed[] = {
+        {"+X", {r2, 0, 0, r2, 0, r2, r2, 0}},      {"-X", {r2, 0, 0, -r2, 0, -r2, r2, 0}},
+        {"+Y", {r2, 0, r2, 0, -r2, 0, r2, 0}},     {"-Y", {r2, 0, -r2, 0, r2, 0, r2, 0}},
+        {"SX", {0.5, 0.5, 0.5, -0.5, 0.5, -0.5, 0.5, 0.5}},
+        {"SX+", {0.5, -0.5, 0.5, 0.5, 0.5, 0.5, 0.5, -0.5}},
+        {"SY", {0.5, 0.5, -0.5, -0.5, 0.5, 0.5, 0.5, 0.5}},
+        {"SY+", {0.5, -0.5, 0.5, -0.5, -0.5, 0.5, 0.5, -0.5}}};
+    for (const auto& g : fixed)
+      if (base == g.name) {
+        expect_columns(2 + nc);
+        operands(nc, 1, c, t);
+        return add_matrix(t[0], c, g.m);
+      }
+    // exponentials of Pauli strings: EX t theta, EXX t1 t2 theta, EX3 ..., and their adjoints
+    if (base.size() >= 2 && base[0] == 'E' && base != "ESWAP" && base != "ESWAP+") {
+      const bool adjoint = base.back() == '+';
+      const std::string core = adjoint ? base.substr(0, base.size() - 1) : base;
+      for (const char letter : {'X', 'Y', 'Z'}) {
+        const size_t k = multiplicity(core, 1, letter);
+        if (k >= 1) {
+          expect_columns(2 + nc + k);
+          operands(nc, k, c, t);
+          const double theta = real(1 + nc + k);
+          const int op = letter == 'X' ? BRK_OP_EX : letter == 'Y' ? BRK_OP_EY : BRK_OP_EZ;
+          return add(op, t, c, {adjoint ? -theta : theta});
+        }
+      }
+    }
+    unsupported("mnemonic " + m);
+  }
+};
+
+}  // namespace
+
+using brk::fail;
+
+extern "C" {
+
+int brk_qcx_parse(const char* text, brk_qcx** out) {
+  if (!text || !out) return fail(BRK_ERR_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  auto prog = std::make_unique<brk_qcx>();
+  parser p(*prog);
+  try {
+    std::istringstream in(text);
+    std::string raw;
+    size_t number = 0;
+    while (std::getline(in, raw)) p.line(++number, raw);
+    p.finish();
+  } catch (const parse_error& e) {
+    return fail(e.status, "%s", e.message.c_str());
+  }
+  if (!prog->num_qubits) return fail(BRK_ERR_INVALID_ARGUMENT, "the program has no QUBITS statement");
+  if (prog->initial_state >> prog->num_qubits) return fail(BRK_ERR_INVALID_ARGUMENT, "INITIAL STATE out of range");
+  *out = prog.release();
+  return BRK_OK;
+}
+
+int brk_qcx_destroy(brk_qcx* program) {
+  delete program;
+  return BRK_OK;
+}
+
+int brk_qcx_num_qubits(const brk_qcx* program, unsigned* num_qubits) {
+  if (!program || !num_qubits) return fail(BRK_ERR_INVALID_ARGUMENT, "null argument");
+  *num_qubits = program->num_qubits;
+  return BRK_OK;
+}
+
+int brk_qcx_initial_state(const brk_qcx* program, uint64_t* basis_index) {
+  if (!program || !basis_index) return fail(BRK_ERR_INVALID_ARGUMENT, "null argument");
+  *basis_index = program->initial_state;
+  return BRK_OK;
+}
+
+int brk_qcx_num_statements(const brk_qcx* program, unsigned* num_statements) {
+  if (!program || !num_statements) return fail(BRK_ERR_INVALID_ARGUMENT, "null argument");
+  *num_statements = (unsigned)program->statements.size();
+  return BRK_OK;
+}
+
+int brk_qcx_statement(const brk_qcx* program, unsigned i, int* kind, const brk_circuit** gates, int* precision,
+                      unsigned* num_indices, const uint64_t** indices) {
+  if (!program) return fail(BRK_ERR_INVALID_ARGUMENT, "program is null");
+  if (i >= program->statements.size()) return fail(BRK_ERR_INVALID_ARGUMENT, "statement %u out of range", i);
+  const auto& s = program->statements[i];
+  if (kind) *kind = s.kind;
+  if (gates) *gates = s.gates;
+  if (precision) *precision = s.precision;
+  if (num_indices) *num_indices = (unsigned)s.indices.size();
+  if (indices) *indices = s.indices.empty() ? nullptr : s.indices.data();
+  return BRK_OK;
+}
+
+}  // extern "C"

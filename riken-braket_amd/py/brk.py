@@ -87,6 +87,14 @@ _SIGS = {
     "brk_fused_apply_timed": [_p, _p, _i, C.POINTER(C.c_float), _p],
     "brk_circuit_num_parameters": [_p, _up],
     "brk_adjoint_ising_gradient": [_p, _p, _p, _i, _dp, _dp, _dp, _dp, _u, _p],
+    "brk_circuit_op": [_p, _u, C.POINTER(C.c_int), _up, _up, _up, _up, _dp, _up],
+    "brk_qcx_parse": [C.c_char_p, C.POINTER(C.c_void_p)],
+    "brk_qcx_destroy": [_p],
+    "brk_qcx_num_qubits": [_p, _up],
+    "brk_qcx_initial_state": [_p, C.POINTER(_u64)],
+    "brk_qcx_num_statements": [_p, _up],
+    "brk_qcx_statement": [_p, _u, C.POINTER(C.c_int), C.POINTER(C.c_void_p), C.POINTER(C.c_int), _up,
+                          C.POINTER(C.POINTER(_u64))],
     "brk_batch_create": [C.POINTER(C.c_void_p), _u, C.POINTER(_dp), C.POINTER(_dp), C.POINTER(C.c_void_p)],
     "brk_batch_destroy": [_p],
     "brk_batch_num_stages": [_p, _u, _up],
@@ -485,6 +493,66 @@ class Batch:
         try:
             if self.h:
                 load().brk_batch_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+OP_NAMES = ("H", "X", "Y", "Z", "U1", "U2", "U3", "EX", "EY", "EZ", "RX", "RY", "RZ", "SWAP", "LOCAL_SWAP", "MATRIX")
+
+
+def circuit_ops(handle):
+    """The ops of a brk_circuit handle as (name, targets, controls, params) tuples."""
+    n = C.c_uint()
+    call("brk_circuit_num_ops", handle, C.byref(n))
+    out = []
+    for i in range(n.value):
+        code, nt, nc, npar = C.c_int(), C.c_uint(), C.c_uint(), C.c_uint()
+        t, c, p = (C.c_uint * 16)(), (C.c_uint * 16)(), (C.c_double * 8)()
+        call("brk_circuit_op", handle, i, C.byref(code), t, C.byref(nt), c, C.byref(nc), p, C.byref(npar))
+        out.append((OP_NAMES[code.value], tuple(t[: nt.value]), tuple(c[: nc.value]), tuple(p[: npar.value])))
+    return out
+
+
+QCX_GATES, QCX_MEASUREMENT, QCX_AMPLITUDES, QCX_BEGIN_FUSION, QCX_END_FUSION = 0, 1, 2, 3, 4
+
+
+class Qcx:
+    """A parsed quantum-assembler program (bra nompi subset)."""
+
+    def __init__(self, text):
+        h = C.c_void_p()
+        call("brk_qcx_parse", text.encode(), C.byref(h))
+        self.h = h.value
+        n, init, cnt = C.c_uint(), C.c_uint64(), C.c_uint()
+        call("brk_qcx_num_qubits", self.h, C.byref(n))
+        call("brk_qcx_initial_state", self.h, C.byref(init))
+        call("brk_qcx_num_statements", self.h, C.byref(cnt))
+        self.num_qubits, self.initial_state = n.value, init.value
+        self.statements = []
+        for i in range(cnt.value):
+            kind, prec, nidx = C.c_int(), C.c_int(), C.c_uint()
+            gates = C.c_void_p()
+            idx = C.POINTER(_u64)()
+            call("brk_qcx_statement", self.h, i, C.byref(kind), C.byref(gates), C.byref(prec), C.byref(nidx),
+                 C.byref(idx))
+            if kind.value == QCX_GATES:
+                self.statements.append((kind.value, circuit_ops(gates.value)))
+            elif kind.value == QCX_MEASUREMENT:
+                self.statements.append((kind.value, prec.value))
+            elif kind.value == QCX_AMPLITUDES:
+                self.statements.append((kind.value, [idx[k] for k in range(nidx.value)]))
+            else:
+                self.statements.append((kind.value, None))
+
+    def ops(self):
+        """Every gate of the program in order."""
+        return [op for kind, v in self.statements if kind == QCX_GATES for op in v]
+
+    def __del__(self):
+        try:
+            if self.h:
+                load().brk_qcx_destroy(self.h)
                 self.h = None
         except Exception:
             pass

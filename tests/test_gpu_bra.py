@@ -1,0 +1,97 @@
+"""GPU checks of brk-bra, bra's nompi front end on the engine: the reference's
+sample programs and synthetic ones are run by the binary and its printed
+expectation values / amplitudes are compared with the CPU oracle running the
+same parsed gates (and with closed forms where they exist)."""
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import brk
+from oracle_circuits import run_ops
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "riken-braket_amd", "brk-bra")
+GOLDEN = os.path.join(ROOT, "tests", "golden", "qcx")
+TIMING = re.compile(r"^(Operations|Expectation values|Amplitudes) finished: [0-9.e+-]+ \([0-9.e+-]+\)$")
+
+
+def run_bra(text):
+    if not os.path.isfile(EXE):
+        pytest.fail(f"{EXE} is not built (run __graft_entry__.build())")
+    print(f"gpu bra: running brk-bra on {text.splitlines()[0]!r} ...", flush=True)
+    sys.stdout.flush()
+    r = subprocess.run([EXE], input=text, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    return r.stdout.splitlines()
+
+
+def oracle_state(oracle, text):
+    q = brk.Qcx(text)
+    s = np.zeros(1 << q.num_qubits, dtype=np.complex128)
+    s[q.initial_state] = 1
+    run_ops(oracle, s, q.ops())
+    return q, s
+
+
+def spins(oracle, s, n):
+    return [[0.5 - v for v in oracle.spin_expectation_value(s, k)] for k in range(n)]
+
+
+def table(lines, n):
+    i = lines.index("Expectation values of spins:")
+    return lines[i + 1], [[float(x) for x in row.split()] for row in lines[i + 2:i + 2 + n]]
+
+
+def test_hadamards08_closed_form():
+    lines = run_bra(open(os.path.join(GOLDEN, "hadamards08.qcx")).read())
+    assert TIMING.match(lines[0]) and TIMING.match(lines[-1])
+    header, rows = table(lines, 8)
+    # fmt "{:^8s}" columns for precision 3 (bra/src/state.cpp:1997-2006)
+    assert header == " ".join(f"{h:^8}" for h in ("<Qx>", "<Qy>", "<Qz>"))
+    # |+> on every qubit: <Qx> = 0, <Qy> = <Qz> = 1/2
+    assert all(abs(v - w) < 1e-12 for row in rows for v, w in zip(row, (0.0, 0.5, 0.5)))
+    row = lines[lines.index(header) + 1]
+    assert row.replace("-0.000", " 0.000") == " ".join(f"{v:^8}" for v in (" 0.000", " 0.500", " 0.500"))
+
+
+@pytest.mark.parametrize("name", ["toffoli12.qcx", "adder6x2.qcx"])
+def test_sample_programs_vs_oracle(oracle, name):
+    text = open(os.path.join(GOLDEN, name)).read()
+    lines = run_bra(text)
+    q, s = oracle_state(oracle, text)
+    _, rows = table(lines, q.num_qubits)
+    ref = spins(oracle, s, q.num_qubits)
+    assert np.max(np.abs(np.array(rows) - np.array(ref))) <= 0.5e-3 + 1e-12
+
+
+def test_full_precision_measurement_and_amplitudes(oracle):
+    text = ("QUBITS 5\nINITIAL STATE 3\nH 0\nCNOT 0 4\nU3 2 0.3 1.1 -0.4\nEXX 1 3 0.7\nCCZ 0 2 4\nSX 1\n"
+            "+Y 3\nU 2 1 3\nT 4\nSWAP 0 3\nDO MEASUREMENT -17\nDO AMPLITUDES\nDO AMPLITUDES 3 17\n")
+    lines = run_bra(text)
+    q, s = oracle_state(oracle, text)
+    i = lines.index("<Qx> <Qy> <Qz>")
+    rows = np.array([[float(x) for x in lines[i + 1 + k].split()] for k in range(5)])
+    assert np.max(np.abs(rows - np.array(spins(oracle, s, 5)))) < 1e-12
+    amp = [l for l in lines if "=>" in l]
+    assert len(amp) == 32 + 2
+    pat = re.compile(r"^([01]{5}) => (\S+) \+ (\S+) i$")
+    for k, line in enumerate(amp):
+        m = pat.match(line)
+        assert m, line
+        index = int(m.group(1), 2)
+        assert index == (k if k < 32 else (3, 17)[k - 32])
+        # default stream formatting: 6 significant digits
+        assert abs(complex(float(m.group(2)), float(m.group(3))) - s[index]) < 1e-5
+
+
+def test_initial_state_and_errors():
+    lines = run_bra("QUBITS 3\nINITIAL STATE 5\nDO AMPLITUDES 5 0\n")
+    assert [l for l in lines if "=>" in l] == ["101 => 1 + 0 i", "000 => 0 + 0 i"]
+    r = subprocess.run([EXE], input="QUBITS 2\nSHORBOX 1 2 3\n", capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "SHORBOX" in r.stderr
