@@ -349,7 +349,7 @@ def main():
     circuit = brk.Circuit(n, ops)
     op_bytes = [W.algorithmic_bytes(op, n) for op in ops]
     total_bytes = sum(op_bytes)
-    log(f"device {hip.device_name()}; {n} qubits c128 ({2 ** n * 16 / 2 ** 30:.0f} GiB), {len(ops)} ops, "
+    log(f"device {brk.device_name()}; {n} qubits c128 ({2 ** n * 16 / 2 ** 30:.0f} GiB), {len(ops)} ops, "
         f"{total_bytes / 1e12:.3f} TB algorithmic per step")
 
     stream = hip.create_stream()
@@ -450,6 +450,7 @@ def main():
             "mean_launch_ms": round(mean_ms, 4),
             "traffic": None,
         },
+        "device": brk.device_name(),
         "kernels": table,
         "norm_error": abs(norm - 1.0),
     }

@@ -56,6 +56,8 @@ const char* brk_last_error(void);
 /* 1 when the library was built for gfx950 and a device is visible. */
 int brk_device_count(int* count);
 const char* brk_version(void);
+/* "<marketing name> (<gfx arch>, <CUs> CUs)" of the current device. */
+int brk_device_name(char* buffer, size_t size);
 
 /* Launch tuning of the streaming gate kernels (no reference counterpart; the
    reference's analogue is its compile-time cache blocking,

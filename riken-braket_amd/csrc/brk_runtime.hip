@@ -3,6 +3,8 @@
 // dtype conversion and the streaming copy kernel.
 #include "brk_internal.hpp"
 
+#include <cstdio>
+
 #include <mutex>
 #include <cstdlib>
 #include <type_traits>
@@ -265,6 +267,17 @@ int brk_device_count(int* count) {
     return fail(BRK_ERR_HIP, "hipGetDeviceCount: %s", hipGetErrorString(e));
   }
   *count = c;
+  return BRK_OK;
+}
+
+int brk_device_name(char* buffer, size_t size) {
+  if (!buffer || size == 0) return fail(BRK_ERR_INVALID_ARGUMENT, "empty buffer");
+  int dev = 0;
+  BRK_HIP_CHECK(hipGetDevice(&dev));
+  hipDeviceProp_t props;
+  BRK_HIP_CHECK(hipGetDeviceProperties(&props, dev));
+  std::snprintf(buffer, size, "%s (%s, %d CUs)", props.name[0] ? props.name : "AMD GPU", props.gcnArchName,
+                props.multiProcessorCount);
   return BRK_OK;
 }
 

@@ -36,6 +36,7 @@ _SIGS = {
     "brk_last_error": [],
     "brk_version": [],
     "brk_device_count": [C.POINTER(C.c_int)],
+    "brk_device_name": [C.c_char_p, C.c_size_t],
     "brk_tune": [C.c_char_p, C.c_int64],
     "brk_get_tune": [C.c_char_p, C.POINTER(C.c_int64)],
     "brk_malloc": [C.POINTER(C.c_void_p), C.c_size_t],
@@ -556,3 +557,9 @@ class Qcx:
                 self.h = None
         except Exception:
             pass
+
+
+def device_name():
+    buf = C.create_string_buffer(256)
+    call("brk_device_name", buf, 256)
+    return buf.value.decode()

@@ -63,12 +63,6 @@ def synchronize(stream=0):
     _check("hipStreamSynchronize", lib().hipStreamSynchronize(stream))
 
 
-def device_name(dev=0):
-    buf = C.create_string_buffer(256)
-    _check("hipDeviceGetName", lib().hipDeviceGetName(buf, 256, dev))
-    return buf.value.decode()
-
-
 def mem_info():
     free, total = C.c_size_t(), C.c_size_t()
     _check("hipMemGetInfo", lib().hipMemGetInfo(C.byref(free), C.byref(total)))
