@@ -81,6 +81,17 @@ int brk_memcpy_htod(void* device_dst, const void* host_src, size_t bytes, void* 
 int brk_memcpy_dtoh(void* host_dst, const void* device_src, size_t bytes, void* stream);
 int brk_memcpy_dtod(void* device_dst, const void* device_src, size_t bytes, void* stream);
 int brk_stream_synchronize(void* stream);
+/* Runtime helpers for callers that reach HIP only through this library (one
+   HIP runtime per process): free/total device memory, default-priority
+   streams, and events for timing work on a stream. */
+int brk_mem_info(size_t* free_bytes, size_t* total_bytes);
+int brk_stream_create(void** stream);
+int brk_stream_destroy(void* stream);
+int brk_event_create(void** event);
+int brk_event_destroy(void* event);
+int brk_event_record(void* event, void* stream);
+int brk_event_synchronize(void* event);
+int brk_event_elapsed_ms(void* start, void* end, float* ms);
 
 /* state := |basis_index> */
 int brk_set_basis_state(void* first, uint64_t size, int dtype, uint64_t basis_index, void* stream);

@@ -270,6 +270,56 @@ int brk_device_count(int* count) {
   return BRK_OK;
 }
 
+int brk_mem_info(size_t* free_bytes, size_t* total_bytes) {
+  if (!free_bytes || !total_bytes) return fail(BRK_ERR_INVALID_ARGUMENT, "null argument");
+  BRK_HIP_CHECK(hipMemGetInfo(free_bytes, total_bytes));
+  return BRK_OK;
+}
+
+int brk_stream_create(void** stream) {
+  if (!stream) return fail(BRK_ERR_INVALID_ARGUMENT, "null argument");
+  hipStream_t s = nullptr;
+  BRK_HIP_CHECK(hipStreamCreate(&s));  // default priority
+  *stream = s;
+  return BRK_OK;
+}
+
+int brk_stream_destroy(void* stream) {
+  if (stream) BRK_HIP_CHECK(hipStreamDestroy(as_stream(stream)));
+  return BRK_OK;
+}
+
+int brk_event_create(void** event) {
+  if (!event) return fail(BRK_ERR_INVALID_ARGUMENT, "null argument");
+  hipEvent_t e = nullptr;
+  BRK_HIP_CHECK(hipEventCreate(&e));
+  *event = e;
+  return BRK_OK;
+}
+
+int brk_event_destroy(void* event) {
+  if (event) BRK_HIP_CHECK(hipEventDestroy(reinterpret_cast<hipEvent_t>(event)));
+  return BRK_OK;
+}
+
+int brk_event_record(void* event, void* stream) {
+  if (!event) return fail(BRK_ERR_INVALID_ARGUMENT, "null event");
+  BRK_HIP_CHECK(hipEventRecord(reinterpret_cast<hipEvent_t>(event), as_stream(stream)));
+  return BRK_OK;
+}
+
+int brk_event_synchronize(void* event) {
+  if (!event) return fail(BRK_ERR_INVALID_ARGUMENT, "null event");
+  BRK_HIP_CHECK(hipEventSynchronize(reinterpret_cast<hipEvent_t>(event)));
+  return BRK_OK;
+}
+
+int brk_event_elapsed_ms(void* start, void* end, float* ms) {
+  if (!start || !end || !ms) return fail(BRK_ERR_INVALID_ARGUMENT, "null argument");
+  BRK_HIP_CHECK(hipEventElapsedTime(ms, reinterpret_cast<hipEvent_t>(start), reinterpret_cast<hipEvent_t>(end)));
+  return BRK_OK;
+}
+
 int brk_device_name(char* buffer, size_t size) {
   if (!buffer || size == 0) return fail(BRK_ERR_INVALID_ARGUMENT, "empty buffer");
   int dev = 0;

@@ -37,6 +37,14 @@ _SIGS = {
     "brk_version": [],
     "brk_device_count": [C.POINTER(C.c_int)],
     "brk_device_name": [C.c_char_p, C.c_size_t],
+    "brk_mem_info": [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)],
+    "brk_stream_create": [C.POINTER(C.c_void_p)],
+    "brk_stream_destroy": [_p],
+    "brk_event_create": [C.POINTER(C.c_void_p)],
+    "brk_event_destroy": [_p],
+    "brk_event_record": [_p, _p],
+    "brk_event_synchronize": [_p],
+    "brk_event_elapsed_ms": [_p, _p, C.POINTER(C.c_float)],
     "brk_tune": [C.c_char_p, C.c_int64],
     "brk_get_tune": [C.c_char_p, C.POINTER(C.c_int64)],
     "brk_malloc": [C.POINTER(C.c_void_p), C.c_size_t],
