@@ -27,6 +27,11 @@
 //    and t + DT/2, or registers v and v + 8 when DT = 1), so each result is
 //    stored as one complex element, in place.
 //
+// Loads and stores are ordinary (temporal): when the block holds some of qubits
+// 0..2 each store instruction covers only part of every 64-byte line, and the
+// L2 merges those pieces before write-back (non-temporal stores went to memory
+// as partial lines and cost up to 2.7x the write traffic).
+//
 // MFMA 32x32x2 f32 operand layout (lane l, accumulator register v):
 //   A[i][k] = lane (i + 32k), B[k][j] = lane (j + 32k),
 //   D[8(v/4) + 4(l/32) + v%4][l%32] = acc[v] of lane l.
@@ -39,8 +44,11 @@ typedef float f16v __attribute__((ext_vector_type(16)));
 
 constexpr int FUSED_THREADS = 256;  // 4 wavefronts per workgroup, one tile each at a time
 
-template <int KB>
-__global__ __launch_bounds__(FUSED_THREADS, 2) void k_fused(cpx<float>* __restrict__ s, fused_launch L) {
+// OCC = workgroups per CU the register budget is cut for: 2 (software-pipelined,
+// the next tile's loads in flight during this tile's MFMAs) or 4 (no pipelining,
+// more wavefronts per SIMD hide the memory latency instead).
+template <int KB, int OCC>
+__global__ __launch_bounds__(FUSED_THREADS, OCC) void k_fused(cpx<float>* __restrict__ s, fused_launch L) {
   constexpr int R = 1 << KB;   // complex rows
   constexpr int W = 2 * R;     // real rows of the embedding
   constexpr int HALF = R / 2;  // complex elements per lane
@@ -70,12 +78,12 @@ __global__ __launch_bounds__(FUSED_THREADS, 2) void k_fused(cpx<float>* __restri
 
   // k <= 5: the next tile's loads are issued before this tile's MFMAs and
   // stores (two tiles in registers); k = 6 has no registers to spare for that.
-  constexpr bool PIPELINE = KB <= 5;
+  constexpr bool PIPELINE = KB <= 5 && OCC == 2;
   cpx<float> xn[HALF];
   if (PIPELINE && tile < tile_end) {
     const uint64_t b0 = expand_index<KB + 5>(tile, L.outer_masks, L.outer_k) | col_bits;
 #pragma unroll
-    for (int j = 0; j < HALF; ++j) xn[j] = load_amp<true>(s + (b0 | L.row_bits[2 * j] | load_half));
+    for (int j = 0; j < HALF; ++j) xn[j] = load_amp<false>(s + (b0 | L.row_bits[2 * j] | load_half));
   }
   for (; tile < tile_end; tile += stride) {
     const uint64_t base = expand_index<KB + 5>(tile, L.outer_masks, L.outer_k) | col_bits;
@@ -87,13 +95,167 @@ __global__ __launch_bounds__(FUSED_THREADS, 2) void k_fused(cpx<float>* __restri
       if (next < tile_end) {
         const uint64_t bn = expand_index<KB + 5>(next, L.outer_masks, L.outer_k) | col_bits;
 #pragma unroll
-        for (int j = 0; j < HALF; ++j) xn[j] = load_amp<true>(s + (bn | L.row_bits[2 * j] | load_half));
+        for (int j = 0; j < HALF; ++j) xn[j] = load_amp<false>(s + (bn | L.row_bits[2 * j] | load_half));
       }
     } else {
 #pragma unroll
-      for (int j = 0; j < HALF; ++j) x[j] = load_amp<true>(s + (base | L.row_bits[2 * j] | load_half));
+      for (int j = 0; j < HALF; ++j) x[j] = load_amp<false>(s + (base | L.row_bits[2 * j] | load_half));
     }
 
+    f16v acc[DT];
+#pragma unroll
+    for (int t = 0; t < DT; ++t) acc[t] = f16v{};
+    // OCC = 4: read the matrix from LDS in every tile instead of holding all of it in
+    // registers (an opaque row offset keeps the reads inside the loop)
+    int arow = 0;
+    if constexpr (OCC == 4) asm volatile("" : "+v"(arow));
+#pragma unroll
+    for (int st = 0; st < R; ++st) {
+      const float b = st < HALF ? x[st].re : x[st - HALF].im;
+#pragma unroll
+      for (int t = 0; t < DT; ++t) {
+        const float a = A[(2 * st + kk) * W + arow + ((32 * t + i32) ^ swz)];
+        acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc[t], 0, 0, 0);
+      }
+      // k = 6: keep the A-fragment reads of a step with their MFMAs (hoisting all
+      // 256 of them would exceed the register budget)
+      if constexpr (KB == 6 || OCC == 4) __builtin_amdgcn_sched_barrier(0);
+    }
+
+    if constexpr (DT == 1) {
+      // rows 0..15 real parts (v < 8), rows 16..31 imaginary parts (v + 8)
+#pragma unroll
+      for (int v = 0; v < 8; ++v) {
+        const int row = 8 * (v >> 2) + (v & 3);  // + 4*kk through store_half
+        store_amp<false>(s + (base | L.row_bits[row] | store_half), cpx<float>{acc[0][v], acc[0][v + 8]});
+      }
+    } else {
+#pragma unroll
+      for (int t = 0; t < DT / 2; ++t) {
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+          const int row = 32 * t + 8 * (v >> 2) + (v & 3);
+          store_amp<false>(s + (base | L.row_bits[row] | store_half), cpx<float>{acc[t][v], acc[t + DT / 2][v]});
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Staged form (k = 4, 5): one 256-thread workgroup owns a tile of 2^k rows x
+// 128 columns (the 7 lowest qubits outside the block).  HBM <-> LDS copies
+// run in address order, 16 bytes per thread per access, so every instruction
+// covers whole 64-byte lines (qubits 0..6 are always tile bits); wave w reads
+// the B operands of columns 32w..32w+31 from LDS, runs the same MFMA chain as
+// the register-direct kernel, and writes its D rows back over its own
+// columns.  The next tile's loads are in flight during the MFMAs.  Address
+// order <-> LDS position: both are sums over the set bits of the address-order
+// index e, looked up in two 64-entry tables each (bits 0..5 and 6..11 of e).
+// ---------------------------------------------------------------------------
+constexpr int STAGED_THREADS = 256;
+
+struct f4 {
+  float x, y, z, w;
+} __attribute__((aligned(16)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// Whole-line traffic streams past the caches (non-temporal, 16 bytes per lane).
+__device__ __forceinline__ f4 load_f4(const cpx<float>* p) {
+  const v4f v = __builtin_nontemporal_load(reinterpret_cast<const v4f*>(p));
+  return f4{v.x, v.y, v.z, v.w};
+}
+__device__ __forceinline__ void store_f4(cpx<float>* p, f4 a) {
+  const v4f v = {a.x, a.y, a.z, a.w};
+  __builtin_nontemporal_store(v, reinterpret_cast<v4f*>(p));
+}
+
+template <int KB>
+__global__ __launch_bounds__(STAGED_THREADS, 3) void k_fused_staged(cpx<float>* __restrict__ s, fused_launch L) {
+  constexpr int R = 1 << KB, W = 2 * R, HALF = R / 2, DT = W / 32;
+  constexpr int COLS = FUSED_STAGED_COLUMNS;
+  constexpr int ELEMS = R * COLS;
+  constexpr int PIECES = ELEMS / 2 / STAGED_THREADS;  // 16-byte accesses per thread per tile
+  extern __shared__ float smem[];
+  float* A = smem;                                            // W*W floats (layout of the register-direct kernel)
+  cpx<float>* buf = reinterpret_cast<cpx<float>*>(smem + W * W);  // the tile, [row][COLS]
+  uint32_t* l_lo = reinterpret_cast<uint32_t*>(buf + ELEMS);  // LDS index of bits 0..5 of e
+  uint32_t* l_hi = l_lo + 64;                                 // ... of bits 6..11
+  uint64_t* g_lo = reinterpret_cast<uint64_t*>(l_hi + 64);    // address bits of bits 0..5 of e
+  uint64_t* g_hi = g_lo + 64;                                 // ... of bits 6..11
+
+  const int tid = threadIdx.x;
+  for (int i = tid; i < W * W; i += STAGED_THREADS) A[i] = L.matrix[i];
+  for (int t = tid; t < 128; t += STAGED_THREADS) {
+    const int v = t & 63, first = t < 64 ? 0 : 6, count = t < 64 ? 6 : L.souter_k - 6;
+    uint32_t li = 0;
+    uint64_t gi = 0;
+    for (int b = 0; b < count; ++b)
+      if ((v >> b) & 1) {
+        li |= 1u << L.stile_lds_bit[first + b];
+        gi |= 1ull << L.stile_qubit[first + b];
+      }
+    if (t < 64) {
+      l_lo[v] = li;
+      g_lo[v] = gi;
+    } else {
+      l_hi[v] = li;
+      g_hi[v] = gi;
+    }
+  }
+  __syncthreads();
+
+  const int lane = tid & 63, wave = tid >> 6;
+  const int i32 = lane & 31, kk = lane >> 5;
+  const int swz = (W >= 64 && kk) ? 32 : 0;
+  const uint32_t one = l_lo[1];  // LDS index bit of e's bit 0 (qubit 0)
+  const uint64_t ntiles = L.sntiles;
+
+  // Workgroup b runs on XCD b % 8; with the grid a multiple of 8 and at least 8
+  // tiles per XCD, XCD x walks the contiguous tile range [x T/8, (x+1) T/8)
+  // with its G/8 workgroups; otherwise the workgroups interleave over all tiles.
+  const bool by_xcd = (gridDim.x % 8) == 0 && ntiles >= 8 * (uint64_t)(gridDim.x / 8) && (ntiles % 8) == 0;
+  const uint64_t g8 = gridDim.x / 8, per_xcd = ntiles / 8, xcd = blockIdx.x % 8, lw = blockIdx.x / 8;
+  const uint64_t iters = by_xcd ? (per_xcd + g8 - 1 - lw) / g8 : (ntiles + gridDim.x - 1 - blockIdx.x) / gridDim.x;
+  auto tile_of = [&](uint64_t it) { return by_xcd ? xcd * per_xcd + lw + it * g8 : blockIdx.x + it * gridDim.x; };
+
+  f4 pre[PIECES];
+  uint64_t tile = tile_of(0);
+  if (iters > 0) {
+    const uint64_t base = expand_index<KB + 7>(tile, L.souter_masks, L.souter_k);
+#pragma unroll
+    for (int p = 0; p < PIECES; ++p) {
+      const int e = 2 * (tid + STAGED_THREADS * p);
+      pre[p] = load_f4(s + (base | g_lo[e & 63] | g_hi[e >> 6]));
+    }
+  }
+  for (uint64_t it = 0; it < iters; ++it) {
+    tile = tile_of(it);
+    const uint64_t base = expand_index<KB + 7>(tile, L.souter_masks, L.souter_k);
+    // 1. the tile into LDS
+#pragma unroll
+    for (int p = 0; p < PIECES; ++p) {
+      const int e = 2 * (tid + STAGED_THREADS * p);
+      const uint32_t li = l_lo[e & 63] | l_hi[e >> 6];
+      buf[li] = cpx<float>{pre[p].x, pre[p].y};
+      buf[li | one] = cpx<float>{pre[p].z, pre[p].w};
+    }
+    __syncthreads();
+    // 2. this wave's B operands: lane (col, kk) holds X[2j + kk][col]
+    cpx<float> x[HALF];
+#pragma unroll
+    for (int j = 0; j < HALF; ++j) x[j] = buf[(2 * j + kk) * COLS + 32 * wave + i32];
+    // 3. the next tile's loads
+    const uint64_t next = tile_of(it + 1);
+    if (it + 1 < iters) {
+      const uint64_t nbase = expand_index<KB + 7>(next, L.souter_masks, L.souter_k);
+#pragma unroll
+      for (int p = 0; p < PIECES; ++p) {
+        const int e = 2 * (tid + STAGED_THREADS * p);
+        pre[p] = load_f4(s + (nbase | g_lo[e & 63] | g_hi[e >> 6]));
+      }
+    }
+    // 4. Y = U X on the wave's 32 columns
     f16v acc[DT];
 #pragma unroll
     for (int t = 0; t < DT; ++t) acc[t] = f16v{};
@@ -105,39 +267,67 @@ __global__ __launch_bounds__(FUSED_THREADS, 2) void k_fused(cpx<float>* __restri
         const float a = A[(2 * st + kk) * W + ((32 * t + i32) ^ swz)];
         acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc[t], 0, 0, 0);
       }
-      // k = 6: keep the A-fragment reads of a step with their MFMAs (hoisting all
-      // 256 of them would exceed the register budget)
-      if constexpr (KB == 6) __builtin_amdgcn_sched_barrier(0);
     }
-
+    // 5. D over the wave's own columns (no other wave reads or writes them before the barrier)
     if constexpr (DT == 1) {
-      // rows 0..15 real parts (v < 8), rows 16..31 imaginary parts (v + 8)
 #pragma unroll
       for (int v = 0; v < 8; ++v) {
-        const int row = 8 * (v >> 2) + (v & 3);  // + 4*kk through store_half
-        store_amp<true>(s + (base | L.row_bits[row] | store_half), cpx<float>{acc[0][v], acc[0][v + 8]});
+        const int row = 8 * (v >> 2) + (v & 3) + 4 * kk;
+        buf[row * COLS + 32 * wave + i32] = cpx<float>{acc[0][v], acc[0][v + 8]};
       }
     } else {
 #pragma unroll
       for (int t = 0; t < DT / 2; ++t) {
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
-          const int row = 32 * t + 8 * (v >> 2) + (v & 3);
-          store_amp<true>(s + (base | L.row_bits[row] | store_half), cpx<float>{acc[t][v], acc[t + DT / 2][v]});
+          const int row = 32 * t + 8 * (v >> 2) + (v & 3) + 4 * kk;
+          buf[row * COLS + 32 * wave + i32] = cpx<float>{acc[t][v], acc[t + DT / 2][v]};
         }
       }
     }
+    __syncthreads();
+    // 6. the tile back to HBM in address order
+#pragma unroll
+    for (int p = 0; p < PIECES; ++p) {
+      const int e = 2 * (tid + STAGED_THREADS * p);
+      const uint32_t li = l_lo[e & 63] | l_hi[e >> 6];
+      const cpx<float> a0 = buf[li], a1 = buf[li | one];
+      store_f4(s + (base | g_lo[e & 63] | g_hi[e >> 6]), f4{a0.re, a0.im, a1.re, a1.im});
+    }
+    __syncthreads();
   }
 }
 
 template <int KB>
+static size_t staged_lds_bytes() {
+  constexpr int W = 2 << KB;
+  return sizeof(float) * W * W + sizeof(cpx<float>) * (size_t)(1 << KB) * FUSED_STAGED_COLUMNS +
+         128 * sizeof(uint32_t) + 128 * sizeof(uint64_t);
+}
+
+template <int KB>
+static int launch_staged(const fused_launch& L, void* state, void* stream) {
+  const size_t lds = staged_lds_bytes<KB>();
+  static bool configured = false;
+  if (!configured) {
+    BRK_HIP_CHECK(hipFuncSetAttribute((const void*)k_fused_staged<KB>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)lds));
+    configured = true;
+  }
+  const unsigned grid = (unsigned)std::min<uint64_t>(L.sntiles, (uint64_t)get_tuning().fused_staged_groups);
+  hipLaunchKernelGGL(k_fused_staged<KB>, dim3(grid), dim3(STAGED_THREADS), lds, as_stream(stream),
+                     (cpx<float>*)state, L);
+  return check_launch("fused block (staged)");
+}
+
+template <int KB, int OCC>
 static int launch_k(const fused_launch& L, void* state, void* stream) {
   constexpr int W = 2 << KB;
   const size_t lds = sizeof(float) * (size_t)W * W;
   static bool configured = false;
   if (!configured) {
     // above 64 KiB of dynamic LDS a kernel has to opt in
-    BRK_HIP_CHECK(hipFuncSetAttribute((const void*)k_fused<KB>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    BRK_HIP_CHECK(hipFuncSetAttribute((const void*)k_fused<KB, OCC>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)lds));
     configured = true;
   }
@@ -145,15 +335,20 @@ static int launch_k(const fused_launch& L, void* state, void* stream) {
   const unsigned grid = (unsigned)std::min<uint64_t>(groups, (uint64_t)get_tuning().fused_groups);
   fused_launch Lc = L;
   Lc.chunked = get_tuning().fused_chunked;
-  hipLaunchKernelGGL(k_fused<KB>, dim3(grid), dim3(FUSED_THREADS), lds, as_stream(stream), (cpx<float>*)state, Lc);
+  hipLaunchKernelGGL((k_fused<KB, OCC>), dim3(grid), dim3(FUSED_THREADS), lds, as_stream(stream),
+                     (cpx<float>*)state, Lc);
   return check_launch("fused block");
 }
 
 int launch_fused_block(const fused_launch& L, void* state, void* stream) {
+  if (get_tuning().fused_staged && L.sntiles) {
+    if (L.width == 4) return launch_staged<4>(L, state, stream);
+    if (L.width == 5) return launch_staged<5>(L, state, stream);
+  }
   switch (L.width) {
-    case 4: return launch_k<4>(L, state, stream);
-    case 5: return launch_k<5>(L, state, stream);
-    case 6: return launch_k<6>(L, state, stream);
+    case 4: return get_tuning().fused_occupancy == 4 ? launch_k<4, 4>(L, state, stream) : launch_k<4, 2>(L, state, stream);
+    case 5: return get_tuning().fused_occupancy == 4 ? launch_k<5, 4>(L, state, stream) : launch_k<5, 2>(L, state, stream);
+    case 6: return launch_k<6, 2>(L, state, stream);  // 64 KiB of LDS per workgroup: two per CU at most
     default: return fail(BRK_ERR_UNSUPPORTED, "fused block width %d not in 4..6", L.width);
   }
 }

@@ -25,7 +25,20 @@ struct fused_launch {
   uint64_t col_bits[FUSED_COLUMNS];    // deposit of each column value
   const float* matrix;                 // device: real embedding [[Ur, -Ui], [Ui, Ur]], transposed, swizzled
   int chunked;                         // tile order (set at launch from the tuning)
+
+  // Staged form (k <= 5): a workgroup tile is 2^k rows x 128 columns (the 7
+  // lowest qubits outside the block), moved between HBM and LDS in address
+  // order.  Tile bit i (ascending qubit order) is qubit stile_qubit[i]; in the
+  // LDS tile, stored as [row][128 columns], it is index bit stile_lds_bit[i]
+  // (column bit c -> c, block-row bit r -> 7 + r).
+  uint64_t souter_masks[17];           // expand masks inserting zeros at the k + 7 tile bits
+  int souter_k;                        // = k + 7
+  uint64_t sntiles;                    // 2^(n - k - 7); 0 when n < k + 7
+  uint8_t stile_qubit[16];
+  uint8_t stile_lds_bit[16];
 };
+
+constexpr int FUSED_STAGED_COLUMNS = 128;
 
 // Launches one block (state is complex64).  Defined in csrc/brk_fused.hip.
 int launch_fused_block(const fused_launch& L, void* state, void* stream);

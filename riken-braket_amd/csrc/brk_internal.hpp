@@ -108,6 +108,9 @@ struct tuning {
   uint64_t max_blocks;    // grid cap of the gate kernels (larger states loop)
   uint64_t line_bytes;    // write granularity: control bits addressing inside it are tested, not skipped
   uint64_t fused_groups;  // grid cap of the fused-block kernel (workgroups loop over the tiles)
+  int fused_occupancy;    // 2 or 4: workgroups per CU the fused kernel (k <= 5) is compiled for
+  int fused_staged;       // k <= 5: 1 = LDS-staged tiles in address order, 0 = register-direct tiles
+  uint64_t fused_staged_groups;  // grid of the staged kernel (workgroups loop over the tiles)
   int fused_chunked;      // fused tiles: 0 = interleaved over all waves, 1 = contiguous run per wave
 };
 tuning& get_tuning();

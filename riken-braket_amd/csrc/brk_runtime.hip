@@ -56,7 +56,8 @@ int collect_qubits(const unsigned* qubits, unsigned n, unsigned num_qubits, unsi
 // Defaults measured on MI355X with 2^30..2^32-amplitude c128 states.
 static tuning g_tuning = {
     /*pair_lds=*/40960, /*diag_lds=*/0, /*pair_remap=*/1, /*diag_remap=*/1,
-    /*nt_min_bytes=*/256ull << 20, /*max_blocks=*/(1ull << 24) - 1, /*line_bytes=*/64, /*fused_groups=*/8192, /*fused_chunked=*/0,
+    /*nt_min_bytes=*/256ull << 20, /*max_blocks=*/(1ull << 24) - 1, /*line_bytes=*/64, /*fused_groups=*/8192,
+    /*fused_occupancy=*/2, /*fused_staged=*/1, /*fused_staged_groups=*/768, /*fused_chunked=*/0,
 };
 static std::once_flag g_tuning_once;
 
@@ -72,6 +73,9 @@ static void read_tuning_env() {
   get("BRK_TUNE_MAX_BLOCKS", &g_tuning.max_blocks);
   get("BRK_TUNE_LINE_BYTES", &g_tuning.line_bytes);
   get("BRK_TUNE_FUSED_GROUPS", &g_tuning.fused_groups);
+  get("BRK_TUNE_FUSED_OCCUPANCY", &g_tuning.fused_occupancy);
+  get("BRK_TUNE_FUSED_STAGED", &g_tuning.fused_staged);
+  get("BRK_TUNE_FUSED_STAGED_GROUPS", &g_tuning.fused_staged_groups);
 }
 
 tuning& get_tuning() {
@@ -234,6 +238,14 @@ int brk_tune(const char* key, int64_t value) {
   } else if (k == "fused_groups") {
     if (value < 1 || value > (1 << 22)) return fail(BRK_ERR_INVALID_ARGUMENT, "fused_groups must be in [1, 2^22]");
     t.fused_groups = (uint64_t)value;
+  } else if (k == "fused_staged") {
+    t.fused_staged = value != 0;
+  } else if (k == "fused_staged_groups") {
+    if (value < 1 || value > (1 << 22)) return fail(BRK_ERR_INVALID_ARGUMENT, "fused_staged_groups must be in [1, 2^22]");
+    t.fused_staged_groups = (uint64_t)value;
+  } else if (k == "fused_occupancy") {
+    if (value != 2 && value != 4) return fail(BRK_ERR_INVALID_ARGUMENT, "fused_occupancy must be 2 or 4");
+    t.fused_occupancy = (int)value;
   } else if (k == "max_blocks") {
     if (value < 1) return fail(BRK_ERR_INVALID_ARGUMENT, "max_blocks must be >= 1");
     t.max_blocks = (uint64_t)value;
@@ -254,6 +266,9 @@ int brk_get_tune(const char* key, int64_t* value) {
   else if (k == "line_bytes") *value = (int64_t)t.line_bytes;
   else if (k == "fused_groups") *value = (int64_t)t.fused_groups;
   else if (k == "fused_chunked") *value = t.fused_chunked;
+  else if (k == "fused_occupancy") *value = t.fused_occupancy;
+  else if (k == "fused_staged") *value = t.fused_staged;
+  else if (k == "fused_staged_groups") *value = (int64_t)t.fused_staged_groups;
   else return fail(BRK_ERR_INVALID_ARGUMENT, "unknown tuning key '%s'", key);
   return BRK_OK;
 }

@@ -108,6 +108,31 @@ static void build_launch(brk_fused_plan::block& blk, unsigned n, unsigned w) {
   L.ntiles = 1ull << (n - tile.size());
   for (unsigned r = 0; r < FUSED_MAX_ROWS; ++r) L.row_bits[r] = r < (1u << w) ? deposit(r, blk.qubits.data(), w) : 0;
   for (unsigned c = 0; c < FUSED_COLUMNS; ++c) L.col_bits[c] = deposit(c, cols.data(), 5);
+
+  // staged tiles: the 7 lowest qubits outside the block are the columns
+  L.sntiles = 0;
+  std::vector<unsigned> scols;
+  for (unsigned q = 0; scols.size() < 7 && q < n; ++q)
+    if (!std::binary_search(blk.qubits.begin(), blk.qubits.end(), q)) scols.push_back(q);
+  if (w <= 5 && scols.size() == 7) {
+    std::vector<unsigned> stile(blk.qubits);
+    stile.insert(stile.end(), scols.begin(), scols.end());
+    std::sort(stile.begin(), stile.end());
+    expand_masks se = make_expand_masks(stile.data(), (int)stile.size());
+    std::copy(se.m, se.m + BRK_MAX_OPERATED_QUBITS + 1, L.souter_masks);
+    L.souter_k = (int)stile.size();
+    L.sntiles = 1ull << (n - stile.size());
+    for (size_t i = 0; i < stile.size(); ++i) {
+      const unsigned q = stile[i];
+      L.stile_qubit[i] = (uint8_t)q;
+      const auto row = std::lower_bound(blk.qubits.begin(), blk.qubits.end(), q);
+      if (row != blk.qubits.end() && *row == q) {
+        L.stile_lds_bit[i] = (uint8_t)(7 + (row - blk.qubits.begin()));
+      } else {
+        L.stile_lds_bit[i] = (uint8_t)(std::lower_bound(scols.begin(), scols.end(), q) - scols.begin());
+      }
+    }
+  }
 }
 
 // Real embedding [[Ur, -Ui], [Ui, Ur]] of a 2^w block, stored transposed

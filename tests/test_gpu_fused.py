@@ -70,3 +70,19 @@ def test_fused_rejects_c128():
     with pytest.raises(brk.BrkError):
         plan.apply(st.ptr, brk.C128)
     st.free()
+
+
+@pytest.mark.parametrize("n,width", [(12, 5), (15, 4), (17, 5)])
+def test_staged_and_register_direct_kernels_agree(n, width):
+    # the two kernel forms run the same MFMA chain per column: identical results
+    ops = W.random_circuit(n, 80, 4, seed=5 * n + width)
+    plan = brk.FusedPlan(brk.Circuit(n, ops), width)
+    s = D.random_state(n, np.random.default_rng(n))
+    note(f"gpu fused staged vs register-direct n={n} width={width}")
+    brk.tune("fused_staged", 0)
+    try:
+        direct = _run_fused(plan, s)
+    finally:
+        brk.tune("fused_staged", 1)
+    staged = _run_fused(plan, s)
+    assert np.array_equal(direct, staged)
