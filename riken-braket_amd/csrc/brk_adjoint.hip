@@ -109,6 +109,42 @@ static void launch_r(const adj_chain& c, void* psi, void* lam, uint64_t size, bo
   }
 }
 
+// Whole-state sign pass: every amplitude is read and written (full lines), two
+// per thread, block-contiguous with the XCD-contiguous block order.
+template <typename R, bool NT>
+__global__ __launch_bounds__(256) void k_sign_terms(cpx<R>* __restrict__ s, uint64_t size, sign_terms t) {
+  const uint64_t i = xcd_tile(blockIdx.x, gridDim.x, 1) * 512 + 2 * threadIdx.x;
+  if (i >= size) return;
+  cpx<R> v[2];
+#pragma unroll
+  for (int u = 0; u < 2; ++u) v[u] = load_amp<NT>(s + i + u);
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const uint64_t x = i + u;
+    uint32_t par = 0;
+    for (int k = 0; k < t.n; ++k) par += (uint32_t)__popcll(x & (x >> t.d[k]) & t.a[k]);
+    if (par & 1u) v[u] = cpx<R>{-v[u].re, -v[u].im};
+    store_amp<NT>(s + x, v[u]);
+  }
+}
+
+int launch_sign_terms(const sign_terms& t, void* first, uint64_t size, int dtype, void* stream) {
+  if (t.n < 0 || t.n > SIGN_MAX_TERMS) return fail(BRK_ERR_INVALID_ARGUMENT, "bad sign term count");
+  if (t.n == 0) return BRK_OK;
+  const uint64_t blocks = (size + 511) / 512;
+  if (blocks > 0xffffffffull) return fail(BRK_ERR_INVALID_ARGUMENT, "state too large for the sign pass");
+  const bool nt = size * amp_bytes(dtype) >= get_tuning().nt_min_bytes;
+  hipStream_t st = as_stream(stream);
+  if (dtype == BRK_C128) {
+    if (nt) hipLaunchKernelGGL((k_sign_terms<double, true>), dim3((unsigned)blocks), dim3(256), 0, st, (cpx<double>*)first, size, t);
+    else hipLaunchKernelGGL((k_sign_terms<double, false>), dim3((unsigned)blocks), dim3(256), 0, st, (cpx<double>*)first, size, t);
+  } else {
+    if (nt) hipLaunchKernelGGL((k_sign_terms<float, true>), dim3((unsigned)blocks), dim3(256), 0, st, (cpx<float>*)first, size, t);
+    else hipLaunchKernelGGL((k_sign_terms<float, false>), dim3((unsigned)blocks), dim3(256), 0, st, (cpx<float>*)first, size, t);
+  }
+  return check_launch("sign terms");
+}
+
 int launch_adjoint_chain(const adj_chain& c, void* psi, void* lam, uint64_t size, int dtype, double* partials,
                          void* stream) {
   if (c.nops < 1 || c.nops > ADJ_MAX_CHAIN) return fail(BRK_ERR_INVALID_ARGUMENT, "bad adjoint chain length");

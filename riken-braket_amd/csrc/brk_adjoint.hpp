@@ -29,4 +29,16 @@ struct adj_chain {
 int launch_adjoint_chain(const adj_chain& c, void* psi, void* lam, uint64_t size, int dtype, double* partials,
                          void* stream);
 
+// A run of Z-type ops (Z, CZ on two qubits) as one diagonal pass: amplitude x
+// is negated when sum_t popcount(x & (x >> d[t]) & a[t]) is odd (d = 0: Z on
+// each qubit of a; d > 0: CZ(q, q + d) for each qubit q of a).  Terms of equal
+// d merge by XOR, so a repeated gate cancels.
+constexpr int SIGN_MAX_TERMS = 16;
+struct sign_terms {
+  int n = 0;
+  uint32_t d[SIGN_MAX_TERMS];
+  uint64_t a[SIGN_MAX_TERMS];
+};
+int launch_sign_terms(const sign_terms& t, void* first, uint64_t size, int dtype, void* stream);
+
 }  // namespace brk

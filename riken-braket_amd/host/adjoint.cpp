@@ -10,13 +10,18 @@
 //   EX/EY/EZ(phi)   = exp(i phi G):         D = i G
 //   U1(lambda)      = diag(1, e^{i lambda}):  D = diag(0, i)
 // Runs of consecutive single-qubit ops on the same qubit are swept by one
-// fused kernel (csrc/brk_adjoint.hip); other ops are un-applied to both
-// states with the plain kernels.
+// fused kernel (csrc/brk_adjoint.hip); runs of Z / CZ ops (diagonal, self-
+// inverse, mutually commuting) are one sign pass per state; other ops are
+// un-applied to both states with the plain kernels.  The forward pass merges
+// the same runs: a single-qubit run becomes one 2x2 product (FP64 on the host),
+// a Z / CZ run one sign pass.
 #include "circuit.hpp"
 #include "brk_adjoint.hpp"
 
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
+#include <array>
 #include <complex>
 #include <mutex>
 #include <vector>
@@ -104,6 +109,99 @@ static brk_circuit::op inverse_op(const brk_circuit::op& o) {
   return r;
 }
 
+// Z on one qubit or CZ on two (controls count as qubits): one sign term.
+static bool sign_term_of(const brk_circuit::op& o, uint32_t* d, uint64_t* a) {
+  if (o.code != BRK_OP_Z) return false;
+  std::vector<unsigned> q(o.targets);
+  q.insert(q.end(), o.controls.begin(), o.controls.end());
+  if (q.size() == 1) {
+    *d = 0;
+    *a = 1ull << q[0];
+    return true;
+  }
+  if (q.size() != 2) return false;
+  const unsigned lo = std::min(q[0], q[1]), hi = std::max(q[0], q[1]);
+  *d = hi - lo;
+  *a = 1ull << lo;
+  return true;
+}
+
+static bool is_sign(const brk_circuit::op& o) {
+  uint32_t d;
+  uint64_t a;
+  return sign_term_of(o, &d, &a);
+}
+
+// Applies the sign ops ops[b, e) to one or two states, as few passes as the
+// term limit allows (the ops commute, so any grouping is exact).
+static int apply_sign_run(const std::vector<brk_circuit::op>& ops, size_t b, size_t e, void* s1, void* s2,
+                          uint64_t size, int dtype, void* stream) {
+  while (b < e) {
+    sign_terms t;
+    size_t k = b;
+    for (; k < e; ++k) {
+      uint32_t d;
+      uint64_t a;
+      sign_term_of(ops[k], &d, &a);
+      int slot = 0;
+      while (slot < t.n && t.d[slot] != d) ++slot;
+      if (slot == t.n) {
+        if (t.n == SIGN_MAX_TERMS) break;
+        t.d[t.n] = d;
+        t.a[t.n] = 0;
+        ++t.n;
+      }
+      t.a[slot] ^= a;
+    }
+    if (int s = launch_sign_terms(t, s1, size, dtype, stream)) return s;
+    if (s2)
+      if (int s = launch_sign_terms(t, s2, size, dtype, stream)) return s;
+    b = k;
+  }
+  return BRK_OK;
+}
+
+// psi := U psi with the runs merged (forward pass).
+static int forward(const std::vector<brk_circuit::op>& ops, void* psi, uint64_t size, int dtype, void* stream) {
+  size_t i = 0;
+  while (i < ops.size()) {
+    const auto& o = ops[i];
+    if (chainable(o)) {
+      std::vector<unsigned> q;
+      std::vector<std::array<double, 2>> u;
+      if (!op_unitary(o, q, u) || u.size() != 4) return fail(BRK_ERR_UNSUPPORTED, "op %zu has no 2x2 form", i);
+      cd m[4];
+      for (int k = 0; k < 4; ++k) m[k] = cd(u[k][0], u[k][1]);
+      size_t j = i + 1;
+      for (; j < ops.size() && chainable(ops[j]) && ops[j].targets[0] == o.targets[0]; ++j) {
+        if (!op_unitary(ops[j], q, u) || u.size() != 4) return fail(BRK_ERR_UNSUPPORTED, "op %zu has no 2x2 form", j);
+        const cd b0(u[0][0], u[0][1]), b1(u[1][0], u[1][1]), b2(u[2][0], u[2][1]), b3(u[3][0], u[3][1]);
+        const cd a0 = m[0], a1 = m[1], a2 = m[2], a3 = m[3];
+        m[0] = b0 * a0 + b1 * a2;  // later op on the left
+        m[1] = b0 * a1 + b1 * a3;
+        m[2] = b2 * a0 + b3 * a2;
+        m[3] = b2 * a1 + b3 * a3;
+      }
+      double mm[8];
+      for (int k = 0; k < 4; ++k) {
+        mm[2 * k] = m[k].real();
+        mm[2 * k + 1] = m[k].imag();
+      }
+      if (int s = brk_gate_1q(psi, size, dtype, o.targets[0], mm, nullptr, 0, stream)) return s;
+      i = j;
+    } else if (is_sign(o)) {
+      size_t j = i + 1;
+      while (j < ops.size() && is_sign(ops[j])) ++j;
+      if (int s = apply_sign_run(ops, i, j, psi, nullptr, size, dtype, stream)) return s;
+      i = j;
+    } else {
+      if (int s = apply_physical_op(o, psi, size, dtype, stream)) return s;
+      ++i;
+    }
+  }
+  return BRK_OK;
+}
+
 // Per-device FP64 partial-sum buffer, grown as needed and kept until exit.
 static std::mutex g_partials_mutex;
 static std::vector<std::pair<double*, size_t>> g_partials;
@@ -167,7 +265,7 @@ int brk_adjoint_ising_gradient(const brk_circuit* circuit, void* psi, void* work
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
 
   // forward pass, lam = H psi, E = <psi|lam>
-  if (int s = brk_circuit_run(circuit, psi, dtype, nullptr, stream)) return s;
+  if (int s = forward(ops, psi, size, dtype, stream)) return s;
   if (int s = brk_apply_ising(work, psi, size, dtype, J, h, stream)) return s;
   double ip[2];
   if (int s = brk_inner_product(work, dtype, psi, dtype, size, ip, stream)) return s;
@@ -202,6 +300,12 @@ int brk_adjoint_ising_gradient(const brk_circuit* circuit, void* psi, void* work
       }
       if (int s = launch_adjoint_chain(c, psi, work, size, dtype, partials + region, stream)) return s;
       region += (size_t)ADJ_MAX_CHAIN * ADJ_BLOCKS;
+      i = j - 1;
+    } else if (is_sign(o)) {
+      // the run of sign ops ending at i: self-inverse and commuting, so one pass per state
+      long j = i;
+      while (j - 1 >= 0 && is_sign(ops[(size_t)(j - 1)])) --j;
+      if (int s = apply_sign_run(ops, (size_t)j, (size_t)i + 1, psi, work, size, dtype, stream)) return s;
       i = j - 1;
     } else {
       const brk_circuit::op inv = inverse_op(o);
