@@ -109,18 +109,21 @@ static void launch_r(const adj_chain& c, void* psi, void* lam, uint64_t size, bo
   }
 }
 
-// Whole-state sign pass: every amplitude is read and written (full lines), two
-// per thread, block-contiguous with the XCD-contiguous block order.
+// Whole-state sign pass: every amplitude is read and written, two per thread
+// 256 apart so that each wavefront instruction covers one contiguous run
+// (whole lines), blocks in the XCD-contiguous order.
 template <typename R, bool NT>
 __global__ __launch_bounds__(256) void k_sign_terms(cpx<R>* __restrict__ s, uint64_t size, sign_terms t) {
-  const uint64_t i = xcd_tile(blockIdx.x, gridDim.x, 1) * 512 + 2 * threadIdx.x;
-  if (i >= size) return;
+  // size is a power of two: blocks of 512 amplitudes, or one block of `size` when smaller
+  const uint64_t stride = size >= 512 ? 256 : size / 2;
+  const uint64_t i = xcd_tile(blockIdx.x, gridDim.x, 1) * 512 + threadIdx.x;
+  if (threadIdx.x >= stride) return;
   cpx<R> v[2];
 #pragma unroll
-  for (int u = 0; u < 2; ++u) v[u] = load_amp<NT>(s + i + u);
+  for (int u = 0; u < 2; ++u) v[u] = load_amp<NT>(s + i + u * stride);
 #pragma unroll
   for (int u = 0; u < 2; ++u) {
-    const uint64_t x = i + u;
+    const uint64_t x = i + u * stride;
     uint32_t par = 0;
     for (int k = 0; k < t.n; ++k) par += (uint32_t)__popcll(x & (x >> t.d[k]) & t.a[k]);
     if (par & 1u) v[u] = cpx<R>{-v[u].re, -v[u].im};
@@ -131,7 +134,7 @@ __global__ __launch_bounds__(256) void k_sign_terms(cpx<R>* __restrict__ s, uint
 int launch_sign_terms(const sign_terms& t, void* first, uint64_t size, int dtype, void* stream) {
   if (t.n < 0 || t.n > SIGN_MAX_TERMS) return fail(BRK_ERR_INVALID_ARGUMENT, "bad sign term count");
   if (t.n == 0) return BRK_OK;
-  const uint64_t blocks = (size + 511) / 512;
+  const uint64_t blocks = (size + 511) / 512;  // size < 512: one block, threads [0, size/2) each take i, i + size/2
   if (blocks > 0xffffffffull) return fail(BRK_ERR_INVALID_ARGUMENT, "state too large for the sign pass");
   const bool nt = size * amp_bytes(dtype) >= get_tuning().nt_min_bytes;
   hipStream_t st = as_stream(stream);
