@@ -242,6 +242,37 @@ int brk_apply_ising(void* out, const void* psi, uint64_t size, int dtype, const 
                     void* stream);
 
 /* ------------------------------------------------------------------------ */
+/* Measurement and sampling.  The random draws stay with the caller (bra    */
+/* keeps a std::mt19937_64 per state, bra/include/bra/state.hpp:459); these */
+/* calls take the canonical numbers u in [0, 1) that                        */
+/* std::uniform_real_distribution turns into u * maximum.                  */
+/* ------------------------------------------------------------------------ */
+/* result = {P(qubit = 0), P(qubit = 1)}, FP64 sums in a fixed order.  Replaces
+   projective_measurement_detail::zero_one_probabilities
+   (ket/include/ket/gate/projective_measurement.hpp:28-62) and the zero / one
+   probabilities of clear.hpp:26-55 and set.hpp:26-55. */
+int brk_zero_one_probabilities(const void* first, uint64_t size, int dtype, unsigned qubit, double result[2],
+                               void* stream);
+/* Amplitudes whose `qubit` bit equals `value` are multiplied by `multiplier`;
+   the others become +0.  Replaces change_state_after_measuring_zero/one
+   (projective_measurement.hpp:64-135) and the state updates of
+   ket::gate::clear (clear.hpp:57-83) and ket::gate::set (set.hpp:57-86), which
+   pass multiplier = p^(-1/2). */
+int brk_project_qubit(void* first, uint64_t size, int dtype, unsigned qubit, int value, double multiplier,
+                      void* stream);
+/* events[e] = the smallest basis index i whose cumulative probability
+   sum_{j<=i} |a_j|^2 exceeds canonical[e] * total (upper_bound over the
+   inclusive scan).  Replaces ket::generate_events
+   (ket/include/ket/generate_events.hpp:15-48) and the sampling step of
+   ket::measure (ket/include/ket/measure.hpp:14-40).  The reference overwrites
+   the state with the scan; this call leaves it unchanged.  The scan is summed
+   per chunk of 2^14 amplitudes and then across chunks, so an event can differ
+   from a sequential scan only where canonical[e] * total lies within rounding
+   of a cumulative boundary.  Synchronises the stream. */
+int brk_generate_events(const void* first, uint64_t size, int dtype, const double* canonical, unsigned num_events,
+                        uint64_t* events, void* stream);
+
+/* ------------------------------------------------------------------------ */
 /* Circuits: an ordered list of operations on logical qubits, run with the  */
 /* plain per-gate kernels.  Stands in for the gate list bra's interpreter   */
 /* builds (bra/src/interpreter.cpp:371-825) and executes through            */
@@ -406,7 +437,9 @@ int brk_batch_run_timed(brk_batch* batch, double* energies, double* elapsed_ms, 
 /*   SY SY+, EX EY EZ (+ adjoints, EXX../EXn forms), TOFFOLI, and every one */
 /*   of them behind a C..C / Cn control prefix (controls come first);       */
 /*   DO MEASUREMENT [p], BEGIN MEASUREMENT, DO AMPLITUDES [i..],            */
-/*   BEGIN FUSION / END FUSION (gates in between run in order).             */
+/*   BEGIN FUSION / END FUSION (gates in between run in order), M, CLEAR,   */
+/*   SET, GENERATE EVENTS n seed and EXIT (reading stops after the last two,*/
+/*   as bra's does).                                                        */
 /* Anything else fails with BRK_ERR_UNSUPPORTED and a message naming the    */
 /* line.                                                                    */
 /* ------------------------------------------------------------------------ */
@@ -418,7 +451,13 @@ typedef enum brk_qcx_kind {
   BRK_QCX_MEASUREMENT = 1,  /* spin expectation values of every qubit (precision) */
   BRK_QCX_AMPLITUDES = 2,   /* amplitudes (all, or the listed indices)          */
   BRK_QCX_BEGIN_FUSION = 3,
-  BRK_QCX_END_FUSION = 4
+  BRK_QCX_END_FUSION = 4,
+  BRK_QCX_PROJECTIVE_MEASUREMENT = 5, /* M q: indices = {q}                  */
+  BRK_QCX_CLEAR = 6,                  /* CLEAR q: indices = {q}              */
+  BRK_QCX_SET = 7,                    /* SET q: indices = {q}                */
+  BRK_QCX_GENERATE_EVENTS = 8,        /* indices = {num_events}, precision = seed
+                                         (negative: continue the state's generator) */
+  BRK_QCX_EXIT = 9                    /* sample one basis state and collapse onto it */
 } brk_qcx_kind;
 
 int brk_qcx_parse(const char* text, brk_qcx** out);
@@ -429,7 +468,8 @@ int brk_qcx_initial_state(const brk_qcx* program, uint64_t* basis_index);
 int brk_qcx_num_statements(const brk_qcx* program, unsigned* num_statements);
 /* Statement i: kind; for BRK_QCX_GATES its circuit (owned by the program);
    for BRK_QCX_MEASUREMENT the precision; for BRK_QCX_AMPLITUDES the indices
-   (num_indices 0 = all).  Output pointers may be NULL. */
+   (num_indices 0 = all); for the other kinds the operands listed with the
+   kind.  Output pointers may be NULL. */
 int brk_qcx_statement(const brk_qcx* program, unsigned i, int* kind, const brk_circuit** gates, int* precision,
                       unsigned* num_indices, const uint64_t** indices);
 

@@ -562,3 +562,92 @@ void or_apply_ising(double* out, const double* psi, uint64_t size, const double*
     cd_store(out, k, acc);
   }
 }
+
+/* ---- measurement and sampling ------------------------------------------ */
+
+/* MT19937-64 (Nishimura, ACM TOMACS 10(4), 2000); the parameters are those
+   of std::mt19937_64 ([rand.predef]). */
+void or_mt64_seed(or_mt64* g, uint64_t seed) {
+  g->mt[0] = seed;
+  for (unsigned i = 1; i < 312; ++i)
+    g->mt[i] = 6364136223846793005ull * (g->mt[i - 1] ^ (g->mt[i - 1] >> 62)) + i;
+  g->index = 312;
+}
+
+uint64_t or_mt64_next(or_mt64* g) {
+  const uint64_t upper = ~0ull << 31, lower = ~upper;
+  if (g->index >= 312) {
+    for (unsigned i = 0; i < 312; ++i) {
+      const uint64_t y = (g->mt[i] & upper) | (g->mt[(i + 1) % 312] & lower);
+      g->mt[i] = g->mt[(i + 156) % 312] ^ (y >> 1) ^ ((y & 1u) ? 0xB5026F5AA96619E9ull : 0ull);
+    }
+    g->index = 0;
+  }
+  uint64_t z = g->mt[g->index++];
+  z ^= (z >> 29) & 0x5555555555555555ull;
+  z ^= (z << 17) & 0x71D67FFFEDA60000ull;
+  z ^= (z << 37) & 0xFFF7EEE000000000ull;
+  z ^= z >> 43;
+  return z;
+}
+
+double or_mt64_canonical(or_mt64* g) {
+  const double r = (double)or_mt64_next(g) / 18446744073709551616.0;
+  return r >= 1.0 ? nextafter(1.0, 0.0) : r;
+}
+
+void or_zero_one_probabilities(const double* s, uint64_t size, unsigned qubit, double out[2]) {
+  const uint64_t qmask = 1ull << qubit, lower = qmask - 1ull;
+  long double p0 = 0.0L, p1 = 0.0L;
+  for (uint64_t g = 0; g < size / 2; ++g) {
+    const uint64_t i0 = ((g & ~lower) << 1) | (g & lower), i1 = i0 | qmask;
+    p0 += (long double)(s[2 * i0] * s[2 * i0] + s[2 * i0 + 1] * s[2 * i0 + 1]);
+    p1 += (long double)(s[2 * i1] * s[2 * i1] + s[2 * i1 + 1] * s[2 * i1 + 1]);
+  }
+  out[0] = (double)p0;
+  out[1] = (double)p1;
+}
+
+void or_project(double* s, uint64_t size, unsigned qubit, int value, double multiplier) {
+  const uint64_t qmask = 1ull << qubit, lower = qmask - 1ull;
+#pragma omp parallel for schedule(static)
+  for (uint64_t g = 0; g < size / 2; ++g) {
+    const uint64_t i0 = ((g & ~lower) << 1) | (g & lower);
+    const uint64_t keep = value ? (i0 | qmask) : i0, drop = value ? i0 : (i0 | qmask);
+    s[2 * keep] *= multiplier;
+    s[2 * keep + 1] *= multiplier;
+    s[2 * drop] = 0.0;
+    s[2 * drop + 1] = 0.0;
+  }
+}
+
+int or_projective_measurement(double* s, uint64_t size, unsigned qubit, double canonical) {
+  double p[2];
+  or_zero_one_probabilities(s, size, qubit, p);
+  const double total = p[0] + p[1];
+  const int outcome = canonical * total < p[0] ? 0 : 1;
+  or_project(s, size, qubit, outcome, pow(p[outcome], -0.5));
+  return outcome;
+}
+
+void or_generate_events(const double* s, uint64_t size, const double* canonical, unsigned num_events,
+                        uint64_t* events) {
+  double* scan = (double*)malloc(sizeof(double) * size);
+  double run = 0.0;
+  for (uint64_t i = 0; i < size; ++i) {
+    run += s[2 * i] * s[2 * i] + s[2 * i + 1] * s[2 * i + 1];
+    scan[i] = run;
+  }
+  const double total = scan[size - 1];
+  for (unsigned e = 0; e < num_events; ++e) {
+    const double r = canonical[e] * total;
+    uint64_t lo = 0, hi = size;  /* first i with scan[i] > r */
+    while (lo < hi) {
+      const uint64_t mid = lo + (hi - lo) / 2;
+      if (r < scan[mid]) hi = mid;
+      else lo = mid + 1;
+    }
+    events[e] = lo;
+  }
+  free(scan);
+}

@@ -73,6 +73,31 @@ void or_expectation_pauli_string(const double* s, uint64_t size, const char* pau
 double or_expectation_ising(const double* s, uint64_t size, const double* J, const double* h);
 void or_apply_ising(double* out, const double* psi, uint64_t size, const double* J, const double* h);
 
+/* measurement and sampling.  Random numbers come from std::mt19937_64
+   (bra/include/bra/state.hpp:459) drawn through std::uniform_real_distribution
+   (ket/utility/positive_random_value_upto.hpp:12-18), whose libstdc++ form is
+   generate_canonical: one 64-bit draw g, canonical = (double)g / 2^64, and the
+   value canonical * maximum. */
+typedef struct or_mt64 {
+  uint64_t mt[312];
+  unsigned index;
+} or_mt64;
+void or_mt64_seed(or_mt64* g, uint64_t seed);
+uint64_t or_mt64_next(or_mt64* g);
+double or_mt64_canonical(or_mt64* g);
+/* P(0), P(1) of a qubit, summed in long double (projective_measurement.hpp:28-62) */
+void or_zero_one_probabilities(const double* s, uint64_t size, unsigned qubit, double out[2]);
+/* keep the half where the qubit equals value, times multiplier; +0 elsewhere
+   (projective_measurement.hpp:64-135, clear.hpp:60-83, set.hpp:61-86) */
+void or_project(double* s, uint64_t size, unsigned qubit, int value, double multiplier);
+/* ket::gate::projective_measurement with the canonical draw given (projective_measurement.hpp:137-176) */
+int or_projective_measurement(double* s, uint64_t size, unsigned qubit, double canonical);
+/* ket::generate_events without the in-place scan: a sequential FP64 inclusive
+   scan of |a|^2 and, per event, upper_bound of canonical[e] * total
+   (generate_events.hpp:15-48); the state is left unchanged. */
+void or_generate_events(const double* s, uint64_t size, const double* canonical, unsigned num_events,
+                        uint64_t* events);
+
 #ifdef __cplusplus
 }
 #endif

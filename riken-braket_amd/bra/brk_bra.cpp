@@ -7,9 +7,12 @@
 // bra prints for the supported statements:
 //   DO MEASUREMENT [p] / BEGIN MEASUREMENT  bra::state::measurement (bra/src/state.cpp:1984-2038)
 //   DO AMPLITUDES [i..]                     bra::nompi_state::do_amplitudes (bra/src/nompi_state.cpp:1108-1133)
+//   GENERATE EVENTS n seed                  bra::state::generate_events (bra/src/state.cpp:2064-2092)
+//   EXIT                                    bra::state::exit (bra/src/state.cpp:2094-2120)
 // each framed by the "... finished: <seconds since start> (<seconds since last>)"
-// lines.  --threads and --seed are accepted for command-line compatibility;
-// the program uses no random numbers.
+// lines; M, CLEAR and SET print nothing.  --seed seeds the state's
+// std::mt19937_64 (default 1, bra/src/bra.cpp:73); --threads is accepted for
+// command-line compatibility.
 #include <chrono>
 #include <cstdio>
 #include <fstream>
@@ -87,6 +90,22 @@ void amplitudes(brk::bra::hip_state& st, const std::vector<std::uint64_t>& indic
   std::cout << oss.str() << t.lap("Amplitudes") << std::flush;
 }
 
+// "Events:" then "<index> <bits>" per event (bra/src/state.cpp:2078-2081)
+void generate_events(brk::bra::hip_state& st, int num_events, int seed, timer& t) {
+  std::cout << t.lap("Operations") << std::flush;
+  const auto events = st.do_generate_events(num_events, seed);
+  std::ostringstream oss;
+  oss << "Events:\n";
+  for (size_t i = 0; i < events.size(); ++i) oss << i << ' ' << bits_string(events[i], st.num_qubits()) << '\n';
+  std::cout << oss.str() << t.lap("Events") << std::flush;
+}
+
+void exit_statement(brk::bra::hip_state& st, timer& t) {
+  std::cout << t.lap("Operations") << std::flush;
+  const std::uint64_t value = st.do_measure();
+  std::cout << "Measurement result: " << value << '\n' << t.lap("Measurement") << std::flush;
+}
+
 int usage() {
   std::cerr << "usage: brk-bra [--file <path>] [--threads <n>] [--seed <n>]\n";
   return 2;
@@ -96,11 +115,20 @@ int usage() {
 
 int main(int argc, char** argv) {
   std::string file;
+  brk::bra::hip_state::seed_type seed = 1;
   for (int i = 1; i < argc; ++i) {
     const std::string a = argv[i];
     if ((a == "--file" || a == "-f") && i + 1 < argc) file = argv[++i];
-    else if ((a == "--threads" || a == "--seed") && i + 1 < argc) ++i;
-    else return usage();
+    else if (a == "--threads" && i + 1 < argc) ++i;
+    else if (a == "--seed" && i + 1 < argc) {
+      try {
+        size_t used = 0;
+        seed = std::stoull(argv[++i], &used);
+        if (argv[i][used] != '\0') return usage();
+      } catch (const std::exception&) {
+        return usage();
+      }
+    } else return usage();
   }
   std::string text;
   if (file.empty()) {
@@ -127,7 +155,7 @@ int main(int argc, char** argv) {
     brk::bra::check(brk_qcx_initial_state(prog, &initial));
     brk::bra::check(brk_qcx_num_statements(prog, &count));
     timer t;
-    brk::bra::hip_state st(n, initial);
+    brk::bra::hip_state st(n, initial, seed);
     for (unsigned i = 0; i < count; ++i) {
       int kind = 0, precision = 3;
       const brk_circuit* gates = nullptr;
@@ -138,6 +166,11 @@ int main(int argc, char** argv) {
         case BRK_QCX_GATES: st.apply(gates); break;
         case BRK_QCX_MEASUREMENT: measurement(st, precision, t); break;
         case BRK_QCX_AMPLITUDES: amplitudes(st, std::vector<std::uint64_t>(idx, idx + nidx), t); break;
+        case BRK_QCX_PROJECTIVE_MEASUREMENT: st.do_projective_measurement((unsigned)idx[0]); break;
+        case BRK_QCX_CLEAR: st.do_clear((unsigned)idx[0]); break;
+        case BRK_QCX_SET: st.do_set((unsigned)idx[0]); break;
+        case BRK_QCX_GENERATE_EVENTS: generate_events(st, (int)idx[0], precision, t); break;
+        case BRK_QCX_EXIT: exit_statement(st, t); break;
         default: break;  // fusion brackets: the gates inside run in order either way
       }
     }

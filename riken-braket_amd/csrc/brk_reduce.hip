@@ -11,6 +11,7 @@
 // values use long double, spin_expectation_value.hpp:31-64).
 #include "brk_internal.hpp"
 
+#include <algorithm>
 #include <complex>
 #include <vector>
 
@@ -157,6 +158,147 @@ __global__ __launch_bounds__(256) void k_apply_ising(cpx<R>* __restrict__ out, c
     im += args.h[i] * (double)y.im;
   }
   out[k] = cpx<R>{(R)re, (R)im};
+}
+
+// P(qubit = 0) and P(qubit = 1): sums of |a|^2 over the two halves of the pairs
+// (projective_measurement.hpp:36-72, clear.hpp zero_probability).
+template <typename R>
+__global__ __launch_bounds__(RED_THREADS) void k_zero_one(const cpx<R>* __restrict__ a, uint64_t npairs, uint64_t lower,
+                                                         uint64_t qmask, double* partial) {
+  dvec<2> acc{{0.0, 0.0}};
+  const uint64_t stride = (uint64_t)RED_BLOCKS * RED_THREADS;
+  for (uint64_t g = (uint64_t)blockIdx.x * RED_THREADS + threadIdx.x; g < npairs; g += stride) {
+    const uint64_t i0 = ((g & ~lower) << 1) | (g & lower);
+    const cpx<R> x0 = a[i0], x1 = a[i0 | qmask];
+    acc.v[0] += (double)x0.re * x0.re + (double)x0.im * x0.im;
+    acc.v[1] += (double)x1.re * x1.re + (double)x1.im * x1.im;
+  }
+  block_store<2>(acc, partial);
+}
+
+// Keeps the half of the pairs where the qubit equals `value`, scaled by m, and
+// sets the other half to +0 (change_state_after_measuring_zero/one,
+// projective_measurement.hpp:74-150; clear.hpp; set.hpp).
+template <typename R>
+__global__ __launch_bounds__(256) void k_project(cpx<R>* __restrict__ a, uint64_t npairs, uint64_t lower,
+                                                 uint64_t qmask, int value, R m) {
+  const uint64_t g = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (g >= npairs) return;
+  const uint64_t i0 = ((g & ~lower) << 1) | (g & lower);
+  const uint64_t keep = value ? (i0 | qmask) : i0, drop = value ? i0 : (i0 | qmask);
+  const cpx<R> x = a[keep];
+  a[keep] = cpx<R>{x.re * m, x.im * m};
+  a[drop] = cpx<R>{R(0), R(0)};
+}
+
+// Sampling (generate_events.hpp:15-48).  The cumulative probability of basis
+// index i is a nested left fold: the state is cut into chunks of SAMPLE_CHUNK
+// amplitudes, a chunk into 16 waves, a wave into 64 threads and a thread into
+// `per` consecutive amplitudes.  With S the running sum over whole chunks,
+// W over the waves of the chunk, L over the threads of the wave and c over the
+// thread's own amplitudes (each summed left to right from 0.0),
+//   C(i) = S + (W + (L + c)).
+// The last value of each level equals the next level's running sum exactly,
+// so C is monotone and continuous across chunk, wave and thread boundaries:
+// an index whose probability is zero repeats its predecessor's C and is
+// never the first to exceed a target.  It matches a sequential scan up to
+// rounding.
+constexpr int SAMPLE_CHUNK = 1 << 13;
+constexpr int SAMPLE_THREADS = 1024;
+constexpr int SAMPLE_WAVES = SAMPLE_THREADS / 64;
+
+__device__ inline double readlane_f64(double v, int lane) {
+  const uint64_t b = (uint64_t)__double_as_longlong(v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, lane);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), lane);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// Coalesced load of one chunk's |a|^2 into LDS.
+template <typename R>
+__device__ inline void load_norms(const cpx<R>* __restrict__ a, uint64_t base, uint32_t chunk, double* nrm) {
+  for (uint32_t j = threadIdx.x; j < chunk; j += SAMPLE_THREADS) {
+    const cpx<R> x = a[base + j];
+    nrm[j] = (double)x.re * x.re + (double)x.im * x.im;
+  }
+}
+
+struct chunk_fold {
+  double lane_before;  // L before this thread
+  double wave_before;  // W before this wave
+  double total;        // the chunk's sum
+};
+
+// Called by every thread after the norms are in LDS (and a barrier).
+__device__ inline chunk_fold fold_chunk(const double* nrm, uint32_t chunk, uint32_t per, double* wave_sums) {
+  const uint32_t first = threadIdx.x * per;
+  double t = 0.0;
+  for (uint32_t k = 0; k < per; ++k)
+    if (first + k < chunk) t += nrm[first + k];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double run = 0.0, before = 0.0;
+  for (int k = 0; k < 64; ++k) {
+    if (k == lane) before = run;
+    run += readlane_f64(t, k);
+  }
+  if (lane == 0) wave_sums[wave] = run;
+  __syncthreads();
+  chunk_fold f{before, 0.0, 0.0};
+  for (int k = 0; k < SAMPLE_WAVES; ++k) {
+    if (k == wave) f.wave_before = f.total;
+    f.total += wave_sums[k];
+  }
+  return f;
+}
+
+template <typename R>
+__global__ __launch_bounds__(SAMPLE_THREADS) void k_chunk_sums(const cpx<R>* __restrict__ a, uint32_t chunk,
+                                                              double* __restrict__ sums) {
+  __shared__ double nrm[SAMPLE_CHUNK];
+  __shared__ double wave_sums[SAMPLE_WAVES];
+  const uint32_t per = chunk >= SAMPLE_THREADS ? chunk / SAMPLE_THREADS : 1;
+  load_norms(a, (uint64_t)blockIdx.x * chunk, chunk, nrm);
+  __syncthreads();
+  const chunk_fold f = fold_chunk(nrm, chunk, per, wave_sums);
+  if (threadIdx.x == 0) sums[blockIdx.x] = f.total;
+}
+
+// One block per event: the first index of chunk chunk_of[e] whose C exceeds
+// target[e] (chunk_start[e] is S before the chunk).  When none does (a target
+// at the total, which the host only sends for the last chunk holding
+// probability), the chunk's last index of non-zero probability.
+template <typename R>
+__global__ __launch_bounds__(SAMPLE_THREADS) void k_chunk_search(const cpx<R>* __restrict__ a, uint32_t chunk,
+                                                                const uint64_t* __restrict__ chunk_of,
+                                                                const double* __restrict__ chunk_start,
+                                                                const double* __restrict__ target,
+                                                                uint64_t* __restrict__ events) {
+  __shared__ double nrm[SAMPLE_CHUNK];
+  __shared__ double wave_sums[SAMPLE_WAVES];
+  __shared__ unsigned found, last_nonzero;
+  const uint64_t base = chunk_of[blockIdx.x] * chunk;
+  const double S = chunk_start[blockIdx.x], r = target[blockIdx.x];
+  const uint32_t per = chunk >= SAMPLE_THREADS ? chunk / SAMPLE_THREADS : 1;
+  if (threadIdx.x == 0) {
+    found = ~0u;
+    last_nonzero = 0u;
+  }
+  load_norms(a, base, chunk, nrm);
+  __syncthreads();
+  const chunk_fold f = fold_chunk(nrm, chunk, per, wave_sums);
+  const uint32_t first = threadIdx.x * per;
+  double c = 0.0;
+  for (uint32_t k = 0; k < per && first + k < chunk; ++k) {
+    const double p = nrm[first + k];
+    c += p;
+    if (p > 0.0) atomicMax(&last_nonzero, first + k);
+    if (S + (f.wave_before + (f.lane_before + c)) > r) {
+      atomicMin(&found, first + k);
+      break;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) events[blockIdx.x] = base + (found != ~0u ? found : last_nonzero);
 }
 
 // Launches `launch(partial)` on the fixed reduction grid and sums the NV
@@ -363,6 +505,124 @@ int brk_apply_ising(void* out, const void* psi, uint64_t size, int dtype, const 
     hipLaunchKernelGGL(k_apply_ising<float>, dim3((unsigned)blocks), dim3(256), 0, as_stream(stream),
                        (cpx<float>*)out, (const cpx<float>*)psi, size, a);
   return check_launch("apply_ising");
+}
+
+int brk_zero_one_probabilities(const void* first, uint64_t size, int dtype, unsigned qubit, double result[2],
+                               void* stream) {
+  unsigned nq;
+  if (int st = check_state(first, size, dtype, &nq)) return st;
+  if (!result) return fail(BRK_ERR_INVALID_ARGUMENT, "result is null");
+  if (qubit >= nq) return fail(BRK_ERR_QUBIT_OUT_OF_RANGE, "qubit %u out of range for %u qubits", qubit, nq);
+  const uint64_t qmask = 1ull << qubit, lower = qmask - 1ull, npairs = size >> 1;
+  return run_reduction<2>(
+      stream,
+      [&](double* partial, hipStream_t s) {
+        if (dtype == BRK_C128)
+          hipLaunchKernelGGL(k_zero_one<double>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s,
+                             (const cpx<double>*)first, npairs, lower, qmask, partial);
+        else
+          hipLaunchKernelGGL(k_zero_one<float>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s,
+                             (const cpx<float>*)first, npairs, lower, qmask, partial);
+      },
+      result);
+}
+
+int brk_project_qubit(void* first, uint64_t size, int dtype, unsigned qubit, int value, double multiplier,
+                      void* stream) {
+  unsigned nq;
+  if (int st = check_state(first, size, dtype, &nq)) return st;
+  if (qubit >= nq) return fail(BRK_ERR_QUBIT_OUT_OF_RANGE, "qubit %u out of range for %u qubits", qubit, nq);
+  if (value != 0 && value != 1) return fail(BRK_ERR_INVALID_ARGUMENT, "value must be 0 or 1");
+  const uint64_t qmask = 1ull << qubit, lower = qmask - 1ull, npairs = size >> 1;
+  const uint64_t blocks = (npairs + 255) / 256;
+  if (blocks > 0xffffffffull) return fail(BRK_ERR_INVALID_ARGUMENT, "state too large");
+  if (dtype == BRK_C128)
+    hipLaunchKernelGGL(k_project<double>, dim3((unsigned)blocks), dim3(256), 0, as_stream(stream), (cpx<double>*)first,
+                       npairs, lower, qmask, value, multiplier);
+  else
+    hipLaunchKernelGGL(k_project<float>, dim3((unsigned)blocks), dim3(256), 0, as_stream(stream), (cpx<float>*)first,
+                       npairs, lower, qmask, value, (float)multiplier);
+  return check_launch("project");
+}
+
+int brk_generate_events(const void* first, uint64_t size, int dtype, const double* canonical, unsigned num_events,
+                        uint64_t* events, void* stream) {
+  unsigned nq;
+  if (int st = check_state(first, size, dtype, &nq)) return st;
+  if (num_events == 0) return BRK_OK;
+  if (!canonical || !events) return fail(BRK_ERR_INVALID_ARGUMENT, "null argument");
+  for (unsigned e = 0; e < num_events; ++e)
+    if (!(canonical[e] >= 0.0 && canonical[e] < 1.0))
+      return fail(BRK_ERR_INVALID_ARGUMENT, "canonical[%u] = %g is not in [0, 1)", e, canonical[e]);
+  const uint32_t chunk = (uint32_t)std::min<uint64_t>(size, SAMPLE_CHUNK);
+  const uint64_t nchunks = size / chunk;
+  if (nchunks > 0x7fffffffull) return fail(BRK_ERR_INVALID_ARGUMENT, "state too large");
+  hipStream_t s = as_stream(stream);
+  // device buffers: chunk sums | per event: chunk, chunk start, target, result
+  const size_t bytes = sizeof(double) * nchunks + (sizeof(uint64_t) * 2 + sizeof(double) * 2) * num_events;
+  void* scratch = nullptr;
+  BRK_HIP_CHECK(hipMalloc(&scratch, bytes));
+  double* d_sums = static_cast<double*>(scratch);
+  uint64_t* d_chunk = reinterpret_cast<uint64_t*>(d_sums + nchunks);
+  double* d_start = reinterpret_cast<double*>(d_chunk + num_events);
+  double* d_target = d_start + num_events;
+  uint64_t* d_events = reinterpret_cast<uint64_t*>(d_target + num_events);
+  if (dtype == BRK_C128)
+    hipLaunchKernelGGL(k_chunk_sums<double>, dim3((unsigned)nchunks), dim3(SAMPLE_THREADS), 0, s,
+                       (const cpx<double>*)first, chunk, d_sums);
+  else
+    hipLaunchKernelGGL(k_chunk_sums<float>, dim3((unsigned)nchunks), dim3(SAMPLE_THREADS), 0, s,
+                       (const cpx<float>*)first, chunk, d_sums);
+  std::vector<double> sums(nchunks);
+  int status = check_launch("chunk sums");
+  if (!status && (hipMemcpyAsync(sums.data(), d_sums, sizeof(double) * nchunks, hipMemcpyDeviceToHost, s) != hipSuccess ||
+                  hipStreamSynchronize(s) != hipSuccess))
+    status = fail(BRK_ERR_HIP, "copying the chunk sums failed");
+  std::vector<double> S(nchunks);  // running sum through each chunk, left to right
+  double run = 0.0;
+  for (uint64_t c = 0; c < nchunks; ++c) S[c] = run = run + sums[c];
+  uint64_t last_chunk = nchunks;  // the last chunk holding probability
+  for (uint64_t c = nchunks; c-- > 0;)
+    if (sums[c] > 0.0) {
+      last_chunk = c;
+      break;
+    }
+  if (!status && last_chunk == nchunks) status = fail(BRK_ERR_INVALID_ARGUMENT, "the state has zero norm");
+  if (!status) {
+    const double total = S[nchunks - 1];
+    std::vector<uint64_t> chunk_of(num_events);
+    std::vector<double> start(num_events), target(num_events);
+    for (unsigned e = 0; e < num_events; ++e) {
+      target[e] = canonical[e] * total;
+      uint64_t c = (uint64_t)(std::upper_bound(S.begin(), S.end(), target[e]) - S.begin());
+      if (c >= nchunks) c = last_chunk;
+      chunk_of[e] = c;
+      start[e] = c ? S[c - 1] : 0.0;
+    }
+    if (hipMemcpyAsync(d_chunk, chunk_of.data(), sizeof(uint64_t) * num_events, hipMemcpyHostToDevice, s) !=
+            hipSuccess ||
+        hipMemcpyAsync(d_start, start.data(), sizeof(double) * num_events, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(d_target, target.data(), sizeof(double) * num_events, hipMemcpyHostToDevice, s) != hipSuccess)
+      status = fail(BRK_ERR_HIP, "uploading the event chunks failed");
+    // the host vectors must outlive the copies
+    if (!status && hipStreamSynchronize(s) != hipSuccess) status = fail(BRK_ERR_HIP, "upload failed");
+  }
+  if (!status) {
+    if (dtype == BRK_C128)
+      hipLaunchKernelGGL(k_chunk_search<double>, dim3(num_events), dim3(SAMPLE_THREADS), 0, s,
+                         (const cpx<double>*)first, chunk, d_chunk, d_start, d_target, d_events);
+    else
+      hipLaunchKernelGGL(k_chunk_search<float>, dim3(num_events), dim3(SAMPLE_THREADS), 0, s,
+                         (const cpx<float>*)first, chunk, d_chunk, d_start, d_target, d_events);
+    status = check_launch("chunk search");
+  }
+  if (!status && (hipMemcpyAsync(events, d_events, sizeof(uint64_t) * num_events, hipMemcpyDeviceToHost, s) !=
+                      hipSuccess ||
+                  hipStreamSynchronize(s) != hipSuccess))
+    status = fail(BRK_ERR_HIP, "copying the events failed");
+  (void)hipStreamSynchronize(s);
+  (void)hipFree(scratch);
+  return status;
 }
 
 }  // extern "C"

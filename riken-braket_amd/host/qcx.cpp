@@ -74,6 +74,7 @@ class parser {
   explicit parser(brk_qcx& p) : prog_(p) {}
 
   void line(size_t number, const std::string& raw) {
+    if (stopped_) return;  // bra stops reading after GENERATE EVENTS and EXIT (interpreter.cpp:579-590, 764-784)
     line_ = number;
     std::string text = raw.substr(0, raw.find('!'));
     std::istringstream in(text);
@@ -96,6 +97,15 @@ class parser {
     if (m == "BEGIN") return begin_statement();
     if (m == "END") return end_statement();
     need_qubits();
+    if (m == "M" || m == "CLEAR" || m == "SET") return single_qubit_statement(m);
+    if (m == "GENERATE") return generate_statement();
+    if (m == "EXIT") {
+      expect_columns(1);
+      not_in_fusion("EXIT");
+      directive(BRK_QCX_EXIT);
+      stopped_ = true;
+      return;
+    }
     gate(m);
   }
 
@@ -106,6 +116,7 @@ class parser {
   size_t line_ = 0;
   std::vector<std::string> cols_;
   brk_circuit* current_ = nullptr;
+  bool in_fusion_ = false, stopped_ = false;
 
   [[noreturn]] void error(const std::string& what) const {
     throw parse_error{BRK_ERR_INVALID_ARGUMENT, "line " + std::to_string(line_) + ": " + what};
@@ -198,18 +209,49 @@ class parser {
       unsupported("DO " + what);
     }
   }
+  // bra::state rejects these between BEGIN FUSION and END FUSION (bra/src/state.cpp:1973-1983, 2064-2067, 2219-2238)
+  void not_in_fusion(const std::string& what) const {
+    if (in_fusion_) unsupported(what + " inside BEGIN FUSION / END FUSION");
+  }
+  // M, CLEAR, SET <qubit> (interpreter.cpp:3199-3206, 3335-3339)
+  void single_qubit_statement(const std::string& m) {
+    expect_columns(2);
+    not_in_fusion(m);
+    const long long q = integer(1);
+    if (q < 0 || q >= (long long)prog_.num_qubits) error("qubit " + cols_[1] + " out of range");
+    directive(m == "M" ? BRK_QCX_PROJECTIVE_MEASUREMENT : m == "CLEAR" ? BRK_QCX_CLEAR : BRK_QCX_SET, 3,
+              {(uint64_t)q});
+  }
+  // GENERATE EVENTS <num_events> <seed> (interpreter.cpp:764-784, 1875-1887)
+  void generate_statement() {
+    if (cols_.size() != 4 || upper(cols_[1]) != "EVENTS") error("expected GENERATE EVENTS <num_events> <seed>");
+    not_in_fusion("GENERATE EVENTS");
+    const long long n = integer(2), seed = integer(3);
+    if (n < 0 || n > 0x7fffffffll) error("number of events out of range");
+    if (seed < -0x80000000ll || seed > 0x7fffffffll) error("seed out of range");
+    directive(BRK_QCX_GENERATE_EVENTS, (int)seed, {(uint64_t)n});
+    stopped_ = true;
+  }
   void begin_statement() {
     if (cols_.size() < 2) error("BEGIN needs a statement");
     const std::string what = upper(cols_[1]);
     need_qubits();
     if (what == "MEASUREMENT") directive(BRK_QCX_MEASUREMENT, 3);
-    else if (what == "FUSION") directive(BRK_QCX_BEGIN_FUSION);
+    else if (what == "FUSION") {
+      if (in_fusion_) error("BEGIN FUSION inside BEGIN FUSION");
+      in_fusion_ = true;
+      directive(BRK_QCX_BEGIN_FUSION);
+    }
     else unsupported("BEGIN " + what);
   }
   void end_statement() {
     if (cols_.size() < 2) error("END needs a statement");
     const std::string what = upper(cols_[1]);
-    if (what == "FUSION") directive(BRK_QCX_END_FUSION);
+    if (what == "FUSION") {
+      if (!in_fusion_) error("END FUSION without BEGIN FUSION");
+      in_fusion_ = false;
+      directive(BRK_QCX_END_FUSION);
+    }
     else unsupported("END " + what);
   }
 

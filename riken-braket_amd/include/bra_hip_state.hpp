@@ -11,8 +11,11 @@
 #pragma once
 
 #include <array>
+#include <cmath>
 #include <complex>
 #include <cstdint>
+#include <limits>
+#include <random>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -37,10 +40,14 @@ inline void check(int status) {
 class hip_state {
  public:
   using complex_type = std::complex<double>;
+  // bra's generator and seed types (bra/include/bra/state.hpp:459-460)
+  using random_number_generator_type = std::mt19937_64;
+  using seed_type = random_number_generator_type::result_type;
 
-  // nompi_state(initial_integer, total_num_qubits, ...) (bra/src/nompi_state.cpp:102-117)
-  hip_state(unsigned num_qubits, std::uint64_t initial_integer, void* stream = nullptr)
-      : n_(num_qubits), size_(std::uint64_t{1} << num_qubits), stream_(stream) {
+  // nompi_state(initial_integer, total_num_qubits, ..., seed, ...) (bra/src/nompi_state.cpp:102-117);
+  // bra's --seed defaults to 1 (bra/src/bra.cpp:73)
+  hip_state(unsigned num_qubits, std::uint64_t initial_integer, seed_type seed = 1, void* stream = nullptr)
+      : n_(num_qubits), size_(std::uint64_t{1} << num_qubits), stream_(stream), rng_(seed) {
     check(brk_malloc(&data_, size_ * sizeof(complex_type)));
     check(brk_set_basis_state(data_, size_, BRK_C128, initial_integer, stream_));
   }
@@ -114,6 +121,56 @@ class hip_state {
   // bra/src/interpreter.cpp:851-872); nompi mode applies gates at their own qubits
   void apply(const brk_circuit* circuit) { check(brk_circuit_run(circuit, data_, BRK_C128, nullptr, stream_)); }
 
+  // --- measurement and sampling ----------------------------------------------
+  // ket::gate::projective_measurement (ket/include/ket/gate/projective_measurement.hpp:137-176):
+  // one draw u * (P0 + P1) decides the outcome; the kept half is scaled by P^(-1/2).
+  int do_projective_measurement(unsigned qubit) {
+    double p[2];
+    check(brk_zero_one_probabilities(data_, size_, BRK_C128, qubit, p, stream_));
+    const double r = std::uniform_real_distribution<double>{0.0, p[0] + p[1]}(rng_);
+    const int outcome = r < p[0] ? 0 : 1;
+    check(brk_project_qubit(data_, size_, BRK_C128, qubit, outcome, std::pow(p[outcome], -0.5), stream_));
+    return outcome;
+  }
+  // ket::gate::clear / ket::gate::set (clear.hpp:85-108, set.hpp:88-111): project onto |0> / |1>
+  void do_clear(unsigned qubit) {
+    double p[2];
+    check(brk_zero_one_probabilities(data_, size_, BRK_C128, qubit, p, stream_));
+    check(brk_project_qubit(data_, size_, BRK_C128, qubit, 0, std::pow(p[0], -0.5), stream_));
+  }
+  void do_set(unsigned qubit) {
+    double p[2];
+    check(brk_zero_one_probabilities(data_, size_, BRK_C128, qubit, p, stream_));
+    check(brk_project_qubit(data_, size_, BRK_C128, qubit, 1, std::pow(p[1], -0.5), stream_));
+  }
+  // ket::generate_events (generate_events.hpp:15-66) as bra::nompi_state::do_generate_events
+  // calls it (nompi_state.cpp:1143-1153): a negative seed continues the state's generator,
+  // any other seeds a fresh one.  Each event takes one draw; the state is left unchanged.
+  std::vector<std::uint64_t> do_generate_events(int num_events, int seed) {
+    if (num_events <= 0) return {};
+    std::vector<double> canonical(num_events);
+    if (seed < 0) {
+      draw_canonical(rng_, canonical);
+    } else {
+      random_number_generator_type fresh(static_cast<seed_type>(seed));
+      draw_canonical(fresh, canonical);
+    }
+    std::vector<std::uint64_t> events(num_events);
+    check(brk_generate_events(data_, size_, BRK_C128, canonical.data(), (unsigned)num_events, events.data(),
+                              stream_));
+    return events;
+  }
+  // ket::measure (ket/include/ket/measure.hpp:14-40): sample one basis state and collapse onto it
+  std::uint64_t do_measure() {
+    std::vector<double> canonical(1);
+    draw_canonical(rng_, canonical);
+    std::uint64_t index = 0;
+    check(brk_generate_events(data_, size_, BRK_C128, canonical.data(), 1u, &index, stream_));
+    check(brk_set_basis_state(data_, size_, BRK_C128, index, stream_));
+    return index;
+  }
+  random_number_generator_type& random_number_generator() { return rng_; }
+
   // --- observables ----------------------------------------------------------
   // ket::all_spin_expectation_values (ket/include/ket/all_spin_expectation_values.hpp:22-42)
   std::vector<std::array<double, 3>> do_expectation_values() const {
@@ -137,9 +194,16 @@ class hip_state {
   }
 
  private:
+  // std::uniform_real_distribution<double>{0, total} draws generate_canonical * total; the
+  // device applies the total, so only the canonical part is drawn here.
+  static void draw_canonical(random_number_generator_type& g, std::vector<double>& out) {
+    for (auto& u : out) u = std::generate_canonical<double, std::numeric_limits<double>::digits>(g);
+  }
+
   unsigned n_;
   std::uint64_t size_;
   void* stream_;
+  random_number_generator_type rng_;
   void* data_ = nullptr;
 };
 

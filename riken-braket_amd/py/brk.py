@@ -80,6 +80,9 @@ _SIGS = {
     "brk_expectation_pauli_string": [_p, _u64, _i, C.c_char_p, _up, _u, _dp, _p],
     "brk_expectation_ising": [_p, _u64, _i, _dp, _dp, _dp, _p],
     "brk_apply_ising": [_p, _p, _u64, _i, _dp, _dp, _p],
+    "brk_zero_one_probabilities": [_p, _u64, _i, _u, _dp, _p],
+    "brk_project_qubit": [_p, _u64, _i, _u, _i, _d, _p],
+    "brk_generate_events": [_p, _u64, _i, _dp, _u, C.POINTER(C.c_uint64), _p],
     "brk_circuit_create": [_u, C.POINTER(C.c_void_p)],
     "brk_circuit_destroy": [_p],
     "brk_circuit_add": [_p, _i, _up, _u, _up, _u, _dp, _u],
@@ -298,6 +301,26 @@ def spin_expectation_value(ptr, size, dtype, qubit, stream=0):
     out = (C.c_double * 3)()
     call("brk_spin_expectation_value", ptr, size, dtype, qubit, out, stream)
     return tuple(out)
+
+
+def zero_one_probabilities(ptr, size, dtype, qubit, stream=0):
+    out = (C.c_double * 2)()
+    call("brk_zero_one_probabilities", ptr, size, dtype, qubit, out, stream)
+    return out[0], out[1]
+
+
+def project_qubit(ptr, size, dtype, qubit, value, multiplier, stream=0):
+    call("brk_project_qubit", ptr, size, dtype, qubit, value, multiplier, stream)
+
+
+def generate_events(ptr, size, dtype, canonical, stream=0):
+    """canonical: sequence of numbers in [0, 1); returns a list of basis indices."""
+    canonical = list(canonical)
+    n = len(canonical)
+    c = (C.c_double * max(n, 1))(*canonical)
+    out = (C.c_uint64 * max(n, 1))()
+    call("brk_generate_events", ptr, size, dtype, c, n, out, stream)
+    return list(out)[:n]
 
 
 def expectation_pauli_string(ptr, size, dtype, pauli, qubits, stream=0):
@@ -524,6 +547,7 @@ def circuit_ops(handle):
 
 
 QCX_GATES, QCX_MEASUREMENT, QCX_AMPLITUDES, QCX_BEGIN_FUSION, QCX_END_FUSION = 0, 1, 2, 3, 4
+QCX_PROJECTIVE_MEASUREMENT, QCX_CLEAR, QCX_SET, QCX_GENERATE_EVENTS, QCX_EXIT = 5, 6, 7, 8, 9
 
 
 class Qcx:
@@ -551,6 +575,10 @@ class Qcx:
                 self.statements.append((kind.value, prec.value))
             elif kind.value == QCX_AMPLITUDES:
                 self.statements.append((kind.value, [idx[k] for k in range(nidx.value)]))
+            elif kind.value in (QCX_PROJECTIVE_MEASUREMENT, QCX_CLEAR, QCX_SET):
+                self.statements.append((kind.value, idx[0]))
+            elif kind.value == QCX_GENERATE_EVENTS:
+                self.statements.append((kind.value, (idx[0], prec.value)))  # (num_events, seed)
             else:
                 self.statements.append((kind.value, None))
 

@@ -47,7 +47,18 @@ _SIGS = {
     "or_expectation_pauli_string": ([_p, _u64, C.c_char_p, _up, _u, C.POINTER(C.c_double)], None),
     "or_expectation_ising": ([_p, _u64, C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_double),
     "or_apply_ising": ([_p, _p, _u64, C.POINTER(C.c_double), C.POINTER(C.c_double)], None),
+    "or_mt64_seed": ([_p, _u64], None),
+    "or_mt64_next": ([_p], C.c_uint64),
+    "or_mt64_canonical": ([_p], C.c_double),
+    "or_zero_one_probabilities": ([_p, _u64, _u, C.POINTER(C.c_double)], None),
+    "or_project": ([_p, _u64, _u, C.c_int, _d], None),
+    "or_projective_measurement": ([_p, _u64, _u, _d], C.c_int),
+    "or_generate_events": ([_p, _u64, C.POINTER(C.c_double), _u, C.POINTER(C.c_uint64)], None),
 }
+
+
+class _Mt64State(C.Structure):
+    _fields_ = [("mt", C.c_uint64 * 312), ("index", C.c_uint)]
 
 _libs = {}
 
@@ -217,4 +228,40 @@ class Oracle:
     def apply_ising(self, psi, J, h):
         out = np.empty_like(psi)
         self.lib.or_apply_ising(_ptr(out), _ptr(psi), psi.size, _da(J), _da(h))
+        return out
+
+    # --- measurement and sampling -------------------------------------------
+    def mt64(self, seed):
+        """A std::mt19937_64 seeded with `seed` (an object with next() and canonical())."""
+        lib = self.lib
+
+        class Mt64:
+            def __init__(self):
+                self.state = _Mt64State()
+                lib.or_mt64_seed(C.byref(self.state), seed)
+
+            def next(self):
+                return lib.or_mt64_next(C.byref(self.state))
+
+            def canonical(self):
+                return lib.or_mt64_canonical(C.byref(self.state))
+
+        return Mt64()
+
+    def zero_one_probabilities(self, s, qubit):
+        out = (C.c_double * 2)()
+        self.lib.or_zero_one_probabilities(_ptr(s), s.size, qubit, out)
+        return out[0], out[1]
+
+    def project(self, s, qubit, value, multiplier):
+        self.lib.or_project(_ptr(s), s.size, qubit, value, multiplier)
+
+    def projective_measurement(self, s, qubit, canonical):
+        return self.lib.or_projective_measurement(_ptr(s), s.size, qubit, canonical)
+
+    def generate_events(self, s, canonical):
+        canonical = np.ascontiguousarray(canonical, dtype=np.float64)
+        out = np.zeros(canonical.size, dtype=np.uint64)
+        self.lib.or_generate_events(_ptr(s), s.size, canonical.ctypes.data_as(C.POINTER(C.c_double)), canonical.size,
+                                    out.ctypes.data_as(C.POINTER(C.c_uint64)))
         return out

@@ -18,7 +18,7 @@ pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "riken-braket_amd", "brk-bra")
 GOLDEN = os.path.join(ROOT, "tests", "golden", "qcx")
-TIMING = re.compile(r"^(Operations|Expectation values|Amplitudes) finished: [0-9.e+-]+ \([0-9.e+-]+\)$")
+TIMING = re.compile(r"^(Operations|Expectation values|Amplitudes|Events|Measurement) finished: [0-9.e+-]+ \([0-9.e+-]+\)$")
 
 
 def run_bra(text):
@@ -95,3 +95,87 @@ def test_initial_state_and_errors():
     assert [l for l in lines if "=>" in l] == ["101 => 1 + 0 i", "000 => 0 + 0 i"]
     r = subprocess.run([EXE], input="QUBITS 2\nSHORBOX 1 2 3\n", capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "SHORBOX" in r.stderr
+
+
+# --- measurement and sampling statements ----------------------------------------------------
+
+def run_bra_args(text, args):
+    print(f"gpu bra: running brk-bra {' '.join(args)} on {text.splitlines()[0]!r} ...", flush=True)
+    r = subprocess.run([EXE] + args, input=text, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    return r.stdout.splitlines()
+
+
+def replay(oracle, text, seed):
+    """The oracle runs the program's statements with bra's generator (std::mt19937_64(seed)):
+    M / CLEAR / SET / GENERATE EVENTS / EXIT as bra::nompi_state does."""
+    q = brk.Qcx(text)
+    s = np.zeros(1 << q.num_qubits, dtype=np.complex128)
+    s[q.initial_state] = 1
+    g = oracle.mt64(seed)
+    out = {"events": None, "exit": None, "amplitudes": []}
+    for kind, v in q.statements:
+        if kind == brk.QCX_GATES:
+            run_ops(oracle, s, v)
+        elif kind == brk.QCX_PROJECTIVE_MEASUREMENT:
+            oracle.projective_measurement(s, v, g.canonical())
+        elif kind in (brk.QCX_CLEAR, brk.QCX_SET):
+            value = 0 if kind == brk.QCX_CLEAR else 1
+            oracle.project(s, v, value, oracle.zero_one_probabilities(s, v)[value] ** -0.5)
+        elif kind == brk.QCX_AMPLITUDES:
+            out["amplitudes"].append(s.copy())
+        elif kind == brk.QCX_GENERATE_EVENTS:
+            num, ev_seed = v
+            src = g if ev_seed < 0 else oracle.mt64(ev_seed)
+            out["events"] = oracle.generate_events(s, [src.canonical() for _ in range(num)])
+        elif kind == brk.QCX_EXIT:
+            out["exit"] = int(oracle.generate_events(s, [g.canonical()])[0])
+    return q, out
+
+
+def printed_events(lines, n):
+    i = lines.index("Events:")
+    assert TIMING.match(lines[i - 1]) and lines[i - 1].startswith("Operations finished")
+    ev = []
+    for k, line in enumerate(lines[i + 1:]):
+        if line.startswith("Events finished"):
+            break
+        index, bits = line.split()
+        assert int(index) == k and len(bits) == n
+        ev.append(int(bits, 2))
+    return ev
+
+
+MEASURE_PROGRAM = ("QUBITS 7\nINITIAL STATE 9\nH 0\nH 1\nU3 2 0.3 1.1 -0.4\nCNOT 0 3\nEXX 4 5 0.7\nH 6\n"
+                   "CCZ 0 2 4\nSX 5\nM 2\nCLEAR 4\nSET 1\nM 0\nH 4\nDO AMPLITUDES\n")
+
+
+@pytest.mark.parametrize("seed_args,seed", [([], 1), (["--seed", "12345"], 12345)])
+def test_measurement_statements_and_events(oracle, seed_args, seed):
+    text = MEASURE_PROGRAM + "GENERATE EVENTS 300 -1\nH 0\n"
+    lines = run_bra_args(text, seed_args)
+    q, ref = replay(oracle, text, seed)
+    amp = [l for l in lines if "=>" in l]
+    assert len(amp) == 128
+    for k, line in enumerate(amp):
+        bits, rest = line.split(" => ")
+        assert int(bits, 2) == k
+        re_, im_ = rest[:-2].split(" + ")
+        assert abs(complex(float(re_), float(im_)) - ref["amplitudes"][0][k]) < 1e-5
+    ev = printed_events(lines, 7)
+    assert len(ev) == 300
+    assert ev == ref["events"].tolist()
+    assert lines[-1].startswith("Events finished") and TIMING.match(lines[-1])
+
+
+def test_generate_events_fresh_seed_and_exit(oracle):
+    text = MEASURE_PROGRAM + "GENERATE EVENTS 50 77\n"
+    lines = run_bra_args(text, [])
+    _, ref = replay(oracle, text, 1)
+    assert printed_events(lines, 7) == ref["events"].tolist()
+    text = MEASURE_PROGRAM + "EXIT\nGARBAGE AFTER EXIT\n"
+    lines = run_bra_args(text, ["--seed", "5"])
+    _, ref = replay(oracle, text, 5)
+    i = [k for k, l in enumerate(lines) if l.startswith("Measurement result: ")]
+    assert len(i) == 1 and int(lines[i[0]].split(": ")[1]) == ref["exit"]
+    assert lines[i[0] - 1].startswith("Operations finished") and lines[i[0] + 1].startswith("Measurement finished")

@@ -160,7 +160,7 @@ def test_directives_and_comments():
     ("QUBITS 2\nU1 0 pi\n", 1, "not a number"),
     ("QUBITS 2\nH 0 1\n", 1, "takes 1 operands"),
     ("QUBITS 2\nBIT ASSIGNMENT 0 0\n", 1, "repeats"),
-    ("QUBITS 2\nGENERATE EVENTS 10 1\n", 8, "GENERATE"),
+    ("QUBITS 2\nDEPOLARIZING CHANNEL P = 0.1\n", 8, "DEPOLARIZING"),
 ])
 def test_errors(text, status, fragment):
     with pytest.raises(brk.BrkError) as e:
