@@ -191,6 +191,13 @@ int brk_exponential_pauli_z(void* first, uint64_t size, int dtype, double phase,
 int brk_swap(void* first, uint64_t size, int dtype, unsigned qubit1, unsigned qubit2,
              const unsigned* controls, unsigned num_controls, void* stream);
 
+/* exp(i phase SWAP) on qubit1, qubit2 under controls: |00> and |11> times
+   e^{i phase}; (|01>, |10>) mixed by [[cos, i sin], [i sin, cos]].  Replaces
+   ket::gate::exponential_swap (ket/include/ket/gate/exponential_swap.hpp:31-90,
+   controlled form :92-…, wrappers :300-350). */
+int brk_exponential_swap(void* first, uint64_t size, int dtype, double phase, unsigned qubit1, unsigned qubit2,
+                         const unsigned* controls, unsigned num_controls, void* stream);
+
 /* General single-qubit gate: matrix = {m00.re, m00.im, m01.re, m01.im,
    m10.re, m10.im, m11.re, m11.im}, (a0', a1') = M (a0, a1), under controls.
    Replaces the generic driver ket::gate::gate applied with a 2x2 functor
@@ -298,7 +305,8 @@ typedef enum brk_op {
   BRK_OP_SWAP = 13,      /* SWAP(targets[0], targets[1]), controls                 */
   BRK_OP_LOCAL_SWAP = 14,/* data swap of PHYSICAL qubits targets[0], targets[1] and
                             permutate(inverse[t0], inverse[t1]) of the permutation  */
-  BRK_OP_MATRIX = 15     /* 2x2 matrix p0..p7 (as brk_gate_1q) on targets[0]       */
+  BRK_OP_MATRIX = 15,    /* 2x2 matrix p0..p7 (as brk_gate_1q) on targets[0]       */
+  BRK_OP_ESWAP = 16      /* exp(i p0 SWAP) on targets[0], targets[1], controls      */
 } brk_op;
 
 int brk_circuit_create(unsigned num_qubits, brk_circuit** out);
@@ -432,7 +440,8 @@ int brk_batch_run_timed(brk_batch* batch, double* energies, double* elapsed_ms, 
 /* the gates between two directives form one brk_circuit.  Subset:          */
 /*   QUBITS, CIRCUITS 1, INITIAL STATE, BIT ASSIGNMENT (accepted: nompi     */
 /*   mode does not permute), comments after '!', case-insensitive mnemonics;*/
-/*   gates I IC In, H, NOT, X, Y, Z, Xn/XX.., Yn/YY.., Zn/ZZ.., SWAP, S S+,  */
+/*   gates I IC In, H, NOT, X, Y, Z, Xn/XX.., Yn/YY.., Zn/ZZ.., SWAP, ESWAP  */
+/*   ESWAP+, S S+,                                                          */
 /*   SZ SZ+, T T+, U1 U1+, U2 U2+, U3 U3+, R R+, U U+, +X -X +Y -Y, SX SX+,  */
 /*   SY SY+, EX EY EZ (+ adjoints, EXX../EXn forms), TOFFOLI, and every one */
 /*   of them behind a C..C / Cn control prefix (controls come first);       */

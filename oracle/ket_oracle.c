@@ -291,6 +291,32 @@ void or_swap(double* s, uint64_t size, unsigned q1, unsigned q2, const unsigned*
   }
 }
 
+/* exp(i phase SWAP) under controls -- exponential_swap.hpp:31-90: |00>, |11> times
+   e^{i phase}; then a_{10} <- cos a_{10} + i sin a_{01}, a_{01} <- cos a_{01} + i sin a_{10}(old),
+   in the reference's order (qubit1's bit set = "qubit1_on") */
+void or_exponential_swap(double* s, uint64_t size, double phase, unsigned q1, unsigned q2, const unsigned* controls,
+                         unsigned nc) {
+  unsigned ops[64];
+  ops[0] = q1;
+  ops[1] = q2;
+  unsigned k = 2 + join_qubits(ops + 2, controls, nc, NULL, 0);
+  uint64_t im[65];
+  or_make_index_masks(ops, k, im);
+  const uint64_t cm = mask_of(controls, nc);
+  const uint64_t m1 = (uint64_t)1 << q1, m2 = (uint64_t)1 << q2;
+  const cd e = cd_expi(phase), i_sin = cd_make(0.0, e.im);
+  const int64_t n = (int64_t)(size >> k);
+#pragma omp parallel for schedule(static)
+  for (int64_t g = 0; g < n; ++g) {
+    const uint64_t b = base_index((uint64_t)g, im, k) | cm;
+    cd_store(s, b, cd_mul(cd_load(s, b), e));
+    cd_store(s, b | m1 | m2, cd_mul(cd_load(s, b | m1 | m2), e));
+    const cd one = cd_load(s, b | m1), two = cd_load(s, b | m2);
+    cd_store(s, b | m1, cd_add(cd_scale(one, e.re), cd_mul(two, i_sin)));
+    cd_store(s, b | m2, cd_add(cd_scale(two, e.re), cd_mul(one, i_sin)));
+  }
+}
+
 /* ------------------------------------------------------------------------- */
 /* Diagonal gates                                                            */
 /* ------------------------------------------------------------------------- */

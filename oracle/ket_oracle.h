@@ -52,6 +52,8 @@ void or_exponential_pauli_y(double* s, uint64_t size, double phase, const unsign
 void or_exponential_pauli_z(double* s, uint64_t size, double phase, const unsigned* targets, unsigned nt,
                             const unsigned* controls, unsigned nc);
 void or_swap(double* s, uint64_t size, unsigned q1, unsigned q2, const unsigned* controls, unsigned nc);
+void or_exponential_swap(double* s, uint64_t size, double phase, unsigned q1, unsigned q2, const unsigned* controls,
+                         unsigned nc);
 void or_gate_1q(double* s, uint64_t size, unsigned target, const double m[8], const unsigned* controls,
                 unsigned nc);
 

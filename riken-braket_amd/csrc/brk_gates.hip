@@ -34,7 +34,7 @@
 
 namespace brk {
 
-enum pair_mode { MODE_H = 0, MODE_SWAP = 1, MODE_MATRIX = 2 };
+enum pair_mode { MODE_H = 0, MODE_SWAP = 1, MODE_MATRIX = 2, MODE_ESWAP = 3 };
 
 struct gate_args {
   uint64_t m[BRK_MAX_OPERATED_QUBITS + 1];  // expand masks
@@ -89,6 +89,17 @@ __device__ __forceinline__ void pair_math(const gate_args& a, uint64_t i0, cpx<R
   } else if constexpr (MODE == MODE_SWAP) {
     y0 = x1;
     y1 = x0;
+  } else if constexpr (MODE == MODE_ESWAP) {
+    // exp(i theta SWAP) (ket/include/ket/gate/exponential_swap.hpp:31-90): the pair
+    // (i0, i0 ^ flipmask) is (00, 11) when the higher target bit of i0 is 0 -- both
+    // get e^{i theta} -- and (10, 01) otherwise: a' = cos a + i sin b, b' = cos b + i sin a.
+    const cpx<R> e = {(R)a.u[0], (R)a.u[1]};
+    const R c = (R)a.u[2], sn = (R)a.u[3];
+    const bool mix = (i0 & a.signmask) != 0;
+    const cpx<R> m0 = {x0.re * c - x1.im * sn, x0.im * c + x1.re * sn};
+    const cpx<R> m1 = {x1.re * c - x0.im * sn, x1.im * c + x0.re * sn};
+    y0 = csel(mix, m0, cmul(x0, e));
+    y1 = csel(mix, m1, cmul(x1, e));
   } else {
     cpx<R> m00, m01, m10, m11;
     load_matrix(a, m00, m01, m10, m11);
@@ -266,6 +277,7 @@ static void launch_family(int family, int mode, const launch_ctx& c, cpx<R>* s, 
     case FAM_PAIR:
       if (mode == MODE_H) pair_k<R, MODE_H, NT, COND>(c, s, n, a);
       else if (mode == MODE_SWAP) pair_k<R, MODE_SWAP, NT, COND>(c, s, n, a);
+      else if (mode == MODE_ESWAP) pair_k<R, MODE_ESWAP, NT, COND>(c, s, n, a);
       else pair_k<R, MODE_MATRIX, NT, COND>(c, s, n, a);
       break;
     case FAM_LANE:
@@ -602,6 +614,26 @@ int brk_swap(void* first, uint64_t size, int dtype, unsigned qubit1, unsigned qu
   a = make_args(p);
   a.flipmask = (1ull << qubit1) | (1ull << qubit2);
   return launch(dtype, FAM_PAIR, MODE_SWAP, first, size, size >> p.np, a, stream);
+}
+
+int brk_exponential_swap(void* first, uint64_t size, int dtype, double phase, unsigned qubit1, unsigned qubit2,
+                         const unsigned* controls, unsigned num_controls, void* stream) {
+  const unsigned t[2] = {qubit1, qubit2};
+  validated v;
+  if (int st = validate(first, size, dtype, t, 2, controls, num_controls, &v)) return st;
+  const unsigned lo = std::min(qubit1, qubit2), hi = std::max(qubit1, qubit2);
+  plan p;
+  p.insert(lo);
+  for (unsigned i = 0; i < num_controls; ++i) p.control(controls[i], line_bits(dtype));
+  gate_args a = make_args(p);
+  a.flipmask = (1ull << lo) | (1ull << hi);
+  a.signmask = 1ull << hi;
+  // e^{i phase} once on the host, as ket::utility::exp_i (exp_i.hpp:24-29); cos and sin are its parts
+  const std::complex<double> e = std::exp(std::complex<double>{0.0, phase});
+  put(a.u, 0, e);
+  a.u[2] = e.real();
+  a.u[3] = e.imag();
+  return launch(dtype, FAM_PAIR, MODE_ESWAP, first, size, size >> p.np, a, stream);
 }
 
 int brk_swap_permutated_local_qubits(void* first, uint64_t size, int dtype, unsigned permutated_qubit1,

@@ -82,7 +82,7 @@ static brk_circuit::op inverse_op(const brk_circuit::op& o) {
   brk_circuit::op r = o;
   switch (o.code) {
     case BRK_OP_U1: case BRK_OP_EX: case BRK_OP_EY: case BRK_OP_EZ:
-    case BRK_OP_RX: case BRK_OP_RY: case BRK_OP_RZ:
+    case BRK_OP_RX: case BRK_OP_RY: case BRK_OP_RZ: case BRK_OP_ESWAP:
       r.params[0] = -o.params[0];
       break;
     case BRK_OP_U3:

@@ -39,7 +39,7 @@ bool permutation::load(const unsigned* p, unsigned n) {
 static unsigned expected_params(int code) {
   switch (code) {
     case BRK_OP_U1: case BRK_OP_EX: case BRK_OP_EY: case BRK_OP_EZ:
-    case BRK_OP_RX: case BRK_OP_RY: case BRK_OP_RZ: return 1;
+    case BRK_OP_RX: case BRK_OP_RY: case BRK_OP_RZ: case BRK_OP_ESWAP: return 1;
     case BRK_OP_U2: return 2;
     case BRK_OP_U3: return 3;
     case BRK_OP_MATRIX: return 8;
@@ -77,6 +77,7 @@ int apply_physical_op(const brk_circuit::op& o, void* first, uint64_t size, int 
     case BRK_OP_SWAP: return brk_swap(first, size, dtype, t[0], t[1], c, nc, stream);
     case BRK_OP_LOCAL_SWAP: return brk_swap_permutated_local_qubits(first, size, dtype, t[0], t[1], stream);
     case BRK_OP_MATRIX: return brk_gate_1q(first, size, dtype, t[0], p, c, nc, stream);
+    case BRK_OP_ESWAP: return brk_exponential_swap(first, size, dtype, p[0], t[0], t[1], c, nc, stream);
     default: return fail(BRK_ERR_INVALID_ARGUMENT, "unknown op code %d", o.code);
   }
 }
@@ -150,6 +151,12 @@ bool op_unitary(const brk_circuit::op& o, std::vector<unsigned>& qubits, std::ve
     case BRK_OP_RY: pauli_exp(Y, -0.5 * p[0]); break;
     case BRK_OP_RZ: pauli_exp(Z, -0.5 * p[0]); break;
     case BRK_OP_SWAP: core = {1, 0, 0, 0, 0, 0, 1, 0, 0, 1, 0, 0, 0, 0, 0, 1}; break;
+    case BRK_OP_ESWAP: {
+      const cd e = std::exp(I * p[0]), is = I * std::sin(p[0]);
+      const double c = std::cos(p[0]);
+      core = {e, 0, 0, 0, 0, c, is, 0, 0, is, c, 0, 0, 0, 0, e};
+      break;
+    }
     case BRK_OP_MATRIX:
       two_by_two(cd(p[0], p[1]), cd(p[2], p[3]), cd(p[4], p[5]), cd(p[6], p[7]));
       break;
@@ -198,14 +205,14 @@ int brk_circuit_destroy(brk_circuit* circuit) {
 int brk_circuit_add(brk_circuit* circuit, int op, const unsigned* targets, unsigned num_targets,
                     const unsigned* controls, unsigned num_controls, const double* params, unsigned num_params) {
   if (!circuit) return fail(BRK_ERR_INVALID_ARGUMENT, "circuit is null");
-  if (op < BRK_OP_H || op > BRK_OP_MATRIX) return fail(BRK_ERR_INVALID_ARGUMENT, "unknown op code %d", op);
+  if (op < BRK_OP_H || op > BRK_OP_ESWAP) return fail(BRK_ERR_INVALID_ARGUMENT, "unknown op code %d", op);
   if (num_targets == 0 || !targets) return fail(BRK_ERR_INVALID_ARGUMENT, "op %d has no targets", op);
   if (num_controls && !controls) return fail(BRK_ERR_INVALID_ARGUMENT, "controls is null");
   const unsigned need = expected_params(op);
   if (num_params != need || (need && !params))
     return fail(BRK_ERR_INVALID_ARGUMENT, "op %d takes %u parameters, got %u", op, need, num_params);
   const bool single = op == BRK_OP_H || op == BRK_OP_U2 || op == BRK_OP_U3 || op == BRK_OP_MATRIX;
-  const bool pair = op == BRK_OP_SWAP || op == BRK_OP_LOCAL_SWAP;
+  const bool pair = op == BRK_OP_SWAP || op == BRK_OP_LOCAL_SWAP || op == BRK_OP_ESWAP;
   if (single && num_targets != 1) return fail(BRK_ERR_INVALID_ARGUMENT, "op %d takes one target", op);
   if (pair && num_targets != 2) return fail(BRK_ERR_INVALID_ARGUMENT, "op %d takes two targets", op);
   if (op == BRK_OP_LOCAL_SWAP && num_controls)

@@ -360,6 +360,13 @@ class parser {
       operands(nc, 2, c, t);
       return add(BRK_OP_SWAP, t, c);
     }
+    // ESWAP t1 t2 theta = exp(i theta SWAP); ESWAP+ its adjoint (interpreter.cpp:3169-3187)
+    if (base == "ESWAP" || base == "ESWAP+") {
+      expect_columns(4 + nc);
+      operands(nc, 2, c, t);
+      const double theta = real(3 + nc);
+      return add(BRK_OP_ESWAP, t, c, {base == "ESWAP" ? theta : -theta});
+    }
     // phases on control qubits: the controls of the prefix and the operand are all "control" qubits
     struct phase_gate {
       const char* name;

@@ -79,6 +79,10 @@ class hip_state {
     do_pauli_z(q2);
   }
   void do_swap(unsigned q1, unsigned q2) { check(brk_swap(data_, size_, BRK_C128, q1, q2, nullptr, 0, stream_)); }
+  void do_exponential_swap(double phase, unsigned q1, unsigned q2) {
+    check(brk_exponential_swap(data_, size_, BRK_C128, phase, q1, q2, nullptr, 0, stream_));
+  }
+  void do_adj_exponential_swap(double phase, unsigned q1, unsigned q2) { do_exponential_swap(-phase, q1, q2); }
   void do_u1(double phase, unsigned control) {
     check(brk_phase_shift(data_, size_, BRK_C128, phase, &control, 1, stream_));
   }

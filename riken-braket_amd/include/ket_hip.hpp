@@ -322,6 +322,17 @@ inline void swap(hip::policy p, hip::device_iterator<Complex> first, hip::device
                       o.targets[1], hip::detail::ptr_or_null(o.controls, o.nc), o.nc, p.stream));
 }
 
+// exp(i phase SWAP) under controls -- exponential_swap.hpp:31-90, 300-350
+template <typename Complex, typename... Qubits>
+inline void exponential_swap(hip::policy p, hip::device_iterator<Complex> first, hip::device_iterator<Complex> last,
+                             double phase, Qubits... qubits) {
+  auto o = hip::detail::split(qubits...);
+  if (o.nt != 2) throw hip::error(BRK_ERR_INVALID_ARGUMENT, "exponential_swap takes two target qubits");
+  hip::check(brk_exponential_swap(first.get(), hip::detail::span(first, last), hip::dtype_of<Complex>::value, phase,
+                                  o.targets[0], o.targets[1], hip::detail::ptr_or_null(o.controls, o.nc), o.nc,
+                                  p.stream));
+}
+
 namespace ranges {
 // range forms: (policy, state, ...) -> state&, as ket::gate::ranges::*
 #define KET_HIP_RANGE(NAME)                                                        \
@@ -344,6 +355,7 @@ KET_HIP_RANGE(exponential_pauli_x)
 KET_HIP_RANGE(exponential_pauli_y)
 KET_HIP_RANGE(exponential_pauli_z)
 KET_HIP_RANGE(swap)
+KET_HIP_RANGE(exponential_swap)
 #undef KET_HIP_RANGE
 }  // namespace ranges
 }  // namespace gate

@@ -73,6 +73,7 @@ _SIGS = {
     "brk_swap": [_p, _u64, _i, _u, _u, _up, _u, _p],
     "brk_gate_1q": [_p, _u64, _i, _u, _dp, _up, _u, _p],
     "brk_swap_permutated_local_qubits": [_p, _u64, _i, _u, _u, _p],
+    "brk_exponential_swap": [_p, _u64, _i, _d, _u, _u, _up, _u, _p],
     "brk_inner_product": [_p, _i, _p, _i, _u64, _dp, _p],
     "brk_fidelity": [_p, _i, _p, _i, _u64, _dp, _p],
     "brk_norm2": [_p, _u64, _i, _dp, _p],
@@ -253,6 +254,11 @@ def swap(ptr, size, dtype, q1, q2, controls=(), stream=0):
     call("brk_swap", ptr, size, dtype, q1, q2, c, nc, stream)
 
 
+def exponential_swap(ptr, size, dtype, phase, q1, q2, controls=(), stream=0):
+    c, nc = uarr(controls)
+    call("brk_exponential_swap", ptr, size, dtype, phase, q1, q2, c, nc, stream)
+
+
 def gate_1q(ptr, size, dtype, target, matrix, controls=(), stream=0):
     m = darr([v for z in matrix for v in (z.real, z.imag)])
     c, nc = uarr(controls)
@@ -353,7 +359,7 @@ def convert(dst, dt_dst, src, dt_src, size, stream=0):
 # ---------------------------------------------------------------------------
 
 OP = dict(H=0, X=1, Y=2, Z=3, U1=4, U2=5, U3=6, EX=7, EY=8, EZ=9, RX=10, RY=11, RZ=12, SWAP=13,
-          LOCAL_SWAP=14, MATRIX=15)
+          LOCAL_SWAP=14, MATRIX=15, ESWAP=16)
 
 
 class Circuit:
@@ -530,7 +536,8 @@ class Batch:
             pass
 
 
-OP_NAMES = ("H", "X", "Y", "Z", "U1", "U2", "U3", "EX", "EY", "EZ", "RX", "RY", "RZ", "SWAP", "LOCAL_SWAP", "MATRIX")
+OP_NAMES = ("H", "X", "Y", "Z", "U1", "U2", "U3", "EX", "EY", "EZ", "RX", "RY", "RZ", "SWAP", "LOCAL_SWAP", "MATRIX",
+            "ESWAP")
 
 
 def circuit_ops(handle):

@@ -48,6 +48,8 @@ int run(const std::string& prefix, const char* tag) {
   r::exponential_pauli_z(p, s, 0.45, 1_q, 5_q, 7_cq);
   r::swap(p, s, 2_q, 6_q);
   r::swap(p, s, 0_q, 7_q, 4_cq);
+  r::exponential_swap(p, s, 0.55, 3_q, 5_q);
+  g::exponential_swap(p, s.begin(), s.end(), -0.8, 1_q, 6_q, 2_cq);
 
   // local swap of physical qubits 1 and 6, then gates addressed through the permutation
   ket::mpi::qubit_permutation<> perm(8);

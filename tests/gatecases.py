@@ -63,6 +63,8 @@ class Case:
             return D.pauli_exp(p[0], D.Z, m), t, c
         if k == "SWAP":
             return D.SWAP, t, c
+        if k == "ESWAP":  # exp(i theta SWAP) = cos theta I + i sin theta SWAP
+            return math.cos(p[0]) * np.eye(4) + 1j * math.sin(p[0]) * D.SWAP, t, c
         if k == "M1":
             return np.array(p, dtype=np.complex128).reshape(2, 2), t, c
         raise ValueError(k)
@@ -94,6 +96,8 @@ class Case:
             o.exponential_pauli_z(s, p[0], t, c)
         elif k == "SWAP":
             o.swap(s, t[0], t[1], c)
+        elif k == "ESWAP":
+            o.exponential_swap(s, p[0], t[0], t[1], c)
         elif k == "M1":
             o.gate_1q(s, t[0], p, c)
         else:
@@ -130,6 +134,8 @@ class Case:
             brk.exponential_pauli_z(ptr, size, dtype, p[0], t, c, stream)
         elif k == "SWAP":
             brk.swap(ptr, size, dtype, t[0], t[1], c, stream)
+        elif k == "ESWAP":
+            brk.exponential_swap(ptr, size, dtype, p[0], t[0], t[1], c, stream)
         elif k == "M1":
             brk.gate_1q(ptr, size, dtype, t[0], p, c, stream)
         else:
@@ -139,9 +145,10 @@ class Case:
 R2 = 1.0 / math.sqrt(2.0)
 
 # ket/test/runtime_gate_numerical.cpp:97-257 (same gates, qubits, controls and angles;
-# controlled_v and exponential_swap are outside the gate set of this engine,
-# x/y_rotation_half_pi are given as explicit matrices through gate_1q, sqrt_pauli_z
-# as the phase coefficient i).
+# controlled_v (controlled_v.hpp:30-90: V(theta) = [[1+e, 1-e], [1-e, 1+e]] / 2 with
+# e = e^{i theta}) and x/y_rotation_half_pi are given as explicit matrices through
+# gate_1q, sqrt_pauli_z as the phase coefficient i).
+_E025 = cmath.exp(0.25j)
 REFERENCE_CASES = [
     Case("ref:hadamard", "H", [3], [0]),
     Case("ref:pauli_x", "X", [0, 3], [1]),
@@ -155,6 +162,8 @@ REFERENCE_CASES = [
     Case("ref:exponential_pauli_x", "EX", [0, 3], [1], (0.25,)),
     Case("ref:exponential_pauli_y", "EY", [0, 3], [1], (0.25,)),
     Case("ref:exponential_pauli_z", "EZ", [0, 3], [1], (0.25,)),
+    Case("ref:controlled_v", "M1", [3], [0, 1], ((1 + _E025) / 2, (1 - _E025) / 2, (1 - _E025) / 2, (1 + _E025) / 2)),
+    Case("ref:exponential_swap", "ESWAP", [0, 3], [1], (0.25,)),
 ]
 
 KINDS_1T = ["H", "U2", "U3", "M1"]  # single target with controls
@@ -168,7 +177,7 @@ def _params(kind, rng):
         return (a(), a())
     if kind == "U3":
         return (a(), a(), a())
-    if kind in ("EX", "EY", "EZ", "U1"):
+    if kind in ("EX", "EY", "EZ", "U1", "ESWAP"):
         return (a(),)
     if kind == "U1coeff":
         return (cmath.exp(1j * a()),)
@@ -180,13 +189,13 @@ def _params(kind, rng):
 
 def random_cases(n, count, seed, max_targets=3, max_controls=2):
     rng = random.Random(seed)
-    kinds = KINDS_1T + KINDS_MT + KINDS_Q + (["SWAP"] if n >= 2 else [])
+    kinds = KINDS_1T + KINDS_MT + KINDS_Q + (["SWAP", "ESWAP"] if n >= 2 else [])
     out = []
     for i in range(count):
         kind = kinds[i % len(kinds)]
         if kind in KINDS_1T:
             nt = 1
-        elif kind == "SWAP":
+        elif kind in ("SWAP", "ESWAP"):
             nt = 2
         elif kind in KINDS_Q:
             nt = rng.randint(1, min(3, n))

@@ -36,6 +36,7 @@ _SIGS = {
     "or_exponential_pauli_y": ([_p, _u64, _d, _up, _u, _up, _u], None),
     "or_exponential_pauli_z": ([_p, _u64, _d, _up, _u, _up, _u], None),
     "or_swap": ([_p, _u64, _u, _u, _up, _u], None),
+    "or_exponential_swap": ([_p, _u64, _d, _u, _u, _up, _u], None),
     "or_gate_1q": ([_p, _u64, _u, C.POINTER(C.c_double), _up, _u], None),
     "or_swap_permutated_local_qubits": ([_p, _u64, _u, _u], None),
     "or_permutate": ([_up, _up, _u, _u, _u], None),
@@ -165,6 +166,10 @@ class Oracle:
     def swap(self, s, q1, q2, controls=()):
         c, nc = _ua(controls)
         self.lib.or_swap(_ptr(s), s.size, q1, q2, c, nc)
+
+    def exponential_swap(self, s, phase, q1, q2, controls=()):
+        c, nc = _ua(controls)
+        self.lib.or_exponential_swap(_ptr(s), s.size, phase, q1, q2, c, nc)
 
     def gate_1q(self, s, target, matrix, controls=()):
         m = _da([v for z in matrix for v in (z.real, z.imag)])

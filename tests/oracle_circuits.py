@@ -47,6 +47,8 @@ def apply_op(o, s, op, perm):
         o.swap(s, pt[0], pt[1], pc)
     elif name == "MATRIX":
         o.gate_1q(s, pt[0], [complex(p[2 * i], p[2 * i + 1]) for i in range(4)], pc)
+    elif name == "ESWAP":
+        o.exponential_swap(s, p[0], pt[0], pt[1], pc)
     else:
         raise ValueError(name)
 

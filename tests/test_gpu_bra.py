@@ -72,7 +72,8 @@ def test_sample_programs_vs_oracle(oracle, name):
 
 def test_full_precision_measurement_and_amplitudes(oracle):
     text = ("QUBITS 5\nINITIAL STATE 3\nH 0\nCNOT 0 4\nU3 2 0.3 1.1 -0.4\nEXX 1 3 0.7\nCCZ 0 2 4\nSX 1\n"
-            "+Y 3\nU 2 1 3\nT 4\nSWAP 0 3\nDO MEASUREMENT -17\nDO AMPLITUDES\nDO AMPLITUDES 3 17\n")
+            "+Y 3\nU 2 1 3\nT 4\nSWAP 0 3\nESWAP 1 4 0.3\nCESWAP 0 2 3 -0.6\nESWAP+ 3 4 1.2\n"
+            "DO MEASUREMENT -17\nDO AMPLITUDES\nDO AMPLITUDES 3 17\n")
     lines = run_bra(text)
     q, s = oracle_state(oracle, text)
     i = lines.index("<Qx> <Qy> <Qz>")

@@ -40,6 +40,8 @@ def oracle_sequence(o):
     o.exponential_pauli_z(s, 0.45, [1, 5], [7])
     o.swap(s, 2, 6)
     o.swap(s, 0, 7, [4])
+    o.exponential_swap(s, 0.55, 3, 5)
+    o.exponential_swap(s, -0.8, 1, 6, [2])
     perm, inv = list(range(8)), list(range(8))
     o.swap_permutated_local_qubits(s, 1, 6)
     o.permutate(perm, inv, inv[1], inv[6])

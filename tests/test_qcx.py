@@ -166,3 +166,22 @@ def test_errors(text, status, fragment):
     with pytest.raises(brk.BrkError) as e:
         parse(text)
     assert e.value.status == status and fragment in str(e.value)
+
+
+def test_exponential_swap_forms(oracle):
+    ops = only_ops("ESWAP 1 2 0.3\nESWAP+ 1 2 0.3\nCESWAP 0 3 4 -0.2\nC2ESWAP 5 6 0 1 0.7")
+    assert ops[0] == ("ESWAP", (1, 2), (), (0.3,))
+    assert ops[1] == ("ESWAP", (1, 2), (), (-0.3,))
+    assert ops[2] == ("ESWAP", (3, 4), (0,), (-0.2,))
+    assert ops[3] == ("ESWAP", (0, 1), (5, 6), (0.7,))
+    # ESWAP(theta) ESWAP+(theta) = I, and ESWAP(pi/2) = i SWAP (checked on the oracle)
+    rng = np.random.default_rng(1)
+    s = rng.normal(size=8) + 1j * rng.normal(size=8)
+    t = s.copy()
+    run_ops(oracle, t, only_ops("ESWAP 0 2 0.9\nESWAP+ 0 2 0.9", 3))
+    assert np.max(np.abs(t - s)) < 1e-15
+    u = s.copy()
+    run_ops(oracle, u, only_ops(f"ESWAP 0 2 {math.pi / 2!r}", 3))
+    v = s.copy()
+    oracle.swap(v, 0, 2)
+    assert np.max(np.abs(u - 1j * v)) < 1e-15
