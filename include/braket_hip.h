@@ -248,6 +248,18 @@ int brk_expectation_ising(const void* first, uint64_t size, int dtype, const dou
 int brk_apply_ising(void* out, const void* psi, uint64_t size, int dtype, const double* J, const double* h,
                     void* stream);
 
+/* Overwrites the state with Shor's modular-exponentiation superposition:
+   2^{-m/2} sum_{x < 2^m} |reverse_m(x)> on the top m qubits (m =
+   num_exponent_qubits) times |v_x> on the n - m low qubits, where v_0 = 1,
+   v_x = base^x mod divisor and only the low n - m bits of v_x are kept.
+   Replaces ket::shor_box (ket/include/ket/shor_box.hpp:60-98) as
+   bra::state::shor_box calls it (bra/src/state.cpp:2173-2191).  Needs
+   (divisor - 1) * base < 2^64 (the reference's 64-bit recurrence does not
+   wrap there; BRK_ERR_UNSUPPORTED otherwise).  Allocates a 2^m-entry table
+   for the call and synchronises the stream. */
+int brk_shor_box(void* first, uint64_t size, int dtype, unsigned num_exponent_qubits, uint64_t divisor,
+                 uint64_t base, void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* Measurement and sampling.  The random draws stay with the caller (bra    */
 /* keeps a std::mt19937_64 per state, bra/include/bra/state.hpp:459); these */
@@ -447,8 +459,8 @@ int brk_batch_run_timed(brk_batch* batch, double* energies, double* elapsed_ms, 
 /*   of them behind a C..C / Cn control prefix (controls come first);       */
 /*   DO MEASUREMENT [p], BEGIN MEASUREMENT, DO AMPLITUDES [i..],            */
 /*   BEGIN FUSION / END FUSION (gates in between run in order), M, CLEAR,   */
-/*   SET, GENERATE EVENTS n seed and EXIT (reading stops after the last two,*/
-/*   as bra's does).                                                        */
+/*   SET, SHORBOX m divisor base, GENERATE EVENTS n seed and EXIT (reading */
+/*   stops after the last two, as bra's does).                              */
 /* Anything else fails with BRK_ERR_UNSUPPORTED and a message naming the    */
 /* line.                                                                    */
 /* ------------------------------------------------------------------------ */
@@ -466,7 +478,8 @@ typedef enum brk_qcx_kind {
   BRK_QCX_SET = 7,                    /* SET q: indices = {q}                */
   BRK_QCX_GENERATE_EVENTS = 8,        /* indices = {num_events}, precision = seed
                                          (negative: continue the state's generator) */
-  BRK_QCX_EXIT = 9                    /* sample one basis state and collapse onto it */
+  BRK_QCX_EXIT = 9,                   /* sample one basis state and collapse onto it */
+  BRK_QCX_SHOR_BOX = 10               /* indices = {num_exponent_qubits, divisor, base} */
 } brk_qcx_kind;
 
 int brk_qcx_parse(const char* text, brk_qcx** out);

@@ -677,3 +677,26 @@ void or_generate_events(const double* s, uint64_t size, const double* canonical,
   }
   free(scan);
 }
+
+/* ---- Shor box ----------------------------------------------------------- */
+
+/* ket::shor_box (shor_box.hpp:60-98): zero the state, then for x = 0 .. 2^m - 1
+   set the amplitude at (reverse_m(x) << (n - m)) | (v_x & (2^(n-m) - 1)) to
+   2^{-m/2}, with v_0 = 1 and v_{x+1} = (v_x * base) mod divisor in uint64. */
+void or_shor_box(double* s, uint64_t size, unsigned m, uint64_t divisor, uint64_t base) {
+  unsigned n = 0;
+  while ((1ull << n) < size) ++n;
+  memset(s, 0, sizeof(double) * 2 * size);
+  const uint64_t count = 1ull << m, low_mask = (1ull << (n - m)) - 1ull;
+  const double coefficient = pow((double)count, -0.5);
+  uint64_t v = 1;
+  for (uint64_t x = 0; x < count; ++x) {
+    uint64_t r = 0;
+    for (unsigned b = 0; b < m; ++b) r |= ((x >> (m - 1 - b)) & 1ull) << b;
+    const uint64_t i = (r << (n - m)) | (v & low_mask);
+    s[2 * i] = coefficient;
+    s[2 * i + 1] = 0.0;
+    v *= base;
+    v %= divisor;
+  }
+}

@@ -100,6 +100,9 @@ int or_projective_measurement(double* s, uint64_t size, unsigned qubit, double c
 void or_generate_events(const double* s, uint64_t size, const double* canonical, unsigned num_events,
                         uint64_t* events);
 
+/* ket::shor_box (shor_box.hpp:60-98): exponent qubits are the top m */
+void or_shor_box(double* s, uint64_t size, unsigned m, uint64_t divisor, uint64_t base);
+
 #ifdef __cplusplus
 }
 #endif

@@ -171,6 +171,7 @@ int main(int argc, char** argv) {
         case BRK_QCX_SET: st.do_set((unsigned)idx[0]); break;
         case BRK_QCX_GENERATE_EVENTS: generate_events(st, (int)idx[0], precision, t); break;
         case BRK_QCX_EXIT: exit_statement(st, t); break;
+        case BRK_QCX_SHOR_BOX: st.do_shor_box((unsigned)idx[0], idx[1], idx[2]); break;
         default: break;  // fusion brackets: the gates inside run in order either way
       }
     }

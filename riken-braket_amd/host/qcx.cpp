@@ -99,6 +99,7 @@ class parser {
     need_qubits();
     if (m == "M" || m == "CLEAR" || m == "SET") return single_qubit_statement(m);
     if (m == "GENERATE") return generate_statement();
+    if (m == "SHORBOX") return shor_box_statement();
     if (m == "EXIT") {
       expect_columns(1);
       not_in_fusion("EXIT");
@@ -221,6 +222,15 @@ class parser {
     if (q < 0 || q >= (long long)prog_.num_qubits) error("qubit " + cols_[1] + " out of range");
     directive(m == "M" ? BRK_QCX_PROJECTIVE_MEASUREMENT : m == "CLEAR" ? BRK_QCX_CLEAR : BRK_QCX_SET, 3,
               {(uint64_t)q});
+  }
+  // SHORBOX <num_exponent_qubits> <divisor> <base> (interpreter.cpp:3208-3216, 1861-1873)
+  void shor_box_statement() {
+    expect_columns(4);
+    not_in_fusion("SHORBOX");
+    const long long m = integer(1), divisor = integer(2), base = integer(3);
+    if (m < 0 || m > (long long)prog_.num_qubits) error("number of exponent qubits out of range");
+    if (divisor <= 0 || base < 0) error("SHORBOX needs a positive divisor and a non-negative base");
+    directive(BRK_QCX_SHOR_BOX, 3, {(uint64_t)m, (uint64_t)divisor, (uint64_t)base});
   }
   // GENERATE EVENTS <num_events> <seed> (interpreter.cpp:764-784, 1875-1887)
   void generate_statement() {

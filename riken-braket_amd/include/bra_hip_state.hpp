@@ -125,6 +125,12 @@ class hip_state {
   // bra/src/interpreter.cpp:851-872); nompi mode applies gates at their own qubits
   void apply(const brk_circuit* circuit) { check(brk_circuit_run(circuit, data_, BRK_C128, nullptr, stream_)); }
 
+  // bra::state::shor_box: exponent qubits are the top m, the rest hold base^x mod divisor
+  // (bra/src/state.cpp:2173-2191, bra/src/nompi_state.cpp:1276-1280)
+  void do_shor_box(unsigned num_exponent_qubits, std::uint64_t divisor, std::uint64_t base) {
+    check(brk_shor_box(data_, size_, BRK_C128, num_exponent_qubits, divisor, base, stream_));
+  }
+
   // --- measurement and sampling ----------------------------------------------
   // ket::gate::projective_measurement (ket/include/ket/gate/projective_measurement.hpp:137-176):
   // one draw u * (P0 + P1) decides the outcome; the kept half is scaled by P^(-1/2).

@@ -82,6 +82,7 @@ _SIGS = {
     "brk_expectation_ising": [_p, _u64, _i, _dp, _dp, _dp, _p],
     "brk_apply_ising": [_p, _p, _u64, _i, _dp, _dp, _p],
     "brk_zero_one_probabilities": [_p, _u64, _i, _u, _dp, _p],
+    "brk_shor_box": [_p, _u64, _i, _u, _u64, _u64, _p],
     "brk_project_qubit": [_p, _u64, _i, _u, _i, _d, _p],
     "brk_generate_events": [_p, _u64, _i, _dp, _u, C.POINTER(C.c_uint64), _p],
     "brk_circuit_create": [_u, C.POINTER(C.c_void_p)],
@@ -307,6 +308,10 @@ def spin_expectation_value(ptr, size, dtype, qubit, stream=0):
     out = (C.c_double * 3)()
     call("brk_spin_expectation_value", ptr, size, dtype, qubit, out, stream)
     return tuple(out)
+
+
+def shor_box(ptr, size, dtype, num_exponent_qubits, divisor, base, stream=0):
+    call("brk_shor_box", ptr, size, dtype, num_exponent_qubits, divisor, base, stream)
 
 
 def zero_one_probabilities(ptr, size, dtype, qubit, stream=0):
@@ -555,6 +560,7 @@ def circuit_ops(handle):
 
 QCX_GATES, QCX_MEASUREMENT, QCX_AMPLITUDES, QCX_BEGIN_FUSION, QCX_END_FUSION = 0, 1, 2, 3, 4
 QCX_PROJECTIVE_MEASUREMENT, QCX_CLEAR, QCX_SET, QCX_GENERATE_EVENTS, QCX_EXIT = 5, 6, 7, 8, 9
+QCX_SHOR_BOX = 10
 
 
 class Qcx:
@@ -584,6 +590,8 @@ class Qcx:
                 self.statements.append((kind.value, [idx[k] for k in range(nidx.value)]))
             elif kind.value in (QCX_PROJECTIVE_MEASUREMENT, QCX_CLEAR, QCX_SET):
                 self.statements.append((kind.value, idx[0]))
+            elif kind.value == QCX_SHOR_BOX:
+                self.statements.append((kind.value, (idx[0], idx[1], idx[2])))  # (m, divisor, base)
             elif kind.value == QCX_GENERATE_EVENTS:
                 self.statements.append((kind.value, (idx[0], prec.value)))  # (num_events, seed)
             else:

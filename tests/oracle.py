@@ -55,6 +55,7 @@ _SIGS = {
     "or_project": ([_p, _u64, _u, C.c_int, _d], None),
     "or_projective_measurement": ([_p, _u64, _u, _d], C.c_int),
     "or_generate_events": ([_p, _u64, C.POINTER(C.c_double), _u, C.POINTER(C.c_uint64)], None),
+    "or_shor_box": ([_p, _u64, _u, _u64, _u64], None),
 }
 
 
@@ -270,3 +271,6 @@ class Oracle:
         self.lib.or_generate_events(_ptr(s), s.size, canonical.ctypes.data_as(C.POINTER(C.c_double)), canonical.size,
                                     out.ctypes.data_as(C.POINTER(C.c_uint64)))
         return out
+
+    def shor_box(self, s, m, divisor, base):
+        self.lib.or_shor_box(_ptr(s), s.size, m, divisor, base)

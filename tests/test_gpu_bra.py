@@ -35,7 +35,11 @@ def oracle_state(oracle, text):
     q = brk.Qcx(text)
     s = np.zeros(1 << q.num_qubits, dtype=np.complex128)
     s[q.initial_state] = 1
-    run_ops(oracle, s, q.ops())
+    for kind, v in q.statements:
+        if kind == brk.QCX_GATES:
+            run_ops(oracle, s, v)
+        elif kind == brk.QCX_SHOR_BOX:
+            oracle.shor_box(s, *v)
     return q, s
 
 
@@ -60,7 +64,7 @@ def test_hadamards08_closed_form():
     assert row.replace("-0.000", " 0.000") == " ".join(f"{v:^8}" for v in (" 0.000", " 0.500", " 0.500"))
 
 
-@pytest.mark.parametrize("name", ["toffoli12.qcx", "adder6x2.qcx"])
+@pytest.mark.parametrize("name", ["toffoli12.qcx", "adder6x2.qcx", "shor12.qcx"])
 def test_sample_programs_vs_oracle(oracle, name):
     text = open(os.path.join(GOLDEN, name)).read()
     lines = run_bra(text)
@@ -94,8 +98,8 @@ def test_full_precision_measurement_and_amplitudes(oracle):
 def test_initial_state_and_errors():
     lines = run_bra("QUBITS 3\nINITIAL STATE 5\nDO AMPLITUDES 5 0\n")
     assert [l for l in lines if "=>" in l] == ["101 => 1 + 0 i", "000 => 0 + 0 i"]
-    r = subprocess.run([EXE], input="QUBITS 2\nSHORBOX 1 2 3\n", capture_output=True, text=True, timeout=60)
-    assert r.returncode == 1 and "SHORBOX" in r.stderr
+    r = subprocess.run([EXE], input="QUBITS 2\nVAR X INT\n", capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "VAR" in r.stderr
 
 
 # --- measurement and sampling statements ----------------------------------------------------

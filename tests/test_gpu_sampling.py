@@ -175,3 +175,31 @@ def test_projective_measurement_sequence(oracle):
     assert np.max(np.abs(st.download() - s)) < 1e-12
     assert abs(np.vdot(s, s).real - 1) < 1e-12
     st.free()
+
+
+@pytest.mark.parametrize("n,m,divisor,base,dtype", [(12, 8, 15, 7, brk.C128), (20, 10, 1021, 3, brk.C128),
+                                                    (24, 12, 4093, 5, brk.C128), (10, 10, 5, 3, brk.C128),
+                                                    (9, 0, 7, 3, brk.C128), (18, 9, 511, 2, brk.C64)])
+def test_shor_box_exact(oracle, n, m, divisor, base, dtype):
+    note(f"gpu shor box n={n} m={m} divisor={divisor} base={base} dtype={dtype}")
+    st = DeviceState(n, dtype)
+    brk.fill_modular(st.ptr, st.size, dtype, (1e-3, 2e-3), (3, 1, 97, 5, 2, 89))  # overwritten entirely
+    brk.shor_box(st.ptr, st.size, dtype, m, divisor, base)
+    ref = np.zeros(1 << n, dtype=np.complex128)
+    oracle.shor_box(ref, m, divisor, base)
+    out = st.download()
+    if dtype == brk.C128:
+        assert np.array_equal(out.view(np.uint64), ref.view(np.uint64))
+    else:  # the coefficient is std::pow in float: equal to the rounded double up to an ulp
+        assert np.array_equal(out != 0, ref != 0)
+        assert np.max(np.abs(out - ref)) < 1e-8
+    st.free()
+
+
+def test_shor_box_errors():
+    st = DeviceState(4)
+    for args, status in (((5, 15, 7), 1), ((2, 0, 7), 1), ((2, 2**40, 2**30), 8)):
+        with pytest.raises(brk.BrkError) as e:
+            brk.shor_box(st.ptr, st.size, brk.C128, *args)
+        assert e.value.status == status
+    st.free()

@@ -1,5 +1,6 @@
-"""CPU checks of the oracle's measurement and sampling functions and of the bra
-statements that use them (M, CLEAR, SET, GENERATE EVENTS, EXIT).
+"""CPU checks of the oracle's measurement, sampling and Shor-box functions and
+of the bra statements that use them (M, CLEAR, SET, GENERATE EVENTS, EXIT,
+SHORBOX).
 
 Pinned by:
 - the C++ standard's known answer for std::mt19937_64 ([rand.predef]: the
@@ -12,8 +13,11 @@ Pinned by:
   product state replaces one factor by a basis vector);
 - an independent numpy restatement of sampling (sequential cumsum +
   searchsorted(side="right"), which is std::upper_bound), plus the invariant
-  that no event lands on an amplitude of probability zero.
-No reference test or fixture covers these paths."""
+  that no event lands on an amplitude of probability zero;
+- for the Shor box, Python's exact modular power pow(base, x, divisor) and an
+  explicit bit reversal (v_0 = 1 unreduced, only the low n - m bits kept).
+No reference test or fixture covers these paths; the reference's sample
+program shor12.qcx is run end to end by the GPU bra tests."""
 import math
 import os
 import shutil
@@ -183,3 +187,33 @@ def test_statement_errors(text, status, fragment):
     with pytest.raises(brk.BrkError) as e:
         brk.Qcx(text)
     assert e.value.status == status and fragment in str(e.value)
+
+
+# --- Shor box --------------------------------------------------------------------------------
+
+def shor_closed_form(n, m, divisor, base):
+    """2^{-m/2} sum_x |reverse_m(x)> (top m qubits) |base^x mod divisor> (low bits kept), v_0 = 1."""
+    s = np.zeros(1 << n, dtype=np.complex128)
+    for x in range(1 << m):
+        r = int(format(x, f"0{m}b")[::-1], 2) if m else 0
+        v = 1 if x == 0 else pow(base, x, divisor)
+        s[(r << (n - m)) | (v & ((1 << (n - m)) - 1))] = (2.0 ** m) ** -0.5
+    return s
+
+
+@pytest.mark.parametrize("n,m,divisor,base", [(12, 8, 15, 7), (10, 4, 21, 2), (6, 6, 5, 3), (5, 0, 7, 3),
+                                              (7, 3, 1, 4), (9, 5, 9, 9), (8, 5, 200, 13)])
+def test_shor_box_closed_form(oracle, n, m, divisor, base):
+    s = np.full(1 << n, 0.3 + 0.1j)  # overwritten entirely
+    oracle.shor_box(s, m, divisor, base)
+    assert np.array_equal(s, shor_closed_form(n, m, divisor, base))
+    assert abs(np.vdot(s, s).real - 1) < 1e-14
+
+
+def test_parse_shor_box():
+    q = brk.Qcx("QUBITS 6\nSHORBOX 3 5 2\nH 0\n")
+    assert q.statements[0] == (brk.QCX_SHOR_BOX, (3, 5, 2))
+    for text in ("QUBITS 6\nSHORBOX 7 5 2\n", "QUBITS 6\nSHORBOX 3 0 2\n", "QUBITS 6\nSHORBOX 3 5\n"):
+        with pytest.raises(brk.BrkError) as e:
+            brk.Qcx(text)
+        assert e.value.status == 1

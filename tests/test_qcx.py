@@ -66,10 +66,19 @@ def test_adder_controlled_phase():
     assert ops[-2] == ("U1", (7,), (6,), (-math.pi / 2,))
 
 
+def test_shor12():
+    q = parse(sample("shor12.qcx"))
+    assert q.num_qubits == 12
+    assert q.statements[0] == (brk.QCX_SHOR_BOX, (8, 15, 7))
+    assert q.statements[-1] == (brk.QCX_MEASUREMENT, 3)
+    ops = q.ops()
+    assert ops[0] == ("H", (7,), (), ()) and ops[1] == ("U1", (7,), (6,), (-math.pi / 2,))
+
+
 def test_unsupported_statement_names_line():
     with pytest.raises(brk.BrkError) as e:
-        parse(sample("shor12.qcx"))
-    assert e.value.status == 8 and "line 2" in str(e.value) and "SHORBOX" in str(e.value)
+        parse("QUBITS 4\nH 0\nVAR X INT\n")
+    assert e.value.status == 8 and "line 3" in str(e.value) and "VAR" in str(e.value)
 
 
 def test_fixed_gates_match_their_definitions():
