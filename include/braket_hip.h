@@ -344,6 +344,43 @@ int brk_circuit_run_timed(const brk_circuit* circuit, void* first, int dtype, un
                           float* op_ms, void* stream);
 
 /* ------------------------------------------------------------------------ */
+/* Gate blocks: runs of consecutive gates applied per chunk of the state    */
+/* held in LDS (2^12 amplitudes for complex<double>, 2^13 for               */
+/* complex<float>), each gate with the per-gate kernels' arithmetic, so the */
+/* amplitudes are bit-identical to brk_circuit_run's.  One HBM read and     */
+/* write per block instead of per gate.  Replaces the gate application of   */
+/* bra's BEGIN/END FUSION (bra/src/state.cpp:2193-2217,                     */
+/* bra/src/nompi_state.cpp:1282-1357) and of ket's cache-aware gate         */
+/* functions (ket/include/ket/gate/detail/gate_bitmasks.hpp:587-2134).      */
+/* ------------------------------------------------------------------------ */
+
+typedef struct brk_block_plan brk_block_plan;
+
+/* Groups the circuit's ops (greedily, in order) into blocks whose qubits,
+   with the 3 (c128) or 4 (c64) lowest qubits, fit in the local qubits of a
+   chunk; single ops and local swaps stay plain steps.  `permutation` (may be
+   NULL = identity) is the logical->physical map at the start, as for
+   brk_circuit_run.  Planning needs no device. */
+int brk_block_plan_create(const brk_circuit* circuit, int dtype, const unsigned* permutation,
+                          brk_block_plan** out);
+/* Call after the work of brk_block_plan_run on its stream has completed. */
+int brk_block_plan_destroy(brk_block_plan* plan);
+int brk_block_plan_num_steps(const brk_block_plan* plan, unsigned* num_steps);
+/* Step i covers ops [first_op, first_op + num_ops) of the circuit; num_local
+   is 0 for a plain step, else the block's local (physical) qubits, ascending,
+   are written to local_qubits (up to 13 entries). */
+int brk_block_plan_step(const brk_block_plan* plan, unsigned i, unsigned* first_op, unsigned* num_ops,
+                        unsigned* num_local, unsigned* local_qubits);
+int brk_block_plan_final_permutation(const brk_block_plan* plan, unsigned* permutation);
+/* Runs the plan on a state of 2^num_qubits amplitudes of the plan's dtype;
+   the first run uploads the block programs to the current device. */
+int brk_block_plan_run(brk_block_plan* plan, void* first, void* stream);
+/* Plans, runs and destroys in one call (synchronises the stream); the
+   permutation is updated as by brk_circuit_run. */
+int brk_circuit_run_blocked(const brk_circuit* circuit, void* first, int dtype, unsigned* permutation,
+                            void* stream);
+
+/* ------------------------------------------------------------------------ */
 /* Precision form: greedy gate fusion into dense 4..6-qubit unitaries,      */
 /* applied to a complex64 state as batched complex GEMMs on MFMA (FP32      */
 /* multiply, FP32 accumulate).  Semantics follow bra's BEGIN/END FUSION     */

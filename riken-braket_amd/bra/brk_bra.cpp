@@ -10,7 +10,9 @@
 //   GENERATE EVENTS n seed                  bra::state::generate_events (bra/src/state.cpp:2064-2092)
 //   EXIT                                    bra::state::exit (bra/src/state.cpp:2094-2120)
 // each framed by the "... finished: <seconds since start> (<seconds since last>)"
-// lines; M, CLEAR and SET print nothing.  --seed seeds the state's
+// lines; M, CLEAR and SET print nothing.  Runs of gates (inside BEGIN/END
+// FUSION or not) are applied in LDS-resident blocks, which leaves every
+// amplitude bit-identical to gate-by-gate application.  --seed seeds the state's
 // std::mt19937_64 (default 1, bra/src/bra.cpp:73); --threads is accepted for
 // command-line compatibility.
 #include <chrono>
@@ -163,7 +165,7 @@ int main(int argc, char** argv) {
       const std::uint64_t* idx = nullptr;
       brk::bra::check(brk_qcx_statement(prog, i, &kind, &gates, &precision, &nidx, &idx));
       switch (kind) {
-        case BRK_QCX_GATES: st.apply(gates); break;
+        case BRK_QCX_GATES: st.apply_blocked(gates); break;
         case BRK_QCX_MEASUREMENT: measurement(st, precision, t); break;
         case BRK_QCX_AMPLITUDES: amplitudes(st, std::vector<std::uint64_t>(idx, idx + nidx), t); break;
         case BRK_QCX_PROJECTIVE_MEASUREMENT: st.do_projective_measurement((unsigned)idx[0]); break;
@@ -172,7 +174,7 @@ int main(int argc, char** argv) {
         case BRK_QCX_GENERATE_EVENTS: generate_events(st, (int)idx[0], precision, t); break;
         case BRK_QCX_EXIT: exit_statement(st, t); break;
         case BRK_QCX_SHOR_BOX: st.do_shor_box((unsigned)idx[0], idx[1], idx[2]); break;
-        default: break;  // fusion brackets: the gates inside run in order either way
+        default: break;  // fusion brackets: every run of gates is applied in blocks already
       }
     }
     brk::bra::check(brk_stream_synchronize(nullptr));

@@ -1,0 +1,67 @@
+// Gate blocks: runs of consecutive gates applied to one LDS-resident chunk of
+// the state per workgroup.
+//
+// A gate is first described in physical-qubit terms (gate_desc) by the same
+// code that computes the plain kernels' coefficients, so a block applies each
+// gate with the plain path's arithmetic and gives bit-identical amplitudes.
+// The planner (host/block.cpp) turns the descriptions of one block into a
+// block_program over local bit positions; the kernel (brk_gates.hip) loads
+// the 2^L amplitudes of a chunk, applies the program in LDS and stores them.
+// Plain C++ only: included by host code compiled without HIP.
+#pragma once
+
+#include <cstdint>
+
+namespace brk {
+
+constexpr int BLOCK_LDS_BYTES = 65536;            // LDS per workgroup (two workgroups per CU)
+constexpr int BLOCK_THREADS = 512;
+constexpr int BLOCK_MAX_LOCAL_C128 = 12;          // 2^12 x 16 B = 64 KiB
+constexpr int BLOCK_MAX_LOCAL_C64 = 13;           // 2^13 x 8 B  = 64 KiB
+constexpr int BLOCK_MAX_OPS = 64;
+constexpr int BLOCK_THREAD_BITS = 9;              // log2(BLOCK_THREADS)
+
+// modes of a described gate: the pair modes of the plain kernels, or diagonal
+enum block_mode { BLOCK_H = 0, BLOCK_SWAP = 1, BLOCK_MATRIX = 2, BLOCK_ESWAP = 3, BLOCK_DIAG = 4 };
+
+// One gate in physical-qubit terms.  Pair modes act on (i0, i0 ^ flip) for
+// every i0 with bit pair_bit clear and all `cond` bits set; diagonal mode
+// multiplies every amplitude with all `cond` bits set by u[0..1] (even parity
+// of i & zmask) or u[2..3] (odd).  `sign` is the matrix mode's sign-flip
+// parity mask and ESWAP's higher-qubit bit.
+struct gate_desc {
+  int mode;
+  unsigned pair_bit;
+  uint64_t flip, cond, sign, zmask;
+  uint64_t qubits;  // every qubit the gate reads or writes
+  double u[8];
+};
+
+// Describes an op (physical qubits; op codes of brk_op_code).  Returns
+// BRK_OK, or BRK_ERR_UNSUPPORTED for ops without a description (local swap).
+int describe_physical_op(int code, const unsigned* targets, unsigned num_targets, const unsigned* controls,
+                         unsigned num_controls, const double* params, gate_desc* out);
+
+// A gate of a block with its masks over local bit positions (bit r of a
+// chunk index <-> physical qubit pos[r]).
+struct block_op {
+  int32_t mode;
+  uint32_t pair_bit, flip, cond, sign, zmask;
+  double u[8];
+};
+
+struct block_program {
+  uint32_t num_ops;
+  uint32_t num_local;                   // L
+  uint32_t pos[16];                     // ascending physical qubits of the local bits
+  uint64_t deposit_high[16];            // physical offset of local index k << BLOCK_THREAD_BITS
+  block_op ops[BLOCK_MAX_OPS];
+};
+
+// Launches one block over a state of `size` amplitudes: 2^(n - L) workgroups.
+// `device_program` points to a block_program in device memory whose copy on
+// the host is `program`.
+int launch_block(const block_program* device_program, const block_program& program, void* first, uint64_t size,
+                 int dtype, void* stream);
+
+}  // namespace brk

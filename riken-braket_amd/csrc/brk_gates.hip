@@ -28,6 +28,7 @@
 // Group -> index mapping: brk::expand_index (bit insertion, the same
 // decomposition as ket's index masks, gate_bitmasks.hpp:50-130).
 #include "brk_internal.hpp"
+#include "brk_block.hpp"
 
 #include <cmath>
 #include <complex>
@@ -69,18 +70,20 @@ __device__ __forceinline__ cpx<R> csel(bool c, cpx<R> a, cpx<R> b) {
   return cpx<R>{c ? a.re : b.re, c ? a.im : b.im};
 }
 
-template <typename R>
-__device__ __forceinline__ void load_matrix(const gate_args& a, cpx<R>& m00, cpx<R>& m01, cpx<R>& m10,
-                                            cpx<R>& m11) {
-  m00 = {(R)a.u[0], (R)a.u[1]};
-  m01 = {(R)a.u[2], (R)a.u[3]};
-  m10 = {(R)a.u[4], (R)a.u[5]};
-  m11 = {(R)a.u[6], (R)a.u[7]};
+// u: the gate's 8 coefficients (any indexable type: kernel argument or program memory)
+template <typename R, typename U>
+__device__ __forceinline__ void load_matrix(const U& u, cpx<R>& m00, cpx<R>& m01, cpx<R>& m10, cpx<R>& m11) {
+  m00 = {(R)u[0], (R)u[1]};
+  m01 = {(R)u[2], (R)u[3]};
+  m10 = {(R)u[4], (R)u[5]};
+  m11 = {(R)u[6], (R)u[7]};
 }
 
-template <typename R, int MODE>
-__device__ __forceinline__ void pair_math(const gate_args& a, uint64_t i0, cpx<R> x0, cpx<R> x1, cpx<R>& y0,
-                                          cpx<R>& y1) {
+// The arithmetic of one amplitude pair (i0, i1), shared by every kernel that
+// applies a pair gate: the plain families and the LDS-resident gate blocks.
+template <typename R, int MODE, typename U>
+__device__ __forceinline__ void pair_math(const U& u, uint64_t signmask, uint64_t i0, cpx<R> x0, cpx<R> x1,
+                                          cpx<R>& y0, cpx<R>& y1) {
   if constexpr (MODE == MODE_H) {
     // (a0 + a1)/sqrt2, (a0 - a1)/sqrt2 -- ket/include/ket/gate/hadamard.hpp:66-70
     const R r = (R)0.707106781186547524400844362104849039;
@@ -93,17 +96,17 @@ __device__ __forceinline__ void pair_math(const gate_args& a, uint64_t i0, cpx<R
     // exp(i theta SWAP) (ket/include/ket/gate/exponential_swap.hpp:31-90): the pair
     // (i0, i0 ^ flipmask) is (00, 11) when the higher target bit of i0 is 0 -- both
     // get e^{i theta} -- and (10, 01) otherwise: a' = cos a + i sin b, b' = cos b + i sin a.
-    const cpx<R> e = {(R)a.u[0], (R)a.u[1]};
-    const R c = (R)a.u[2], sn = (R)a.u[3];
-    const bool mix = (i0 & a.signmask) != 0;
+    const cpx<R> e = {(R)u[0], (R)u[1]};
+    const R c = (R)u[2], sn = (R)u[3];
+    const bool mix = (i0 & signmask) != 0;
     const cpx<R> m0 = {x0.re * c - x1.im * sn, x0.im * c + x1.re * sn};
     const cpx<R> m1 = {x1.re * c - x0.im * sn, x1.im * c + x0.re * sn};
     y0 = csel(mix, m0, cmul(x0, e));
     y1 = csel(mix, m1, cmul(x1, e));
   } else {
     cpx<R> m00, m01, m10, m11;
-    load_matrix(a, m00, m01, m10, m11);
-    const bool flip = (__popcll(i0 & a.signmask) & 1) != 0;
+    load_matrix(u, m00, m01, m10, m11);
+    const bool flip = (__popcll(i0 & signmask) & 1) != 0;
     m01 = csel(flip, cpx<R>{-m01.re, -m01.im}, m01);
     m10 = csel(flip, cpx<R>{-m10.re, -m10.im}, m10);
     y0 = cadd(cmul(m00, x0), cmul(m01, x1));
@@ -123,7 +126,7 @@ __global__ __launch_bounds__(GATE_THREADS) void k_pair(cpx<R>* __restrict__ s, u
     const cpx<R> x0 = load_amp<NT>(s + i0);
     const cpx<R> x1 = load_amp<NT>(s + i1);
     cpx<R> y0, y1;
-    pair_math<R, MODE>(a, i0, x0, x1, y0, y1);
+    pair_math<R, MODE>(a.u, a.signmask, i0, x0, x1, y0, y1);
     if (COND) {
       const bool on = (i0 & a.condmask) == a.condmask;
       y0 = csel(on, y0, x0);
@@ -155,7 +158,7 @@ __global__ __launch_bounds__(GATE_THREADS) void k_lane(cpx<R>* __restrict__ s, u
       const bool one = (i >> a.lane_bit) & 1ull;
       cpx<R> y0, y1;
       // pair (a0, a1) = (this, partner) on the 0 side, (partner, this) on the 1 side
-      pair_math<R, MODE>(a, i & ~(1ull << a.lane_bit), csel(one, p, x), csel(one, x, p), y0, y1);
+      pair_math<R, MODE>(a.u, a.signmask, i & ~(1ull << a.lane_bit), csel(one, p, x), csel(one, x, p), y0, y1);
       y = csel(one, y1, y0);
     }
     if (COND) y = csel((i & a.condmask) == a.condmask, y, x);
@@ -212,8 +215,102 @@ __global__ __launch_bounds__(GATE_THREADS) void k_diag(cpx<R>* __restrict__ s, u
 }
 
 // ---------------------------------------------------------------------------
+// Gate blocks: one workgroup per chunk of 2^L amplitudes (the local bits of a
+// block_program), held in LDS while the block's gates run in order with the
+// arithmetic above; one HBM read and one write per amplitude per block.
+// ---------------------------------------------------------------------------
+
+typedef const __attribute__((address_space(4))) block_program* cprogram;
+
+template <typename R, bool NT>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_block(cpx<R>* __restrict__ s, const block_program* program) {
+  constexpr uint32_t CAP = BLOCK_LDS_BYTES / sizeof(cpx<R>);
+  __shared__ cpx<R> lds[CAP];
+  const cprogram prog = (cprogram)program;  // uniform: scalar loads
+  const uint32_t L = prog->num_local, count = 1u << L;
+  // chunk index -> global bits outside the local positions
+  uint64_t base = blockIdx.x;
+  for (uint32_t r = 0; r < L; ++r) {
+    const uint32_t q = prog->pos[r];
+    base = ((base >> q) << (q + 1)) | (base & ((1ull << q) - 1ull));
+  }
+  // local index j = t + BLOCK_THREADS * k: the bits of t go to pos[0 .. 8]
+  const uint32_t t = threadIdx.x;
+  uint64_t low = 0;
+  for (uint32_t r = 0; r < L && r < (uint32_t)BLOCK_THREAD_BITS; ++r) low |= (uint64_t)((t >> r) & 1u) << prog->pos[r];
+  const uint32_t per = count > BLOCK_THREADS ? count / BLOCK_THREADS : 1;
+  const bool active = t < count;
+  if (active)
+    for (uint32_t k = 0; k < per; ++k)
+      lds[t + BLOCK_THREADS * k] = load_amp<NT>(s + (base | low | prog->deposit_high[k]));
+  __syncthreads();
+  const uint32_t nops = prog->num_ops;
+  for (uint32_t o = 0; o < nops; ++o) {
+    const auto& op = prog->ops[o];
+    const int mode = op.mode;
+    const uint32_t cond = op.cond;
+    if (mode == BLOCK_DIAG) {
+      const cpx<R> c0 = {(R)op.u[0], (R)op.u[1]}, c1 = {(R)op.u[2], (R)op.u[3]};
+      const uint32_t zmask = op.zmask;
+      for (uint32_t j = t; j < count; j += BLOCK_THREADS) {
+        if ((j & cond) != cond) continue;
+        const cpx<R> c = csel((__popc(j & zmask) & 1) != 0, c1, c0);
+        lds[j] = cmul(lds[j], c);
+      }
+    } else {
+      const uint32_t b = op.pair_bit, flip = op.flip, sign = op.sign;
+      const uint32_t lower = (1u << b) - 1u;
+      double u[8];
+      for (int k = 0; k < 8; ++k) u[k] = op.u[k];
+      for (uint32_t p = t; p < (count >> 1); p += BLOCK_THREADS) {
+        const uint32_t j0 = ((p & ~lower) << 1) | (p & lower);
+        if ((j0 & cond) != cond) continue;
+        const uint32_t j1 = j0 ^ flip;
+        const cpx<R> x0 = lds[j0], x1 = lds[j1];
+        cpx<R> y0, y1;
+        switch (mode) {
+          case BLOCK_H: pair_math<R, MODE_H>(u, sign, j0, x0, x1, y0, y1); break;
+          case BLOCK_SWAP: pair_math<R, MODE_SWAP>(u, sign, j0, x0, x1, y0, y1); break;
+          case BLOCK_ESWAP: pair_math<R, MODE_ESWAP>(u, sign, j0, x0, x1, y0, y1); break;
+          default: pair_math<R, MODE_MATRIX>(u, sign, j0, x0, x1, y0, y1); break;
+        }
+        lds[j0] = y0;
+        lds[j1] = y1;
+      }
+    }
+    __syncthreads();
+  }
+  if (active)
+    for (uint32_t k = 0; k < per; ++k)
+      store_amp<NT>(s + (base | low | prog->deposit_high[k]), lds[t + BLOCK_THREADS * k]);
+}
+
+// ---------------------------------------------------------------------------
 // Launch helpers
 // ---------------------------------------------------------------------------
+
+int launch_block(const block_program* device_program, const block_program& program, void* first, uint64_t size,
+                 int dtype, void* stream) {
+  unsigned nq;
+  if (int st = check_state(first, size, dtype, &nq)) return st;
+  const uint32_t cap = dtype == BRK_C128 ? BLOCK_MAX_LOCAL_C128 : BLOCK_MAX_LOCAL_C64;
+  if (program.num_local > cap || program.num_local > nq || program.num_ops > (uint32_t)BLOCK_MAX_OPS)
+    return fail(BRK_ERR_INVALID_ARGUMENT, "block program does not fit (%u local qubits, %u ops)", program.num_local,
+                program.num_ops);
+  const uint64_t chunks = size >> program.num_local;
+  if (chunks > 0x7fffffffull) return fail(BRK_ERR_INVALID_ARGUMENT, "too many chunks");
+  const bool nt = size * amp_bytes(dtype) >= get_tuning().nt_min_bytes;
+  const dim3 grid((unsigned)chunks), block(BLOCK_THREADS);
+  hipStream_t st = as_stream(stream);
+  if (dtype == BRK_C128) {
+    if (nt) hipLaunchKernelGGL((k_block<double, true>), grid, block, 0, st, (cpx<double>*)first, device_program);
+    else hipLaunchKernelGGL((k_block<double, false>), grid, block, 0, st, (cpx<double>*)first, device_program);
+  } else {
+    if (nt) hipLaunchKernelGGL((k_block<float, true>), grid, block, 0, st, (cpx<float>*)first, device_program);
+    else hipLaunchKernelGGL((k_block<float, false>), grid, block, 0, st, (cpx<float>*)first, device_program);
+  }
+  return check_launch("block");
+}
 
 static unsigned grid_blocks(uint64_t items) {
   uint64_t b = (items + GATE_THREADS - 1) / GATE_THREADS;
@@ -441,6 +538,148 @@ static void put(double* u, int slot, std::complex<double> v) {
   u[2 * slot + 1] = v.imag();
 }
 
+// ---------------------------------------------------------------------------
+// Gate coefficients, computed once on the host.  Every path that applies a
+// gate (the entry points below, gate blocks) takes them from these helpers.
+// ---------------------------------------------------------------------------
+
+// exp(i phase) with std::exp, as ket::utility::exp_i does (ket/include/ket/utility/exp_i.hpp:24-29).
+static std::complex<double> exp_i(double phase) { return std::exp(std::complex<double>{0.0, phase}); }
+
+// Y...Y: (Y^m a)[i] = (-i)^m (-1)^{popcount(i & T)} a[i ^ T]; for one target
+// Y = [[0, -i], [i, 0]] (docs/bra.md "Y t").  Sign by the targets.
+static void coeffs_pauli_y(unsigned num_targets, double* u) {
+  const std::complex<double> ph = minus_i_pow(num_targets);
+  const double sgn_m = (num_targets & 1u) ? -1.0 : 1.0;
+  put(u, 0, {0.0, 0.0});
+  put(u, 1, ph);
+  put(u, 2, ph * sgn_m);
+  put(u, 3, {0.0, 0.0});
+}
+
+// U2(p1, p2) = 1/sqrt2 [[1, -e^{i p2}], [e^{i p1}, e^{i(p1+p2)}]] (ket/include/ket/gate/phase_shift.hpp:649-652)
+static void coeffs_u2(double phase1, double phase2, double* u) {
+  const double r = 0.707106781186547524400844362104849039;
+  const std::complex<double> e1 = exp_i(phase1), e2 = exp_i(phase2);
+  put(u, 0, {r, 0.0});
+  put(u, 1, -e2 * r);
+  put(u, 2, e1 * r);
+  put(u, 3, (e2 * e1) * r);
+}
+
+// U3(t, p, l) = [[c, -e^{il} s], [e^{ip} s, e^{i(p+l)} c]], c = cos(t/2), s = sin(t/2)
+// (ket/include/ket/gate/phase_shift.hpp:1195-1198, coefficients :1219-1229)
+static void coeffs_u3(double phase1, double phase2, double phase3, double* u) {
+  const double sine = std::sin(0.5 * phase1);
+  const double cosine = std::cos(0.5 * phase1);
+  const std::complex<double> e2 = exp_i(phase2), e3 = exp_i(phase3);
+  put(u, 0, {cosine, 0.0});
+  put(u, 1, -(sine * e3));
+  put(u, 2, sine * e2);
+  put(u, 3, (cosine * e3) * e2);
+}
+
+// exp(i phase X..X) = cos I + i sin X..X; pairs (i, i ^ T).
+static void coeffs_exp_x(double phase, double* u) {
+  const std::complex<double> e = exp_i(phase);
+  const double c = e.real(), s = e.imag();
+  put(u, 0, {c, 0.0});
+  put(u, 1, {0.0, s});
+  put(u, 2, {0.0, s});
+  put(u, 3, {c, 0.0});
+}
+
+// exp(i phase Y..Y) = cos I + i sin Y..Y, with (Y^m a)[i] = (-i)^m (-1)^{popcount(i & T)} a[i ^ T].
+// For one target: a0' = c a0 + s a1, a1' = -s a0 + c a1
+// (ket/include/ket/gate/exponential_pauli_y.hpp:36-38, :84-87).  Sign by the targets.
+static void coeffs_exp_y(double phase, unsigned num_targets, double* u) {
+  const std::complex<double> e = exp_i(phase);
+  const double c = e.real(), s = e.imag();
+  const std::complex<double> ph = minus_i_pow(num_targets);
+  const double sgn_m = (num_targets & 1u) ? -1.0 : 1.0;
+  put(u, 0, {c, 0.0});
+  put(u, 1, std::complex<double>{0.0, s} * ph);
+  put(u, 2, std::complex<double>{0.0, s} * ph * sgn_m);
+  put(u, 3, {c, 0.0});
+}
+
+// exp(i phase SWAP): e^{i phase} for (00, 11), cos and sin of the same coefficient for (01, 10)
+// (ket/include/ket/gate/exponential_swap.hpp:31-90).
+static void coeffs_exp_swap(double phase, double* u) {
+  const std::complex<double> e = exp_i(phase);
+  put(u, 0, e);
+  u[2] = e.real();
+  u[3] = e.imag();
+}
+
+int describe_physical_op(int code, const unsigned* t, unsigned nt, const unsigned* c, unsigned nc,
+                         const double* p, gate_desc* d) {
+  *d = gate_desc{};
+  const uint64_t tmask = nt ? mask_of(t, nt) : 0, cmask = nc ? mask_of(c, nc) : 0;
+  d->qubits = tmask | cmask;
+  d->cond = cmask;
+  unsigned lowest = 64;
+  for (unsigned i = 0; i < nt; ++i) lowest = std::min(lowest, t[i]);
+  auto pair = [&](int mode) {
+    d->mode = mode;
+    d->pair_bit = lowest;
+    d->flip = tmask;
+  };
+  auto diag = [&](uint64_t cond, uint64_t zmask, std::complex<double> d0, std::complex<double> d1) {
+    d->mode = BLOCK_DIAG;
+    d->cond = cond;
+    d->zmask = zmask;
+    put(d->u, 0, d0);
+    put(d->u, 1, d1);
+  };
+  switch (code) {
+    case BRK_OP_H: pair(BLOCK_H); break;
+    case BRK_OP_X: pair(BLOCK_SWAP); break;
+    case BRK_OP_Y:
+      pair(BLOCK_MATRIX);
+      coeffs_pauli_y(nt, d->u);
+      d->sign = tmask;
+      break;
+    case BRK_OP_Z: diag(tmask | cmask, 0, {-1.0, 0.0}, {-1.0, 0.0}); break;
+    case BRK_OP_U1: {
+      const std::complex<double> e = exp_i(p[0]);
+      diag(tmask | cmask, 0, e, e);
+      break;
+    }
+    case BRK_OP_U2: pair(BLOCK_MATRIX); coeffs_u2(p[0], p[1], d->u); break;
+    case BRK_OP_U3: pair(BLOCK_MATRIX); coeffs_u3(p[0], p[1], p[2], d->u); break;
+    case BRK_OP_MATRIX: pair(BLOCK_MATRIX); std::copy(p, p + 8, d->u); break;
+    case BRK_OP_EX: pair(BLOCK_MATRIX); coeffs_exp_x(p[0], d->u); break;
+    case BRK_OP_RX: pair(BLOCK_MATRIX); coeffs_exp_x(-0.5 * p[0], d->u); break;
+    case BRK_OP_EY:
+    case BRK_OP_RY:
+      pair(BLOCK_MATRIX);
+      coeffs_exp_y(code == BRK_OP_EY ? p[0] : -0.5 * p[0], nt, d->u);
+      d->sign = tmask;
+      break;
+    case BRK_OP_EZ:
+    case BRK_OP_RZ: {
+      const std::complex<double> e = exp_i(code == BRK_OP_EZ ? p[0] : -0.5 * p[0]);
+      diag(cmask, tmask, e, std::conj(e));
+      break;
+    }
+    case BRK_OP_SWAP:
+      // pairs ...1(t0)...0(t1)... <-> ...0(t0)...1(t1)...: t1 is the inserted bit, t0 must be set
+      d->mode = BLOCK_SWAP;
+      d->pair_bit = t[1];
+      d->flip = tmask;
+      d->cond = cmask | (1ull << t[0]);
+      break;
+    case BRK_OP_ESWAP:
+      pair(BLOCK_ESWAP);
+      d->sign = 1ull << std::max(t[0], t[1]);
+      coeffs_exp_swap(p[0], d->u);
+      break;
+    default: return fail(BRK_ERR_UNSUPPORTED, "op %d has no gate description", code);
+  }
+  return BRK_OK;
+}
+
 }  // namespace brk
 
 using namespace brk;
@@ -471,15 +710,8 @@ int brk_controlled_not(void* first, uint64_t size, int dtype, unsigned target, c
 
 int brk_pauli_y(void* first, uint64_t size, int dtype, const unsigned* targets, unsigned num_targets,
                 const unsigned* controls, unsigned num_controls, void* stream) {
-  // Y...Y: (Y^m a)[i] = (-i)^m (-1)^{popcount(i & T)} a[i ^ T]; for one target
-  // Y = [[0, -i], [i, 0]] (docs/bra.md "Y t").
-  const std::complex<double> ph = minus_i_pow(num_targets);
-  const double sgn_m = (num_targets & 1u) ? -1.0 : 1.0;
   double u[8];
-  put(u, 0, {0.0, 0.0});
-  put(u, 1, ph);
-  put(u, 2, ph * sgn_m);
-  put(u, 3, {0.0, 0.0});
+  coeffs_pauli_y(num_targets, u);
   return multi_target_flip(first, size, dtype, targets, num_targets, controls, num_controls, MODE_MATRIX, u, true,
                            stream);
 }
@@ -502,72 +734,37 @@ int brk_phase_shift_coeff(void* first, uint64_t size, int dtype, const double co
 
 int brk_phase_shift(void* first, uint64_t size, int dtype, double phase, const unsigned* qubits,
                     unsigned num_qubits, void* stream) {
-  // exp(i phase) computed once on the host with std::exp, as ket::utility::exp_i
-  // does (ket/include/ket/utility/exp_i.hpp:24-29).
-  const std::complex<double> c = std::exp(std::complex<double>{0.0, phase});
+  const std::complex<double> c = exp_i(phase);
   const double cc[2] = {c.real(), c.imag()};
   return brk_phase_shift_coeff(first, size, dtype, cc, qubits, num_qubits, stream);
 }
 
 int brk_phase_shift2(void* first, uint64_t size, int dtype, double phase1, double phase2, unsigned target,
                      const unsigned* controls, unsigned num_controls, void* stream) {
-  // U2(p1, p2) = 1/sqrt2 [[1, -e^{i p2}], [e^{i p1}, e^{i(p1+p2)}]]
-  // (ket/include/ket/gate/phase_shift.hpp:649-652)
-  const double r = 0.707106781186547524400844362104849039;
-  const std::complex<double> e1 = std::exp(std::complex<double>{0.0, phase1});
-  const std::complex<double> e2 = std::exp(std::complex<double>{0.0, phase2});
   double u[8];
-  put(u, 0, {r, 0.0});
-  put(u, 1, -e2 * r);
-  put(u, 2, e1 * r);
-  put(u, 3, (e2 * e1) * r);
+  coeffs_u2(phase1, phase2, u);
   return single_target(first, size, dtype, target, controls, num_controls, MODE_MATRIX, u, stream);
 }
 
 int brk_phase_shift3(void* first, uint64_t size, int dtype, double phase1, double phase2, double phase3,
                      unsigned target, const unsigned* controls, unsigned num_controls, void* stream) {
-  // U3(t, p, l) = [[c, -e^{il} s], [e^{ip} s, e^{i(p+l)} c]], c = cos(t/2), s = sin(t/2)
-  // (ket/include/ket/gate/phase_shift.hpp:1195-1198, coefficients :1219-1229)
-  const double sine = std::sin(0.5 * phase1);
-  const double cosine = std::cos(0.5 * phase1);
-  const std::complex<double> e2 = std::exp(std::complex<double>{0.0, phase2});
-  const std::complex<double> e3 = std::exp(std::complex<double>{0.0, phase3});
   double u[8];
-  put(u, 0, {cosine, 0.0});
-  put(u, 1, -(sine * e3));
-  put(u, 2, sine * e2);
-  put(u, 3, (cosine * e3) * e2);
+  coeffs_u3(phase1, phase2, phase3, u);
   return single_target(first, size, dtype, target, controls, num_controls, MODE_MATRIX, u, stream);
 }
 
 int brk_exponential_pauli_x(void* first, uint64_t size, int dtype, double phase, const unsigned* targets,
                             unsigned num_targets, const unsigned* controls, unsigned num_controls, void* stream) {
-  // exp(i phase X..X) = cos I + i sin X..X; pairs (i, i ^ T).
-  const std::complex<double> e = std::exp(std::complex<double>{0.0, phase});
-  const double c = e.real(), s = e.imag();
   double u[8];
-  put(u, 0, {c, 0.0});
-  put(u, 1, {0.0, s});
-  put(u, 2, {0.0, s});
-  put(u, 3, {c, 0.0});
+  coeffs_exp_x(phase, u);
   return multi_target_flip(first, size, dtype, targets, num_targets, controls, num_controls, MODE_MATRIX, u, false,
                            stream);
 }
 
 int brk_exponential_pauli_y(void* first, uint64_t size, int dtype, double phase, const unsigned* targets,
                             unsigned num_targets, const unsigned* controls, unsigned num_controls, void* stream) {
-  // exp(i phase Y..Y) = cos I + i sin Y..Y, with (Y^m a)[i] = (-i)^m (-1)^{popcount(i & T)} a[i ^ T].
-  // For one target: a0' = c a0 + s a1, a1' = -s a0 + c a1
-  // (ket/include/ket/gate/exponential_pauli_y.hpp:36-38, :84-87).
-  const std::complex<double> e = std::exp(std::complex<double>{0.0, phase});
-  const double c = e.real(), s = e.imag();
-  const std::complex<double> ph = minus_i_pow(num_targets);
-  const double sgn_m = (num_targets & 1u) ? -1.0 : 1.0;
   double u[8];
-  put(u, 0, {c, 0.0});
-  put(u, 1, std::complex<double>{0.0, s} * ph);
-  put(u, 2, std::complex<double>{0.0, s} * ph * sgn_m);
-  put(u, 3, {c, 0.0});
+  coeffs_exp_y(phase, num_targets, u);
   return multi_target_flip(first, size, dtype, targets, num_targets, controls, num_controls, MODE_MATRIX, u, true,
                            stream);
 }
@@ -579,7 +776,7 @@ int brk_exponential_pauli_z(void* first, uint64_t size, int dtype, double phase,
   if (num_targets == 0) return fail(BRK_ERR_INVALID_ARGUMENT, "no target qubits");
   validated v;
   if (int st = validate(first, size, dtype, targets, num_targets, controls, num_controls, &v)) return st;
-  const std::complex<double> e = std::exp(std::complex<double>{0.0, phase});
+  const std::complex<double> e = exp_i(phase);
   return diagonal(first, size, dtype, controls, num_controls, mask_of(targets, num_targets), e, std::conj(e),
                   stream);
 }
@@ -628,11 +825,7 @@ int brk_exponential_swap(void* first, uint64_t size, int dtype, double phase, un
   gate_args a = make_args(p);
   a.flipmask = (1ull << lo) | (1ull << hi);
   a.signmask = 1ull << hi;
-  // e^{i phase} once on the host, as ket::utility::exp_i (exp_i.hpp:24-29); cos and sin are its parts
-  const std::complex<double> e = std::exp(std::complex<double>{0.0, phase});
-  put(a.u, 0, e);
-  a.u[2] = e.real();
-  a.u[3] = e.imag();
+  coeffs_exp_swap(phase, a.u);
   return launch(dtype, FAM_PAIR, MODE_ESWAP, first, size, size >> p.np, a, stream);
 }
 

@@ -1,0 +1,273 @@
+// Gate-block planner and its C ABI (brk_block_plan_*, brk_circuit_run_blocked).
+//
+// The circuit's ops are walked in order with the logical -> physical qubit
+// map (as brk_circuit_run does).  Consecutive ops are grouped greedily while
+// the union of their qubits, together with the qubits that address one
+// 128-byte line (so every chunk access is whole lines), fits in the local
+// bits of a chunk; the local set is then filled up with the lowest remaining
+// qubits.  A group of two or more ops becomes one block launch; a single op
+// and every local swap run on the plain kernels.  This is the role of bra's
+// BEGIN/END FUSION (bra/src/state.cpp:2193-2217, nompi_state.cpp:1282-1357)
+// and of ket's cache-aware gate functions (gate_bitmasks.hpp:587-2134): a
+// run of gates applied per group of amplitudes held near the core, with the
+// per-gate arithmetic unchanged.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "brk_block.hpp"
+#include "braket_hip.h"
+#include "circuit.hpp"
+
+namespace brk {
+int fail(int status, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+}
+
+struct brk_block_plan {
+  struct step {
+    unsigned first_op = 0, num_ops = 0;  // ops of the circuit covered by this step
+    int program = -1;                    // index into programs, or -1: plain op
+    brk_circuit::op physical;            // the op at physical qubits (plain steps)
+  };
+  unsigned num_qubits = 0;
+  int dtype = BRK_C128;
+  std::vector<step> steps;
+  std::vector<brk::block_program> programs;
+  std::vector<unsigned> final_permutation;
+  brk::block_program* device = nullptr;
+  int device_id = -1;
+  ~brk_block_plan() {
+    if (device) (void)hipFree(device);
+  }
+};
+
+namespace brk {
+
+static unsigned popcount64(uint64_t x) { return (unsigned)__builtin_popcountll(x); }
+
+struct block_builder {
+  unsigned n, max_local, line_bits;
+  uint64_t qubits = 0;  // union of the ops' qubits
+  std::vector<gate_desc> descs;
+  std::vector<brk_circuit::op> ops;
+  unsigned first_op = 0;
+
+  uint64_t line_mask() const { return (1ull << std::min(line_bits, n)) - 1ull; }
+  bool fits(const gate_desc& d) const {
+    return descs.size() < (size_t)BLOCK_MAX_OPS && popcount64(qubits | d.qubits | line_mask()) <= max_local;
+  }
+};
+
+// Local positions: the block's qubits, the line qubits, then the lowest others up to L.
+static block_program make_program(const block_builder& b) {
+  block_program prog{};
+  const unsigned L = std::min(b.max_local, b.n);
+  uint64_t local = b.qubits | b.line_mask();
+  for (unsigned q = 0; q < b.n && popcount64(local) < L; ++q) local |= 1ull << q;
+  unsigned r = 0, rank[64] = {};
+  for (unsigned q = 0; q < b.n; ++q)
+    if (local >> q & 1u) {
+      rank[q] = r;
+      prog.pos[r++] = q;
+    }
+  prog.num_local = r;
+  auto to_local = [&](uint64_t m) {
+    uint32_t out = 0;
+    for (unsigned q = 0; q < b.n; ++q)
+      if (m >> q & 1u) out |= 1u << rank[q];
+    return out;
+  };
+  // local index j = t + BLOCK_THREADS * k: the physical offset of the k part
+  const uint32_t count = 1u << r, per = count > (uint32_t)BLOCK_THREADS ? count / BLOCK_THREADS : 1;
+  for (uint32_t k = 0; k < per; ++k) {
+    const uint32_t j = k << BLOCK_THREAD_BITS;
+    uint64_t dep = 0;
+    for (unsigned bit = 0; bit < r; ++bit)
+      if (j >> bit & 1u) dep |= 1ull << prog.pos[bit];
+    prog.deposit_high[k] = dep;
+  }
+  prog.num_ops = (uint32_t)b.descs.size();
+  for (size_t i = 0; i < b.descs.size(); ++i) {
+    const gate_desc& d = b.descs[i];
+    block_op& o = prog.ops[i];
+    o.mode = d.mode;
+    o.pair_bit = d.mode == BLOCK_DIAG ? 0 : rank[d.pair_bit];
+    o.flip = to_local(d.flip);
+    o.cond = to_local(d.cond);
+    o.sign = to_local(d.sign);
+    o.zmask = to_local(d.zmask);
+    std::copy(d.u, d.u + 8, o.u);
+  }
+  return prog;
+}
+
+static brk_circuit::op map_op(const brk_circuit::op& o, const permutation& perm) {
+  brk_circuit::op m = o;
+  for (auto& q : m.targets) q = perm.perm[q];
+  for (auto& q : m.controls) q = perm.perm[q];
+  return m;
+}
+
+static void flush(block_builder& b, brk_block_plan& plan) {
+  if (b.ops.empty()) return;
+  if (b.ops.size() == 1) {
+    brk_block_plan::step s;
+    s.first_op = b.first_op;
+    s.num_ops = 1;
+    s.physical = b.ops[0];
+    plan.steps.push_back(std::move(s));
+  } else {
+    brk_block_plan::step s;
+    s.first_op = b.first_op;
+    s.num_ops = (unsigned)b.ops.size();
+    s.program = (int)plan.programs.size();
+    plan.programs.push_back(make_program(b));
+    plan.steps.push_back(std::move(s));
+  }
+  b.qubits = 0;
+  b.descs.clear();
+  b.ops.clear();
+}
+
+static int build_plan(const brk_circuit* circuit, int dtype, const unsigned* permutation_in, brk_block_plan* plan) {
+  const unsigned n = circuit->num_qubits;
+  plan->num_qubits = n;
+  plan->dtype = dtype;
+  permutation perm(n);
+  if (permutation_in && !perm.load(permutation_in, n))
+    return fail(BRK_ERR_INVALID_ARGUMENT, "permutation is not a permutation of 0..%u", n - 1);
+  block_builder b;
+  b.n = n;
+  b.max_local = dtype == BRK_C128 ? BLOCK_MAX_LOCAL_C128 : BLOCK_MAX_LOCAL_C64;
+  b.line_bits = dtype == BRK_C128 ? 3 : 4;  // 128-byte lines
+  for (unsigned i = 0; i < circuit->ops.size(); ++i) {
+    const auto& o = circuit->ops[i];
+    if (o.code == BRK_OP_LOCAL_SWAP) {
+      flush(b, *plan);
+      brk_block_plan::step s;
+      s.first_op = i;
+      s.num_ops = 1;
+      s.physical = o;
+      plan->steps.push_back(std::move(s));
+      perm.permutate(perm.inverse[o.targets[0]], perm.inverse[o.targets[1]]);
+      continue;
+    }
+    const brk_circuit::op m = map_op(o, perm);
+    gate_desc d;
+    if (int st = describe_physical_op(m.code, m.targets.data(), (unsigned)m.targets.size(),
+                                      m.controls.empty() ? nullptr : m.controls.data(), (unsigned)m.controls.size(),
+                                      m.params.data(), &d))
+      return st;
+    if (!b.fits(d)) flush(b, *plan);
+    if (b.ops.empty()) b.first_op = i;
+    b.qubits |= d.qubits;
+    b.descs.push_back(d);
+    b.ops.push_back(m);
+  }
+  flush(b, *plan);
+  plan->final_permutation = perm.perm;
+  return BRK_OK;
+}
+
+static int upload(brk_block_plan* plan) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return fail(BRK_ERR_HIP, "hipGetDevice failed");
+  if (plan->device && plan->device_id == dev) return BRK_OK;
+  if (plan->device) (void)hipFree(plan->device);
+  plan->device = nullptr;
+  if (plan->programs.empty()) {
+    plan->device_id = dev;
+    return BRK_OK;
+  }
+  const size_t bytes = sizeof(block_program) * plan->programs.size();
+  if (hipMalloc((void**)&plan->device, bytes) != hipSuccess) return fail(BRK_ERR_OUT_OF_MEMORY, "block programs");
+  if (hipMemcpy(plan->device, plan->programs.data(), bytes, hipMemcpyHostToDevice) != hipSuccess)
+    return fail(BRK_ERR_HIP, "uploading the block programs failed");
+  plan->device_id = dev;
+  return BRK_OK;
+}
+
+static int run_plan(brk_block_plan* plan, void* first, void* stream) {
+  if (int st = upload(plan)) return st;
+  const uint64_t size = 1ull << plan->num_qubits;
+  for (const auto& s : plan->steps) {
+    int st;
+    if (s.program < 0) st = apply_physical_op(s.physical, first, size, plan->dtype, stream);
+    else st = launch_block(plan->device + s.program, plan->programs[s.program], first, size, plan->dtype, stream);
+    if (st) return st;
+  }
+  return BRK_OK;
+}
+
+}  // namespace brk
+
+using namespace brk;
+
+extern "C" {
+
+int brk_block_plan_create(const brk_circuit* circuit, int dtype, const unsigned* permutation, brk_block_plan** out) {
+  if (!circuit || !out) return fail(BRK_ERR_INVALID_ARGUMENT, "null argument");
+  if (dtype != BRK_C128 && dtype != BRK_C64) return fail(BRK_ERR_UNSUPPORTED_DTYPE, "dtype %d", dtype);
+  *out = nullptr;
+  auto* plan = new (std::nothrow) brk_block_plan;
+  if (!plan) return fail(BRK_ERR_OUT_OF_MEMORY, "cannot allocate the plan");
+  if (int st = build_plan(circuit, dtype, permutation, plan)) {
+    delete plan;
+    return st;
+  }
+  *out = plan;
+  return BRK_OK;
+}
+
+int brk_block_plan_destroy(brk_block_plan* plan) {
+  delete plan;
+  return BRK_OK;
+}
+
+int brk_block_plan_num_steps(const brk_block_plan* plan, unsigned* num_steps) {
+  if (!plan || !num_steps) return fail(BRK_ERR_INVALID_ARGUMENT, "null argument");
+  *num_steps = (unsigned)plan->steps.size();
+  return BRK_OK;
+}
+
+int brk_block_plan_step(const brk_block_plan* plan, unsigned i, unsigned* first_op, unsigned* num_ops,
+                        unsigned* num_local, unsigned* local_qubits) {
+  if (!plan) return fail(BRK_ERR_INVALID_ARGUMENT, "plan is null");
+  if (i >= plan->steps.size()) return fail(BRK_ERR_INVALID_ARGUMENT, "step %u out of range", i);
+  const auto& s = plan->steps[i];
+  if (first_op) *first_op = s.first_op;
+  if (num_ops) *num_ops = s.num_ops;
+  const unsigned L = s.program < 0 ? 0 : plan->programs[s.program].num_local;
+  if (num_local) *num_local = L;
+  if (local_qubits && L) std::copy(plan->programs[s.program].pos, plan->programs[s.program].pos + L, local_qubits);
+  return BRK_OK;
+}
+
+int brk_block_plan_final_permutation(const brk_block_plan* plan, unsigned* permutation) {
+  if (!plan || !permutation) return fail(BRK_ERR_INVALID_ARGUMENT, "null argument");
+  std::copy(plan->final_permutation.begin(), plan->final_permutation.end(), permutation);
+  return BRK_OK;
+}
+
+int brk_block_plan_run(brk_block_plan* plan, void* first, void* stream) {
+  if (!plan) return fail(BRK_ERR_INVALID_ARGUMENT, "plan is null");
+  return run_plan(plan, first, stream);
+}
+
+int brk_circuit_run_blocked(const brk_circuit* circuit, void* first, int dtype, unsigned* permutation,
+                            void* stream) {
+  brk_block_plan* plan = nullptr;
+  if (int st = brk_block_plan_create(circuit, dtype, permutation, &plan)) return st;
+  int st = run_plan(plan, first, stream);
+  // the programs must stay resident until the launches that read them are done
+  if (hipStreamSynchronize(reinterpret_cast<hipStream_t>(stream)) != hipSuccess && !st)
+    st = fail(BRK_ERR_HIP, "hipStreamSynchronize failed");
+  if (!st && permutation) std::copy(plan->final_permutation.begin(), plan->final_permutation.end(), permutation);
+  delete plan;
+  return st;
+}
+
+}  // extern "C"

@@ -124,6 +124,11 @@ class hip_state {
   // every gate of a parsed circuit, in order (what interpreter::apply_circuit does,
   // bra/src/interpreter.cpp:851-872); nompi mode applies gates at their own qubits
   void apply(const brk_circuit* circuit) { check(brk_circuit_run(circuit, data_, BRK_C128, nullptr, stream_)); }
+  // the same gates in blocks held in LDS (one HBM pass per block; bit-identical amplitudes):
+  // what do_end_fusion does for a fused run of gates (bra/src/nompi_state.cpp:1282-1357)
+  void apply_blocked(const brk_circuit* circuit) {
+    check(brk_circuit_run_blocked(circuit, data_, BRK_C128, nullptr, stream_));
+  }
 
   // bra::state::shor_box: exponent qubits are the top m, the rest hold base^x mod divisor
   // (bra/src/state.cpp:2173-2191, bra/src/nompi_state.cpp:1276-1280)

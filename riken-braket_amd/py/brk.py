@@ -92,6 +92,13 @@ _SIGS = {
     "brk_circuit_num_qubits": [_p, _up],
     "brk_circuit_run": [_p, _p, _i, _up, _p],
     "brk_circuit_run_timed": [_p, _p, _i, _up, C.POINTER(C.c_float), _p],
+    "brk_block_plan_create": [_p, _i, _up, C.POINTER(C.c_void_p)],
+    "brk_block_plan_destroy": [_p],
+    "brk_block_plan_num_steps": [_p, _up],
+    "brk_block_plan_step": [_p, _u, _up, _up, _up, _up],
+    "brk_block_plan_final_permutation": [_p, _up],
+    "brk_block_plan_run": [_p, _p, _p],
+    "brk_circuit_run_blocked": [_p, _p, _i, _up, _p],
     "brk_fused_plan_create": [_p, _u, _up, C.POINTER(C.c_void_p)],
     "brk_fused_plan_destroy": [_p],
     "brk_fused_plan_num_blocks": [_p, _up],
@@ -367,6 +374,41 @@ OP = dict(H=0, X=1, Y=2, Z=3, U1=4, U2=5, U3=6, EX=7, EY=8, EZ=9, RX=10, RY=11, 
           LOCAL_SWAP=14, MATRIX=15, ESWAP=16)
 
 
+class BlockPlan:
+    """brk_block_plan of a Circuit: steps are (first_op, num_ops, local_qubits), local_qubits empty for a
+    plain step."""
+
+    def __init__(self, circuit: "Circuit", dtype=C128, permutation=None):
+        self.circuit = circuit
+        n = circuit.num_qubits
+        pa = None if permutation is None else (C.c_uint * n)(*permutation)
+        h = C.c_void_p()
+        call("brk_block_plan_create", circuit.h, dtype, pa, C.byref(h))
+        self.h = h.value
+        cnt = C.c_uint()
+        call("brk_block_plan_num_steps", self.h, C.byref(cnt))
+        self.steps = []
+        for i in range(cnt.value):
+            f, k, nl = C.c_uint(), C.c_uint(), C.c_uint()
+            lq = (C.c_uint * 16)()
+            call("brk_block_plan_step", self.h, i, C.byref(f), C.byref(k), C.byref(nl), lq)
+            self.steps.append((f.value, k.value, tuple(lq[: nl.value])))
+        fp = (C.c_uint * n)()
+        call("brk_block_plan_final_permutation", self.h, fp)
+        self.final_permutation = list(fp)
+
+    def run(self, ptr, stream=0):
+        call("brk_block_plan_run", self.h, ptr, stream)
+
+    def __del__(self):
+        try:
+            if self.h:
+                load().brk_block_plan_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
 class Circuit:
     """Owns a brk_circuit built from (name, targets, controls, params) tuples."""
 
@@ -394,6 +436,13 @@ class Circuit:
         perm = list(range(self.num_qubits)) if permutation is None else list(permutation)
         pa = (C.c_uint * self.num_qubits)(*perm)
         call("brk_circuit_run", self.h, ptr, dtype, pa, stream)
+        return list(pa)
+
+    def run_blocked(self, ptr, dtype=C128, permutation=None, stream=0):
+        """brk_circuit_run_blocked: gate blocks in LDS, same amplitudes as run()."""
+        perm = list(range(self.num_qubits)) if permutation is None else list(permutation)
+        pa = (C.c_uint * self.num_qubits)(*perm)
+        call("brk_circuit_run_blocked", self.h, ptr, dtype, pa, stream)
         return list(pa)
 
     def run_timed(self, ptr, dtype=C128, permutation=None, stream=0):
