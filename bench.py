@@ -204,6 +204,53 @@ def fused_leg(circuit, plain_state, args, stream):
     }
 
 
+def blocked_leg(circuit, plain_state, args, stream):
+    """The same circuit through gate blocks (brk_block_plan_run) on a second complex128 state.
+
+    Amplitudes must equal the per-gate run bit for bit: 4096 seeded indices are compared
+    exactly and the inner product with the per-gate state must be 1 within 1e-12."""
+    import numpy as np
+
+    import brk
+    import hip
+    from devstate import DeviceState
+    n = circuit.num_qubits
+    t0 = time.perf_counter()
+    plan = brk.BlockPlan(circuit, brk.C128)
+    plan_s = time.perf_counter() - t0
+    blocks = [k for _, k, local in plan.steps if local]
+    st = DeviceState(n)
+    times = []
+    for k in range(args.warmup + args.steps):
+        brk.set_basis_state(st.ptr, st.size, brk.C128, 0, stream)
+        hip.synchronize(stream)
+        e0, e1 = hip.Event(), hip.Event()
+        e0.record(stream)
+        plan.run(st.ptr, stream)
+        e1.record(stream)
+        e1.synchronize()
+        if k >= args.warmup:
+            times.append(e0.elapsed_ms(e1))
+            log(f"blocked step {k - args.warmup + 1}/{args.steps}: {times[-1]:.1f} ms")
+    rng = np.random.default_rng(3)
+    idx = sorted(set(int(i) for i in rng.integers(0, 1 << n, 4096)))
+    same = bool(np.array_equal(st.download_indices(idx).view(np.uint64),
+                               plain_state.download_indices(idx).view(np.uint64)))
+    ip = brk.inner_product(st.ptr, brk.C128, plain_state.ptr, brk.C128, st.size, stream)
+    st.free()
+    ms = sum(times) / len(times)
+    return {
+        "ms_per_circuit": round(ms, 2),
+        "blocks": len(blocks),
+        "ops_in_blocks": int(sum(blocks)),
+        "plain_steps": len(plan.steps) - len(blocks),
+        "block_hbm_GB_per_launch": round(2 * 16 * (1 << n) / 1e9, 2),
+        "plan_s": round(plan_s, 4),
+        "sampled_amplitudes_bit_identical": same,
+        "inner_product_error": abs(ip - 1.0),
+    }
+
+
 def plain_energy(n, ops, J, h, stream):
     import brk
     from devstate import DeviceState
@@ -329,6 +376,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fused", action="store_true", help="skip the fused complex64 (precision form) leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batching-form leg")
+    ap.add_argument("--no-blocked", action="store_true", help="skip the gate-block (complex128, LDS) leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiation-form leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run the streaming copy kernel once on 2^30 c128 amplitudes (16 GiB read + "
@@ -389,6 +437,9 @@ def main():
     fused = None
     if not args.no_fused:
         fused = fused_leg(circuit, st, args, stream)
+    blocked = None
+    if not args.no_blocked:
+        blocked = blocked_leg(circuit, st, args, stream)
     st.free()
 
     ms_per_step = sum(step_ms) / len(step_ms)
@@ -457,6 +508,9 @@ def main():
     if fused is not None:
         fused["speedup_vs_c128"] = round(ms_per_step / fused["ms_per_circuit"], 2)
         result["fused_c64"] = fused
+    if blocked is not None:
+        blocked["speedup_vs_per_gate"] = round(ms_per_step / blocked["ms_per_circuit"], 2)
+        result["blocked_c128"] = blocked
     if not args.no_batch:
         result["batch_c64"] = batch_leg(args, stream)
     if not args.no_adjoint:

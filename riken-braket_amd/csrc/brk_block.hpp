@@ -14,7 +14,7 @@
 
 namespace brk {
 
-constexpr int BLOCK_LDS_BYTES = 65536;            // LDS per workgroup (two workgroups per CU)
+constexpr int BLOCK_LDS_BYTES = 65536;            // largest chunk: 64 KiB of LDS (two workgroups per CU)
 constexpr int BLOCK_THREADS = 512;
 constexpr int BLOCK_MAX_LOCAL_C128 = 12;          // 2^12 x 16 B = 64 KiB
 constexpr int BLOCK_MAX_LOCAL_C64 = 13;           // 2^13 x 8 B  = 64 KiB
@@ -43,10 +43,14 @@ int describe_physical_op(int code, const unsigned* targets, unsigned num_targets
                          unsigned num_controls, const double* params, gate_desc* out);
 
 // A gate of a block with its masks over local bit positions (bit r of a
-// chunk index <-> physical qubit pos[r]).
+// chunk index <-> physical qubit pos[r]).  The gate visits only the local
+// indices it acts on: item v -> j = v with zeros inserted at ins[0..nins)
+// (ascending: the pair bit and the cond bits), then the cond bits set.
 struct block_op {
   int32_t mode;
   uint32_t pair_bit, flip, cond, sign, zmask;
+  uint32_t nins;
+  uint32_t ins[15];
   double u[8];
 };
 
@@ -57,6 +61,11 @@ struct block_program {
   uint64_t deposit_high[16];            // physical offset of local index k << BLOCK_THREAD_BITS
   block_op ops[BLOCK_MAX_OPS];
 };
+
+// Local qubits of a chunk for this dtype under the current tuning (block_lds).
+unsigned block_max_local(int dtype);
+// Number of lowest qubits every chunk keeps local (contiguous runs of block_run_bytes).
+unsigned block_run_qubits(int dtype);
 
 // Launches one block over a state of `size` amplitudes: 2^(n - L) workgroups.
 // `device_program` points to a block_program in device memory whose copy on

@@ -222,50 +222,68 @@ __global__ __launch_bounds__(GATE_THREADS) void k_diag(cpx<R>* __restrict__ s, u
 
 typedef const __attribute__((address_space(4))) block_program* cprogram;
 
-template <typename R, bool NT>
-__global__ __launch_bounds__(BLOCK_THREADS) void k_block(cpx<R>* __restrict__ s, const block_program* program) {
-  constexpr uint32_t CAP = BLOCK_LDS_BYTES / sizeof(cpx<R>);
-  __shared__ cpx<R> lds[CAP];
-  const cprogram prog = (cprogram)program;  // uniform: scalar loads
-  const uint32_t L = prog->num_local, count = 1u << L;
-  // chunk index -> global bits outside the local positions
-  uint64_t base = blockIdx.x;
+// Chunk index -> global bits outside the local positions.
+__device__ __forceinline__ uint64_t chunk_base(uint64_t c, cprogram prog, uint32_t L) {
   for (uint32_t r = 0; r < L; ++r) {
     const uint32_t q = prog->pos[r];
-    base = ((base >> q) << (q + 1)) | (base & ((1ull << q) - 1ull));
+    c = ((c >> q) << (q + 1)) | (c & ((1ull << q) - 1ull));
   }
+  return c;
+}
+
+// Local index of item v of a gate: zeros inserted at op.ins[0 .. nins), then
+// the cond bits set.  The common cases (no insertion: uncontrolled diagonal
+// gates; one: uncontrolled pair gates) avoid the loop.
+__device__ __forceinline__ uint32_t op_index(uint32_t v, const __attribute__((address_space(4))) block_op& op,
+                                             uint32_t nins, uint32_t set, uint32_t lower0) {
+  if (nins == 0) return v;
+  if (nins == 1) return ((v & ~lower0) << 1) | (v & lower0) | set;
+  for (uint32_t r = 0; r < nins; ++r) {
+    const uint32_t q = op.ins[r];
+    v = ((v >> q) << (q + 1)) | (v & ((1u << q) - 1u));
+  }
+  return v | set;
+}
+
+// One workgroup per chunk of exactly PER * BLOCK_THREADS amplitudes
+// (L = 9 + log2 PER), so every global access of the load and store phases is
+// unconditional.  A gate visits only the local indices it acts on (controls
+// are inserted, not tested), in a strided loop over the workgroup.
+template <typename R, bool NT, int PER>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_block(cpx<R>* __restrict__ s, const block_program* program) {
+  constexpr uint32_t CAP = PER * BLOCK_THREADS;
+  __shared__ cpx<R> lds[CAP];
+  const cprogram prog = (cprogram)program;  // uniform: scalar loads
+  const uint32_t L = prog->num_local;
+  const uint64_t base = chunk_base(blockIdx.x, prog, L);
   // local index j = t + BLOCK_THREADS * k: the bits of t go to pos[0 .. 8]
   const uint32_t t = threadIdx.x;
   uint64_t low = 0;
-  for (uint32_t r = 0; r < L && r < (uint32_t)BLOCK_THREAD_BITS; ++r) low |= (uint64_t)((t >> r) & 1u) << prog->pos[r];
-  const uint32_t per = count > BLOCK_THREADS ? count / BLOCK_THREADS : 1;
-  const bool active = t < count;
-  if (active)
-    for (uint32_t k = 0; k < per; ++k)
-      lds[t + BLOCK_THREADS * k] = load_amp<NT>(s + (base | low | prog->deposit_high[k]));
+  for (uint32_t r = 0; r < (uint32_t)BLOCK_THREAD_BITS; ++r) low |= (uint64_t)((t >> r) & 1u) << prog->pos[r];
+#pragma unroll
+  for (uint32_t k = 0; k < PER; ++k)
+    lds[t + BLOCK_THREADS * k] = load_amp<NT>(s + (base | low | prog->deposit_high[k]));
   __syncthreads();
   const uint32_t nops = prog->num_ops;
   for (uint32_t o = 0; o < nops; ++o) {
     const auto& op = prog->ops[o];
     const int mode = op.mode;
-    const uint32_t cond = op.cond;
+    const uint32_t nins = op.nins, set = op.cond;
+    const uint32_t items = CAP >> nins;
+    const uint32_t lower0 = (1u << op.ins[0]) - 1u;  // below the first inserted bit
     if (mode == BLOCK_DIAG) {
       const cpx<R> c0 = {(R)op.u[0], (R)op.u[1]}, c1 = {(R)op.u[2], (R)op.u[3]};
       const uint32_t zmask = op.zmask;
-      for (uint32_t j = t; j < count; j += BLOCK_THREADS) {
-        if ((j & cond) != cond) continue;
-        const cpx<R> c = csel((__popc(j & zmask) & 1) != 0, c1, c0);
-        lds[j] = cmul(lds[j], c);
+      for (uint32_t v = t; v < items; v += BLOCK_THREADS) {
+        const uint32_t j = op_index(v, op, nins, set, lower0);
+        lds[j] = cmul(lds[j], csel((__popc(j & zmask) & 1) != 0, c1, c0));
       }
     } else {
-      const uint32_t b = op.pair_bit, flip = op.flip, sign = op.sign;
-      const uint32_t lower = (1u << b) - 1u;
+      const uint32_t flip = op.flip, sign = op.sign;
       double u[8];
       for (int k = 0; k < 8; ++k) u[k] = op.u[k];
-      for (uint32_t p = t; p < (count >> 1); p += BLOCK_THREADS) {
-        const uint32_t j0 = ((p & ~lower) << 1) | (p & lower);
-        if ((j0 & cond) != cond) continue;
-        const uint32_t j1 = j0 ^ flip;
+      for (uint32_t v = t; v < items; v += BLOCK_THREADS) {
+        const uint32_t j0 = op_index(v, op, nins, set, lower0), j1 = j0 ^ flip;
         const cpx<R> x0 = lds[j0], x1 = lds[j1];
         cpx<R> y0, y1;
         switch (mode) {
@@ -280,34 +298,65 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_block(cpx<R>* __restrict__ s,
     }
     __syncthreads();
   }
-  if (active)
-    for (uint32_t k = 0; k < per; ++k)
-      store_amp<NT>(s + (base | low | prog->deposit_high[k]), lds[t + BLOCK_THREADS * k]);
+#pragma unroll
+  for (uint32_t k = 0; k < PER; ++k)
+    store_amp<NT>(s + (base | low | prog->deposit_high[k]), lds[t + BLOCK_THREADS * k]);
 }
 
 // ---------------------------------------------------------------------------
 // Launch helpers
 // ---------------------------------------------------------------------------
 
+unsigned block_max_local(int dtype) {
+  const unsigned amp_log2 = dtype == BRK_C128 ? 4 : 3;
+  return (unsigned)__builtin_ctz((unsigned)get_tuning().block_lds) - amp_log2;
+}
+
+unsigned block_run_qubits(int dtype) {
+  const unsigned amp_log2 = dtype == BRK_C128 ? 4 : 3;
+  const unsigned run_log2 = (unsigned)__builtin_ctz((unsigned)get_tuning().block_run_bytes);
+  return run_log2 > amp_log2 ? run_log2 - amp_log2 : 0;
+}
+
+template <typename R, bool NT, int PER>
+static void launch_block_inst(uint64_t chunks, hipStream_t st, cpx<R>* s, const block_program* p) {
+  hipLaunchKernelGGL((k_block<R, NT, PER>), dim3((unsigned)chunks), dim3(BLOCK_THREADS), 0, st, s, p);
+}
+
+// Picks the instantiation whose chunk is exactly 2^L amplitudes (L = 9 .. 13).
+template <typename R, bool NT>
+static void launch_block_per(uint64_t chunks, hipStream_t st, cpx<R>* s, const block_program* p, unsigned L) {
+  switch (L - BLOCK_THREAD_BITS) {
+    case 0: launch_block_inst<R, NT, 1>(chunks, st, s, p); break;
+    case 1: launch_block_inst<R, NT, 2>(chunks, st, s, p); break;
+    case 2: launch_block_inst<R, NT, 4>(chunks, st, s, p); break;
+    case 3: launch_block_inst<R, NT, 8>(chunks, st, s, p); break;
+    default:
+      if constexpr (sizeof(R) == 4) launch_block_inst<R, NT, 16>(chunks, st, s, p);
+      break;
+  }
+}
+
 int launch_block(const block_program* device_program, const block_program& program, void* first, uint64_t size,
                  int dtype, void* stream) {
   unsigned nq;
   if (int st = check_state(first, size, dtype, &nq)) return st;
   const uint32_t cap = dtype == BRK_C128 ? BLOCK_MAX_LOCAL_C128 : BLOCK_MAX_LOCAL_C64;
-  if (program.num_local > cap || program.num_local > nq || program.num_ops > (uint32_t)BLOCK_MAX_OPS)
+  if (program.num_local > cap || program.num_local < (uint32_t)BLOCK_THREAD_BITS || program.num_local > nq ||
+      program.num_ops > (uint32_t)BLOCK_MAX_OPS)
     return fail(BRK_ERR_INVALID_ARGUMENT, "block program does not fit (%u local qubits, %u ops)", program.num_local,
                 program.num_ops);
   const uint64_t chunks = size >> program.num_local;
   if (chunks > 0x7fffffffull) return fail(BRK_ERR_INVALID_ARGUMENT, "too many chunks");
   const bool nt = size * amp_bytes(dtype) >= get_tuning().nt_min_bytes;
-  const dim3 grid((unsigned)chunks), block(BLOCK_THREADS);
   hipStream_t st = as_stream(stream);
+  const unsigned L = program.num_local;
   if (dtype == BRK_C128) {
-    if (nt) hipLaunchKernelGGL((k_block<double, true>), grid, block, 0, st, (cpx<double>*)first, device_program);
-    else hipLaunchKernelGGL((k_block<double, false>), grid, block, 0, st, (cpx<double>*)first, device_program);
+    if (nt) launch_block_per<double, true>(chunks, st, (cpx<double>*)first, device_program, L);
+    else launch_block_per<double, false>(chunks, st, (cpx<double>*)first, device_program, L);
   } else {
-    if (nt) hipLaunchKernelGGL((k_block<float, true>), grid, block, 0, st, (cpx<float>*)first, device_program);
-    else hipLaunchKernelGGL((k_block<float, false>), grid, block, 0, st, (cpx<float>*)first, device_program);
+    if (nt) launch_block_per<float, true>(chunks, st, (cpx<float>*)first, device_program, L);
+    else launch_block_per<float, false>(chunks, st, (cpx<float>*)first, device_program, L);
   }
   return check_launch("block");
 }

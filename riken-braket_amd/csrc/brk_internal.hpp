@@ -112,8 +112,13 @@ struct tuning {
   int fused_staged;       // k <= 5: 1 = LDS-staged tiles in address order, 0 = register-direct tiles
   uint64_t fused_staged_groups;  // grid of the staged kernel (workgroups loop over the tiles)
   int fused_chunked;      // fused tiles: 0 = interleaved over all waves, 1 = contiguous run per wave
+  int block_lds;          // gate blocks: LDS bytes per chunk (65536, 32768 or 16384) -> local qubits
+  int block_run_bytes;    // gate blocks: the lowest qubits addressing this many bytes are always local
 };
 tuning& get_tuning();
+
+// Compute units of the current device (cached per device).
+unsigned device_cus();
 
 // Bit insertion: see brk_bits.hpp.
 

@@ -58,6 +58,7 @@ static tuning g_tuning = {
     /*pair_lds=*/40960, /*diag_lds=*/0, /*pair_remap=*/1, /*diag_remap=*/1,
     /*nt_min_bytes=*/256ull << 20, /*max_blocks=*/(1ull << 24) - 1, /*line_bytes=*/64, /*fused_groups=*/8192,
     /*fused_occupancy=*/2, /*fused_staged=*/1, /*fused_staged_groups=*/768, /*fused_chunked=*/0,
+    /*block_lds=*/65536, /*block_run_bytes=*/128,
 };
 static std::once_flag g_tuning_once;
 
@@ -76,11 +77,28 @@ static void read_tuning_env() {
   get("BRK_TUNE_FUSED_OCCUPANCY", &g_tuning.fused_occupancy);
   get("BRK_TUNE_FUSED_STAGED", &g_tuning.fused_staged);
   get("BRK_TUNE_FUSED_STAGED_GROUPS", &g_tuning.fused_staged_groups);
+  get("BRK_TUNE_BLOCK_LDS", &g_tuning.block_lds);
+  get("BRK_TUNE_BLOCK_RUN_BYTES", &g_tuning.block_run_bytes);
 }
 
 tuning& get_tuning() {
   std::call_once(g_tuning_once, read_tuning_env);
   return g_tuning;
+}
+
+unsigned device_cus() {
+  static std::mutex m;
+  static std::vector<unsigned> cus;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0) return 1;
+  std::lock_guard<std::mutex> lock(m);
+  if ((int)cus.size() <= dev) cus.resize(dev + 1, 0);
+  if (!cus[dev]) {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n < 1) n = 1;
+    cus[dev] = (unsigned)n;
+  }
+  return cus[dev];
 }
 
 static std::mutex g_scratch_mutex;
@@ -235,6 +253,14 @@ int brk_tune(const char* key, int64_t value) {
     t.line_bytes = (uint64_t)value;
   } else if (k == "fused_chunked") {
     t.fused_chunked = value != 0;
+  } else if (k == "block_lds") {
+    if (value != 65536 && value != 32768 && value != 16384)
+      return fail(BRK_ERR_INVALID_ARGUMENT, "block_lds must be 65536, 32768 or 16384");
+    t.block_lds = (int)value;
+  } else if (k == "block_run_bytes") {
+    if (value < 16 || value > 4096 || (value & (value - 1)))
+      return fail(BRK_ERR_INVALID_ARGUMENT, "block_run_bytes must be a power of two in [16, 4096]");
+    t.block_run_bytes = (int)value;
   } else if (k == "fused_groups") {
     if (value < 1 || value > (1 << 22)) return fail(BRK_ERR_INVALID_ARGUMENT, "fused_groups must be in [1, 2^22]");
     t.fused_groups = (uint64_t)value;
@@ -266,6 +292,8 @@ int brk_get_tune(const char* key, int64_t* value) {
   else if (k == "line_bytes") *value = (int64_t)t.line_bytes;
   else if (k == "fused_groups") *value = (int64_t)t.fused_groups;
   else if (k == "fused_chunked") *value = t.fused_chunked;
+  else if (k == "block_lds") *value = t.block_lds;
+  else if (k == "block_run_bytes") *value = t.block_run_bytes;
   else if (k == "fused_occupancy") *value = t.fused_occupancy;
   else if (k == "fused_staged") *value = t.fused_staged;
   else if (k == "fused_staged_groups") *value = (int64_t)t.fused_staged_groups;

@@ -56,8 +56,11 @@ struct block_builder {
   unsigned first_op = 0;
 
   uint64_t line_mask() const { return (1ull << std::min(line_bits, n)) - 1ull; }
+  // a chunk is 2^L amplitudes with L >= 9 (one per thread of a workgroup at least)
+  bool enabled() const { return std::min(max_local, n) >= (unsigned)BLOCK_THREAD_BITS; }
   bool fits(const gate_desc& d) const {
-    return descs.size() < (size_t)BLOCK_MAX_OPS && popcount64(qubits | d.qubits | line_mask()) <= max_local;
+    return enabled() && descs.size() < (size_t)BLOCK_MAX_OPS &&
+           popcount64(qubits | d.qubits | line_mask()) <= max_local;
   }
 };
 
@@ -99,6 +102,10 @@ static block_program make_program(const block_builder& b) {
     o.cond = to_local(d.cond);
     o.sign = to_local(d.sign);
     o.zmask = to_local(d.zmask);
+    const uint32_t inserted = o.cond | (d.mode == BLOCK_DIAG ? 0u : 1u << o.pair_bit);
+    o.nins = 0;
+    for (unsigned bit = 0; bit < r; ++bit)
+      if (inserted >> bit & 1u) o.ins[o.nins++] = bit;
     std::copy(d.u, d.u + 8, o.u);
   }
   return prog;
@@ -113,7 +120,7 @@ static brk_circuit::op map_op(const brk_circuit::op& o, const permutation& perm)
 
 static void flush(block_builder& b, brk_block_plan& plan) {
   if (b.ops.empty()) return;
-  if (b.ops.size() == 1) {
+  if (b.ops.size() == 1 || !b.enabled()) {
     brk_block_plan::step s;
     s.first_op = b.first_op;
     s.num_ops = 1;
@@ -141,8 +148,8 @@ static int build_plan(const brk_circuit* circuit, int dtype, const unsigned* per
     return fail(BRK_ERR_INVALID_ARGUMENT, "permutation is not a permutation of 0..%u", n - 1);
   block_builder b;
   b.n = n;
-  b.max_local = dtype == BRK_C128 ? BLOCK_MAX_LOCAL_C128 : BLOCK_MAX_LOCAL_C64;
-  b.line_bits = dtype == BRK_C128 ? 3 : 4;  // 128-byte lines
+  b.max_local = block_max_local(dtype);
+  b.line_bits = block_run_qubits(dtype);  // default: one 128-byte line
   for (unsigned i = 0; i < circuit->ops.size(); ++i) {
     const auto& o = circuit->ops[i];
     if (o.code == BRK_OP_LOCAL_SWAP) {

@@ -2,7 +2,7 @@
 
 Invariants: the steps cover the ops in order; a block holds at least two ops
 and at most 64; its local qubits are ascending, min(L, n) of them (L = 12 for
-complex<double>, 13 for complex<float>), include the qubits addressing one
+complex<double>, 13 for complex<float>; no blocks when that is below 9), include the qubits addressing one
 128-byte line and every physical qubit its ops touch; local swaps are plain
 steps; the final permutation is the one the ops' local swaps produce
 (ket::mpi::qubit_permutation::permutate, qubit_permutation.hpp:242-252)."""
@@ -53,7 +53,7 @@ def test_plan_invariants(n, dtype, seed, swaps):
         assert line <= set(local)
         for k in range(first, first + count):
             assert ops[k][0] != "LOCAL_SWAP" and phys[k] <= set(local)
-    assert nxt == len(ops) and blocks > 0
+    assert nxt == len(ops) and (blocks > 0) == (L >= 9)  # a chunk holds at least 2^9 amplitudes
     assert plan.final_permutation == final
 
 
