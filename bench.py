@@ -251,6 +251,41 @@ def blocked_leg(circuit, plain_state, args, stream):
     }
 
 
+def sampling_leg(state, args, stream):
+    """Measurement and sampling on the circuit's final complex128 state (overwritten at the end):
+    GENERATE EVENTS with 1 and 100 000 seeded canonical draws, one projective measurement
+    (P(0), P(1) then the projection) and a Shor box.  Every event must land on an amplitude of
+    non-zero probability."""
+    import numpy as np
+
+    import brk
+    n = state.num_qubits
+    rng = np.random.default_rng(4)
+    out = {}
+    for count in (1, 100000):
+        c = list(rng.random(count))
+        brk.generate_events(state.ptr, state.size, brk.C128, c, stream)  # warm-up
+        t0 = time.perf_counter()
+        ev = brk.generate_events(state.ptr, state.size, brk.C128, c, stream)
+        out[f"events_{count}_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
+    sample = sorted(set(int(e) for e in ev[:256]))
+    amps = state.download_indices(sample)
+    out["events_on_nonzero_amplitudes"] = bool(np.all(np.abs(amps) > 0))
+    out["events_per_s"] = round(100000 / (out["events_100000_ms"] * 1e-3), 1)
+    t0 = time.perf_counter()
+    p = brk.zero_one_probabilities(state.ptr, state.size, brk.C128, n // 2, stream)
+    brk.project_qubit(state.ptr, state.size, brk.C128, n // 2, 0, p[0] ** -0.5, stream)
+    norm = brk.norm2(state.ptr, state.size, brk.C128, stream)
+    out["projective_measurement_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
+    out["norm_after_projection_error"] = abs(norm - 1.0)
+    t0 = time.perf_counter()
+    brk.shor_box(state.ptr, state.size, brk.C128, n // 2, 65521, 7, stream)
+    out["shor_box_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
+    log(f"sampling: 100000 events in {out['events_100000_ms']} ms, projective measurement "
+        f"{out['projective_measurement_ms']} ms, Shor box {out['shor_box_ms']} ms")
+    return out
+
+
 def plain_energy(n, ops, J, h, stream):
     import brk
     from devstate import DeviceState
@@ -377,6 +412,7 @@ def main():
     ap.add_argument("--no-fused", action="store_true", help="skip the fused complex64 (precision form) leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batching-form leg")
     ap.add_argument("--no-blocked", action="store_true", help="skip the gate-block (complex128, LDS) leg")
+    ap.add_argument("--no-sampling", action="store_true", help="skip the measurement and sampling leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiation-form leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run the streaming copy kernel once on 2^30 c128 amplitudes (16 GiB read + "
@@ -440,6 +476,9 @@ def main():
     blocked = None
     if not args.no_blocked:
         blocked = blocked_leg(circuit, st, args, stream)
+    sampling = None
+    if not args.no_sampling:
+        sampling = sampling_leg(st, args, stream)
     st.free()
 
     ms_per_step = sum(step_ms) / len(step_ms)
@@ -511,6 +550,8 @@ def main():
     if blocked is not None:
         blocked["speedup_vs_per_gate"] = round(ms_per_step / blocked["ms_per_circuit"], 2)
         result["blocked_c128"] = blocked
+    if sampling is not None:
+        result["sampling_c128"] = sampling
     if not args.no_batch:
         result["batch_c64"] = batch_leg(args, stream)
     if not args.no_adjoint:
