@@ -186,6 +186,19 @@ int brk_exponential_pauli_z(void* first, uint64_t size, int dtype, double phase,
                             const unsigned* targets, unsigned num_targets,
                             const unsigned* controls, unsigned num_controls, void* stream);
 
+/* The same three gates from the coefficient coeff = {cos, sin} = e^{i phase}
+   instead of the phase: the _coeff forms of ket
+   (exponential_pauli_{x,y,z}.hpp: exponential_pauli_*_coeff). */
+int brk_exponential_pauli_x_coeff(void* first, uint64_t size, int dtype, const double coeff[2],
+                                  const unsigned* targets, unsigned num_targets,
+                                  const unsigned* controls, unsigned num_controls, void* stream);
+int brk_exponential_pauli_y_coeff(void* first, uint64_t size, int dtype, const double coeff[2],
+                                  const unsigned* targets, unsigned num_targets,
+                                  const unsigned* controls, unsigned num_controls, void* stream);
+int brk_exponential_pauli_z_coeff(void* first, uint64_t size, int dtype, const double coeff[2],
+                                  const unsigned* targets, unsigned num_targets,
+                                  const unsigned* controls, unsigned num_controls, void* stream);
+
 /* SWAP of qubit1 and qubit2 under controls.  Replaces ket::gate::swap
    (ket/include/ket/gate/swap.hpp:30-75 and the controlled form after :76). */
 int brk_swap(void* first, uint64_t size, int dtype, unsigned qubit1, unsigned qubit2,
@@ -197,6 +210,9 @@ int brk_swap(void* first, uint64_t size, int dtype, unsigned qubit1, unsigned qu
    controlled form :92-…, wrappers :300-350). */
 int brk_exponential_swap(void* first, uint64_t size, int dtype, double phase, unsigned qubit1, unsigned qubit2,
                          const unsigned* controls, unsigned num_controls, void* stream);
+/* The same from coeff = e^{i phase} (exponential_swap_coeff, exponential_swap.hpp:31-90). */
+int brk_exponential_swap_coeff(void* first, uint64_t size, int dtype, const double coeff[2], unsigned qubit1,
+                               unsigned qubit2, const unsigned* controls, unsigned num_controls, void* stream);
 
 /* General single-qubit gate: matrix = {m00.re, m00.im, m01.re, m01.im,
    m10.re, m10.im, m11.re, m11.im}, (a0', a1') = M (a0, a1), under controls.

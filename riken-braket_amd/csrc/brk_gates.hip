@@ -628,9 +628,8 @@ static void coeffs_u3(double phase1, double phase2, double phase3, double* u) {
   put(u, 3, (cosine * e3) * e2);
 }
 
-// exp(i phase X..X) = cos I + i sin X..X; pairs (i, i ^ T).
-static void coeffs_exp_x(double phase, double* u) {
-  const std::complex<double> e = exp_i(phase);
+// exp(i phase X..X) = cos I + i sin X..X; pairs (i, i ^ T).  e = e^{i phase}.
+static void coeffs_exp_x(std::complex<double> e, double* u) {
   const double c = e.real(), s = e.imag();
   put(u, 0, {c, 0.0});
   put(u, 1, {0.0, s});
@@ -641,8 +640,7 @@ static void coeffs_exp_x(double phase, double* u) {
 // exp(i phase Y..Y) = cos I + i sin Y..Y, with (Y^m a)[i] = (-i)^m (-1)^{popcount(i & T)} a[i ^ T].
 // For one target: a0' = c a0 + s a1, a1' = -s a0 + c a1
 // (ket/include/ket/gate/exponential_pauli_y.hpp:36-38, :84-87).  Sign by the targets.
-static void coeffs_exp_y(double phase, unsigned num_targets, double* u) {
-  const std::complex<double> e = exp_i(phase);
+static void coeffs_exp_y(std::complex<double> e, unsigned num_targets, double* u) {
   const double c = e.real(), s = e.imag();
   const std::complex<double> ph = minus_i_pow(num_targets);
   const double sgn_m = (num_targets & 1u) ? -1.0 : 1.0;
@@ -654,8 +652,7 @@ static void coeffs_exp_y(double phase, unsigned num_targets, double* u) {
 
 // exp(i phase SWAP): e^{i phase} for (00, 11), cos and sin of the same coefficient for (01, 10)
 // (ket/include/ket/gate/exponential_swap.hpp:31-90).
-static void coeffs_exp_swap(double phase, double* u) {
-  const std::complex<double> e = exp_i(phase);
+static void coeffs_exp_swap(std::complex<double> e, double* u) {
   put(u, 0, e);
   u[2] = e.real();
   u[3] = e.imag();
@@ -698,12 +695,12 @@ int describe_physical_op(int code, const unsigned* t, unsigned nt, const unsigne
     case BRK_OP_U2: pair(BLOCK_MATRIX); coeffs_u2(p[0], p[1], d->u); break;
     case BRK_OP_U3: pair(BLOCK_MATRIX); coeffs_u3(p[0], p[1], p[2], d->u); break;
     case BRK_OP_MATRIX: pair(BLOCK_MATRIX); std::copy(p, p + 8, d->u); break;
-    case BRK_OP_EX: pair(BLOCK_MATRIX); coeffs_exp_x(p[0], d->u); break;
-    case BRK_OP_RX: pair(BLOCK_MATRIX); coeffs_exp_x(-0.5 * p[0], d->u); break;
+    case BRK_OP_EX: pair(BLOCK_MATRIX); coeffs_exp_x(exp_i(p[0]), d->u); break;
+    case BRK_OP_RX: pair(BLOCK_MATRIX); coeffs_exp_x(exp_i(-0.5 * p[0]), d->u); break;
     case BRK_OP_EY:
     case BRK_OP_RY:
       pair(BLOCK_MATRIX);
-      coeffs_exp_y(code == BRK_OP_EY ? p[0] : -0.5 * p[0], nt, d->u);
+      coeffs_exp_y(exp_i(code == BRK_OP_EY ? p[0] : -0.5 * p[0]), nt, d->u);
       d->sign = tmask;
       break;
     case BRK_OP_EZ:
@@ -722,7 +719,7 @@ int describe_physical_op(int code, const unsigned* t, unsigned nt, const unsigne
     case BRK_OP_ESWAP:
       pair(BLOCK_ESWAP);
       d->sign = 1ull << std::max(t[0], t[1]);
-      coeffs_exp_swap(p[0], d->u);
+      coeffs_exp_swap(exp_i(p[0]), d->u);
       break;
     default: return fail(BRK_ERR_UNSUPPORTED, "op %d has no gate description", code);
   }
@@ -802,32 +799,93 @@ int brk_phase_shift3(void* first, uint64_t size, int dtype, double phase1, doubl
   return single_target(first, size, dtype, target, controls, num_controls, MODE_MATRIX, u, stream);
 }
 
-int brk_exponential_pauli_x(void* first, uint64_t size, int dtype, double phase, const unsigned* targets,
-                            unsigned num_targets, const unsigned* controls, unsigned num_controls, void* stream) {
+}  // extern "C"
+
+namespace brk {
+
+static int exp_x(void* first, uint64_t size, int dtype, std::complex<double> e, const unsigned* targets,
+                 unsigned num_targets, const unsigned* controls, unsigned num_controls, void* stream) {
   double u[8];
-  coeffs_exp_x(phase, u);
+  coeffs_exp_x(e, u);
   return multi_target_flip(first, size, dtype, targets, num_targets, controls, num_controls, MODE_MATRIX, u, false,
                            stream);
 }
 
-int brk_exponential_pauli_y(void* first, uint64_t size, int dtype, double phase, const unsigned* targets,
-                            unsigned num_targets, const unsigned* controls, unsigned num_controls, void* stream) {
+static int exp_y(void* first, uint64_t size, int dtype, std::complex<double> e, const unsigned* targets,
+                 unsigned num_targets, const unsigned* controls, unsigned num_controls, void* stream) {
   double u[8];
-  coeffs_exp_y(phase, num_targets, u);
+  coeffs_exp_y(e, num_targets, u);
   return multi_target_flip(first, size, dtype, targets, num_targets, controls, num_controls, MODE_MATRIX, u, true,
                            stream);
 }
 
-int brk_exponential_pauli_z(void* first, uint64_t size, int dtype, double phase, const unsigned* targets,
-                            unsigned num_targets, const unsigned* controls, unsigned num_controls, void* stream) {
-  // exp(i phase Z..Z): e^{i phase} on even parity of the target bits, e^{-i phase} on odd
-  // (ket/include/ket/gate/exponential_pauli_z.hpp:36-38, :73-74).
+// exp(i phase Z..Z): e^{i phase} on even parity of the target bits, e^{-i phase} on odd
+// (ket/include/ket/gate/exponential_pauli_z.hpp:36-38, :73-74).
+static int exp_z(void* first, uint64_t size, int dtype, std::complex<double> e, const unsigned* targets,
+                 unsigned num_targets, const unsigned* controls, unsigned num_controls, void* stream) {
   if (num_targets == 0) return fail(BRK_ERR_INVALID_ARGUMENT, "no target qubits");
   validated v;
   if (int st = validate(first, size, dtype, targets, num_targets, controls, num_controls, &v)) return st;
-  const std::complex<double> e = exp_i(phase);
   return diagonal(first, size, dtype, controls, num_controls, mask_of(targets, num_targets), e, std::conj(e),
                   stream);
+}
+
+static int exp_swap(void* first, uint64_t size, int dtype, std::complex<double> e, unsigned qubit1, unsigned qubit2,
+                    const unsigned* controls, unsigned num_controls, void* stream) {
+  const unsigned t[2] = {qubit1, qubit2};
+  validated v;
+  if (int st = validate(first, size, dtype, t, 2, controls, num_controls, &v)) return st;
+  const unsigned lo = std::min(qubit1, qubit2), hi = std::max(qubit1, qubit2);
+  plan p;
+  p.insert(lo);
+  for (unsigned i = 0; i < num_controls; ++i) p.control(controls[i], line_bits(dtype));
+  gate_args a = make_args(p);
+  a.flipmask = (1ull << lo) | (1ull << hi);
+  a.signmask = 1ull << hi;
+  coeffs_exp_swap(e, a.u);
+  return launch(dtype, FAM_PAIR, MODE_ESWAP, first, size, size >> p.np, a, stream);
+}
+
+static std::complex<double> coefficient(const double* c) { return {c[0], c[1]}; }
+
+}  // namespace brk
+
+extern "C" {
+
+int brk_exponential_pauli_x(void* first, uint64_t size, int dtype, double phase, const unsigned* targets,
+                            unsigned num_targets, const unsigned* controls, unsigned num_controls, void* stream) {
+  return exp_x(first, size, dtype, exp_i(phase), targets, num_targets, controls, num_controls, stream);
+}
+
+int brk_exponential_pauli_y(void* first, uint64_t size, int dtype, double phase, const unsigned* targets,
+                            unsigned num_targets, const unsigned* controls, unsigned num_controls, void* stream) {
+  return exp_y(first, size, dtype, exp_i(phase), targets, num_targets, controls, num_controls, stream);
+}
+
+int brk_exponential_pauli_z(void* first, uint64_t size, int dtype, double phase, const unsigned* targets,
+                            unsigned num_targets, const unsigned* controls, unsigned num_controls, void* stream) {
+  return exp_z(first, size, dtype, exp_i(phase), targets, num_targets, controls, num_controls, stream);
+}
+
+int brk_exponential_pauli_x_coeff(void* first, uint64_t size, int dtype, const double coeff[2],
+                                  const unsigned* targets, unsigned num_targets, const unsigned* controls,
+                                  unsigned num_controls, void* stream) {
+  if (!coeff) return fail(BRK_ERR_INVALID_ARGUMENT, "coeff is null");
+  return exp_x(first, size, dtype, coefficient(coeff), targets, num_targets, controls, num_controls, stream);
+}
+
+int brk_exponential_pauli_y_coeff(void* first, uint64_t size, int dtype, const double coeff[2],
+                                  const unsigned* targets, unsigned num_targets, const unsigned* controls,
+                                  unsigned num_controls, void* stream) {
+  if (!coeff) return fail(BRK_ERR_INVALID_ARGUMENT, "coeff is null");
+  return exp_y(first, size, dtype, coefficient(coeff), targets, num_targets, controls, num_controls, stream);
+}
+
+int brk_exponential_pauli_z_coeff(void* first, uint64_t size, int dtype, const double coeff[2],
+                                  const unsigned* targets, unsigned num_targets, const unsigned* controls,
+                                  unsigned num_controls, void* stream) {
+  if (!coeff) return fail(BRK_ERR_INVALID_ARGUMENT, "coeff is null");
+  return exp_z(first, size, dtype, coefficient(coeff), targets, num_targets, controls, num_controls, stream);
 }
 
 int brk_swap(void* first, uint64_t size, int dtype, unsigned qubit1, unsigned qubit2, const unsigned* controls,
@@ -864,18 +922,13 @@ int brk_swap(void* first, uint64_t size, int dtype, unsigned qubit1, unsigned qu
 
 int brk_exponential_swap(void* first, uint64_t size, int dtype, double phase, unsigned qubit1, unsigned qubit2,
                          const unsigned* controls, unsigned num_controls, void* stream) {
-  const unsigned t[2] = {qubit1, qubit2};
-  validated v;
-  if (int st = validate(first, size, dtype, t, 2, controls, num_controls, &v)) return st;
-  const unsigned lo = std::min(qubit1, qubit2), hi = std::max(qubit1, qubit2);
-  plan p;
-  p.insert(lo);
-  for (unsigned i = 0; i < num_controls; ++i) p.control(controls[i], line_bits(dtype));
-  gate_args a = make_args(p);
-  a.flipmask = (1ull << lo) | (1ull << hi);
-  a.signmask = 1ull << hi;
-  coeffs_exp_swap(phase, a.u);
-  return launch(dtype, FAM_PAIR, MODE_ESWAP, first, size, size >> p.np, a, stream);
+  return exp_swap(first, size, dtype, exp_i(phase), qubit1, qubit2, controls, num_controls, stream);
+}
+
+int brk_exponential_swap_coeff(void* first, uint64_t size, int dtype, const double coeff[2], unsigned qubit1,
+                               unsigned qubit2, const unsigned* controls, unsigned num_controls, void* stream) {
+  if (!coeff) return fail(BRK_ERR_INVALID_ARGUMENT, "coeff is null");
+  return exp_swap(first, size, dtype, coefficient(coeff), qubit1, qubit2, controls, num_controls, stream);
 }
 
 int brk_swap_permutated_local_qubits(void* first, uint64_t size, int dtype, unsigned permutated_qubit1,

@@ -16,8 +16,12 @@
 #pragma once
 
 #include <array>
+#include <cmath>
 #include <complex>
+#include <limits>
+#include <random>
 #include <cstdint>
+#include <iterator>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -196,168 +200,394 @@ inline std::uint64_t span(device_iterator<Complex> first, device_iterator<Comple
 }
 inline const unsigned* ptr_or_null(const unsigned* p, unsigned n) { return n ? p : nullptr; }
 
+// runtime forms: targets are one ket::qubit or a range of them, controls a range of ket::control
+template <typename S, typename B>
+inline void add_all(operands& o, const ::ket::qubit<S, B>& q) {
+  collect(o, q);
+}
+template <typename Range>
+inline auto add_all(operands& o, const Range& r) -> decltype(std::begin(r), std::end(r), void()) {
+  for (const auto& q : r) collect(o, q);
+}
+template <typename Targets>
+inline operands from_ranges(const Targets& targets) {
+  operands o;
+  add_all(o, targets);
+  return o;
+}
+template <typename Targets, typename Controls>
+inline operands from_ranges(const Targets& targets, const Controls& controls) {
+  operands o;
+  add_all(o, targets);
+  add_all(o, controls);
+  return o;
+}
+
 }  // namespace detail
 }  // namespace hip
 
 namespace gate {
 
+// Core of every gate: (policy, first, last, real parameters, operands).  The public
+// forms below -- variadic ket::qubit / ket::control arguments, ranges::, runtime::
+// qubit ranges -- all build the operand list and call these.
+namespace hip_core {
+using hip::detail::operands;
+using hip::detail::ptr_or_null;
+using hip::detail::span;
+template <typename Complex>
+using iter = hip::device_iterator<Complex>;
+template <typename Complex>
+constexpr int dtype() { return hip::dtype_of<Complex>::value; }
+
+inline void one_target(const operands& o, const char* name) {
+  if (o.nt != 1) throw hip::error(BRK_ERR_INVALID_ARGUMENT, std::string(name) + " takes one target qubit");
+}
+inline std::complex<double> exp_i(double phase) { return std::exp(std::complex<double>{0.0, phase}); }
+
 // H -- ket/include/ket/gate/hadamard.hpp:34-223
-template <typename Complex, typename... Qubits>
-inline void hadamard(hip::policy p, hip::device_iterator<Complex> first, hip::device_iterator<Complex> last,
-                     Qubits... qubits) {
-  auto o = hip::detail::split(qubits...);
-  if (o.nt != 1) throw hip::error(BRK_ERR_INVALID_ARGUMENT, "hadamard takes one target qubit");
-  hip::check(brk_hadamard(first.get(), hip::detail::span(first, last), hip::dtype_of<Complex>::value, o.targets[0],
-                          hip::detail::ptr_or_null(o.controls, o.nc), o.nc, p.stream));
+template <typename Complex>
+inline void hadamard(hip::policy p, iter<Complex> first, iter<Complex> last, const operands& o) {
+  one_target(o, "hadamard");
+  hip::check(brk_hadamard(first.get(), span(first, last), dtype<Complex>(), o.targets[0],
+                          ptr_or_null(o.controls, o.nc), o.nc, p.stream));
 }
-
-// X...X under controls -- pauli_x.hpp
-template <typename Complex, typename... Qubits>
-inline void pauli_x(hip::policy p, hip::device_iterator<Complex> first, hip::device_iterator<Complex> last,
-                    Qubits... qubits) {
-  auto o = hip::detail::split(qubits...);
-  hip::check(brk_pauli_x(first.get(), hip::detail::span(first, last), hip::dtype_of<Complex>::value, o.targets, o.nt,
-                         hip::detail::ptr_or_null(o.controls, o.nc), o.nc, p.stream));
+// X...X under controls -- pauli_x.hpp; NOT / CNOT / TOFFOLI -- not_.hpp, controlled_not.hpp, toffoli.hpp
+template <typename Complex>
+inline void pauli_x(hip::policy p, iter<Complex> first, iter<Complex> last, const operands& o) {
+  hip::check(brk_pauli_x(first.get(), span(first, last), dtype<Complex>(), o.targets, o.nt,
+                         ptr_or_null(o.controls, o.nc), o.nc, p.stream));
 }
-
-// NOT / CNOT -- not_.hpp:33-51, controlled_not.hpp:30-37
-template <typename Complex, typename... Qubits>
-inline void not_(hip::policy p, hip::device_iterator<Complex> first, hip::device_iterator<Complex> last,
-                 Qubits... qubits) {
-  pauli_x(p, first, last, qubits...);
+template <typename Complex>
+inline void not_(hip::policy p, iter<Complex> first, iter<Complex> last, const operands& o) {
+  pauli_x(p, first, last, o);
 }
-template <typename Complex, typename... Qubits>
-inline void controlled_not(hip::policy p, hip::device_iterator<Complex> first, hip::device_iterator<Complex> last,
-                           Qubits... qubits) {
-  auto o = hip::detail::split(qubits...);
+template <typename Complex>
+inline void controlled_not(hip::policy p, iter<Complex> first, iter<Complex> last, const operands& o) {
   if (o.nt != 1 || o.nc == 0) throw hip::error(BRK_ERR_INVALID_ARGUMENT, "controlled_not takes a target and controls");
-  hip::check(brk_controlled_not(first.get(), hip::detail::span(first, last), hip::dtype_of<Complex>::value,
-                                o.targets[0], o.controls, o.nc, p.stream));
+  hip::check(brk_controlled_not(first.get(), span(first, last), dtype<Complex>(), o.targets[0], o.controls, o.nc,
+                                p.stream));
 }
-
+template <typename Complex>
+inline void toffoli(hip::policy p, iter<Complex> first, iter<Complex> last, const operands& o) {
+  if (o.nt != 1 || o.nc != 2) throw hip::error(BRK_ERR_INVALID_ARGUMENT, "toffoli takes a target and two controls");
+  controlled_not(p, first, last, o);
+}
 // Y...Y under controls -- pauli_y.hpp
-template <typename Complex, typename... Qubits>
-inline void pauli_y(hip::policy p, hip::device_iterator<Complex> first, hip::device_iterator<Complex> last,
-                    Qubits... qubits) {
-  auto o = hip::detail::split(qubits...);
-  hip::check(brk_pauli_y(first.get(), hip::detail::span(first, last), hip::dtype_of<Complex>::value, o.targets, o.nt,
-                         hip::detail::ptr_or_null(o.controls, o.nc), o.nc, p.stream));
+template <typename Complex>
+inline void pauli_y(hip::policy p, iter<Complex> first, iter<Complex> last, const operands& o) {
+  hip::check(brk_pauli_y(first.get(), span(first, last), dtype<Complex>(), o.targets, o.nt,
+                         ptr_or_null(o.controls, o.nc), o.nc, p.stream));
 }
-
 // Z on control qubits (Z, CZ, CnZ) -- pauli_z.hpp:55-210; targets and controls are treated alike
-template <typename Complex, typename... Qubits>
-inline void pauli_z(hip::policy p, hip::device_iterator<Complex> first, hip::device_iterator<Complex> last,
-                    Qubits... qubits) {
-  auto o = hip::detail::split(qubits...);
+template <typename Complex>
+inline void pauli_z(hip::policy p, iter<Complex> first, iter<Complex> last, const operands& o) {
   unsigned all[2 * BRK_MAX_OPERATED_QUBITS], n = 0;
   hip::detail::all_qubits(o, all, &n);
-  hip::check(brk_pauli_z(first.get(), hip::detail::span(first, last), hip::dtype_of<Complex>::value, all, n, p.stream));
+  hip::check(brk_pauli_z(first.get(), span(first, last), dtype<Complex>(), all, n, p.stream));
 }
-
-// U1 from its coefficient e^{i theta} -- phase_shift.hpp:37-93
-template <typename Complex, typename... Qubits>
-inline void phase_shift_coeff(hip::policy p, hip::device_iterator<Complex> first, hip::device_iterator<Complex> last,
-                              std::complex<double> coeff, Qubits... qubits) {
-  auto o = hip::detail::split(qubits...);
+// a coefficient where every listed qubit is 1: U1, controlled U1, sqrt(Z) -- phase_shift.hpp:37-93,
+// controlled_phase_shift.hpp:30-110, sqrt_pauli_z.hpp:35-230
+template <typename Complex>
+inline void phase_shift_coeff(hip::policy p, iter<Complex> first, iter<Complex> last, std::complex<double> coeff,
+                              const operands& o) {
   unsigned all[2 * BRK_MAX_OPERATED_QUBITS], n = 0;
   hip::detail::all_qubits(o, all, &n);
   const double c[2] = {coeff.real(), coeff.imag()};
-  hip::check(brk_phase_shift_coeff(first.get(), hip::detail::span(first, last), hip::dtype_of<Complex>::value, c, all,
-                                   n, p.stream));
+  hip::check(brk_phase_shift_coeff(first.get(), span(first, last), dtype<Complex>(), c, all, n, p.stream));
 }
-
-// U1(theta) -- phase_shift.hpp:510-521
-template <typename Complex, typename... Qubits>
-inline void phase_shift(hip::policy p, hip::device_iterator<Complex> first, hip::device_iterator<Complex> last,
-                        double phase, Qubits... qubits) {
-  auto o = hip::detail::split(qubits...);
+template <typename Complex>
+inline void phase_shift(hip::policy p, iter<Complex> first, iter<Complex> last, double phase, const operands& o) {
   unsigned all[2 * BRK_MAX_OPERATED_QUBITS], n = 0;
   hip::detail::all_qubits(o, all, &n);
-  hip::check(brk_phase_shift(first.get(), hip::detail::span(first, last), hip::dtype_of<Complex>::value, phase, all, n,
-                             p.stream));
+  hip::check(brk_phase_shift(first.get(), span(first, last), dtype<Complex>(), phase, all, n, p.stream));
 }
-
-// U2(phase1, phase2) -- phase_shift.hpp:653-709
-template <typename Complex, typename... Qubits>
-inline void phase_shift2(hip::policy p, hip::device_iterator<Complex> first, hip::device_iterator<Complex> last,
-                         double phase1, double phase2, Qubits... qubits) {
-  auto o = hip::detail::split(qubits...);
-  if (o.nt != 1) throw hip::error(BRK_ERR_INVALID_ARGUMENT, "phase_shift2 takes one target qubit");
-  hip::check(brk_phase_shift2(first.get(), hip::detail::span(first, last), hip::dtype_of<Complex>::value, phase1,
-                              phase2, o.targets[0], hip::detail::ptr_or_null(o.controls, o.nc), o.nc, p.stream));
+template <typename Complex>
+inline void controlled_phase_shift_coeff(hip::policy p, iter<Complex> first, iter<Complex> last,
+                                         std::complex<double> coeff, const operands& o) {
+  phase_shift_coeff(p, first, last, coeff, o);
 }
-
-// U3(phase1, phase2, phase3) -- phase_shift.hpp:1195-1254
-template <typename Complex, typename... Qubits>
-inline void phase_shift3(hip::policy p, hip::device_iterator<Complex> first, hip::device_iterator<Complex> last,
-                         double phase1, double phase2, double phase3, Qubits... qubits) {
-  auto o = hip::detail::split(qubits...);
-  if (o.nt != 1) throw hip::error(BRK_ERR_INVALID_ARGUMENT, "phase_shift3 takes one target qubit");
-  hip::check(brk_phase_shift3(first.get(), hip::detail::span(first, last), hip::dtype_of<Complex>::value, phase1,
-                              phase2, phase3, o.targets[0], hip::detail::ptr_or_null(o.controls, o.nc), o.nc,
-                              p.stream));
+template <typename Complex>
+inline void controlled_phase_shift(hip::policy p, iter<Complex> first, iter<Complex> last, double phase,
+                                   const operands& o) {
+  phase_shift(p, first, last, phase, o);
 }
-
-#define KET_HIP_EXPONENTIAL_PAULI(NAME, FN)                                                                      \
-  template <typename Complex, typename... Qubits>                                                                 \
-  inline void NAME(hip::policy p, hip::device_iterator<Complex> first, hip::device_iterator<Complex> last,        \
-                   double phase, Qubits... qubits) {                                                              \
-    auto o = hip::detail::split(qubits...);                                                                       \
-    hip::check(FN(first.get(), hip::detail::span(first, last), hip::dtype_of<Complex>::value, phase, o.targets,  \
-                  o.nt, hip::detail::ptr_or_null(o.controls, o.nc), o.nc, p.stream));                             \
+template <typename Complex>
+inline void sqrt_pauli_z(hip::policy p, iter<Complex> first, iter<Complex> last, const operands& o) {
+  phase_shift_coeff(p, first, last, {0.0, 1.0}, o);
+}
+// U2, U3 -- phase_shift.hpp:653-709, 1195-1254
+template <typename Complex>
+inline void phase_shift2(hip::policy p, iter<Complex> first, iter<Complex> last, double phase1, double phase2,
+                         const operands& o) {
+  one_target(o, "phase_shift2");
+  hip::check(brk_phase_shift2(first.get(), span(first, last), dtype<Complex>(), phase1, phase2, o.targets[0],
+                              ptr_or_null(o.controls, o.nc), o.nc, p.stream));
+}
+template <typename Complex>
+inline void phase_shift3(hip::policy p, iter<Complex> first, iter<Complex> last, double phase1, double phase2,
+                         double phase3, const operands& o) {
+  one_target(o, "phase_shift3");
+  hip::check(brk_phase_shift3(first.get(), span(first, last), dtype<Complex>(), phase1, phase2, phase3,
+                              o.targets[0], ptr_or_null(o.controls, o.nc), o.nc, p.stream));
+}
+// exp(i phase P...P) -- exponential_pauli_{x,y,z}.hpp; the _coeff forms take e^{i phase}
+#define KET_HIP_EXP_PAULI(NAME, FN)                                                                         \
+  template <typename Complex>                                                                               \
+  inline void NAME(hip::policy p, iter<Complex> first, iter<Complex> last, double phase, const operands& o) { \
+    hip::check(FN(first.get(), span(first, last), dtype<Complex>(), phase, o.targets, o.nt,                  \
+                  ptr_or_null(o.controls, o.nc), o.nc, p.stream));                                          \
+  }                                                                                                         \
+  template <typename Complex>                                                                               \
+  inline void NAME##_coeff(hip::policy p, iter<Complex> first, iter<Complex> last, std::complex<double> coeff, \
+                           const operands& o) {                                                             \
+    const double c[2] = {coeff.real(), coeff.imag()};                                                       \
+    hip::check(FN##_coeff(first.get(), span(first, last), dtype<Complex>(), c, o.targets, o.nt,             \
+                          ptr_or_null(o.controls, o.nc), o.nc, p.stream));                                  \
   }
-// exp(i phase P...P) -- exponential_pauli_{x,y,z}.hpp
-KET_HIP_EXPONENTIAL_PAULI(exponential_pauli_x, brk_exponential_pauli_x)
-KET_HIP_EXPONENTIAL_PAULI(exponential_pauli_y, brk_exponential_pauli_y)
-KET_HIP_EXPONENTIAL_PAULI(exponential_pauli_z, brk_exponential_pauli_z)
-#undef KET_HIP_EXPONENTIAL_PAULI
-
-// SWAP under controls -- swap.hpp:30-75
-template <typename Complex, typename... Qubits>
-inline void swap(hip::policy p, hip::device_iterator<Complex> first, hip::device_iterator<Complex> last,
-                 Qubits... qubits) {
-  auto o = hip::detail::split(qubits...);
+KET_HIP_EXP_PAULI(exponential_pauli_x, brk_exponential_pauli_x)
+KET_HIP_EXP_PAULI(exponential_pauli_y, brk_exponential_pauli_y)
+KET_HIP_EXP_PAULI(exponential_pauli_z, brk_exponential_pauli_z)
+#undef KET_HIP_EXP_PAULI
+// SWAP and exp(i phase SWAP) under controls -- swap.hpp:30-75, exponential_swap.hpp:31-350
+template <typename Complex>
+inline void swap(hip::policy p, iter<Complex> first, iter<Complex> last, const operands& o) {
   if (o.nt != 2) throw hip::error(BRK_ERR_INVALID_ARGUMENT, "swap takes two target qubits");
-  hip::check(brk_swap(first.get(), hip::detail::span(first, last), hip::dtype_of<Complex>::value, o.targets[0],
-                      o.targets[1], hip::detail::ptr_or_null(o.controls, o.nc), o.nc, p.stream));
+  hip::check(brk_swap(first.get(), span(first, last), dtype<Complex>(), o.targets[0], o.targets[1],
+                      ptr_or_null(o.controls, o.nc), o.nc, p.stream));
 }
-
-// exp(i phase SWAP) under controls -- exponential_swap.hpp:31-90, 300-350
-template <typename Complex, typename... Qubits>
-inline void exponential_swap(hip::policy p, hip::device_iterator<Complex> first, hip::device_iterator<Complex> last,
-                             double phase, Qubits... qubits) {
-  auto o = hip::detail::split(qubits...);
+template <typename Complex>
+inline void exponential_swap_coeff(hip::policy p, iter<Complex> first, iter<Complex> last,
+                                   std::complex<double> coeff, const operands& o) {
   if (o.nt != 2) throw hip::error(BRK_ERR_INVALID_ARGUMENT, "exponential_swap takes two target qubits");
-  hip::check(brk_exponential_swap(first.get(), hip::detail::span(first, last), hip::dtype_of<Complex>::value, phase,
-                                  o.targets[0], o.targets[1], hip::detail::ptr_or_null(o.controls, o.nc), o.nc,
-                                  p.stream));
+  const double c[2] = {coeff.real(), coeff.imag()};
+  hip::check(brk_exponential_swap_coeff(first.get(), span(first, last), dtype<Complex>(), c, o.targets[0],
+                                        o.targets[1], ptr_or_null(o.controls, o.nc), o.nc, p.stream));
+}
+template <typename Complex>
+inline void exponential_swap(hip::policy p, iter<Complex> first, iter<Complex> last, double phase,
+                             const operands& o) {
+  if (o.nt != 2) throw hip::error(BRK_ERR_INVALID_ARGUMENT, "exponential_swap takes two target qubits");
+  hip::check(brk_exponential_swap(first.get(), span(first, last), dtype<Complex>(), phase, o.targets[0],
+                                  o.targets[1], ptr_or_null(o.controls, o.nc), o.nc, p.stream));
+}
+// fixed 2x2 gates on one target under controls, rows (m00, m01; m10, m11)
+template <typename Complex>
+inline void matrix_1q(hip::policy p, iter<Complex> first, iter<Complex> last, std::complex<double> m00,
+                      std::complex<double> m01, std::complex<double> m10, std::complex<double> m11,
+                      const operands& o, const char* name) {
+  one_target(o, name);
+  const double m[8] = {m00.real(), m00.imag(), m01.real(), m01.imag(), m10.real(), m10.imag(), m11.real(), m11.imag()};
+  hip::check(brk_gate_1q(first.get(), span(first, last), dtype<Complex>(), o.targets[0], m,
+                         ptr_or_null(o.controls, o.nc), o.nc, p.stream));
+}
+// sqrt(X), sqrt(Y) and adjoints -- sqrt_pauli_x.hpp:31-32, 258-259; sqrt_pauli_y.hpp:32-33, 255-256
+template <typename Complex>
+inline void sqrt_pauli_x(hip::policy p, iter<Complex> first, iter<Complex> last, const operands& o) {
+  const std::complex<double> a{0.5, 0.5}, b{0.5, -0.5};
+  matrix_1q(p, first, last, a, b, b, a, o, "sqrt_pauli_x");
+}
+template <typename Complex>
+inline void adj_sqrt_pauli_x(hip::policy p, iter<Complex> first, iter<Complex> last, const operands& o) {
+  const std::complex<double> a{0.5, -0.5}, b{0.5, 0.5};
+  matrix_1q(p, first, last, a, b, b, a, o, "adj_sqrt_pauli_x");
+}
+template <typename Complex>
+inline void sqrt_pauli_y(hip::policy p, iter<Complex> first, iter<Complex> last, const operands& o) {
+  const std::complex<double> a{0.5, 0.5};
+  matrix_1q(p, first, last, a, -a, a, a, o, "sqrt_pauli_y");
+}
+template <typename Complex>
+inline void adj_sqrt_pauli_y(hip::policy p, iter<Complex> first, iter<Complex> last, const operands& o) {
+  const std::complex<double> a{0.5, -0.5};
+  matrix_1q(p, first, last, a, a, -a, a, o, "adj_sqrt_pauli_y");
+}
+// +X / -X and +Y / -Y half-pi rotations -- x_rotation_half_pi.hpp:33-34, 253-254; y_rotation_half_pi.hpp:32-33, 250-251
+template <typename Complex>
+inline void x_rotation_half_pi(hip::policy p, iter<Complex> first, iter<Complex> last, const operands& o) {
+  const double r = 0.707106781186547524400844362104849039;
+  matrix_1q(p, first, last, {r, 0.0}, {0.0, r}, {0.0, r}, {r, 0.0}, o, "x_rotation_half_pi");
+}
+template <typename Complex>
+inline void adj_x_rotation_half_pi(hip::policy p, iter<Complex> first, iter<Complex> last, const operands& o) {
+  const double r = 0.707106781186547524400844362104849039;
+  matrix_1q(p, first, last, {r, 0.0}, {0.0, -r}, {0.0, -r}, {r, 0.0}, o, "adj_x_rotation_half_pi");
+}
+template <typename Complex>
+inline void y_rotation_half_pi(hip::policy p, iter<Complex> first, iter<Complex> last, const operands& o) {
+  const double r = 0.707106781186547524400844362104849039;
+  matrix_1q(p, first, last, {r, 0.0}, {r, 0.0}, {-r, 0.0}, {r, 0.0}, o, "y_rotation_half_pi");
+}
+template <typename Complex>
+inline void adj_y_rotation_half_pi(hip::policy p, iter<Complex> first, iter<Complex> last, const operands& o) {
+  const double r = 0.707106781186547524400844362104849039;
+  matrix_1q(p, first, last, {r, 0.0}, {-r, 0.0}, {r, 0.0}, {r, 0.0}, o, "adj_y_rotation_half_pi");
+}
+// controlled V(theta) = [[1+e, 1-e], [1-e, 1+e]] / 2 on the target under controls -- controlled_v.hpp:30-90
+template <typename Complex>
+inline void controlled_v_coeff(hip::policy p, iter<Complex> first, iter<Complex> last, std::complex<double> coeff,
+                               const operands& o) {
+  if (o.nc == 0) throw hip::error(BRK_ERR_INVALID_ARGUMENT, "controlled_v takes at least one control qubit");
+  const std::complex<double> plus = 0.5 * (1.0 + coeff), minus = 0.5 * (1.0 - coeff);
+  matrix_1q(p, first, last, plus, minus, minus, plus, o, "controlled_v");
+}
+template <typename Complex>
+inline void controlled_v(hip::policy p, iter<Complex> first, iter<Complex> last, double phase, const operands& o) {
+  controlled_v_coeff(p, first, last, exp_i(phase), o);
 }
 
-namespace ranges {
-// range forms: (policy, state, ...) -> state&, as ket::gate::ranges::*
-#define KET_HIP_RANGE(NAME)                                                        \
-  template <typename Complex, typename... Args>                                    \
-  inline hip::state<Complex>& NAME(hip::policy p, hip::state<Complex>& s, Args... args) { \
-    ::ket::gate::NAME(p, s.begin(), s.end(), args...);                             \
-    return s;                                                                      \
+// adjoints (adj_* of each gate header): self-inverse gates map to themselves, the
+// others to the same gate with negated phase / conjugated coefficient; U2(phi, lambda)^+ =
+// U3(-pi/2, -lambda, -phi), U3(theta, phi, lambda)^+ = U3(-theta, -lambda, -phi)
+#define KET_HIP_SELF_ADJOINT(NAME)                                                                       \
+  template <typename Complex>                                                                            \
+  inline void adj_##NAME(hip::policy p, iter<Complex> first, iter<Complex> last, const operands& o) {    \
+    NAME(p, first, last, o);                                                                             \
   }
-KET_HIP_RANGE(hadamard)
-KET_HIP_RANGE(pauli_x)
-KET_HIP_RANGE(not_)
-KET_HIP_RANGE(controlled_not)
-KET_HIP_RANGE(pauli_y)
-KET_HIP_RANGE(pauli_z)
-KET_HIP_RANGE(phase_shift_coeff)
-KET_HIP_RANGE(phase_shift)
-KET_HIP_RANGE(phase_shift2)
-KET_HIP_RANGE(phase_shift3)
-KET_HIP_RANGE(exponential_pauli_x)
-KET_HIP_RANGE(exponential_pauli_y)
-KET_HIP_RANGE(exponential_pauli_z)
-KET_HIP_RANGE(swap)
-KET_HIP_RANGE(exponential_swap)
-#undef KET_HIP_RANGE
-}  // namespace ranges
+KET_HIP_SELF_ADJOINT(hadamard)
+KET_HIP_SELF_ADJOINT(pauli_x)
+KET_HIP_SELF_ADJOINT(not_)
+KET_HIP_SELF_ADJOINT(controlled_not)
+KET_HIP_SELF_ADJOINT(toffoli)
+KET_HIP_SELF_ADJOINT(pauli_y)
+KET_HIP_SELF_ADJOINT(pauli_z)
+KET_HIP_SELF_ADJOINT(swap)
+#undef KET_HIP_SELF_ADJOINT
+#define KET_HIP_PHASE_ADJOINT(NAME)                                                                      \
+  template <typename Complex>                                                                            \
+  inline void adj_##NAME(hip::policy p, iter<Complex> first, iter<Complex> last, double phase,           \
+                         const operands& o) {                                                            \
+    NAME(p, first, last, -phase, o);                                                                     \
+  }
+#define KET_HIP_COEFF_ADJOINT(NAME)                                                                      \
+  template <typename Complex>                                                                            \
+  inline void adj_##NAME(hip::policy p, iter<Complex> first, iter<Complex> last, std::complex<double> c, \
+                         const operands& o) {                                                            \
+    NAME(p, first, last, std::conj(c), o);                                                               \
+  }
+KET_HIP_PHASE_ADJOINT(phase_shift)
+KET_HIP_PHASE_ADJOINT(controlled_phase_shift)
+KET_HIP_PHASE_ADJOINT(exponential_pauli_x)
+KET_HIP_PHASE_ADJOINT(exponential_pauli_y)
+KET_HIP_PHASE_ADJOINT(exponential_pauli_z)
+KET_HIP_PHASE_ADJOINT(exponential_swap)
+KET_HIP_PHASE_ADJOINT(controlled_v)
+KET_HIP_COEFF_ADJOINT(phase_shift_coeff)
+KET_HIP_COEFF_ADJOINT(controlled_phase_shift_coeff)
+KET_HIP_COEFF_ADJOINT(exponential_pauli_x_coeff)
+KET_HIP_COEFF_ADJOINT(exponential_pauli_y_coeff)
+KET_HIP_COEFF_ADJOINT(exponential_pauli_z_coeff)
+KET_HIP_COEFF_ADJOINT(exponential_swap_coeff)
+KET_HIP_COEFF_ADJOINT(controlled_v_coeff)
+#undef KET_HIP_PHASE_ADJOINT
+#undef KET_HIP_COEFF_ADJOINT
+template <typename Complex>
+inline void adj_sqrt_pauli_z(hip::policy p, iter<Complex> first, iter<Complex> last, const operands& o) {
+  phase_shift_coeff(p, first, last, {0.0, -1.0}, o);
+}
+template <typename Complex>
+inline void adj_phase_shift2(hip::policy p, iter<Complex> first, iter<Complex> last, double phase1, double phase2,
+                             const operands& o) {
+  phase_shift3(p, first, last, -1.57079632679489661923132169163975144, -phase2, -phase1, o);
+}
+template <typename Complex>
+inline void adj_phase_shift3(hip::policy p, iter<Complex> first, iter<Complex> last, double phase1, double phase2,
+                             double phase3, const operands& o) {
+  phase_shift3(p, first, last, -phase1, -phase3, -phase2, o);
+}
+}  // namespace hip_core
+
+// Public forms of gate NAME with real parameters PARAMS (declarations) / ARGS (names):
+//   NAME(policy, first, last, params..., qubit, control...)          -- ket::gate::NAME
+//   ranges::NAME(policy, state, params..., qubit, control...)        -- ket::gate::ranges::NAME
+//   runtime::qubit_ranges::NAME(policy, first, last, params..., targets[, controls])
+//   runtime::ranges::NAME(policy, state, params..., targets[, controls])
+// where targets is one ket::qubit or a range of them and controls a range of ket::control
+// (the runtime forms of the reference, e.g. hadamard.hpp:281-470, pauli_x.hpp:288-572).
+#define KET_HIP_UNPAREN(...) __VA_ARGS__
+#define KET_HIP_GATE(NAME, PARAMS, ARGS)                                                                      \
+  template <typename Complex, typename... Qubits>                                                             \
+  inline void NAME(hip::policy p, hip::device_iterator<Complex> first, hip::device_iterator<Complex> last     \
+                       KET_HIP_UNPAREN PARAMS,                                                                \
+                   Qubits... qubits) {                                                                        \
+    hip_core::NAME(p, first, last KET_HIP_UNPAREN ARGS, hip::detail::split(qubits...));                       \
+  }                                                                                                           \
+  namespace ranges {                                                                                          \
+  template <typename Complex, typename... Qubits>                                                             \
+  inline hip::state<Complex>& NAME(hip::policy p, hip::state<Complex>& s KET_HIP_UNPAREN PARAMS,              \
+                                   Qubits... qubits) {                                                        \
+    hip_core::NAME(p, s.begin(), s.end() KET_HIP_UNPAREN ARGS, hip::detail::split(qubits...));               \
+    return s;                                                                                                 \
+  }                                                                                                           \
+  }                                                                                                           \
+  namespace runtime {                                                                                         \
+  namespace qubit_ranges {                                                                                    \
+  template <typename Complex, typename Targets, typename Controls>                                            \
+  inline void NAME(hip::policy p, hip::device_iterator<Complex> first, hip::device_iterator<Complex> last     \
+                       KET_HIP_UNPAREN PARAMS,                                                                \
+                   const Targets& targets, const Controls& controls) {                                        \
+    hip_core::NAME(p, first, last KET_HIP_UNPAREN ARGS, hip::detail::from_ranges(targets, controls));         \
+  }                                                                                                           \
+  template <typename Complex, typename Targets>                                                               \
+  inline void NAME(hip::policy p, hip::device_iterator<Complex> first, hip::device_iterator<Complex> last     \
+                       KET_HIP_UNPAREN PARAMS,                                                                \
+                   const Targets& targets) {                                                                  \
+    hip_core::NAME(p, first, last KET_HIP_UNPAREN ARGS, hip::detail::from_ranges(targets));                   \
+  }                                                                                                           \
+  }                                                                                                           \
+  namespace ranges {                                                                                          \
+  template <typename Complex, typename Targets, typename Controls>                                            \
+  inline hip::state<Complex>& NAME(hip::policy p, hip::state<Complex>& s KET_HIP_UNPAREN PARAMS,              \
+                                   const Targets& targets, const Controls& controls) {                        \
+    hip_core::NAME(p, s.begin(), s.end() KET_HIP_UNPAREN ARGS, hip::detail::from_ranges(targets, controls));  \
+    return s;                                                                                                 \
+  }                                                                                                           \
+  template <typename Complex, typename Targets>                                                               \
+  inline hip::state<Complex>& NAME(hip::policy p, hip::state<Complex>& s KET_HIP_UNPAREN PARAMS,              \
+                                   const Targets& targets) {                                                  \
+    hip_core::NAME(p, s.begin(), s.end() KET_HIP_UNPAREN ARGS, hip::detail::from_ranges(targets));            \
+    return s;                                                                                                 \
+  }                                                                                                           \
+  }                                                                                                           \
+  }
+
+#define KET_HIP_GATE_AND_ADJOINT(NAME, PARAMS, ARGS) \
+  KET_HIP_GATE(NAME, PARAMS, ARGS)                   \
+  KET_HIP_GATE(adj_##NAME, PARAMS, ARGS)
+
+KET_HIP_GATE_AND_ADJOINT(hadamard, (), ())
+KET_HIP_GATE_AND_ADJOINT(pauli_x, (), ())
+KET_HIP_GATE_AND_ADJOINT(not_, (), ())
+KET_HIP_GATE_AND_ADJOINT(controlled_not, (), ())
+KET_HIP_GATE_AND_ADJOINT(toffoli, (), ())
+KET_HIP_GATE_AND_ADJOINT(pauli_y, (), ())
+KET_HIP_GATE_AND_ADJOINT(pauli_z, (), ())
+KET_HIP_GATE_AND_ADJOINT(swap, (), ())
+KET_HIP_GATE_AND_ADJOINT(sqrt_pauli_x, (), ())
+KET_HIP_GATE_AND_ADJOINT(sqrt_pauli_y, (), ())
+KET_HIP_GATE_AND_ADJOINT(sqrt_pauli_z, (), ())
+KET_HIP_GATE_AND_ADJOINT(x_rotation_half_pi, (), ())
+KET_HIP_GATE_AND_ADJOINT(y_rotation_half_pi, (), ())
+KET_HIP_GATE_AND_ADJOINT(phase_shift_coeff, (, std::complex<double> coeff), (, coeff))
+KET_HIP_GATE_AND_ADJOINT(phase_shift, (, double phase), (, phase))
+KET_HIP_GATE_AND_ADJOINT(controlled_phase_shift_coeff, (, std::complex<double> coeff), (, coeff))
+KET_HIP_GATE_AND_ADJOINT(controlled_phase_shift, (, double phase), (, phase))
+KET_HIP_GATE_AND_ADJOINT(phase_shift2, (, double phase1, double phase2), (, phase1, phase2))
+KET_HIP_GATE_AND_ADJOINT(phase_shift3, (, double phase1, double phase2, double phase3), (, phase1, phase2, phase3))
+KET_HIP_GATE_AND_ADJOINT(exponential_pauli_x, (, double phase), (, phase))
+KET_HIP_GATE_AND_ADJOINT(exponential_pauli_y, (, double phase), (, phase))
+KET_HIP_GATE_AND_ADJOINT(exponential_pauli_z, (, double phase), (, phase))
+KET_HIP_GATE_AND_ADJOINT(exponential_pauli_x_coeff, (, std::complex<double> coeff), (, coeff))
+KET_HIP_GATE_AND_ADJOINT(exponential_pauli_y_coeff, (, std::complex<double> coeff), (, coeff))
+KET_HIP_GATE_AND_ADJOINT(exponential_pauli_z_coeff, (, std::complex<double> coeff), (, coeff))
+KET_HIP_GATE_AND_ADJOINT(exponential_swap, (, double phase), (, phase))
+KET_HIP_GATE_AND_ADJOINT(exponential_swap_coeff, (, std::complex<double> coeff), (, coeff))
+KET_HIP_GATE_AND_ADJOINT(controlled_v, (, double phase), (, phase))
+KET_HIP_GATE_AND_ADJOINT(controlled_v_coeff, (, std::complex<double> coeff), (, coeff))
+#undef KET_HIP_GATE_AND_ADJOINT
+#undef KET_HIP_GATE
+#undef KET_HIP_UNPAREN
+
 }  // namespace gate
 
 // <psi2|psi1> -- ket/include/ket/inner_product.hpp:32-56
@@ -389,6 +619,175 @@ inline std::array<double, 3> spin_expectation_value(hip::policy p, hip::device_i
                                         static_cast<unsigned>(static_cast<B>(q)), r.data(), p.stream));
   return r;
 }
+
+// every qubit's spin_expectation_value -- ket/include/ket/all_spin_expectation_values.hpp:22-42
+template <typename Complex>
+inline std::vector<std::array<double, 3>> all_spin_expectation_values(hip::policy p, hip::device_iterator<Complex> first,
+                                                                      hip::device_iterator<Complex> last) {
+  const std::uint64_t size = hip::detail::span(first, last);
+  unsigned n = 0;
+  while ((std::uint64_t{1} << n) < size) ++n;
+  std::vector<std::array<double, 3>> out(n);
+  for (unsigned q = 0; q < n; ++q)
+    hip::check(brk_spin_expectation_value(first.get(), size, hip::dtype_of<Complex>::value, q, out[q].data(),
+                                          p.stream));
+  return out;
+}
+
+namespace hip {
+namespace detail {
+// std::uniform_real_distribution<double>{0, m}(g) draws generate_canonical(g) * m
+// (positive_random_value_upto.hpp:12-18); the device applies m, so only the canonical part is drawn.
+template <typename RandomNumberGenerator>
+inline double canonical(RandomNumberGenerator& g) {
+  return std::generate_canonical<double, std::numeric_limits<double>::digits>(g);
+}
+template <typename Complex>
+inline unsigned qubit_count(hip::device_iterator<Complex> first, hip::device_iterator<Complex> last) {
+  unsigned n = 0;
+  while ((std::uint64_t{1} << n) < span(first, last)) ++n;
+  return n;
+}
+}  // namespace detail
+}  // namespace hip
+
+// ket::generate_events (generate_events.hpp:15-66): num_events draws; the state is left
+// unchanged (the reference overwrites it with its cumulative probabilities)
+template <typename Complex, typename StateInteger, typename Allocator, typename RandomNumberGenerator>
+inline void generate_events(hip::policy p, std::vector<StateInteger, Allocator>& result,
+                            hip::device_iterator<Complex> first, hip::device_iterator<Complex> last, int num_events,
+                            RandomNumberGenerator& g) {
+  result.clear();
+  if (num_events <= 0) return;
+  std::vector<double> c(static_cast<std::size_t>(num_events));
+  for (auto& u : c) u = hip::detail::canonical(g);
+  std::vector<std::uint64_t> events(c.size());
+  hip::check(brk_generate_events(first.get(), hip::detail::span(first, last), hip::dtype_of<Complex>::value, c.data(),
+                                 static_cast<unsigned>(c.size()), events.data(), p.stream));
+  result.assign(events.begin(), events.end());
+}
+template <typename Complex, typename StateInteger, typename Allocator, typename RandomNumberGenerator>
+inline void generate_events(hip::policy p, std::vector<StateInteger, Allocator>& result,
+                            hip::device_iterator<Complex> first, hip::device_iterator<Complex> last, int num_events,
+                            const RandomNumberGenerator&, typename RandomNumberGenerator::result_type seed) {
+  RandomNumberGenerator g(seed);
+  generate_events(p, result, first, last, num_events, g);
+}
+// ket::measure (measure.hpp:14-40): one draw, then the state becomes that basis state
+template <typename Complex, typename RandomNumberGenerator>
+inline std::uint64_t measure(hip::policy p, hip::device_iterator<Complex> first, hip::device_iterator<Complex> last,
+                             RandomNumberGenerator& g) {
+  const double c = hip::detail::canonical(g);
+  std::uint64_t index = 0;
+  const std::uint64_t size = hip::detail::span(first, last);
+  hip::check(brk_generate_events(first.get(), size, hip::dtype_of<Complex>::value, &c, 1u, &index, p.stream));
+  hip::check(brk_set_basis_state(first.get(), size, hip::dtype_of<Complex>::value, index, p.stream));
+  return index;
+}
+// ket::shor_box (shor_box.hpp:60-98) with the layout bra uses (bra/src/state.cpp:2173-2191): the
+// exponent qubits are the top ones in ascending order, the modular-exponentiation qubits 0, 1, ...
+template <typename Complex, typename StateInteger, typename Qubits>
+inline void shor_box(hip::policy p, hip::device_iterator<Complex> first, hip::device_iterator<Complex> last,
+                     StateInteger base, StateInteger divisor, const Qubits& exponent_qubits,
+                     const Qubits& modular_exponentiation_qubits) {
+  const unsigned n = hip::detail::qubit_count(first, last);
+  const unsigned m = static_cast<unsigned>(std::distance(std::begin(exponent_qubits), std::end(exponent_qubits)));
+  unsigned k = 0;
+  bool layout = true;
+  for (const auto& q : modular_exponentiation_qubits) layout = layout && static_cast<unsigned>(q) == k++;
+  for (const auto& q : exponent_qubits) layout = layout && static_cast<unsigned>(q) == k++;
+  if (!layout || k != n || m > n)
+    throw hip::error(BRK_ERR_UNSUPPORTED, "shor_box supports bra's layout only (exponent qubits on top, ascending)");
+  hip::check(brk_shor_box(first.get(), hip::detail::span(first, last), hip::dtype_of<Complex>::value, m,
+                          static_cast<std::uint64_t>(divisor), static_cast<std::uint64_t>(base), p.stream));
+}
+
+namespace ranges {
+template <typename Complex>
+inline std::vector<std::array<double, 3>> all_spin_expectation_values(hip::policy p, hip::state<Complex>& s) {
+  return ::ket::all_spin_expectation_values(p, s.begin(), s.end());
+}
+template <typename Complex, typename StateInteger, typename Allocator, typename RandomNumberGenerator>
+inline void generate_events(hip::policy p, std::vector<StateInteger, Allocator>& result, hip::state<Complex>& s,
+                            int num_events, RandomNumberGenerator& g) {
+  ::ket::generate_events(p, result, s.begin(), s.end(), num_events, g);
+}
+template <typename Complex, typename StateInteger, typename Allocator, typename RandomNumberGenerator>
+inline void generate_events(hip::policy p, std::vector<StateInteger, Allocator>& result, hip::state<Complex>& s,
+                            int num_events, const RandomNumberGenerator& g,
+                            typename RandomNumberGenerator::result_type seed) {
+  ::ket::generate_events(p, result, s.begin(), s.end(), num_events, g, seed);
+}
+template <typename Complex, typename RandomNumberGenerator>
+inline std::uint64_t measure(hip::policy p, hip::state<Complex>& s, RandomNumberGenerator& g) {
+  return ::ket::measure(p, s.begin(), s.end(), g);
+}
+template <typename Complex, typename StateInteger, typename Qubits>
+inline hip::state<Complex>& shor_box(hip::policy p, hip::state<Complex>& s, StateInteger base, StateInteger divisor,
+                                     const Qubits& exponent_qubits, const Qubits& modular_exponentiation_qubits) {
+  ::ket::shor_box(p, s.begin(), s.end(), base, divisor, exponent_qubits, modular_exponentiation_qubits);
+  return s;
+}
+}  // namespace ranges
+
+namespace gate {
+#ifndef KET_GATE_PROJECTIVE_MEASUREMENT_HPP
+enum class outcome : int { unspecified = -1, zero = 0, one = 1 };  // projective_measurement.hpp:30
+#endif
+// P(qubit = 0 / 1), then the kept half times P^(-1/2) and the other half +0:
+// projective_measurement.hpp:137-176, clear.hpp:85-108, set.hpp:88-111
+template <typename Complex, typename RandomNumberGenerator, typename S, typename B>
+inline outcome projective_measurement(hip::policy p, hip::device_iterator<Complex> first,
+                                      hip::device_iterator<Complex> last, RandomNumberGenerator& g,
+                                      ::ket::qubit<S, B> qubit) {
+  const unsigned q = static_cast<unsigned>(static_cast<B>(qubit));
+  const std::uint64_t size = hip::detail::span(first, last);
+  double pr[2];
+  hip::check(brk_zero_one_probabilities(first.get(), size, hip::dtype_of<Complex>::value, q, pr, p.stream));
+  const double r = std::uniform_real_distribution<double>{0.0, pr[0] + pr[1]}(g);
+  const int v = r < pr[0] ? 0 : 1;
+  hip::check(brk_project_qubit(first.get(), size, hip::dtype_of<Complex>::value, q, v, std::pow(pr[v], -0.5),
+                               p.stream));
+  return v ? outcome::one : outcome::zero;
+}
+template <typename Complex, typename S, typename B>
+inline void clear(hip::policy p, hip::device_iterator<Complex> first, hip::device_iterator<Complex> last,
+                  ::ket::qubit<S, B> qubit) {
+  const unsigned q = static_cast<unsigned>(static_cast<B>(qubit));
+  const std::uint64_t size = hip::detail::span(first, last);
+  double pr[2];
+  hip::check(brk_zero_one_probabilities(first.get(), size, hip::dtype_of<Complex>::value, q, pr, p.stream));
+  hip::check(brk_project_qubit(first.get(), size, hip::dtype_of<Complex>::value, q, 0, std::pow(pr[0], -0.5),
+                               p.stream));
+}
+template <typename Complex, typename S, typename B>
+inline void set(hip::policy p, hip::device_iterator<Complex> first, hip::device_iterator<Complex> last,
+                ::ket::qubit<S, B> qubit) {
+  const unsigned q = static_cast<unsigned>(static_cast<B>(qubit));
+  const std::uint64_t size = hip::detail::span(first, last);
+  double pr[2];
+  hip::check(brk_zero_one_probabilities(first.get(), size, hip::dtype_of<Complex>::value, q, pr, p.stream));
+  hip::check(brk_project_qubit(first.get(), size, hip::dtype_of<Complex>::value, q, 1, std::pow(pr[1], -0.5),
+                               p.stream));
+}
+namespace ranges {
+template <typename Complex, typename RandomNumberGenerator, typename S, typename B>
+inline outcome projective_measurement(hip::policy p, hip::state<Complex>& s, RandomNumberGenerator& g,
+                                      ::ket::qubit<S, B> qubit) {
+  return ::ket::gate::projective_measurement(p, s.begin(), s.end(), g, qubit);
+}
+template <typename Complex, typename S, typename B>
+inline hip::state<Complex>& clear(hip::policy p, hip::state<Complex>& s, ::ket::qubit<S, B> qubit) {
+  ::ket::gate::clear(p, s.begin(), s.end(), qubit);
+  return s;
+}
+template <typename Complex, typename S, typename B>
+inline hip::state<Complex>& set(hip::policy p, hip::state<Complex>& s, ::ket::qubit<S, B> qubit) {
+  ::ket::gate::set(p, s.begin(), s.end(), qubit);
+  return s;
+}
+}  // namespace ranges
+}  // namespace gate
 
 namespace mpi {
 

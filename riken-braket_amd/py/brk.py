@@ -74,6 +74,10 @@ _SIGS = {
     "brk_gate_1q": [_p, _u64, _i, _u, _dp, _up, _u, _p],
     "brk_swap_permutated_local_qubits": [_p, _u64, _i, _u, _u, _p],
     "brk_exponential_swap": [_p, _u64, _i, _d, _u, _u, _up, _u, _p],
+    "brk_exponential_swap_coeff": [_p, _u64, _i, _dp, _u, _u, _up, _u, _p],
+    "brk_exponential_pauli_x_coeff": [_p, _u64, _i, _dp, _up, _u, _up, _u, _p],
+    "brk_exponential_pauli_y_coeff": [_p, _u64, _i, _dp, _up, _u, _up, _u, _p],
+    "brk_exponential_pauli_z_coeff": [_p, _u64, _i, _dp, _up, _u, _up, _u, _p],
     "brk_inner_product": [_p, _i, _p, _i, _u64, _dp, _p],
     "brk_fidelity": [_p, _i, _p, _i, _u64, _dp, _p],
     "brk_norm2": [_p, _u64, _i, _dp, _p],

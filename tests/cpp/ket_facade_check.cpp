@@ -4,8 +4,10 @@
 //   ket_facade_check <out-prefix>
 // writes <out-prefix>_c128.bin and <out-prefix>_c64.bin (the final state as
 // complex128, physical order) and prints the permutation and the reductions as
-// "key value..." lines.  tests/test_gpu_ket_facade.py replays the same
-// sequence on the CPU oracle and compares.
+// "key value..." lines; it also measures / samples a second state with
+// std::mt19937_64(7) and builds a Shor box in a third, each written to its own
+// file.  tests/test_gpu_ket_facade.py replays the same sequence on the CPU
+// oracle and compares.
 #ifdef KET_FACADE_WITH_REFERENCE_TYPES
 #include <ket/control.hpp>
 #include <ket/qubit.hpp>
@@ -13,10 +15,25 @@
 #include "ket_hip.hpp"
 
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
+#include <random>
 #include <string>
 
 namespace {
+
+// the state as complex128, physical order
+template <typename Complex>
+bool write_state(const std::string& path, const std::vector<Complex>& host) {
+  std::FILE* f = std::fopen(path.c_str(), "wb");
+  if (!f) return false;
+  for (const auto& a : host) {
+    const double v[2] = {static_cast<double>(a.real()), static_cast<double>(a.imag())};
+    std::fwrite(v, sizeof(double), 2, f);
+  }
+  std::fclose(f);
+  return true;
+}
 
 template <typename Complex>
 int run(const std::string& prefix, const char* tag) {
@@ -51,21 +68,32 @@ int run(const std::string& prefix, const char* tag) {
   r::exponential_swap(p, s, 0.55, 3_q, 5_q);
   g::exponential_swap(p, s.begin(), s.end(), -0.8, 1_q, 6_q, 2_cq);
 
+  // adjoint forms and the remaining gates of ket/include/ket/gate
+  r::adj_phase_shift3(p, s, 0.1, 0.4, 1.3, 7_q, 2_cq);
+  r::sqrt_pauli_x(p, s, 4_q);
+  r::adj_sqrt_pauli_y(p, s, 2_q, 5_cq);
+  r::sqrt_pauli_z(p, s, 1_cq, 3_cq);
+  r::x_rotation_half_pi(p, s, 6_q);
+  r::adj_y_rotation_half_pi(p, s, 0_q, 7_cq);
+  r::controlled_v(p, s, 0.25, 3_q, 0_cq, 1_cq);
+  r::toffoli(p, s, 5_q, 2_cq, 6_cq);
+  r::controlled_phase_shift(p, s, 0.9, 2_cq, 4_cq);
+  r::exponential_pauli_y_coeff(p, s, std::complex<double>(std::cos(0.3), std::sin(0.3)), 0_q, 4_q);
+  r::adj_exponential_swap(p, s, 0.4, 2_q, 7_q);
+  // runtime (qubit-range) forms
+  const std::vector<ket::qubit<>> targets = {1_q, 6_q};
+  const std::vector<ket::control<ket::qubit<>>> controls = {3_cq};
+  ket::gate::runtime::ranges::pauli_x(p, s, targets, controls);
+  ket::gate::runtime::qubit_ranges::hadamard(p, s.begin(), s.end(), 2_q, controls);
+  ket::gate::runtime::ranges::exponential_pauli_z(p, s, 0.7, targets);
+
   // local swap of physical qubits 1 and 6, then gates addressed through the permutation
   ket::mpi::qubit_permutation<> perm(8);
   ket::mpi::utility::swap_permutated_local_qubits(p, s, perm, 1u, 6u);
   r::hadamard(p, s, ket::qubit<>{perm[1_q]});
   r::exponential_pauli_x(p, s, 0.6, ket::qubit<>{perm[6_q]}, ket::make_control(ket::qubit<>{perm[3_q]}));
 
-  const auto host = s.download();
-  const std::string path = prefix + "_" + tag + ".bin";
-  std::FILE* f = std::fopen(path.c_str(), "wb");
-  if (!f) return 1;
-  for (const auto& a : host) {
-    const double v[2] = {static_cast<double>(a.real()), static_cast<double>(a.imag())};
-    std::fwrite(v, sizeof(double), 2, f);
-  }
-  std::fclose(f);
+  if (!write_state(prefix + "_" + tag + ".bin", s.download())) return 1;
 
   std::printf("%s perm", tag);
   for (unsigned q = 0; q < 8; ++q) std::printf(" %u", perm.data()[q]);
@@ -81,6 +109,34 @@ int run(const std::string& prefix, const char* tag) {
   const auto ip = ket::inner_product(p, s.begin(), s.end(), t.begin());
   std::printf("%s inner %.17g %.17g\n", tag, ip.real(), ip.imag());
   std::printf("%s fidelity %.17g\n", tag, ket::fidelity(p, s.begin(), s.end(), t.begin()));
+
+  // measurement and sampling with bra's generator, then CLEAR / SET, on a second state
+  {
+    state<Complex> m(8);
+    for (unsigned q = 0; q < 8; ++q) r::hadamard(p, m, ket::qubit<>{q});
+    r::phase_shift3(p, m, 0.7, 0.2, -0.5, 3_q);
+    r::controlled_not(p, m, 1_q, 3_cq);
+    std::mt19937_64 rng(7);
+    const auto outcome = r::projective_measurement(p, m, rng, 3_q);
+    std::vector<std::uint64_t> events;
+    ket::ranges::generate_events(p, events, m, 12, rng);
+    r::clear(p, m, 5_q);
+    r::set(p, m, 6_q);
+    std::printf("%s outcome %d\n", tag, static_cast<int>(outcome));
+    std::printf("%s events", tag);
+    for (auto e : events) std::printf(" %llu", static_cast<unsigned long long>(e));
+    std::printf("\n");
+    const auto spins = ket::ranges::all_spin_expectation_values(p, m);
+    std::printf("%s spins7 %.17g %.17g %.17g\n", tag, spins[7][0], spins[7][1], spins[7][2]);
+    write_state(prefix + "_measured_" + tag + ".bin", m.download());
+  }
+  // Shor box with bra's layout: exponent qubits 4..7, modular-exponentiation qubits 0..3
+  {
+    state<Complex> b(8);
+    const std::vector<ket::qubit<>> exponent = {4_q, 5_q, 6_q, 7_q}, modexp = {0_q, 1_q, 2_q, 3_q};
+    ket::ranges::shor_box(p, b, std::uint64_t{7}, std::uint64_t{15}, exponent, modexp);
+    write_state(prefix + "_shor_" + tag + ".bin", b.download());
+  }
 
   // an invalid request throws ket::hip::error
   try {

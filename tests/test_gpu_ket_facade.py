@@ -1,7 +1,8 @@
 """GPU check of the ket-shaped C++ facade: tests/cpp/ket_facade_check drives
-ket::gate::*, ket::gate::ranges::*, ket::mpi::qubit_permutation and the
-reductions through riken-braket_amd/include/ket_hip.hpp; the same sequence is
-replayed here on the CPU oracle."""
+ket::gate::* (with adj_* and runtime:: qubit-range forms), ket::gate::ranges::*,
+ket::mpi::qubit_permutation, the reductions, measurement / sampling with
+std::mt19937_64 and the Shor box through riken-braket_amd/include/ket_hip.hpp;
+the same sequence is replayed here on the CPU oracle."""
 import math
 import os
 import subprocess
@@ -42,12 +43,46 @@ def oracle_sequence(o):
     o.swap(s, 0, 7, [4])
     o.exponential_swap(s, 0.55, 3, 5)
     o.exponential_swap(s, -0.8, 1, 6, [2])
+    # adjoint forms and the remaining gates (matrices of the ket headers' comments)
+    r = 1 / math.sqrt(2)
+    a, b = (1 + 1j) / 2, (1 - 1j) / 2
+    e = complex(math.cos(0.25), math.sin(0.25))
+    o.phase_shift3(s, -0.1, -1.3, -0.4, 7, [2])           # U3(t, p, l)^+ = U3(-t, -l, -p)
+    o.gate_1q(s, 4, [a, b, b, a])                         # sqrt(X)
+    o.gate_1q(s, 2, [b, b, -b, b], [5])                   # sqrt(Y)^+
+    o.phase_shift_coeff(s, 1j, [1, 3])                    # sqrt(Z) on control qubits
+    o.gate_1q(s, 6, [r, 1j * r, 1j * r, r])               # +X
+    o.gate_1q(s, 0, [r, -r, r, r], [7])                   # (+Y)^+ = -Y
+    o.gate_1q(s, 3, [(1 + e) / 2, (1 - e) / 2, (1 - e) / 2, (1 + e) / 2], [0, 1])  # controlled V
+    o.pauli_x(s, [5], [2, 6])                             # Toffoli
+    o.phase_shift(s, 0.9, [2, 4])                         # controlled phase shift
+    o.exponential_pauli_y(s, 0.3, [0, 4])
+    o.exponential_swap(s, -0.4, 2, 7)
+    o.pauli_x(s, [1, 6], [3])                             # runtime forms
+    o.hadamard(s, 2, [3])
+    o.exponential_pauli_z(s, 0.7, [1, 6])
     perm, inv = list(range(8)), list(range(8))
     o.swap_permutated_local_qubits(s, 1, 6)
     o.permutate(perm, inv, inv[1], inv[6])
     o.hadamard(s, perm[1])
     o.exponential_pauli_x(s, 0.6, [perm[6]], [perm[3]])
     return s, perm
+
+
+def oracle_measured(o):
+    """The measured / sampled second state: bra's generator std::mt19937_64(7)."""
+    m = np.zeros(256, dtype=np.complex128)
+    m[0] = 1
+    for q in range(8):
+        o.hadamard(m, q)
+    o.phase_shift3(m, 0.7, 0.2, -0.5, 3)
+    o.pauli_x(m, [1], [3])
+    g = o.mt64(7)
+    outcome = o.projective_measurement(m, 3, g.canonical())
+    events = o.generate_events(m, [g.canonical() for _ in range(12)]).tolist()
+    for q, value in ((5, 0), (6, 1)):
+        o.project(m, q, value, o.zero_one_probabilities(m, q)[value] ** -0.5)
+    return m, outcome, events
 
 
 def parse(stdout):
@@ -84,3 +119,14 @@ def test_facade_program_matches_oracle(oracle, tmp_path):
         assert abs(gi - ip) < tol
         assert abs(float(got[(tag, "fidelity")][0]) - abs(ip) ** 2) < tol
         assert got[(tag, "error")] == ["2"]  # BRK_ERR_QUBIT_OUT_OF_RANGE for 9_q on 8 qubits
+        # measurement, sampling, CLEAR / SET (same draws), all spin expectation values, Shor box
+        m, outcome, events = oracle_measured(oracle)
+        assert int(got[(tag, "outcome")][0]) == outcome
+        assert [int(v) for v in got[(tag, "events")]] == events
+        measured = np.fromfile(tmp_path / f"state_measured_{tag}.bin", dtype=np.complex128)
+        assert np.max(np.abs(measured - m)) < tol, tag
+        assert np.allclose([float(v) for v in got[(tag, "spins7")]], oracle.spin_expectation_value(m, 7),
+                           atol=tol, rtol=0)
+        box = np.zeros(256, dtype=np.complex128)
+        oracle.shor_box(box, 4, 15, 7)
+        assert np.max(np.abs(np.fromfile(tmp_path / f"state_shor_{tag}.bin", dtype=np.complex128) - box)) < tol
