@@ -51,14 +51,29 @@ struct block_op {
   uint32_t pair_bit, flip, cond, sign, zmask;
   uint32_t nins;
   uint32_t ins[15];
+  uint32_t group_flip, group_pair;  // register segments: flip and pair bit over the group index
   double u[8];
+};
+
+// A pass over the chunk in LDS.  A segment pass gives every thread one group of
+// 2^(L - 9) amplitudes -- local indices that differ only in the bits group[0..]
+// -- in registers and applies several gates to it: diagonal gates, and pair
+// gates whose flipped bits are group bits.  A single pass applies one gate
+// with the strided loop.
+enum block_pass_kind { BLOCK_PASS_SEGMENT = 0, BLOCK_PASS_SINGLE = 1 };
+struct block_pass {
+  int32_t kind;
+  uint32_t first_op, num_ops;
+  uint32_t group[4];  // ascending local bits of the group (L - 9 of them)
 };
 
 struct block_program {
   uint32_t num_ops;
   uint32_t num_local;                   // L
+  uint32_t num_passes;
   uint32_t pos[16];                     // ascending physical qubits of the local bits
   uint64_t deposit_high[16];            // physical offset of local index k << BLOCK_THREAD_BITS
+  block_pass passes[BLOCK_MAX_OPS];
   block_op ops[BLOCK_MAX_OPS];
 };
 

@@ -245,10 +245,154 @@ __device__ __forceinline__ uint32_t op_index(uint32_t v, const __attribute__((ad
   return v | set;
 }
 
+typedef const __attribute__((address_space(4))) block_op& cop;
+
+// One gate over the whole chunk: a strided loop over exactly the local indices it changes.
+template <typename R, int PER>
+__device__ __forceinline__ void single_pass(cpx<R>* lds, cop op, uint32_t t) {
+  constexpr uint32_t CAP = PER * BLOCK_THREADS;
+  const int mode = op.mode;
+  const uint32_t nins = op.nins, set = op.cond;
+  const uint32_t items = CAP >> nins;
+  const uint32_t lower0 = (1u << op.ins[0]) - 1u;  // below the first inserted bit
+  if (mode == BLOCK_DIAG) {
+    const cpx<R> c0 = {(R)op.u[0], (R)op.u[1]}, c1 = {(R)op.u[2], (R)op.u[3]};
+    const uint32_t zmask = op.zmask;
+    for (uint32_t v = t; v < items; v += BLOCK_THREADS) {
+      const uint32_t j = op_index(v, op, nins, set, lower0);
+      lds[j] = cmul(lds[j], csel((__popc(j & zmask) & 1) != 0, c1, c0));
+    }
+  } else {
+    const uint32_t flip = op.flip, sign = op.sign;
+    double u[8];
+    for (int k = 0; k < 8; ++k) u[k] = op.u[k];
+    for (uint32_t v = t; v < items; v += BLOCK_THREADS) {
+      const uint32_t j0 = op_index(v, op, nins, set, lower0), j1 = j0 ^ flip;
+      const cpx<R> x0 = lds[j0], x1 = lds[j1];
+      cpx<R> y0, y1;
+      switch (mode) {
+        case BLOCK_H: pair_math<R, MODE_H>(u, sign, j0, x0, x1, y0, y1); break;
+        case BLOCK_SWAP: pair_math<R, MODE_SWAP>(u, sign, j0, x0, x1, y0, y1); break;
+        case BLOCK_ESWAP: pair_math<R, MODE_ESWAP>(u, sign, j0, x0, x1, y0, y1); break;
+        default: pair_math<R, MODE_MATRIX>(u, sign, j0, x0, x1, y0, y1); break;
+      }
+      lds[j0] = y0;
+      lds[j1] = y1;
+    }
+  }
+}
+
+// A pair gate on a thread's group in registers: group index m pairs with m ^ F when the
+// gate's pair bit is clear in m and its condition bits are set in the local index j[m].
+template <int F, typename R, int PER>
+__device__ __forceinline__ void group_pairs(cpx<R> (&x)[PER], const uint32_t (&j)[PER], cop op) {
+  if constexpr (F < PER) {
+    const int mode = op.mode;
+    const uint32_t pair = op.group_pair, cond = op.cond, sign = op.sign;
+    double u[8];
+    for (int k = 0; k < 8; ++k) u[k] = op.u[k];
+    if (pair == (uint32_t)(F & -F)) {
+      // the common case: i0 is the member with the lowest flipped bit clear
+#pragma unroll
+      for (int m = 0; m < PER; ++m) {
+        if ((m & (F & -F)) != 0) continue;
+        const int m1 = m ^ F;
+        cpx<R> y0, y1;
+        switch (mode) {
+          case BLOCK_H: pair_math<R, MODE_H>(u, sign, j[m], x[m], x[m1], y0, y1); break;
+          case BLOCK_SWAP: pair_math<R, MODE_SWAP>(u, sign, j[m], x[m], x[m1], y0, y1); break;
+          case BLOCK_ESWAP: pair_math<R, MODE_ESWAP>(u, sign, j[m], x[m], x[m1], y0, y1); break;
+          default: pair_math<R, MODE_MATRIX>(u, sign, j[m], x[m], x[m1], y0, y1); break;
+        }
+        if (cond == 0) {
+          x[m] = y0;
+          x[m1] = y1;
+        } else {
+          const bool on = (j[m] & cond) == cond;
+          x[m] = csel(on, y0, x[m]);
+          x[m1] = csel(on, y1, x[m1]);
+        }
+      }
+      return;
+    }
+#pragma unroll
+    for (int m = 0; m < PER; ++m) {
+      // each pair once, from its member with the lowest flipped bit clear; i0 is the member
+      // with the gate's pair bit clear (another flipped bit for SWAP)
+      if ((m & (F & -F)) != 0) continue;
+      const int m1 = m ^ F;
+      const bool first_is_i0 = ((uint32_t)m & pair) == 0;
+      const uint32_t i0 = first_is_i0 ? j[m] : j[m1];
+      const bool on = (i0 & cond) == cond;
+      const cpx<R> x0 = csel(first_is_i0, x[m], x[m1]), x1 = csel(first_is_i0, x[m1], x[m]);
+      cpx<R> y0, y1;
+      switch (mode) {
+        case BLOCK_H: pair_math<R, MODE_H>(u, sign, i0, x0, x1, y0, y1); break;
+        case BLOCK_SWAP: pair_math<R, MODE_SWAP>(u, sign, i0, x0, x1, y0, y1); break;
+        case BLOCK_ESWAP: pair_math<R, MODE_ESWAP>(u, sign, i0, x0, x1, y0, y1); break;
+        default: pair_math<R, MODE_MATRIX>(u, sign, i0, x0, x1, y0, y1); break;
+      }
+      const cpx<R> nm = csel(first_is_i0, y0, y1), nm1 = csel(first_is_i0, y1, y0);
+      x[m] = csel(on, nm, x[m]);
+      x[m1] = csel(on, nm1, x[m1]);
+    }
+  }
+}
+
+// A segment: each thread loads one group of PER amplitudes (local indices differing in
+// the group bits only), applies the segment's gates in registers and stores it back.
+template <typename R, int PER>
+__device__ __forceinline__ void segment_pass(cpx<R>* lds, cprogram prog, uint32_t first_op, uint32_t num_ops,
+                                             const __attribute__((address_space(4))) uint32_t* group, uint32_t t) {
+  constexpr int GB = PER == 1 ? 0 : PER == 2 ? 1 : PER == 4 ? 2 : PER == 8 ? 3 : 4;
+  uint32_t base = t;
+#pragma unroll
+  for (int i = 0; i < GB; ++i) {
+    const uint32_t q = group[i];
+    base = ((base >> q) << (q + 1)) | (base & ((1u << q) - 1u));
+  }
+  uint32_t j[PER];
+  cpx<R> x[PER];
+#pragma unroll
+  for (int m = 0; m < PER; ++m) {
+    uint32_t dep = 0;
+#pragma unroll
+    for (int i = 0; i < GB; ++i)
+      if ((m >> i) & 1) dep |= 1u << group[i];
+    j[m] = base | dep;
+    x[m] = lds[j[m]];
+  }
+  for (uint32_t o = 0; o < num_ops; ++o) {
+    cop op = prog->ops[first_op + o];
+    if (op.mode == BLOCK_DIAG) {
+      // in a segment only unconditioned diagonal gates (the planner gives the others a pass)
+      const cpx<R> c0 = {(R)op.u[0], (R)op.u[1]}, c1 = {(R)op.u[2], (R)op.u[3]};
+      const uint32_t zmask = op.zmask;
+#pragma unroll
+      for (int m = 0; m < PER; ++m) x[m] = cmul(x[m], csel((__popc(j[m] & zmask) & 1) != 0, c1, c0));
+    } else {
+      switch (op.group_flip) {
+        case 1: group_pairs<1>(x, j, op); break;
+        case 2: group_pairs<2>(x, j, op); break;
+        case 3: group_pairs<3>(x, j, op); break;
+        case 4: group_pairs<4>(x, j, op); break;
+        case 5: group_pairs<5>(x, j, op); break;
+        case 6: group_pairs<6>(x, j, op); break;
+        case 8: group_pairs<8>(x, j, op); break;
+        case 9: group_pairs<9>(x, j, op); break;
+        case 10: group_pairs<10>(x, j, op); break;
+        default: group_pairs<12>(x, j, op); break;
+      }
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < PER; ++m) lds[j[m]] = x[m];
+}
+
 // One workgroup per chunk of exactly PER * BLOCK_THREADS amplitudes
 // (L = 9 + log2 PER), so every global access of the load and store phases is
-// unconditional.  A gate visits only the local indices it acts on (controls
-// are inserted, not tested), in a strided loop over the workgroup.
+// unconditional.  The gates run as passes over the chunk in LDS: segments
+// (several gates on register-resident groups) and single-gate passes.
 template <typename R, bool NT, int PER>
 __global__ __launch_bounds__(BLOCK_THREADS) void k_block(cpx<R>* __restrict__ s, const block_program* program) {
   constexpr uint32_t CAP = PER * BLOCK_THREADS;
@@ -264,38 +408,11 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_block(cpx<R>* __restrict__ s,
   for (uint32_t k = 0; k < PER; ++k)
     lds[t + BLOCK_THREADS * k] = load_amp<NT>(s + (base | low | prog->deposit_high[k]));
   __syncthreads();
-  const uint32_t nops = prog->num_ops;
-  for (uint32_t o = 0; o < nops; ++o) {
-    const auto& op = prog->ops[o];
-    const int mode = op.mode;
-    const uint32_t nins = op.nins, set = op.cond;
-    const uint32_t items = CAP >> nins;
-    const uint32_t lower0 = (1u << op.ins[0]) - 1u;  // below the first inserted bit
-    if (mode == BLOCK_DIAG) {
-      const cpx<R> c0 = {(R)op.u[0], (R)op.u[1]}, c1 = {(R)op.u[2], (R)op.u[3]};
-      const uint32_t zmask = op.zmask;
-      for (uint32_t v = t; v < items; v += BLOCK_THREADS) {
-        const uint32_t j = op_index(v, op, nins, set, lower0);
-        lds[j] = cmul(lds[j], csel((__popc(j & zmask) & 1) != 0, c1, c0));
-      }
-    } else {
-      const uint32_t flip = op.flip, sign = op.sign;
-      double u[8];
-      for (int k = 0; k < 8; ++k) u[k] = op.u[k];
-      for (uint32_t v = t; v < items; v += BLOCK_THREADS) {
-        const uint32_t j0 = op_index(v, op, nins, set, lower0), j1 = j0 ^ flip;
-        const cpx<R> x0 = lds[j0], x1 = lds[j1];
-        cpx<R> y0, y1;
-        switch (mode) {
-          case BLOCK_H: pair_math<R, MODE_H>(u, sign, j0, x0, x1, y0, y1); break;
-          case BLOCK_SWAP: pair_math<R, MODE_SWAP>(u, sign, j0, x0, x1, y0, y1); break;
-          case BLOCK_ESWAP: pair_math<R, MODE_ESWAP>(u, sign, j0, x0, x1, y0, y1); break;
-          default: pair_math<R, MODE_MATRIX>(u, sign, j0, x0, x1, y0, y1); break;
-        }
-        lds[j0] = y0;
-        lds[j1] = y1;
-      }
-    }
+  const uint32_t npasses = prog->num_passes;
+  for (uint32_t pi = 0; pi < npasses; ++pi) {
+    const auto& pass = prog->passes[pi];
+    if (pass.kind == BLOCK_PASS_SINGLE) single_pass<R, PER>(lds, prog->ops[pass.first_op], t);
+    else segment_pass<R, PER>(lds, prog, pass.first_op, pass.num_ops, pass.group, t);
     __syncthreads();
   }
 #pragma unroll

@@ -64,6 +64,70 @@ struct block_builder {
   }
 };
 
+// Groups the program's gates into passes over LDS.  A segment holds consecutive
+// gates whose flipped bits (at most two per gate) all lie in the segment's
+// L - 9 group bits; diagonal gates join any segment.  Other gates get a pass
+// of their own.
+static void make_passes(block_program& prog) {
+  const unsigned gb = prog.num_local > (unsigned)BLOCK_THREAD_BITS ? prog.num_local - BLOCK_THREAD_BITS : 0;
+  prog.num_passes = 0;
+  uint32_t bits = 0;
+  auto close = [&](uint32_t first, uint32_t count) {
+    if (!count) return;
+    block_pass& p = prog.passes[prog.num_passes++];
+    p = block_pass{};
+    if (count == 1) {  // one gate: the strided pass visits only what it changes
+      p.kind = BLOCK_PASS_SINGLE;
+      p.first_op = first;
+      p.num_ops = 1;
+      return;
+    }
+    p.kind = BLOCK_PASS_SEGMENT;
+    p.first_op = first;
+    p.num_ops = count;
+    for (unsigned b = 0; b < prog.num_local && (unsigned)__builtin_popcount(bits) < gb; ++b) bits |= 1u << b;
+    unsigned k = 0;
+    for (unsigned b = 0; b < prog.num_local; ++b)
+      if (bits >> b & 1u) p.group[k++] = b;
+    for (uint32_t i = first; i < first + count; ++i) {  // flip / pair bit over the group index
+      block_op& o = prog.ops[i];
+      o.group_flip = o.group_pair = 0;
+      for (unsigned g = 0; g < k; ++g) {
+        if (o.mode != BLOCK_DIAG && (o.flip >> p.group[g] & 1u)) o.group_flip |= 1u << g;
+        if (o.mode != BLOCK_DIAG && p.group[g] == o.pair_bit) o.group_pair = 1u << g;
+      }
+    }
+  };
+  uint32_t first = 0, count = 0;
+  for (uint32_t i = 0; i < prog.num_ops; ++i) {
+    const block_op& o = prog.ops[i];
+    const uint32_t mixing = o.mode == BLOCK_DIAG ? 0u : o.flip;
+    // a diagonal gate under conditions visits only its subspace in a pass of its own
+    const bool alone = o.mode == BLOCK_DIAG ? o.cond != 0 : (unsigned)__builtin_popcount(mixing) > 2;
+    if (alone || (unsigned)__builtin_popcount(mixing) > gb) {
+      close(first, count);
+      block_pass& p = prog.passes[prog.num_passes++];
+      p = block_pass{};
+      p.kind = BLOCK_PASS_SINGLE;
+      p.first_op = i;
+      p.num_ops = 1;
+      first = i + 1;
+      count = 0;
+      bits = 0;
+      continue;
+    }
+    if ((unsigned)__builtin_popcount(bits | mixing) > gb) {
+      close(first, count);
+      first = i;
+      count = 0;
+      bits = 0;
+    }
+    bits |= mixing;
+    ++count;
+  }
+  close(first, count);
+}
+
 // Local positions: the block's qubits, the line qubits, then the lowest others up to L.
 static block_program make_program(const block_builder& b) {
   block_program prog{};
@@ -108,6 +172,7 @@ static block_program make_program(const block_builder& b) {
       if (inserted >> bit & 1u) o.ins[o.nins++] = bit;
     std::copy(d.u, d.u + 8, o.u);
   }
+  make_passes(prog);
   return prog;
 }
 
