@@ -77,6 +77,32 @@ struct block_program {
   block_op ops[BLOCK_MAX_OPS];
 };
 
+// Reverse sweep of the adjoint method in blocks: chunks of psi and lambda are
+// held in LDS together (2^11 complex<double> or 2^12 complex<float>
+// amplitudes each, 32 KiB apiece) while the block's ops -- the inverses of
+// the circuit's ops, in reverse order -- are un-applied to both.  Before a
+// parametric op (one target, no controls) is un-applied, its term
+// 2 Re <lambda| D |psi> is summed over the chunk into
+// partials[grad_slot * num_chunks + chunk] (FP64, fixed order).
+constexpr int ADJ_BLOCK_LOCAL_C128 = 11;
+constexpr int ADJ_BLOCK_LOCAL_C64 = 12;
+struct adjoint_block_program {
+  block_program base;                  // the inverse ops as single passes
+  int32_t grad_slot[BLOCK_MAX_OPS];    // -1: no gradient term
+  double d[BLOCK_MAX_OPS][8];          // D = (dU/dtheta) U^dagger, (re, im) of d00, d01, d10, d11
+};
+int launch_adjoint_block(const adjoint_block_program* device_program, const adjoint_block_program& program,
+                         void* psi, void* lam, uint64_t size, int dtype, double* partials, void* stream);
+
+// Whether the adjoint gradient runs in blocks (tuning adjoint_blocks).
+bool adjoint_blocks_enabled();
+
+// A block_program for the described gates (physical qubits of an n-qubit state):
+// local qubits are their qubits, the line_bits lowest qubits, then the lowest others,
+// num_local in all.
+block_program make_block_program(const gate_desc* descs, unsigned count, unsigned n, unsigned num_local,
+                                 unsigned line_bits);
+
 // Local qubits of a chunk for this dtype under the current tuning (block_lds).
 unsigned block_max_local(int dtype);
 // Number of lowest qubits every chunk keeps local (contiguous runs of block_run_bytes).

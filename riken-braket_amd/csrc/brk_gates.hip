@@ -420,6 +420,96 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_block(cpx<R>* __restrict__ s,
     store_amp<NT>(s + (base | low | prog->deposit_high[k]), lds[t + BLOCK_THREADS * k]);
 }
 
+// Reverse sweep of the adjoint method over one chunk of psi and lambda (both in
+// LDS).  For each op of the program (the circuit's inverse ops in reverse
+// order): the gradient term of a parametric op, summed over the chunk in a
+// fixed order, then the op un-applied to both states with the plain
+// arithmetic.
+typedef const __attribute__((address_space(4))) adjoint_block_program* cadjoint;
+
+template <typename R, int PER>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_adjoint_block(cpx<R>* __restrict__ psi, cpx<R>* __restrict__ lam,
+                                                                const adjoint_block_program* program,
+                                                                double* __restrict__ partials) {
+  constexpr uint32_t CAP = PER * BLOCK_THREADS;
+  __shared__ cpx<R> lp[CAP], ll[CAP];
+  __shared__ double wave_sums[BLOCK_THREADS / 64];
+  const cadjoint aprog = (cadjoint)program;
+  const cprogram prog = (cprogram)&program->base;
+  const uint32_t L = prog->num_local;
+  const uint64_t base = chunk_base(blockIdx.x, prog, L);
+  const uint32_t t = threadIdx.x;
+  uint64_t low = 0;
+  for (uint32_t r = 0; r < (uint32_t)BLOCK_THREAD_BITS; ++r) low |= (uint64_t)((t >> r) & 1u) << prog->pos[r];
+#pragma unroll
+  for (uint32_t k = 0; k < PER; ++k) {
+    const uint64_t i = base | low | prog->deposit_high[k];
+    lp[t + BLOCK_THREADS * k] = load_amp<true>(psi + i);
+    ll[t + BLOCK_THREADS * k] = load_amp<true>(lam + i);
+  }
+  __syncthreads();
+  const uint32_t nops = prog->num_ops;
+  for (uint32_t o = 0; o < nops; ++o) {
+    const auto& op = prog->ops[o];
+    const int slot = aprog->grad_slot[o];
+    if (slot >= 0) {
+      // 2 Re <lam| D |psi> over the pairs of the op's target (bit pair_bit of the local index)
+      const uint32_t b = op.pair_bit, lower = (1u << b) - 1u;
+      const double d00r = aprog->d[o][0], d00i = aprog->d[o][1], d01r = aprog->d[o][2], d01i = aprog->d[o][3];
+      const double d10r = aprog->d[o][4], d10i = aprog->d[o][5], d11r = aprog->d[o][6], d11i = aprog->d[o][7];
+      double acc = 0.0;
+      for (uint32_t v = t; v < CAP / 2; v += BLOCK_THREADS) {
+        const uint32_t j0 = ((v & ~lower) << 1) | (v & lower), j1 = j0 | (1u << b);
+        const double p0r = lp[j0].re, p0i = lp[j0].im, p1r = lp[j1].re, p1i = lp[j1].im;
+        const double l0r = ll[j0].re, l0i = ll[j0].im, l1r = ll[j1].re, l1i = ll[j1].im;
+        const double q0r = d00r * p0r - d00i * p0i + d01r * p1r - d01i * p1i;
+        const double q0i = d00r * p0i + d00i * p0r + d01r * p1i + d01i * p1r;
+        const double q1r = d10r * p0r - d10i * p0i + d11r * p1r - d11i * p1i;
+        const double q1i = d10r * p0i + d10i * p0r + d11r * p1i + d11i * p1r;
+        acc += l0r * q0r + l0i * q0i + l1r * q1r + l1i * q1i;
+      }
+      for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+      if ((t & 63) == 0) wave_sums[t >> 6] = acc;
+      __syncthreads();
+      if (t == 0) {
+        double sum = 0.0;
+        for (int w = 0; w < BLOCK_THREADS / 64; ++w) sum += wave_sums[w];
+        partials[(uint64_t)slot * gridDim.x + blockIdx.x] = sum;
+      }
+    }
+    single_pass<R, PER>(lp, op, t);
+    single_pass<R, PER>(ll, op, t);
+    __syncthreads();
+  }
+#pragma unroll
+  for (uint32_t k = 0; k < PER; ++k) {
+    const uint64_t i = base | low | prog->deposit_high[k];
+    store_amp<true>(psi + i, lp[t + BLOCK_THREADS * k]);
+    store_amp<true>(lam + i, ll[t + BLOCK_THREADS * k]);
+  }
+}
+
+int launch_adjoint_block(const adjoint_block_program* device_program, const adjoint_block_program& program,
+                         void* psi, void* lam, uint64_t size, int dtype, double* partials, void* stream) {
+  unsigned nq;
+  if (int st = check_state(psi, size, dtype, &nq)) return st;
+  if (int st = check_state(lam, size, dtype, &nq)) return st;
+  const unsigned L = program.base.num_local;
+  const unsigned want = dtype == BRK_C128 ? ADJ_BLOCK_LOCAL_C128 : ADJ_BLOCK_LOCAL_C64;
+  if (L != want || L > nq || program.base.num_ops > (uint32_t)BLOCK_MAX_OPS)
+    return fail(BRK_ERR_INVALID_ARGUMENT, "adjoint block does not fit (%u local qubits)", L);
+  const uint64_t chunks = size >> L;
+  if (chunks > 0x7fffffffull) return fail(BRK_ERR_INVALID_ARGUMENT, "too many chunks");
+  hipStream_t st = as_stream(stream);
+  if (dtype == BRK_C128)
+    hipLaunchKernelGGL((k_adjoint_block<double, 4>), dim3((unsigned)chunks), dim3(BLOCK_THREADS), 0, st,
+                       (cpx<double>*)psi, (cpx<double>*)lam, device_program, partials);
+  else
+    hipLaunchKernelGGL((k_adjoint_block<float, 8>), dim3((unsigned)chunks), dim3(BLOCK_THREADS), 0, st,
+                       (cpx<float>*)psi, (cpx<float>*)lam, device_program, partials);
+  return check_launch("adjoint block");
+}
+
 // ---------------------------------------------------------------------------
 // Launch helpers
 // ---------------------------------------------------------------------------
@@ -428,6 +518,8 @@ unsigned block_max_local(int dtype) {
   const unsigned amp_log2 = dtype == BRK_C128 ? 4 : 3;
   return (unsigned)__builtin_ctz((unsigned)get_tuning().block_lds) - amp_log2;
 }
+
+bool adjoint_blocks_enabled() { return get_tuning().adjoint_blocks != 0; }
 
 unsigned block_run_qubits(int dtype) {
   const unsigned amp_log2 = dtype == BRK_C128 ? 4 : 3;

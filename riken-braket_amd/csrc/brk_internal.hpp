@@ -114,6 +114,7 @@ struct tuning {
   int fused_chunked;      // fused tiles: 0 = interleaved over all waves, 1 = contiguous run per wave
   int block_lds;          // gate blocks: LDS bytes per chunk (65536, 32768 or 16384) -> local qubits
   int block_run_bytes;    // gate blocks: the lowest qubits addressing this many bytes are always local
+  int adjoint_blocks;     // adjoint gradient: 1 = forward pass and reverse sweep in gate blocks, 0 = per-run passes
 };
 tuning& get_tuning();
 

@@ -58,7 +58,7 @@ static tuning g_tuning = {
     /*pair_lds=*/40960, /*diag_lds=*/0, /*pair_remap=*/1, /*diag_remap=*/1,
     /*nt_min_bytes=*/256ull << 20, /*max_blocks=*/(1ull << 24) - 1, /*line_bytes=*/64, /*fused_groups=*/8192,
     /*fused_occupancy=*/2, /*fused_staged=*/1, /*fused_staged_groups=*/768, /*fused_chunked=*/0,
-    /*block_lds=*/65536, /*block_run_bytes=*/128,
+    /*block_lds=*/65536, /*block_run_bytes=*/128, /*adjoint_blocks=*/1,
 };
 static std::once_flag g_tuning_once;
 
@@ -79,6 +79,7 @@ static void read_tuning_env() {
   get("BRK_TUNE_FUSED_STAGED_GROUPS", &g_tuning.fused_staged_groups);
   get("BRK_TUNE_BLOCK_LDS", &g_tuning.block_lds);
   get("BRK_TUNE_BLOCK_RUN_BYTES", &g_tuning.block_run_bytes);
+  get("BRK_TUNE_ADJOINT_BLOCKS", &g_tuning.adjoint_blocks);
 }
 
 tuning& get_tuning() {
@@ -257,6 +258,8 @@ int brk_tune(const char* key, int64_t value) {
     if (value != 65536 && value != 32768 && value != 16384)
       return fail(BRK_ERR_INVALID_ARGUMENT, "block_lds must be 65536, 32768 or 16384");
     t.block_lds = (int)value;
+  } else if (k == "adjoint_blocks") {
+    t.adjoint_blocks = value != 0;
   } else if (k == "block_run_bytes") {
     if (value < 16 || value > 4096 || (value & (value - 1)))
       return fail(BRK_ERR_INVALID_ARGUMENT, "block_run_bytes must be a power of two in [16, 4096]");
@@ -294,6 +297,7 @@ int brk_get_tune(const char* key, int64_t* value) {
   else if (k == "fused_chunked") *value = t.fused_chunked;
   else if (k == "block_lds") *value = t.block_lds;
   else if (k == "block_run_bytes") *value = t.block_run_bytes;
+  else if (k == "adjoint_blocks") *value = t.adjoint_blocks;
   else if (k == "fused_occupancy") *value = t.fused_occupancy;
   else if (k == "fused_staged") *value = t.fused_staged;
   else if (k == "fused_staged_groups") *value = (int64_t)t.fused_staged_groups;

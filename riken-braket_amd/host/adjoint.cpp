@@ -9,14 +9,20 @@
 //   RX/RY/RZ(theta) = exp(-i theta G / 2):  D = -(i/2) G
 //   EX/EY/EZ(phi)   = exp(i phi G):         D = i G
 //   U1(lambda)      = diag(1, e^{i lambda}):  D = diag(0, i)
-// Runs of consecutive single-qubit ops on the same qubit are swept by one
-// fused kernel (csrc/brk_adjoint.hip); runs of Z / CZ ops (diagonal, self-
-// inverse, mutually commuting) are one sign pass per state; other ops are
-// un-applied to both states with the plain kernels.  The forward pass merges
-// the same runs: a single-qubit run becomes one 2x2 product (FP64 on the host),
-// a Z / CZ run one sign pass.
+// Default (tuning adjoint_blocks = 1): the forward pass runs in gate blocks,
+// and the reverse sweep in blocks of ops whose qubits fit in a chunk of 2^11
+// (complex<double>) or 2^12 (complex<float>) amplitudes: psi and lambda chunks
+// stay in LDS while the block's inverse ops are un-applied and its gradient
+// terms summed (k_adjoint_block), one HBM round trip of both states per block.
+// Otherwise: runs of consecutive single-qubit ops on the same qubit are swept
+// by one fused kernel (csrc/brk_adjoint.hip); runs of Z / CZ ops (diagonal,
+// self-inverse, mutually commuting) are one sign pass per state; other ops
+// are un-applied to both states with the plain kernels.  That forward pass
+// merges the same runs: a single-qubit run becomes one 2x2 product (FP64 on
+// the host), a Z / CZ run one sign pass.
 #include "circuit.hpp"
 #include "brk_adjoint.hpp"
+#include "brk_block.hpp"
 
 #include <hip/hip_runtime_api.h>
 
@@ -225,6 +231,105 @@ static int get_partials(size_t doubles, double** out) {
   return BRK_OK;
 }
 
+// The blocked gradient (see the top of this file).  pidx: parameter index of
+// every op or -1.  Writes the energy and 2 Re <lam| D |psi> per parameter.
+static int blocked_gradient(const brk_circuit* circuit, const std::vector<int>& pidx, unsigned nparams, void* psi,
+                            void* lam, int dtype, const double* J, const double* h, double* energy,
+                            double* gradient, void* stream) {
+  const auto& ops = circuit->ops;
+  const unsigned n = circuit->num_qubits;
+  const uint64_t size = 1ull << n;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  const unsigned L = dtype == BRK_C128 ? ADJ_BLOCK_LOCAL_C128 : ADJ_BLOCK_LOCAL_C64;
+  const unsigned line_bits = dtype == BRK_C128 ? 3 : 4;  // one 128-byte line
+  const uint64_t line = (1ull << line_bits) - 1ull;
+
+  // forward pass in gate blocks, lam = H psi, E = <psi|lam>
+  brk_block_plan* plan = nullptr;
+  if (int s = brk_block_plan_create(circuit, dtype, nullptr, &plan)) return s;
+  int status = brk_block_plan_run(plan, psi, stream);
+  if (!status) status = brk_apply_ising(lam, psi, size, dtype, J, h, stream);
+  double ip[2] = {0.0, 0.0};
+  if (!status) status = brk_inner_product(lam, dtype, psi, dtype, size, ip, stream);
+  (void)brk_block_plan_destroy(plan);  // brk_inner_product synchronised the stream
+  if (status) return status;
+  *energy = ip[0];
+
+  // reverse sweep: blocks of inverse ops, last op first
+  std::vector<adjoint_block_program> programs;
+  std::vector<gate_desc> descs;
+  std::vector<int> slots;
+  std::vector<std::array<double, 8>> dmats;
+  std::vector<unsigned> targets;
+  uint64_t qubits = 0;
+  auto flush = [&]() {
+    if (descs.empty()) return;
+    programs.emplace_back();
+    adjoint_block_program& ap = programs.back();
+    ap = adjoint_block_program{};
+    ap.base = make_block_program(descs.data(), (unsigned)descs.size(), n, L, line_bits);
+    for (size_t k = 0; k < descs.size(); ++k) {
+      ap.grad_slot[k] = slots[k];
+      std::copy(dmats[k].begin(), dmats[k].end(), ap.d[k]);
+      if (slots[k] >= 0)  // the target's local bit (diagonal ops carry no pair bit)
+        for (unsigned r = 0; r < ap.base.num_local; ++r)
+          if (ap.base.pos[r] == targets[k]) ap.base.ops[k].pair_bit = r;
+    }
+    descs.clear();
+    slots.clear();
+    dmats.clear();
+    targets.clear();
+    qubits = 0;
+  };
+  for (size_t i = ops.size(); i-- > 0;) {
+    const brk_circuit::op inv = inverse_op(ops[i]);
+    gate_desc d;
+    if (int s = describe_physical_op(inv.code, inv.targets.data(), (unsigned)inv.targets.size(),
+                                     inv.controls.empty() ? nullptr : inv.controls.data(),
+                                     (unsigned)inv.controls.size(), inv.params.data(), &d))
+      return s;
+    if (__builtin_popcountll(qubits | d.qubits | line) > (int)L || descs.size() == (size_t)BLOCK_MAX_OPS) flush();
+    std::array<double, 8> dm{};
+    if (pidx[i] >= 0) {
+      adj_op a;
+      if (!make_adj_op(ops[i], a)) return fail(BRK_ERR_UNSUPPORTED, "op %zu has no 2x2 form", i);
+      std::copy(a.d, a.d + 8, dm.begin());
+    }
+    descs.push_back(d);
+    slots.push_back(pidx[i]);
+    dmats.push_back(dm);
+    targets.push_back(ops[i].targets[0]);
+    qubits |= d.qubits;
+  }
+  flush();
+
+  const uint64_t chunks = size >> L;
+  double* partials = nullptr;
+  if (int s = get_partials(std::max<size_t>(1, (size_t)nparams * chunks), &partials)) return s;
+  adjoint_block_program* device = nullptr;
+  const size_t bytes = sizeof(adjoint_block_program) * programs.size();
+  if (!programs.empty()) {
+    if (hipMalloc((void**)&device, bytes) != hipSuccess) return fail(BRK_ERR_OUT_OF_MEMORY, "adjoint programs");
+    if (hipMemcpy(device, programs.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) status = BRK_ERR_HIP;
+  }
+  if (status) status = fail(BRK_ERR_HIP, "uploading the adjoint programs failed");
+  for (size_t b = 0; !status && b < programs.size(); ++b)
+    status = launch_adjoint_block(device + b, programs[b], psi, lam, size, dtype, partials, stream);
+  std::vector<double> host((size_t)nparams * chunks);
+  if (!status && nparams &&
+      hipMemcpyAsync(host.data(), partials, host.size() * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess)
+    status = fail(BRK_ERR_HIP, "copying gradient partials failed");
+  if (hipStreamSynchronize(st) != hipSuccess && !status) status = fail(BRK_ERR_HIP, "hipStreamSynchronize failed");
+  if (device) (void)hipFree(device);
+  if (status) return status;
+  for (unsigned p = 0; p < nparams; ++p) {
+    double t = 0.0;
+    for (uint64_t c = 0; c < chunks; ++c) t += host[(size_t)p * chunks + c];
+    gradient[p] = 2.0 * t;
+  }
+  return BRK_OK;
+}
+
 }  // namespace brk
 
 using namespace brk;
@@ -261,6 +366,9 @@ int brk_adjoint_ising_gradient(const brk_circuit* circuit, void* psi, void* work
       pidx[i] = next++;
     }
   }
+  const unsigned L = dtype == BRK_C128 ? ADJ_BLOCK_LOCAL_C128 : ADJ_BLOCK_LOCAL_C64;
+  if (adjoint_blocks_enabled() && circuit->num_qubits >= L)
+    return blocked_gradient(circuit, pidx, nparams, psi, work, dtype, J, h, energy, gradient, stream);
   const uint64_t size = 1ull << circuit->num_qubits;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
 

@@ -176,6 +176,19 @@ static block_program make_program(const block_builder& b) {
   return prog;
 }
 
+block_program make_block_program(const gate_desc* descs, unsigned count, unsigned n, unsigned num_local,
+                                 unsigned line_bits) {
+  block_builder b;
+  b.n = n;
+  b.max_local = num_local;
+  b.line_bits = line_bits;
+  for (unsigned i = 0; i < count; ++i) {
+    b.qubits |= descs[i].qubits;
+    b.descs.push_back(descs[i]);
+  }
+  return make_program(b);
+}
+
 static brk_circuit::op map_op(const brk_circuit::op& o, const permutation& perm) {
   brk_circuit::op m = o;
   for (auto& q : m.targets) q = perm.perm[q];

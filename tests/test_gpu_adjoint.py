@@ -102,6 +102,60 @@ def test_small_ansatz_vs_shift_fd_and_oracle(oracle, dtype):
     assert np.max(np.abs(g - fd)) < 5e-5  # FD truncation + rounding at step 1e-5
 
 
+def mixed_ansatz(n, rng, layers=3):
+    """every parametric kind (RX RY RZ EX EY EZ U1) beside H, CNOT, CZ and SWAP"""
+    ops = []
+    for layer in range(layers):
+        for q in range(n):
+            kind = ("RX", "RY", "RZ", "EX", "EY", "EZ", "U1")[(q + layer) % 7]
+            ops.append((kind, (q,), (), (float(rng.uniform(-math.pi, math.pi)),)))
+        for q in range(layer % 2, n - 1, 2):
+            ops.append(("Z", (q, q + 1), (), ()))
+        ops.append(("H", (int(rng.integers(n)),), (), ()))
+        a, b = (int(x) for x in rng.choice(n, 2, replace=False))
+        ops.append(("X", (b,), (a,), ()))
+        a, b = (int(x) for x in rng.choice(n, 2, replace=False))
+        ops.append(("SWAP", (a, b), (), ()))
+    return ops
+
+
+@pytest.mark.parametrize("n,dtype", [(14, brk.C128), (13, brk.C64)])
+def test_blocked_sweep_matches_chain_sweep(n, dtype):
+    """The blocked reverse sweep (default) against the per-run passes (tuning adjoint_blocks = 0)."""
+    rng = np.random.default_rng(20 + n)
+    ops = mixed_ansatz(n, rng)
+    J, h = rng.uniform(-1, 1, n - 1), rng.uniform(-1, 1, n)
+    note(f"gpu adjoint blocked vs chain n={n} dtype={dtype}")
+    e1, g1, back1 = adjoint(n, ops, J, h, dtype)
+    brk.tune("adjoint_blocks", 0)
+    try:
+        e0, g0, back0 = adjoint(n, ops, J, h, dtype)
+    finally:
+        brk.tune("adjoint_blocks", 1)
+    tol = 1e-12 if dtype == brk.C128 else 2e-5
+    assert abs(e1 - e0) < tol
+    assert np.max(np.abs(g1 - g0)) < tol
+    assert abs(back1[0] - 1) < (1e-12 if dtype == brk.C128 else 1e-5)
+    if dtype == brk.C128:
+        theta = np.array(W.parameters(ops))
+        for k in (0, 5, theta.size // 2, theta.size - 1):
+            kind = [op for op in ops if op[0] in W.PARAMETRIC][k][0]
+            tp, tm = theta.copy(), theta.copy()
+            # exp(-i t G/2) rotations: shift pi/2; exp(i p G): shift pi/4 with factor 1; U1: central FD
+            if kind in ("RX", "RY", "RZ"):
+                tp[k] += math.pi / 2
+                tm[k] -= math.pi / 2
+                ref = 0.5 * (plain_energy(n, W.with_parameters(ops, tp), J, h) -
+                             plain_energy(n, W.with_parameters(ops, tm), J, h))
+                assert abs(g1[k] - ref) < 1e-11
+            else:
+                tp[k] += 1e-5
+                tm[k] -= 1e-5
+                ref = (plain_energy(n, W.with_parameters(ops, tp), J, h) -
+                       plain_energy(n, W.with_parameters(ops, tm), J, h)) / 2e-5
+                assert abs(g1[k] - ref) < 1e-7
+
+
 @pytest.mark.slow
 def test_24q_12_layers_tfim():
     # the differentiation workload: 24 qubits, 12 layers (576 parameters), J = 1.0, h = 0.7
