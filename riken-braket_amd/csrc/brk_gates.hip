@@ -422,9 +422,10 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_block(cpx<R>* __restrict__ s,
 
 // Reverse sweep of the adjoint method over one chunk of psi and lambda (both in
 // LDS).  For each op of the program (the circuit's inverse ops in reverse
-// order): the gradient term of a parametric op, summed over the chunk in a
-// fixed order, then the op un-applied to both states with the plain
-// arithmetic.
+// order): a parametric op reads each amplitude pair of both states once, adds
+// its gradient term (summed over the chunk in a fixed order) and un-applies
+// the op to both; any other op is un-applied to both with a single pass.  The
+// plain arithmetic throughout.
 typedef const __attribute__((address_space(4))) adjoint_block_program* cadjoint;
 
 template <typename R, int PER>
@@ -452,34 +453,57 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_adjoint_block(cpx<R>* __restr
   for (uint32_t o = 0; o < nops; ++o) {
     const auto& op = prog->ops[o];
     const int slot = aprog->grad_slot[o];
-    if (slot >= 0) {
-      // 2 Re <lam| D |psi> over the pairs of the op's target (bit pair_bit of the local index)
-      const uint32_t b = op.pair_bit, lower = (1u << b) - 1u;
-      const double d00r = aprog->d[o][0], d00i = aprog->d[o][1], d01r = aprog->d[o][2], d01i = aprog->d[o][3];
-      const double d10r = aprog->d[o][4], d10i = aprog->d[o][5], d11r = aprog->d[o][6], d11i = aprog->d[o][7];
-      double acc = 0.0;
-      for (uint32_t v = t; v < CAP / 2; v += BLOCK_THREADS) {
-        const uint32_t j0 = ((v & ~lower) << 1) | (v & lower), j1 = j0 | (1u << b);
-        const double p0r = lp[j0].re, p0i = lp[j0].im, p1r = lp[j1].re, p1i = lp[j1].im;
-        const double l0r = ll[j0].re, l0i = ll[j0].im, l1r = ll[j1].re, l1i = ll[j1].im;
-        const double q0r = d00r * p0r - d00i * p0i + d01r * p1r - d01i * p1i;
-        const double q0i = d00r * p0i + d00i * p0r + d01r * p1i + d01i * p1r;
-        const double q1r = d10r * p0r - d10i * p0i + d11r * p1r - d11i * p1i;
-        const double q1i = d10r * p0i + d10i * p0r + d11r * p1i + d11i * p1r;
-        acc += l0r * q0r + l0i * q0i + l1r * q1r + l1i * q1i;
-      }
-      for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
-      if ((t & 63) == 0) wave_sums[t >> 6] = acc;
+    if (slot < 0) {
+      single_pass<R, PER>(lp, op, t);
+      single_pass<R, PER>(ll, op, t);
       __syncthreads();
-      if (t == 0) {
-        double sum = 0.0;
-        for (int w = 0; w < BLOCK_THREADS / 64; ++w) sum += wave_sums[w];
-        partials[(uint64_t)slot * gridDim.x + blockIdx.x] = sum;
-      }
+      continue;
     }
-    single_pass<R, PER>(lp, op, t);
-    single_pass<R, PER>(ll, op, t);
+    // a parametric op (one target, no controls): per pair (j0, j1) of its target bit, read
+    // psi and lambda once, add Re <lam| D |psi>, un-apply the op to both, write both back
+    const uint32_t b = op.pair_bit, lower = (1u << b) - 1u;
+    const double d00r = aprog->d[o][0], d00i = aprog->d[o][1], d01r = aprog->d[o][2], d01i = aprog->d[o][3];
+    const double d10r = aprog->d[o][4], d10i = aprog->d[o][5], d11r = aprog->d[o][6], d11i = aprog->d[o][7];
+    const int mode = op.mode;
+    const uint32_t cond = op.cond, zmask = op.zmask, sign = op.sign;
+    const cpx<R> c0 = {(R)op.u[0], (R)op.u[1]}, c1 = {(R)op.u[2], (R)op.u[3]};
+    double u[8];
+    for (int k = 0; k < 8; ++k) u[k] = op.u[k];
+    double acc = 0.0;
+    for (uint32_t v = t; v < CAP / 2; v += BLOCK_THREADS) {
+      const uint32_t j0 = ((v & ~lower) << 1) | (v & lower), j1 = j0 | (1u << b);
+      const cpx<R> x0 = lp[j0], x1 = lp[j1], y0 = ll[j0], y1 = ll[j1];
+      const double p0r = x0.re, p0i = x0.im, p1r = x1.re, p1i = x1.im;
+      const double q0r = d00r * p0r - d00i * p0i + d01r * p1r - d01i * p1i;
+      const double q0i = d00r * p0i + d00i * p0r + d01r * p1i + d01i * p1r;
+      const double q1r = d10r * p0r - d10i * p0i + d11r * p1r - d11i * p1i;
+      const double q1i = d10r * p0i + d10i * p0r + d11r * p1i + d11i * p1r;
+      acc += (double)y0.re * q0r + (double)y0.im * q0i + (double)y1.re * q1r + (double)y1.im * q1i;
+      cpx<R> a0, a1, e0, e1;
+      if (mode == BLOCK_DIAG) {  // RZ / EZ (parity of the target) or U1 (condition on the target)
+        const bool on0 = (j0 & cond) == cond, on1 = (j1 & cond) == cond;
+        const cpx<R> k0 = csel((__popc(j0 & zmask) & 1) != 0, c1, c0), k1 = csel((__popc(j1 & zmask) & 1) != 0, c1, c0);
+        a0 = csel(on0, cmul(x0, k0), x0);
+        a1 = csel(on1, cmul(x1, k1), x1);
+        e0 = csel(on0, cmul(y0, k0), y0);
+        e1 = csel(on1, cmul(y1, k1), y1);
+      } else {  // RX / RY / EX / EY: the 2x2 of the inverse op
+        pair_math<R, MODE_MATRIX>(u, sign, j0, x0, x1, a0, a1);
+        pair_math<R, MODE_MATRIX>(u, sign, j0, y0, y1, e0, e1);
+      }
+      lp[j0] = a0;
+      lp[j1] = a1;
+      ll[j0] = e0;
+      ll[j1] = e1;
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if ((t & 63) == 0) wave_sums[t >> 6] = acc;
     __syncthreads();
+    if (t == 0) {
+      double sum = 0.0;
+      for (int w = 0; w < BLOCK_THREADS / 64; ++w) sum += wave_sums[w];
+      partials[(uint64_t)slot * gridDim.x + blockIdx.x] = sum;
+    }
   }
 #pragma unroll
   for (uint32_t k = 0; k < PER; ++k) {
