@@ -86,13 +86,25 @@ struct block_program {
 // partials[grad_slot * num_chunks + chunk] (FP64, fixed order).
 constexpr int ADJ_BLOCK_LOCAL_C128 = 11;
 constexpr int ADJ_BLOCK_LOCAL_C64 = 12;
+constexpr int ADJ_SEGMENT_GRADIENTS = 8;  // parametric ops per register segment
+// Forms of a parametric op's arithmetic, from the exact zeros of D and of the inverse op's
+// 2x2: the general formulas with their zero terms left out (the same values).
+//   Z: D = diag(i a, i b), the inverse op diagonal (RZ, EZ, U1);
+//   Y: D = [[0, a], [b, 0]] real, the inverse op a real 2x2 (RY, EY);
+//   X: D = [[0, i a], [i b, 0]], the inverse op [[r, i s], [i s', r']] (RX, EX).
+enum adjoint_grad_kind { ADJ_GRAD_GENERAL = 0, ADJ_GRAD_Z = 1, ADJ_GRAD_Y = 2, ADJ_GRAD_X = 3 };
 struct adjoint_block_program {
-  block_program base;                  // the inverse ops as single passes
+  block_program base;                  // the inverse ops; their passes
   int32_t grad_slot[BLOCK_MAX_OPS];    // -1: no gradient term
+  int32_t grad_kind[BLOCK_MAX_OPS];    // adjoint_grad_kind of a parametric op
   double d[BLOCK_MAX_OPS][8];          // D = (dU/dtheta) U^dagger, (re, im) of d00, d01, d10, d11
 };
 int launch_adjoint_block(const adjoint_block_program* device_program, const adjoint_block_program& program,
                          void* psi, void* lam, uint64_t size, int dtype, double* partials, void* stream);
+// out[p] = scale * (sum over c of partials[p * chunks + c]) for p < rows, on the device in a
+// fixed order (one workgroup per row: strided per-thread sums, then a tree).
+int sum_gradient_partials(const double* partials, unsigned rows, uint64_t chunks, double scale, double* out,
+                          void* stream);
 
 // Whether the adjoint gradient runs in blocks (tuning adjoint_blocks).
 bool adjoint_blocks_enabled();

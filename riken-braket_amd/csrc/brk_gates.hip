@@ -421,12 +421,296 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_block(cpx<R>* __restrict__ s,
 }
 
 // Reverse sweep of the adjoint method over one chunk of psi and lambda (both in
-// LDS).  For each op of the program (the circuit's inverse ops in reverse
-// order): a parametric op reads each amplitude pair of both states once, adds
-// its gradient term (summed over the chunk in a fixed order) and un-applies
-// the op to both; any other op is un-applied to both with a single pass.  The
-// plain arithmetic throughout.
+// LDS).  The program's passes (the circuit's inverse ops in reverse order) are
+// register segments or single ops.  A parametric op reads each amplitude pair of
+// both states once, adds its gradient term (summed over the chunk in a fixed
+// order) and un-applies the op to both.  Non-parametric ops are un-applied with
+// the plain arithmetic; the RX / RY / RZ-type parametric forms use fused
+// multiply-adds (the sweep's results are sums, not amplitudes compared bitwise).
 typedef const __attribute__((address_space(4))) adjoint_block_program* cadjoint;
+
+typedef const __attribute__((address_space(4))) double* cdouble;
+
+__device__ __forceinline__ float fma_r(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_r(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// The gradient term of one pair -- psi (x0, x1), lambda (y0, y1) -- added to two FP64 sums.
+// For the Z, Y and X forms Re <lam| D |psi> = w0 s0 + w1 s1 with the weights of
+// gradient_term:
+//   Z: D = diag(i a, i b):        s0 = Im(conj(l0) p0), s1 = Im(conj(l1) p1);
+//   Y: D = [[0, a], [b, 0]]:      s0 = Re(conj(l0) p1), s1 = Re(conj(l1) p0);
+//   X: D = [[0, i a], [i b, 0]]:  s0 = Im(conj(l0) p1), s1 = Im(conj(l1) p0).
+// The general form adds Re <lam| D |psi> to s0.
+template <int KIND, typename R>
+__device__ __forceinline__ void pair_gradient(cdouble d, cpx<R> x0, cpx<R> x1, cpx<R> y0, cpx<R> y1, double& s0,
+                                              double& s1) {
+  const double p0r = x0.re, p0i = x0.im, p1r = x1.re, p1i = x1.im;
+  const double l0r = y0.re, l0i = y0.im, l1r = y1.re, l1i = y1.im;
+  if constexpr (KIND == ADJ_GRAD_Z) {
+    s0 = __builtin_fma(l0i, p0r, s0);
+    s0 = __builtin_fma(-l0r, p0i, s0);
+    s1 = __builtin_fma(l1i, p1r, s1);
+    s1 = __builtin_fma(-l1r, p1i, s1);
+  } else if constexpr (KIND == ADJ_GRAD_Y) {
+    s0 = __builtin_fma(l0r, p1r, s0);
+    s0 = __builtin_fma(l0i, p1i, s0);
+    s1 = __builtin_fma(l1r, p0r, s1);
+    s1 = __builtin_fma(l1i, p0i, s1);
+  } else if constexpr (KIND == ADJ_GRAD_X) {
+    s0 = __builtin_fma(l0i, p1r, s0);
+    s0 = __builtin_fma(-l0r, p1i, s0);
+    s1 = __builtin_fma(l1i, p0r, s1);
+    s1 = __builtin_fma(-l1r, p0i, s1);
+  } else {
+    const double d00r = d[0], d00i = d[1], d01r = d[2], d01i = d[3], d10r = d[4], d10i = d[5], d11r = d[6],
+                 d11i = d[7];
+    const double q0r = d00r * p0r - d00i * p0i + d01r * p1r - d01i * p1i;
+    const double q0i = d00r * p0i + d00i * p0r + d01r * p1i + d01i * p1r;
+    const double q1r = d10r * p0r - d10i * p0i + d11r * p1r - d11i * p1i;
+    const double q1i = d10r * p0i + d10i * p0r + d11r * p1i + d11i * p1r;
+    s0 += l0r * q0r + l0i * q0i + l1r * q1r + l1i * q1i;
+  }
+}
+
+// Re <lam| D |psi> from the two sums of pair_gradient: the weights are a and b of D.
+template <int KIND>
+__device__ __forceinline__ double gradient_term(cdouble d, double s0, double s1) {
+  if constexpr (KIND == ADJ_GRAD_Z) return d[1] * s0 + d[7] * s1;
+  else if constexpr (KIND == ADJ_GRAD_Y) return d[2] * s0 + d[4] * s1;
+  else if constexpr (KIND == ADJ_GRAD_X) return d[3] * s0 + d[5] * s1;
+  else return s0;
+}
+
+// The inverse of a one-target parametric op in matrix mode on the pair (x0, x1), i0 = j0:
+// the matrix-mode arithmetic in general; the Y form (a real 2x2) and the X form
+// ([[r, i s], [i s', r']]) with fused multiply-adds.
+template <int KIND, typename R>
+__device__ __forceinline__ void unapply_pair(const double (&u)[8], uint64_t sign, uint32_t j0, cpx<R> x0,
+                                             cpx<R> x1, cpx<R>& y0, cpx<R>& y1) {
+  if constexpr (KIND == ADJ_GRAD_Y) {
+    const R m00 = (R)u[0], m01 = (R)u[2], m10 = (R)u[4], m11 = (R)u[6];
+    y0 = {fma_r(m01, x1.re, m00 * x0.re), fma_r(m01, x1.im, m00 * x0.im)};
+    y1 = {fma_r(m11, x1.re, m10 * x0.re), fma_r(m11, x1.im, m10 * x0.im)};
+  } else if constexpr (KIND == ADJ_GRAD_X) {
+    const R m00 = (R)u[0], m01 = (R)u[3], m10 = (R)u[5], m11 = (R)u[6];
+    y0 = {fma_r(-m01, x1.im, m00 * x0.re), fma_r(m01, x1.re, m00 * x0.im)};
+    y1 = {fma_r(m11, x1.re, -(m10 * x0.im)), fma_r(m10, x0.re, m11 * x1.im)};
+  } else {
+    pair_math<R, MODE_MATRIX>(u, sign, j0, x0, x1, y0, y1);
+  }
+}
+
+// The inverse of a diagonal parametric op (RZ / EZ: parity of the target; U1: condition
+// on the target) on one amplitude of each state.
+template <typename R>
+__device__ __forceinline__ void unapply_diag(cop op, uint32_t j, cpx<R>& x, cpx<R>& y) {
+  const cpx<R> c0 = {(R)op.u[0], (R)op.u[1]}, c1 = {(R)op.u[2], (R)op.u[3]};
+  const bool on = (j & op.cond) == op.cond;
+  const cpx<R> k = csel((__popc(j & op.zmask) & 1) != 0, c1, c0);
+  const cpx<R> xk = {fma_r(x.re, k.re, -(x.im * k.im)), fma_r(x.re, k.im, x.im * k.re)};
+  const cpx<R> yk = {fma_r(y.re, k.re, -(y.im * k.im)), fma_r(y.re, k.im, y.im * k.re)};
+  x = csel(on, xk, x);
+  y = csel(on, yk, y);
+}
+
+// A parametric op over the whole chunk: per pair (j0, j1) of its target bit, read psi and
+// lambda once, add Re <lam| D |psi>, un-apply the op to both, write both back.  Returns the
+// thread's share of the gradient term.
+template <int KIND, typename R, int PER>
+__device__ __forceinline__ double param_pass(cpx<R>* lp, cpx<R>* ll, cop op, cdouble d, uint32_t t) {
+  constexpr uint32_t CAP = PER * BLOCK_THREADS;
+  const uint32_t b = op.pair_bit, lower = (1u << b) - 1u;
+  const bool diag = op.mode == BLOCK_DIAG;
+  const uint64_t sign = op.sign;
+  double u[8];
+  for (int k = 0; k < 8; ++k) u[k] = op.u[k];
+  double s0 = 0.0, s1 = 0.0;
+  for (uint32_t v = t; v < CAP / 2; v += BLOCK_THREADS) {
+    const uint32_t j0 = ((v & ~lower) << 1) | (v & lower), j1 = j0 | (1u << b);
+    cpx<R> x0 = lp[j0], x1 = lp[j1], y0 = ll[j0], y1 = ll[j1];
+    pair_gradient<KIND>(d, x0, x1, y0, y1, s0, s1);
+    if (KIND == ADJ_GRAD_Z || (KIND == ADJ_GRAD_GENERAL && diag)) {
+      unapply_diag(op, j0, x0, y0);
+      unapply_diag(op, j1, x1, y1);
+    } else {
+      cpx<R> a0, a1, e0, e1;
+      unapply_pair<KIND>(u, sign, j0, x0, x1, a0, a1);
+      unapply_pair<KIND>(u, sign, j0, y0, y1, e0, e1);
+      x0 = a0;
+      x1 = a1;
+      y0 = e0;
+      y1 = e1;
+    }
+    lp[j0] = x0;
+    lp[j1] = x1;
+    ll[j0] = y0;
+    ll[j1] = y1;
+  }
+  return gradient_term<KIND>(d, s0, s1);
+}
+
+// A parametric op on a thread's group in registers: pairs (m, m | P) of the op's target
+// group bit P.  Adds the gradient term, then un-applies the op to both states.  Every
+// instance reads and writes all PER amplitudes of both groups, so that the instances for
+// different P, merged by the compiler, still address the group arrays statically.
+template <int KIND, int P, typename R, int PER>
+__device__ __forceinline__ double group_param(cpx<R> (&xp)[PER], cpx<R> (&xl)[PER], const uint32_t (&j)[PER],
+                                              cop op, cdouble d) {
+  double s0 = 0.0, s1 = 0.0;
+  if constexpr (P < PER) {
+    cpx<R> a[PER], b[PER];
+#pragma unroll
+    for (int m = 0; m < PER; ++m) {
+      a[m] = xp[m];
+      b[m] = xl[m];
+    }
+    if (KIND == ADJ_GRAD_Z || (KIND == ADJ_GRAD_GENERAL && op.mode == BLOCK_DIAG)) {
+#pragma unroll
+      for (int m = 0; m < PER; ++m) {
+        if ((m & P) != 0) continue;
+        pair_gradient<KIND>(d, a[m], a[m | P], b[m], b[m | P], s0, s1);
+      }
+#pragma unroll
+      for (int m = 0; m < PER; ++m) unapply_diag(op, j[m], a[m], b[m]);
+    } else {
+      const uint64_t sign = op.sign;
+      double u[8];
+      for (int k = 0; k < 8; ++k) u[k] = op.u[k];
+#pragma unroll
+      for (int m = 0; m < PER; ++m) {
+        if ((m & P) != 0) continue;
+        const int m1 = m | P;
+        pair_gradient<KIND>(d, a[m], a[m1], b[m], b[m1], s0, s1);
+        cpx<R> a0, a1, e0, e1;
+        unapply_pair<KIND>(u, sign, j[m], a[m], a[m1], a0, a1);
+        unapply_pair<KIND>(u, sign, j[m], b[m], b[m1], e0, e1);
+        a[m] = a0;
+        a[m1] = a1;
+        b[m] = e0;
+        b[m1] = e1;
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < PER; ++m) {
+      xp[m] = a[m];
+      xl[m] = b[m];
+    }
+  }
+  return gradient_term<KIND>(d, s0, s1);
+}
+
+template <int KIND, typename R, int PER>
+__device__ __forceinline__ double group_param_any(cpx<R> (&xp)[PER], cpx<R> (&xl)[PER], const uint32_t (&j)[PER],
+                                                  cop op, cdouble d) {
+  switch (op.group_pair) {
+    case 1: return group_param<KIND, 1>(xp, xl, j, op, d);
+    case 2: return group_param<KIND, 2>(xp, xl, j, op, d);
+    case 4: return group_param<KIND, 4>(xp, xl, j, op, d);
+    default: return group_param<KIND, 8>(xp, xl, j, op, d);
+  }
+}
+
+// A register segment of the reverse sweep: each thread loads one group of psi and one of
+// lambda (local indices differing in the group bits only), runs the segment's ops on both --
+// the gradient term of a parametric op first, then its inverse -- and stores them; the
+// segment's gradient terms are then reduced over the workgroup in a fixed order.
+template <typename R, int PER>
+__device__ __forceinline__ void adjoint_segment(cpx<R>* lp, cpx<R>* ll, cadjoint aprog, uint32_t first_op,
+                                                uint32_t num_ops,
+                                                const __attribute__((address_space(4))) uint32_t* group, uint32_t t,
+                                                double (*wave_sums)[BLOCK_THREADS / 64], double* partials) {
+  constexpr int GB = PER == 1 ? 0 : PER == 2 ? 1 : PER == 4 ? 2 : PER == 8 ? 3 : 4;
+  const cprogram prog = (cprogram)&aprog->base;
+  uint32_t base = t;
+#pragma unroll
+  for (int i = 0; i < GB; ++i) {
+    const uint32_t q = group[i];
+    base = ((base >> q) << (q + 1)) | (base & ((1u << q) - 1u));
+  }
+  uint32_t j[PER];
+  cpx<R> xp[PER], xl[PER];
+#pragma unroll
+  for (int m = 0; m < PER; ++m) {
+    uint32_t dep = 0;
+#pragma unroll
+    for (int i = 0; i < GB; ++i)
+      if ((m >> i) & 1) dep |= 1u << group[i];
+    j[m] = base | dep;
+    xp[m] = lp[j[m]];
+    xl[m] = ll[j[m]];
+  }
+  double acc[ADJ_SEGMENT_GRADIENTS];
+#pragma unroll
+  for (int k = 0; k < ADJ_SEGMENT_GRADIENTS; ++k) acc[k] = 0.0;
+  uint32_t ngrad = 0;
+  for (uint32_t o = first_op; o < first_op + num_ops; ++o) {
+    cop op = prog->ops[o];
+    if (aprog->grad_slot[o] >= 0) {
+      double g;
+      switch (aprog->grad_kind[o]) {
+        case ADJ_GRAD_Z: g = group_param_any<ADJ_GRAD_Z>(xp, xl, j, op, aprog->d[o]); break;
+        case ADJ_GRAD_Y: g = group_param_any<ADJ_GRAD_Y>(xp, xl, j, op, aprog->d[o]); break;
+        case ADJ_GRAD_X: g = group_param_any<ADJ_GRAD_X>(xp, xl, j, op, aprog->d[o]); break;
+        default: g = group_param_any<ADJ_GRAD_GENERAL>(xp, xl, j, op, aprog->d[o]); break;
+      }
+#pragma unroll
+      for (int k = 0; k < ADJ_SEGMENT_GRADIENTS; ++k)
+        if ((uint32_t)k == ngrad) acc[k] = g;
+      ++ngrad;
+      continue;
+    }
+    if (op.mode == BLOCK_DIAG) {
+      const cpx<R> c0 = {(R)op.u[0], (R)op.u[1]}, c1 = {(R)op.u[2], (R)op.u[3]};
+      const uint32_t cond = op.cond, zmask = op.zmask;
+#pragma unroll
+      for (int m = 0; m < PER; ++m) {
+        const bool on = (j[m] & cond) == cond;
+        const cpx<R> k = csel((__popc(j[m] & zmask) & 1) != 0, c1, c0);
+        xp[m] = csel(on, cmul(xp[m], k), xp[m]);
+        xl[m] = csel(on, cmul(xl[m], k), xl[m]);
+      }
+    } else {
+      switch (op.group_flip) {
+        case 1: group_pairs<1>(xp, j, op); group_pairs<1>(xl, j, op); break;
+        case 2: group_pairs<2>(xp, j, op); group_pairs<2>(xl, j, op); break;
+        case 3: group_pairs<3>(xp, j, op); group_pairs<3>(xl, j, op); break;
+        case 4: group_pairs<4>(xp, j, op); group_pairs<4>(xl, j, op); break;
+        case 5: group_pairs<5>(xp, j, op); group_pairs<5>(xl, j, op); break;
+        case 6: group_pairs<6>(xp, j, op); group_pairs<6>(xl, j, op); break;
+        case 8: group_pairs<8>(xp, j, op); group_pairs<8>(xl, j, op); break;
+        case 9: group_pairs<9>(xp, j, op); group_pairs<9>(xl, j, op); break;
+        case 10: group_pairs<10>(xp, j, op); group_pairs<10>(xl, j, op); break;
+        default: group_pairs<12>(xp, j, op); group_pairs<12>(xl, j, op); break;
+      }
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < PER; ++m) {
+    lp[j[m]] = xp[m];
+    ll[j[m]] = xl[m];
+  }
+  // one reduction round for the segment's gradient terms, in a fixed order
+#pragma unroll
+  for (int k = 0; k < ADJ_SEGMENT_GRADIENTS; ++k) {
+    if ((uint32_t)k >= ngrad) break;
+    double v = acc[k];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((t & 63) == 0) wave_sums[k][t >> 6] = v;
+  }
+  __syncthreads();
+  if (t < ngrad) {
+    // the t-th parametric op of the segment
+    uint32_t seen = 0, slot = 0;
+    for (uint32_t o = first_op; o < first_op + num_ops; ++o)
+      if (aprog->grad_slot[o] >= 0) {
+        if (seen == t) slot = (uint32_t)aprog->grad_slot[o];
+        ++seen;
+      }
+    double sum = 0.0;
+    for (int w = 0; w < BLOCK_THREADS / 64; ++w) sum += wave_sums[t][w];
+    partials[(uint64_t)slot * gridDim.x + blockIdx.x] = sum;
+  }
+}
 
 template <typename R, int PER>
 __global__ __launch_bounds__(BLOCK_THREADS) void k_adjoint_block(cpx<R>* __restrict__ psi, cpx<R>* __restrict__ lam,
@@ -434,7 +718,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_adjoint_block(cpx<R>* __restr
                                                                 double* __restrict__ partials) {
   constexpr uint32_t CAP = PER * BLOCK_THREADS;
   __shared__ cpx<R> lp[CAP], ll[CAP];
-  __shared__ double wave_sums[BLOCK_THREADS / 64];
+  __shared__ double wave_sums[ADJ_SEGMENT_GRADIENTS][BLOCK_THREADS / 64];
   const cadjoint aprog = (cadjoint)program;
   const cprogram prog = (cprogram)&program->base;
   const uint32_t L = prog->num_local;
@@ -449,8 +733,15 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_adjoint_block(cpx<R>* __restr
     ll[t + BLOCK_THREADS * k] = load_amp<true>(lam + i);
   }
   __syncthreads();
-  const uint32_t nops = prog->num_ops;
-  for (uint32_t o = 0; o < nops; ++o) {
+  const uint32_t npasses = prog->num_passes;
+  for (uint32_t pi = 0; pi < npasses; ++pi) {
+    const auto& pass = prog->passes[pi];
+    if (pass.kind == BLOCK_PASS_SEGMENT) {
+      adjoint_segment<R, PER>(lp, ll, aprog, pass.first_op, pass.num_ops, pass.group, t, wave_sums, partials);
+      __syncthreads();
+      continue;
+    }
+    const uint32_t o = pass.first_op;
     const auto& op = prog->ops[o];
     const int slot = aprog->grad_slot[o];
     if (slot < 0) {
@@ -459,51 +750,23 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_adjoint_block(cpx<R>* __restr
       __syncthreads();
       continue;
     }
-    // a parametric op (one target, no controls): per pair (j0, j1) of its target bit, read
-    // psi and lambda once, add Re <lam| D |psi>, un-apply the op to both, write both back
-    const uint32_t b = op.pair_bit, lower = (1u << b) - 1u;
-    const double d00r = aprog->d[o][0], d00i = aprog->d[o][1], d01r = aprog->d[o][2], d01i = aprog->d[o][3];
-    const double d10r = aprog->d[o][4], d10i = aprog->d[o][5], d11r = aprog->d[o][6], d11i = aprog->d[o][7];
-    const int mode = op.mode;
-    const uint32_t cond = op.cond, zmask = op.zmask, sign = op.sign;
-    const cpx<R> c0 = {(R)op.u[0], (R)op.u[1]}, c1 = {(R)op.u[2], (R)op.u[3]};
-    double u[8];
-    for (int k = 0; k < 8; ++k) u[k] = op.u[k];
-    double acc = 0.0;
-    for (uint32_t v = t; v < CAP / 2; v += BLOCK_THREADS) {
-      const uint32_t j0 = ((v & ~lower) << 1) | (v & lower), j1 = j0 | (1u << b);
-      const cpx<R> x0 = lp[j0], x1 = lp[j1], y0 = ll[j0], y1 = ll[j1];
-      const double p0r = x0.re, p0i = x0.im, p1r = x1.re, p1i = x1.im;
-      const double q0r = d00r * p0r - d00i * p0i + d01r * p1r - d01i * p1i;
-      const double q0i = d00r * p0i + d00i * p0r + d01r * p1i + d01i * p1r;
-      const double q1r = d10r * p0r - d10i * p0i + d11r * p1r - d11i * p1i;
-      const double q1i = d10r * p0i + d10i * p0r + d11r * p1i + d11i * p1r;
-      acc += (double)y0.re * q0r + (double)y0.im * q0i + (double)y1.re * q1r + (double)y1.im * q1i;
-      cpx<R> a0, a1, e0, e1;
-      if (mode == BLOCK_DIAG) {  // RZ / EZ (parity of the target) or U1 (condition on the target)
-        const bool on0 = (j0 & cond) == cond, on1 = (j1 & cond) == cond;
-        const cpx<R> k0 = csel((__popc(j0 & zmask) & 1) != 0, c1, c0), k1 = csel((__popc(j1 & zmask) & 1) != 0, c1, c0);
-        a0 = csel(on0, cmul(x0, k0), x0);
-        a1 = csel(on1, cmul(x1, k1), x1);
-        e0 = csel(on0, cmul(y0, k0), y0);
-        e1 = csel(on1, cmul(y1, k1), y1);
-      } else {  // RX / RY / EX / EY: the 2x2 of the inverse op
-        pair_math<R, MODE_MATRIX>(u, sign, j0, x0, x1, a0, a1);
-        pair_math<R, MODE_MATRIX>(u, sign, j0, y0, y1, e0, e1);
-      }
-      lp[j0] = a0;
-      lp[j1] = a1;
-      ll[j0] = e0;
-      ll[j1] = e1;
+    // a parametric op (one target, no controls) with the strided loop over its pairs
+    double acc;
+    switch (aprog->grad_kind[o]) {
+      case ADJ_GRAD_Z: acc = param_pass<ADJ_GRAD_Z, R, PER>(lp, ll, op, aprog->d[o], t); break;
+      case ADJ_GRAD_Y: acc = param_pass<ADJ_GRAD_Y, R, PER>(lp, ll, op, aprog->d[o], t); break;
+      case ADJ_GRAD_X: acc = param_pass<ADJ_GRAD_X, R, PER>(lp, ll, op, aprog->d[o], t); break;
+      default: acc = param_pass<ADJ_GRAD_GENERAL, R, PER>(lp, ll, op, aprog->d[o], t); break;
     }
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
-    if ((t & 63) == 0) wave_sums[t >> 6] = acc;
+    if ((t & 63) == 0) wave_sums[0][t >> 6] = acc;
     __syncthreads();
     if (t == 0) {
       double sum = 0.0;
-      for (int w = 0; w < BLOCK_THREADS / 64; ++w) sum += wave_sums[w];
+      for (int w = 0; w < BLOCK_THREADS / 64; ++w) sum += wave_sums[0][w];
       partials[(uint64_t)slot * gridDim.x + blockIdx.x] = sum;
     }
+    __syncthreads();
   }
 #pragma unroll
   for (uint32_t k = 0; k < PER; ++k) {

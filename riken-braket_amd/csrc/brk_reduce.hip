@@ -10,6 +10,7 @@
 // partials and then std::accumulate's them, inner_product.hpp:43-55; spin
 // values use long double, spin_expectation_value.hpp:31-64).
 #include "brk_internal.hpp"
+#include "brk_block.hpp"
 
 #include <algorithm>
 #include <complex>
@@ -331,6 +332,32 @@ static int spin_raw(const void* first, uint64_t size, int dtype, unsigned qubit,
                              npairs, lower, qmask, partial);
       },
       out);
+}
+
+constexpr int ROW_THREADS = 256;
+
+__global__ __launch_bounds__(ROW_THREADS) void k_row_sums(const double* __restrict__ in, uint64_t cols,
+                                                          double scale, double* __restrict__ out) {
+  __shared__ double part[ROW_THREADS];
+  const double* row = in + (uint64_t)blockIdx.x * cols;
+  double s = 0.0;
+  for (uint64_t c = threadIdx.x; c < cols; c += ROW_THREADS) s += row[c];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = ROW_THREADS / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = scale * part[0];
+}
+
+int sum_gradient_partials(const double* partials, unsigned rows, uint64_t chunks, double scale, double* out,
+                          void* stream) {
+  if (!rows) return BRK_OK;
+  if (!partials || !out || !chunks) return fail(BRK_ERR_INVALID_ARGUMENT, "gradient partials: null or empty");
+  if (rows > 0x7fffffffu) return fail(BRK_ERR_INVALID_ARGUMENT, "too many gradient rows");
+  hipLaunchKernelGGL(k_row_sums, dim3(rows), dim3(ROW_THREADS), 0, as_stream(stream), partials, chunks, scale, out);
+  return check_launch("gradient partial sums");
 }
 
 static int make_ising_args(unsigned nq, const double* J, const double* h, ising_args* a) {

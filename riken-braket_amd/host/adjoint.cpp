@@ -14,6 +14,10 @@
 // (complex<double>) or 2^12 (complex<float>) amplitudes: psi and lambda chunks
 // stay in LDS while the block's inverse ops are un-applied and its gradient
 // terms summed (k_adjoint_block), one HBM round trip of both states per block.
+// Within a block, register segments take runs of ops on at most L - 9 qubits
+// (every thread holds a group of psi and lambda amplitudes) with one LDS round
+// trip; RX / RY / RZ-type ops use the general formulas with their zero terms
+// left out.  Per-chunk gradient terms are summed on the device in a fixed order.
 // Otherwise: runs of consecutive single-qubit ops on the same qubit are swept
 // by one fused kernel (csrc/brk_adjoint.hip); runs of Z / CZ ops (diagonal,
 // self-inverse, mutually commuting) are one sign pass per state; other ops
@@ -231,6 +235,85 @@ static int get_partials(size_t doubles, double** out) {
   return BRK_OK;
 }
 
+// The arithmetic form of parametric op k (adjoint_grad_kind) from the exact zeros of D and of
+// the inverse op's coefficients.
+static int32_t grad_kind(const block_op& op, const double* d) {
+  const bool d00 = d[0] == 0.0 && d[1] == 0.0, d01 = d[2] == 0.0 && d[3] == 0.0, d10 = d[4] == 0.0 && d[5] == 0.0,
+             d11 = d[6] == 0.0 && d[7] == 0.0;
+  if (op.mode == BLOCK_DIAG)
+    return d01 && d10 && d[0] == 0.0 && d[6] == 0.0 ? ADJ_GRAD_Z : ADJ_GRAD_GENERAL;
+  // a sign mask of the target bit alone never flips: the pair's first index has it clear
+  const bool no_flip = (op.sign & ~(1u << op.pair_bit)) == 0;
+  if (op.mode != BLOCK_MATRIX || !no_flip || op.cond != 0 || !d00 || !d11) return ADJ_GRAD_GENERAL;
+  const double* u = op.u;
+  if (u[1] == 0.0 && u[3] == 0.0 && u[5] == 0.0 && u[7] == 0.0 && d[3] == 0.0 && d[5] == 0.0) return ADJ_GRAD_Y;
+  if (u[1] == 0.0 && u[2] == 0.0 && u[4] == 0.0 && u[7] == 0.0 && d[2] == 0.0 && d[4] == 0.0) return ADJ_GRAD_X;
+  return ADJ_GRAD_GENERAL;
+}
+
+// Passes of an adjoint block: register segments (each thread holds one group of
+// psi and lambda amplitudes, L - 9 group bits) take runs of ops whose flipped
+// bits -- and, for a parametric op, its target -- are group bits, with at most
+// ADJ_SEGMENT_GRADIENTS parametric ops; diagonal ops under conditions (CZ) and
+// everything else get a pass of their own, as does a segment of one op.
+static void make_adjoint_passes(adjoint_block_program& ap) {
+  block_program& prog = ap.base;
+  const unsigned gb = prog.num_local - (unsigned)BLOCK_THREAD_BITS;
+  prog.num_passes = 0;
+  uint32_t bits = 0, first = 0, count = 0, grads = 0;
+  auto single = [&](uint32_t i) {
+    block_pass& p = prog.passes[prog.num_passes++];
+    p = block_pass{};
+    p.kind = BLOCK_PASS_SINGLE;
+    p.first_op = i;
+    p.num_ops = 1;
+  };
+  auto close = [&]() {
+    if (count == 1) single(first);
+    if (count > 1) {
+      block_pass& p = prog.passes[prog.num_passes++];
+      p = block_pass{};
+      p.kind = BLOCK_PASS_SEGMENT;
+      p.first_op = first;
+      p.num_ops = count;
+      for (unsigned b = 0; b < prog.num_local && (unsigned)__builtin_popcount(bits) < gb; ++b) bits |= 1u << b;
+      unsigned k = 0;
+      for (unsigned b = 0; b < prog.num_local; ++b)
+        if (bits >> b & 1u) p.group[k++] = b;
+      for (uint32_t i = first; i < first + count; ++i) {
+        block_op& o = prog.ops[i];
+        o.group_flip = o.group_pair = 0;
+        for (unsigned g = 0; g < k; ++g) {
+          if (o.mode != BLOCK_DIAG && (o.flip >> p.group[g] & 1u)) o.group_flip |= 1u << g;
+          if ((o.mode != BLOCK_DIAG || ap.grad_slot[i] >= 0) && p.group[g] == o.pair_bit) o.group_pair = 1u << g;
+        }
+      }
+    }
+    bits = 0;
+    count = 0;
+    grads = 0;
+  };
+  for (uint32_t i = 0; i < prog.num_ops; ++i) {
+    const block_op& o = prog.ops[i];
+    const bool grad = ap.grad_slot[i] >= 0;
+    const uint32_t mixing = (o.mode == BLOCK_DIAG ? 0u : o.flip) | (grad ? 1u << o.pair_bit : 0u);
+    const bool alone = (o.mode == BLOCK_DIAG && !grad && o.cond != 0) ||
+                       (unsigned)__builtin_popcount(mixing) > 2 || (unsigned)__builtin_popcount(mixing) > gb;
+    if (alone) {
+      close();
+      single(i);
+      continue;
+    }
+    if ((unsigned)__builtin_popcount(bits | mixing) > gb || (grad && grads == (uint32_t)ADJ_SEGMENT_GRADIENTS))
+      close();
+    if (count == 0) first = i;
+    bits |= mixing;
+    ++count;
+    grads += grad ? 1 : 0;
+  }
+  close();
+}
+
 // The blocked gradient (see the top of this file).  pidx: parameter index of
 // every op or -1.  Writes the energy and 2 Re <lam| D |psi> per parameter.
 static int blocked_gradient(const brk_circuit* circuit, const std::vector<int>& pidx, unsigned nparams, void* psi,
@@ -271,10 +354,13 @@ static int blocked_gradient(const brk_circuit* circuit, const std::vector<int>& 
     for (size_t k = 0; k < descs.size(); ++k) {
       ap.grad_slot[k] = slots[k];
       std::copy(dmats[k].begin(), dmats[k].end(), ap.d[k]);
-      if (slots[k] >= 0)  // the target's local bit (diagonal ops carry no pair bit)
+      if (slots[k] >= 0) {  // the target's local bit (diagonal ops carry no pair bit)
         for (unsigned r = 0; r < ap.base.num_local; ++r)
           if (ap.base.pos[r] == targets[k]) ap.base.ops[k].pair_bit = r;
+        ap.grad_kind[k] = grad_kind(ap.base.ops[k], ap.d[k]);
+      }
     }
+    make_adjoint_passes(ap);
     descs.clear();
     slots.clear();
     dmats.clear();
@@ -305,7 +391,9 @@ static int blocked_gradient(const brk_circuit* circuit, const std::vector<int>& 
 
   const uint64_t chunks = size >> L;
   double* partials = nullptr;
-  if (int s = get_partials(std::max<size_t>(1, (size_t)nparams * chunks), &partials)) return s;
+  // nparams x chunks partials, then the nparams gradients
+  if (int s = get_partials((size_t)nparams * chunks + std::max(1u, nparams), &partials)) return s;
+  double* sums = partials + (size_t)nparams * chunks;
   adjoint_block_program* device = nullptr;
   const size_t bytes = sizeof(adjoint_block_program) * programs.size();
   if (!programs.empty()) {
@@ -315,19 +403,13 @@ static int blocked_gradient(const brk_circuit* circuit, const std::vector<int>& 
   if (status) status = fail(BRK_ERR_HIP, "uploading the adjoint programs failed");
   for (size_t b = 0; !status && b < programs.size(); ++b)
     status = launch_adjoint_block(device + b, programs[b], psi, lam, size, dtype, partials, stream);
-  std::vector<double> host((size_t)nparams * chunks);
+  if (!status) status = sum_gradient_partials(partials, nparams, chunks, 2.0, sums, stream);
   if (!status && nparams &&
-      hipMemcpyAsync(host.data(), partials, host.size() * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess)
-    status = fail(BRK_ERR_HIP, "copying gradient partials failed");
+      hipMemcpyAsync(gradient, sums, nparams * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess)
+    status = fail(BRK_ERR_HIP, "copying the gradient failed");
   if (hipStreamSynchronize(st) != hipSuccess && !status) status = fail(BRK_ERR_HIP, "hipStreamSynchronize failed");
   if (device) (void)hipFree(device);
-  if (status) return status;
-  for (unsigned p = 0; p < nparams; ++p) {
-    double t = 0.0;
-    for (uint64_t c = 0; c < chunks; ++c) t += host[(size_t)p * chunks + c];
-    gradient[p] = 2.0 * t;
-  }
-  return BRK_OK;
+  return status;
 }
 
 }  // namespace brk
