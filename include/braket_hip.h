@@ -395,6 +395,17 @@ int brk_block_plan_run(brk_block_plan* plan, void* first, void* stream);
    permutation is updated as by brk_circuit_run. */
 int brk_circuit_run_blocked(const brk_circuit* circuit, void* first, int dtype, unsigned* permutation,
                             void* stream);
+/* An order of the circuit's ops that packs into fewer blocks: a topological
+   order of their dependences (ops sharing a qubit, and every op and a local
+   swap, keep their relative order; ops on disjoint qubits commute exactly)
+   built block by block -- ops within the block's qubits first, else the op
+   whose qubits let the most ops run per added qubit -- for blocks of
+   num_local qubits that always hold the line_bits lowest ones.  order[k] is
+   the index of the k-th op (circuit-size entries).  The reordered circuit is
+   the same operator; its amplitudes round differently.  The adjoint gradient
+   sweeps its circuit in this order.  No counterpart in the reference: bra and
+   ket apply gates in the given order. */
+int brk_circuit_block_order(const brk_circuit* circuit, unsigned num_local, unsigned line_bits, unsigned* order);
 
 /* ------------------------------------------------------------------------ */
 /* Precision form: greedy gate fusion into dense 4..6-qubit unitaries,      */

@@ -316,16 +316,27 @@ static void make_adjoint_passes(adjoint_block_program& ap) {
 
 // The blocked gradient (see the top of this file).  pidx: parameter index of
 // every op or -1.  Writes the energy and 2 Re <lam| D |psi> per parameter.
-static int blocked_gradient(const brk_circuit* circuit, const std::vector<int>& pidx, unsigned nparams, void* psi,
-                            void* lam, int dtype, const double* J, const double* h, double* energy,
+static int blocked_gradient(const brk_circuit* given, const std::vector<int>& given_pidx, unsigned nparams,
+                            void* psi, void* lam, int dtype, const double* J, const double* h, double* energy,
                             double* gradient, void* stream) {
-  const auto& ops = circuit->ops;
-  const unsigned n = circuit->num_qubits;
+  const unsigned n = given->num_qubits;
   const uint64_t size = 1ull << n;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   const unsigned L = dtype == BRK_C128 ? ADJ_BLOCK_LOCAL_C128 : ADJ_BLOCK_LOCAL_C64;
   const unsigned line_bits = dtype == BRK_C128 ? 3 : 4;  // one 128-byte line
   const uint64_t line = (1ull << line_bits) - 1ull;
+
+  // the ops reordered into few blocks of L qubits (ops on disjoint qubits commute), with
+  // their parameter indices
+  brk_circuit ordered;
+  ordered.num_qubits = n;
+  std::vector<int> pidx;
+  for (size_t i : block_order(given->ops, L, line_bits)) {
+    ordered.ops.push_back(given->ops[i]);
+    pidx.push_back(given_pidx[i]);
+  }
+  const brk_circuit* circuit = &ordered;
+  const auto& ops = ordered.ops;
 
   // forward pass in gate blocks, lam = H psi, E = <psi|lam>
   brk_block_plan* plan = nullptr;

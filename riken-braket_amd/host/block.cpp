@@ -189,6 +189,113 @@ block_program make_block_program(const gate_desc* descs, unsigned count, unsigne
   return make_program(b);
 }
 
+std::vector<size_t> block_order(const std::vector<brk_circuit::op>& ops, unsigned num_local, unsigned line_bits) {
+  const size_t count = ops.size();
+  std::vector<uint64_t> mask(count, 0);
+  std::vector<std::vector<size_t>> succ(count);
+  std::vector<unsigned> indeg(count, 0);
+  std::vector<long> last(64, -1);
+  auto edge = [&](long from, size_t to) {
+    if (from < 0) return;
+    auto& out = succ[(size_t)from];
+    if (out.empty() || out.back() != to) {  // edges into `to` are added in one go: no duplicates
+      out.push_back(to);
+      ++indeg[to];
+    }
+  };
+  for (size_t i = 0; i < count; ++i) {
+    const auto& o = ops[i];
+    for (unsigned q : o.targets) mask[i] |= 1ull << q;
+    for (unsigned q : o.controls) mask[i] |= 1ull << q;
+    if (o.code == BRK_OP_LOCAL_SWAP) {  // a barrier: it moves qubits between slots
+      std::vector<long> prev(last);
+      std::sort(prev.begin(), prev.end());
+      prev.erase(std::unique(prev.begin(), prev.end()), prev.end());
+      for (long f : prev) edge(f, i);
+      std::fill(last.begin(), last.end(), (long)i);
+      mask[i] = ~0ull;
+      continue;
+    }
+    std::vector<long> prev;
+    for (unsigned q = 0; q < 64; ++q)
+      if (mask[i] >> q & 1ull) prev.push_back(last[q]);
+    std::sort(prev.begin(), prev.end());
+    prev.erase(std::unique(prev.begin(), prev.end()), prev.end());
+    for (long f : prev) edge(f, i);
+    for (unsigned q = 0; q < 64; ++q)
+      if (mask[i] >> q & 1ull) last[q] = (long)i;
+  }
+  std::vector<size_t> ready, order;
+  order.reserve(count);
+  for (size_t i = 0; i < count; ++i)
+    if (!indeg[i]) ready.push_back(i);
+  const uint64_t line = line_bits >= 64 ? ~0ull : (1ull << line_bits) - 1ull;
+  auto take = [&](size_t k) {  // schedule ready[k]
+    const size_t i = ready[k];
+    ready.erase(ready.begin() + (long)k);
+    order.push_back(i);
+    for (size_t j : succ[i])
+      if (--indeg[j] == 0) ready.insert(std::lower_bound(ready.begin(), ready.end(), j), j);
+  };
+  // Lookahead: how many ops (at most `budget`) could run within the local qubits `loc` once
+  // `first` runs, without adding qubits.
+  std::vector<unsigned> tmp_deg(count, 0), stamp(count, 0);
+  unsigned cur_stamp = 0;
+  std::vector<size_t> pend;
+  auto closure = [&](size_t first, uint64_t loc, unsigned budget) {
+    ++cur_stamp;
+    pend.assign(1, first);
+    for (size_t r : ready)
+      if (r != first && !(mask[r] & ~loc)) pend.push_back(r);
+    unsigned got = 0;
+    while (!pend.empty() && got < budget) {
+      const size_t i = pend.back();
+      pend.pop_back();
+      ++got;
+      for (size_t j : succ[i]) {
+        if (stamp[j] != cur_stamp) {
+          stamp[j] = cur_stamp;
+          tmp_deg[j] = indeg[j];
+        }
+        if (--tmp_deg[j] == 0 && !(mask[j] & ~loc)) pend.push_back(j);
+      }
+    }
+    return got;
+  };
+  while (!ready.empty()) {
+    uint64_t local = line;
+    unsigned taken = 0;
+    while (taken < (unsigned)BLOCK_MAX_OPS && !ready.empty()) {
+      // an op within the block's qubits first (the earliest); else the op whose qubits
+      // let the most ops run per added qubit
+      size_t best = ready.size();
+      for (size_t k = 0; k < ready.size() && best == ready.size(); ++k)
+        if (!(mask[ready[k]] & ~local)) best = k;
+      if (best == ready.size()) {
+        double best_score = -1.0;
+        unsigned best_growth = 65;
+        for (size_t k = 0; k < ready.size(); ++k) {
+          const uint64_t m = mask[ready[k]];
+          if (popcount64(local | m) > num_local) continue;
+          const unsigned growth = popcount64(m & ~local);
+          const double score = (double)closure(ready[k], local | m, (unsigned)BLOCK_MAX_OPS - taken) / growth;
+          if (score > best_score || (score == best_score && growth < best_growth)) {
+            best = k;
+            best_score = score;
+            best_growth = growth;
+          }
+        }
+      }
+      if (best == ready.size()) break;
+      local |= mask[ready[best]];
+      take(best);
+      ++taken;
+    }
+    if (!taken) take(0);  // an op no block holds: a step of its own
+  }
+  return order;
+}
+
 static brk_circuit::op map_op(const brk_circuit::op& o, const permutation& perm) {
   brk_circuit::op m = o;
   for (auto& q : m.targets) q = perm.perm[q];
@@ -340,6 +447,15 @@ int brk_block_plan_final_permutation(const brk_block_plan* plan, unsigned* permu
 int brk_block_plan_run(brk_block_plan* plan, void* first, void* stream) {
   if (!plan) return fail(BRK_ERR_INVALID_ARGUMENT, "plan is null");
   return run_plan(plan, first, stream);
+}
+
+int brk_circuit_block_order(const brk_circuit* circuit, unsigned num_local, unsigned line_bits, unsigned* order) {
+  if (!circuit || (!order && !circuit->ops.empty())) return fail(BRK_ERR_INVALID_ARGUMENT, "null argument");
+  if (num_local < 1 || num_local > 64 || line_bits > num_local)
+    return fail(BRK_ERR_INVALID_ARGUMENT, "need 1 <= num_local <= 64 and line_bits <= num_local");
+  const std::vector<size_t> o = block_order(circuit->ops, num_local, line_bits);
+  for (size_t k = 0; k < o.size(); ++k) order[k] = (unsigned)o[k];
+  return BRK_OK;
 }
 
 int brk_circuit_run_blocked(const brk_circuit* circuit, void* first, int dtype, unsigned* permutation,

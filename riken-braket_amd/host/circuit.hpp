@@ -46,4 +46,12 @@ int apply_physical_op(const brk_circuit::op& o, void* first, uint64_t size, int 
 // ops without a dense form (LOCAL_SWAP).
 bool op_unitary(const brk_circuit::op& o, std::vector<unsigned>& qubits, std::vector<std::array<double, 2>>& u);
 
+// An order of the ops for gate blocks: a topological order of their dependences (ops that
+// share a qubit keep their relative order; a local swap keeps its place relative to every
+// op) that a greedy scheduler packs into few blocks of num_local qubits, the line_bits
+// lowest always among them, at most BLOCK_MAX_OPS ops each.  Ops on disjoint qubits
+// commute, so the circuit is the same operator; its amplitudes round differently from
+// those of the given order.
+std::vector<size_t> block_order(const std::vector<brk_circuit::op>& ops, unsigned num_local, unsigned line_bits);
+
 }  // namespace brk

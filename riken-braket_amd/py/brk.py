@@ -103,6 +103,7 @@ _SIGS = {
     "brk_block_plan_final_permutation": [_p, _up],
     "brk_block_plan_run": [_p, _p, _p],
     "brk_circuit_run_blocked": [_p, _p, _i, _up, _p],
+    "brk_circuit_block_order": [_p, _u, _u, _up],
     "brk_fused_plan_create": [_p, _u, _up, C.POINTER(C.c_void_p)],
     "brk_fused_plan_destroy": [_p],
     "brk_fused_plan_num_blocks": [_p, _up],
@@ -448,6 +449,16 @@ class Circuit:
         pa = (C.c_uint * self.num_qubits)(*perm)
         call("brk_circuit_run_blocked", self.h, ptr, dtype, pa, stream)
         return list(pa)
+
+    def block_order(self, num_local, line_bits):
+        """brk_circuit_block_order: op indices in an order that packs into fewer gate blocks."""
+        out = (C.c_uint * max(1, len(self.ops)))()
+        call("brk_circuit_block_order", self.h, num_local, line_bits, out)
+        return list(out)[: len(self.ops)]
+
+    def reordered(self, order):
+        """The same circuit with its ops in `order` (a list of op indices)."""
+        return Circuit(self.num_qubits, [self.ops[i] for i in order])
 
     def run_timed(self, ptr, dtype=C128, permutation=None, stream=0):
         perm = list(range(self.num_qubits)) if permutation is None else list(permutation)

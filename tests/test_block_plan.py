@@ -72,3 +72,53 @@ def test_single_ops_and_local_swaps_are_plain():
     ops = [("H", (0,), (), ()), ("LOCAL_SWAP", (1, 9), (), ()), ("H", (1,), (), ())]
     plan = brk.BlockPlan(brk.Circuit(10, ops))
     assert plan.steps == [(0, 1, ()), (1, 1, ()), (2, 1, ())]
+
+
+def _shares(a, b):
+    qa, qb = set(a[1]) | set(a[2]), set(b[1]) | set(b[2])
+    return a[0] == "LOCAL_SWAP" or b[0] == "LOCAL_SWAP" or bool(qa & qb)
+
+
+@pytest.mark.parametrize("n,seed,swaps,L,line", [(10, 1, 0, 11, 3), (20, 2, 3, 11, 3), (24, 3, 0, 12, 4),
+                                                 (6, 4, 2, 5, 2)])
+def test_block_order_keeps_dependences(n, seed, swaps, L, line):
+    """brk_circuit_block_order is a permutation that keeps every pair of ops sharing a
+    qubit (and every op relative to a local swap) in the given order."""
+    ops = random_ops(n, 250, seed, local_swaps=swaps)
+    order = brk.Circuit(n, ops).block_order(L, line)
+    assert sorted(order) == list(range(len(ops)))
+    pos = {i: k for k, i in enumerate(order)}
+    for i in range(len(ops)):
+        for j in range(i + 1, len(ops)):
+            if _shares(ops[i], ops[j]):
+                assert pos[i] < pos[j], (i, j)
+
+
+def test_block_order_packs_layered_ansatz():
+    """24 qubits, 12 layers of RY, RZ on every qubit and a ZZ ladder: the given order
+    needs 60 blocks of 12 local qubits; the block order needs fewer than half."""
+    import circuits as W
+    ops, _ = W.adjoint_ansatz(24, 12, 5)
+    c = brk.Circuit(24, ops)
+    before = brk.BlockPlan(c)
+    after = brk.BlockPlan(c.reordered(c.block_order(11, 3)))
+    nb = lambda p: sum(1 for _, _, local in p.steps if local)
+    assert nb(before) == 60
+    assert nb(after) < 30
+
+
+def test_block_order_is_the_same_operator(oracle):
+    """The reordered circuit gives the same state on the CPU oracle (ops on disjoint
+    qubits commute): within 1e-12 of the given order's."""
+    import numpy as np
+    import dense as D
+    from oracle_circuits import run_ops
+    n = 10
+    ops = random_ops(n, 200, 7)
+    order = brk.Circuit(n, ops).block_order(6, 2)
+    assert order != list(range(len(ops)))
+    s = D.random_state(n, np.random.default_rng(7))
+    a, b = s.copy(), s.copy()
+    run_ops(oracle, a, ops)
+    run_ops(oracle, b, [ops[i] for i in order])
+    assert np.max(np.abs(a - b)) < 1e-12
