@@ -52,6 +52,7 @@ struct block_op {
   uint32_t nins;
   uint32_t ins[15];
   uint32_t group_flip, group_pair;  // register segments: flip and pair bit over the group index
+  uint32_t negate;                   // diagonal with u = (-1, -1) (Z, CZ): an exact negation on cond
   double u[8];
 };
 

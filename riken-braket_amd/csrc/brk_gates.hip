@@ -106,9 +106,11 @@ __device__ __forceinline__ void pair_math(const U& u, uint64_t signmask, uint64_
   } else {
     cpx<R> m00, m01, m10, m11;
     load_matrix(u, m00, m01, m10, m11);
-    const bool flip = (__popcll(i0 & signmask) & 1) != 0;
-    m01 = csel(flip, cpx<R>{-m01.re, -m01.im}, m01);
-    m10 = csel(flip, cpx<R>{-m10.re, -m10.im}, m10);
+    if (signmask != 0) {  // uniform: most gates have no sign mask
+      const bool flip = (__popcll(i0 & signmask) & 1) != 0;
+      m01 = csel(flip, cpx<R>{-m01.re, -m01.im}, m01);
+      m10 = csel(flip, cpx<R>{-m10.re, -m10.im}, m10);
+    }
     y0 = cadd(cmul(m00, x0), cmul(m01, x1));
     y1 = cadd(cmul(m10, x0), cmul(m11, x1));
   }
@@ -201,6 +203,8 @@ __global__ __launch_bounds__(GATE_THREADS) void k_diag(cpx<R>* __restrict__ s, u
   if (a.k < 0) occupancy_lds[threadIdx.x] = 0;
   const cpx<R> c0 = {(R)a.u[0], (R)a.u[1]};
   const cpx<R> c1 = {(R)a.u[2], (R)a.u[3]};
+  // Z and CZ: both coefficients -1, an exact negation
+  const bool negate = a.u[0] == -1.0 && a.u[1] == 0.0 && a.u[2] == -1.0 && a.u[3] == 0.0;
   const uint64_t stride = (uint64_t)gridDim.x * GATE_THREADS;
   for (uint64_t g = xcd_tile(blockIdx.x, gridDim.x, remap) * GATE_THREADS + threadIdx.x; g < nitems;
        g += stride) {
@@ -208,7 +212,7 @@ __global__ __launch_bounds__(GATE_THREADS) void k_diag(cpx<R>* __restrict__ s, u
     const cpx<R> x = load_amp<NT>(s + i);
     cpx<R> c = c0;
     if constexpr (PARITY) c = csel((__popcll(i & a.zmask) & 1) != 0, c1, c0);
-    cpx<R> y = cmul(x, c);
+    cpx<R> y = negate ? cneg(x) : cmul(x, c);
     if (COND) y = csel((i & a.condmask) == a.condmask, y, x);
     store_amp<NT>(s + i, y);
   }
@@ -258,6 +262,13 @@ __device__ __forceinline__ void single_pass(cpx<R>* lds, cop op, uint32_t t) {
   if (mode == BLOCK_DIAG) {
     const cpx<R> c0 = {(R)op.u[0], (R)op.u[1]}, c1 = {(R)op.u[2], (R)op.u[3]};
     const uint32_t zmask = op.zmask;
+    if (op.negate) {  // the visited indices are exactly the cond subspace
+      for (uint32_t v = t; v < items; v += BLOCK_THREADS) {
+        const uint32_t j = op_index(v, op, nins, set, lower0);
+        lds[j] = cneg(lds[j]);
+      }
+      return;
+    }
     for (uint32_t v = t; v < items; v += BLOCK_THREADS) {
       const uint32_t j = op_index(v, op, nins, set, lower0);
       lds[j] = cmul(lds[j], csel((__popc(j & zmask) & 1) != 0, c1, c0));
@@ -365,7 +376,14 @@ __device__ __forceinline__ void segment_pass(cpx<R>* lds, cprogram prog, uint32_
   for (uint32_t o = 0; o < num_ops; ++o) {
     cop op = prog->ops[first_op + o];
     if (op.mode == BLOCK_DIAG) {
-      // in a segment only unconditioned diagonal gates (the planner gives the others a pass)
+      // in a segment: negations under conditions, and unconditioned diagonal gates (the
+      // planner gives the others a pass)
+      if (op.negate) {
+        const uint32_t cond = op.cond;
+#pragma unroll
+        for (int m = 0; m < PER; ++m) x[m] = csel((j[m] & cond) == cond, cneg(x[m]), x[m]);
+        continue;
+      }
       const cpx<R> c0 = {(R)op.u[0], (R)op.u[1]}, c1 = {(R)op.u[2], (R)op.u[3]};
       const uint32_t zmask = op.zmask;
 #pragma unroll
@@ -660,8 +678,17 @@ __device__ __forceinline__ void adjoint_segment(cpx<R>* lp, cpx<R>* ll, cadjoint
       continue;
     }
     if (op.mode == BLOCK_DIAG) {
-      const cpx<R> c0 = {(R)op.u[0], (R)op.u[1]}, c1 = {(R)op.u[2], (R)op.u[3]};
       const uint32_t cond = op.cond, zmask = op.zmask;
+      if (op.negate) {
+#pragma unroll
+        for (int m = 0; m < PER; ++m) {
+          const bool on = (j[m] & cond) == cond;
+          xp[m] = csel(on, cneg(xp[m]), xp[m]);
+          xl[m] = csel(on, cneg(xl[m]), xl[m]);
+        }
+        continue;
+      }
+      const cpx<R> c0 = {(R)op.u[0], (R)op.u[1]}, c1 = {(R)op.u[2], (R)op.u[3]};
 #pragma unroll
       for (int m = 0; m < PER; ++m) {
         const bool on = (j[m] & cond) == cond;

@@ -59,6 +59,8 @@ template <typename R>
 __device__ __forceinline__ cpx<R> csub(cpx<R> a, cpx<R> b) { return {a.re - b.re, a.im - b.im}; }
 template <typename R>
 __device__ __forceinline__ cpx<R> cscale(cpx<R> a, R s) { return {a.re * s, a.im * s}; }
+template <typename R>
+__device__ __forceinline__ cpx<R> cneg(cpx<R> a) { return {-a.re, -a.im}; }
 
 inline size_t amp_bytes(int dtype) { return dtype == BRK_C128 ? 16 : 8; }
 

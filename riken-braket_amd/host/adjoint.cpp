@@ -254,8 +254,9 @@ static int32_t grad_kind(const block_op& op, const double* d) {
 // Passes of an adjoint block: register segments (each thread holds one group of
 // psi and lambda amplitudes, L - 9 group bits) take runs of ops whose flipped
 // bits -- and, for a parametric op, its target -- are group bits, with at most
-// ADJ_SEGMENT_GRADIENTS parametric ops; diagonal ops under conditions (CZ) and
-// everything else get a pass of their own, as does a segment of one op.
+// ADJ_SEGMENT_GRADIENTS parametric ops; diagonal ops under conditions other than
+// negations (Z, CZ) and everything else get a pass of their own, as does a segment
+// of one op.
 static void make_adjoint_passes(adjoint_block_program& ap) {
   block_program& prog = ap.base;
   const unsigned gb = prog.num_local - (unsigned)BLOCK_THREAD_BITS;
@@ -297,7 +298,7 @@ static void make_adjoint_passes(adjoint_block_program& ap) {
     const block_op& o = prog.ops[i];
     const bool grad = ap.grad_slot[i] >= 0;
     const uint32_t mixing = (o.mode == BLOCK_DIAG ? 0u : o.flip) | (grad ? 1u << o.pair_bit : 0u);
-    const bool alone = (o.mode == BLOCK_DIAG && !grad && o.cond != 0) ||
+    const bool alone = (o.mode == BLOCK_DIAG && !grad && o.cond != 0 && !o.negate) ||
                        (unsigned)__builtin_popcount(mixing) > 2 || (unsigned)__builtin_popcount(mixing) > gb;
     if (alone) {
       close();

@@ -102,8 +102,9 @@ static void make_passes(block_program& prog) {
   for (uint32_t i = 0; i < prog.num_ops; ++i) {
     const block_op& o = prog.ops[i];
     const uint32_t mixing = o.mode == BLOCK_DIAG ? 0u : o.flip;
-    // a diagonal gate under conditions visits only its subspace in a pass of its own
-    const bool alone = o.mode == BLOCK_DIAG ? o.cond != 0 : (unsigned)__builtin_popcount(mixing) > 2;
+    // a diagonal gate under conditions visits only its subspace in a pass of its own, unless
+    // it is a negation (Z, CZ), which a segment applies with a select
+    const bool alone = o.mode == BLOCK_DIAG ? o.cond != 0 && !o.negate : (unsigned)__builtin_popcount(mixing) > 2;
     if (alone || (unsigned)__builtin_popcount(mixing) > gb) {
       close(first, count);
       block_pass& p = prog.passes[prog.num_passes++];
@@ -165,6 +166,9 @@ static block_program make_program(const block_builder& b) {
     o.flip = to_local(d.flip);
     o.cond = to_local(d.cond);
     o.sign = to_local(d.sign);
+    // the pair's first index has the pair bit clear: that bit of the sign mask never counts
+    if (d.mode == BLOCK_MATRIX) o.sign &= ~(1u << o.pair_bit);
+    o.negate = d.mode == BLOCK_DIAG && d.u[0] == -1.0 && d.u[1] == 0.0 && d.u[2] == -1.0 && d.u[3] == 0.0;
     o.zmask = to_local(d.zmask);
     const uint32_t inserted = o.cond | (d.mode == BLOCK_DIAG ? 0u : 1u << o.pair_bit);
     o.nins = 0;
