@@ -90,7 +90,8 @@ constexpr int ADJ_BLOCK_LOCAL_C64 = 12;
 constexpr int ADJ_SEGMENT_GRADIENTS = 8;  // parametric ops per register segment
 // Forms of a parametric op's arithmetic, from the exact zeros of D and of the inverse op's
 // 2x2: the general formulas with their zero terms left out (the same values).
-//   Z: D = diag(i a, i b), the inverse op diagonal (RZ, EZ, U1);
+//   Z: D = diag(i a, i b), the inverse op diagonal on the parity of the target
+//      (RZ, EZ) or on its 1 side (U1);
 //   Y: D = [[0, a], [b, 0]] real, the inverse op a real 2x2 (RY, EY);
 //   X: D = [[0, i a], [i b, 0]], the inverse op [[r, i s], [i s', r']] (RX, EX).
 enum adjoint_grad_kind { ADJ_GRAD_GENERAL = 0, ADJ_GRAD_Z = 1, ADJ_GRAD_Y = 2, ADJ_GRAD_X = 3 };

@@ -518,6 +518,11 @@ __device__ __forceinline__ void unapply_pair(const double (&u)[8], uint64_t sign
   }
 }
 
+template <typename R>
+__device__ __forceinline__ cpx<R> cmul_fma(cpx<R> x, cpx<R> k) {
+  return {fma_r(x.re, k.re, -(x.im * k.im)), fma_r(x.re, k.im, x.im * k.re)};
+}
+
 // The inverse of a diagonal parametric op (RZ / EZ: parity of the target; U1: condition
 // on the target) on one amplitude of each state.
 template <typename R>
@@ -525,10 +530,24 @@ __device__ __forceinline__ void unapply_diag(cop op, uint32_t j, cpx<R>& x, cpx<
   const cpx<R> c0 = {(R)op.u[0], (R)op.u[1]}, c1 = {(R)op.u[2], (R)op.u[3]};
   const bool on = (j & op.cond) == op.cond;
   const cpx<R> k = csel((__popc(j & op.zmask) & 1) != 0, c1, c0);
-  const cpx<R> xk = {fma_r(x.re, k.re, -(x.im * k.im)), fma_r(x.re, k.im, x.im * k.re)};
-  const cpx<R> yk = {fma_r(y.re, k.re, -(y.im * k.im)), fma_r(y.re, k.im, y.im * k.re)};
-  x = csel(on, xk, x);
-  y = csel(on, yk, y);
+  x = csel(on, cmul_fma(x, k), x);
+  y = csel(on, cmul_fma(y, k), y);
+}
+
+// The Z form's inverse on a pair (j0 with the target clear, j1 with it set): RZ / EZ
+// (no condition) multiply the pair's members by c0 and c1; U1 multiplies j1 by c0.
+template <typename R>
+__device__ __forceinline__ void unapply_z_pair(cop op, cpx<R>& x0, cpx<R>& x1, cpx<R>& y0, cpx<R>& y1) {
+  const cpx<R> c0 = {(R)op.u[0], (R)op.u[1]}, c1 = {(R)op.u[2], (R)op.u[3]};
+  if (op.cond == 0) {
+    x0 = cmul_fma(x0, c0);
+    y0 = cmul_fma(y0, c0);
+    x1 = cmul_fma(x1, c1);
+    y1 = cmul_fma(y1, c1);
+  } else {
+    x1 = cmul_fma(x1, c0);
+    y1 = cmul_fma(y1, c0);
+  }
 }
 
 // A parametric op over the whole chunk: per pair (j0, j1) of its target bit, read psi and
@@ -547,7 +566,9 @@ __device__ __forceinline__ double param_pass(cpx<R>* lp, cpx<R>* ll, cop op, cdo
     const uint32_t j0 = ((v & ~lower) << 1) | (v & lower), j1 = j0 | (1u << b);
     cpx<R> x0 = lp[j0], x1 = lp[j1], y0 = ll[j0], y1 = ll[j1];
     pair_gradient<KIND>(d, x0, x1, y0, y1, s0, s1);
-    if (KIND == ADJ_GRAD_Z || (KIND == ADJ_GRAD_GENERAL && diag)) {
+    if (KIND == ADJ_GRAD_Z) {
+      unapply_z_pair(op, x0, x1, y0, y1);
+    } else if (KIND == ADJ_GRAD_GENERAL && diag) {
       unapply_diag(op, j0, x0, y0);
       unapply_diag(op, j1, x1, y1);
     } else {
@@ -582,7 +603,14 @@ __device__ __forceinline__ double group_param(cpx<R> (&xp)[PER], cpx<R> (&xl)[PE
       a[m] = xp[m];
       b[m] = xl[m];
     }
-    if (KIND == ADJ_GRAD_Z || (KIND == ADJ_GRAD_GENERAL && op.mode == BLOCK_DIAG)) {
+    if (KIND == ADJ_GRAD_Z) {
+#pragma unroll
+      for (int m = 0; m < PER; ++m) {
+        if ((m & P) != 0) continue;
+        pair_gradient<KIND>(d, a[m], a[m | P], b[m], b[m | P], s0, s1);
+        unapply_z_pair(op, a[m], a[m | P], b[m], b[m | P]);
+      }
+    } else if (KIND == ADJ_GRAD_GENERAL && op.mode == BLOCK_DIAG) {
 #pragma unroll
       for (int m = 0; m < PER; ++m) {
         if ((m & P) != 0) continue;

@@ -240,8 +240,12 @@ static int get_partials(size_t doubles, double** out) {
 static int32_t grad_kind(const block_op& op, const double* d) {
   const bool d00 = d[0] == 0.0 && d[1] == 0.0, d01 = d[2] == 0.0 && d[3] == 0.0, d10 = d[4] == 0.0 && d[5] == 0.0,
              d11 = d[6] == 0.0 && d[7] == 0.0;
-  if (op.mode == BLOCK_DIAG)
-    return d01 && d10 && d[0] == 0.0 && d[6] == 0.0 ? ADJ_GRAD_Z : ADJ_GRAD_GENERAL;
+  if (op.mode == BLOCK_DIAG) {
+    // RZ / EZ: parity of the target alone; U1: the target's 1 side alone
+    const uint32_t target = 1u << op.pair_bit;
+    const bool form = (op.cond == 0 && op.zmask == target) || (op.cond == target && op.zmask == 0);
+    return form && d01 && d10 && d[0] == 0.0 && d[6] == 0.0 ? ADJ_GRAD_Z : ADJ_GRAD_GENERAL;
+  }
   // a sign mask of the target bit alone never flips: the pair's first index has it clear
   const bool no_flip = (op.sign & ~(1u << op.pair_bit)) == 0;
   if (op.mode != BLOCK_MATRIX || !no_flip || op.cond != 0 || !d00 || !d11) return ADJ_GRAD_GENERAL;
