@@ -237,6 +237,25 @@ def blocked_leg(circuit, plain_state, args, stream):
     same = bool(np.array_equal(st.download_indices(idx).view(np.uint64),
                                plain_state.download_indices(idx).view(np.uint64)))
     ip = brk.inner_product(st.ptr, brk.C128, plain_state.ptr, brk.C128, st.size, stream)
+    # the same circuit in block order (brk_circuit_block_order): fewer blocks, same operator,
+    # amplitudes equal to the per-gate run within rounding
+    ordered = circuit.reordered(circuit.block_order(12, 3))
+    plan2 = brk.BlockPlan(ordered, brk.C128)
+    blocks2 = [k for _, k, local in plan2.steps if local]
+    times2 = []
+    for k in range(args.warmup + args.steps):
+        brk.set_basis_state(st.ptr, st.size, brk.C128, 0, stream)
+        hip.synchronize(stream)
+        e0, e1 = hip.Event(), hip.Event()
+        e0.record(stream)
+        plan2.run(st.ptr, stream)
+        e1.record(stream)
+        e1.synchronize()
+        if k >= args.warmup:
+            times2.append(e0.elapsed_ms(e1))
+            log(f"blocked (block order) step {k - args.warmup + 1}/{args.steps}: {times2[-1]:.1f} ms")
+    diff2 = float(np.max(np.abs(st.download_indices(idx) - plain_state.download_indices(idx))))
+    ip2 = brk.inner_product(st.ptr, brk.C128, plain_state.ptr, brk.C128, st.size, stream)
     st.free()
     ms = sum(times) / len(times)
     return {
@@ -248,6 +267,13 @@ def blocked_leg(circuit, plain_state, args, stream):
         "plan_s": round(plan_s, 4),
         "sampled_amplitudes_bit_identical": same,
         "inner_product_error": abs(ip - 1.0),
+        "block_order": {
+            "ms_per_circuit": round(sum(times2) / len(times2), 2),
+            "blocks": len(blocks2),
+            "ops_in_blocks": int(sum(blocks2)),
+            "max_abs_diff_sampled": diff2,
+            "inner_product_error": abs(ip2 - 1.0),
+        },
     }
 
 
