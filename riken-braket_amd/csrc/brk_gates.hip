@@ -381,7 +381,7 @@ __device__ __forceinline__ void segment_pass(cpx<R>* lds, cprogram prog, uint32_
       if (op.negate) {
         const uint32_t cond = op.cond;
 #pragma unroll
-        for (int m = 0; m < PER; ++m) x[m] = csel((j[m] & cond) == cond, cneg(x[m]), x[m]);
+        for (int m = 0; m < PER; ++m) x[m] = cneg_if(x[m], (j[m] & cond) == cond);
         continue;
       }
       const cpx<R> c0 = {(R)op.u[0], (R)op.u[1]}, c1 = {(R)op.u[2], (R)op.u[3]};
@@ -685,9 +685,6 @@ __device__ __forceinline__ void adjoint_segment(cpx<R>* lp, cpx<R>* ll, cadjoint
     xp[m] = lp[j[m]];
     xl[m] = ll[j[m]];
   }
-  double acc[ADJ_SEGMENT_GRADIENTS];
-#pragma unroll
-  for (int k = 0; k < ADJ_SEGMENT_GRADIENTS; ++k) acc[k] = 0.0;
   uint32_t ngrad = 0;
   for (uint32_t o = first_op; o < first_op + num_ops; ++o) {
     cop op = prog->ops[o];
@@ -699,9 +696,9 @@ __device__ __forceinline__ void adjoint_segment(cpx<R>* lp, cpx<R>* ll, cadjoint
         case ADJ_GRAD_X: g = group_param_any<ADJ_GRAD_X>(xp, xl, j, op, aprog->d[o]); break;
         default: g = group_param_any<ADJ_GRAD_GENERAL>(xp, xl, j, op, aprog->d[o]); break;
       }
-#pragma unroll
-      for (int k = 0; k < ADJ_SEGMENT_GRADIENTS; ++k)
-        if ((uint32_t)k == ngrad) acc[k] = g;
+      // the term's wave sum now (fixed order), kept per wave until the segment ends
+      for (int off = 32; off > 0; off >>= 1) g += __shfl_xor(g, off, 64);
+      if ((t & 63) == 0) wave_sums[ngrad][t >> 6] = g;
       ++ngrad;
       continue;
     }
@@ -711,8 +708,8 @@ __device__ __forceinline__ void adjoint_segment(cpx<R>* lp, cpx<R>* ll, cadjoint
 #pragma unroll
         for (int m = 0; m < PER; ++m) {
           const bool on = (j[m] & cond) == cond;
-          xp[m] = csel(on, cneg(xp[m]), xp[m]);
-          xl[m] = csel(on, cneg(xl[m]), xl[m]);
+          xp[m] = cneg_if(xp[m], on);
+          xl[m] = cneg_if(xl[m], on);
         }
         continue;
       }
@@ -743,14 +740,6 @@ __device__ __forceinline__ void adjoint_segment(cpx<R>* lp, cpx<R>* ll, cadjoint
   for (int m = 0; m < PER; ++m) {
     lp[j[m]] = xp[m];
     ll[j[m]] = xl[m];
-  }
-  // one reduction round for the segment's gradient terms, in a fixed order
-#pragma unroll
-  for (int k = 0; k < ADJ_SEGMENT_GRADIENTS; ++k) {
-    if ((uint32_t)k >= ngrad) break;
-    double v = acc[k];
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    if ((t & 63) == 0) wave_sums[k][t >> 6] = v;
   }
   __syncthreads();
   if (t < ngrad) {

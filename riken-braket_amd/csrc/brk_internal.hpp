@@ -61,6 +61,16 @@ template <typename R>
 __device__ __forceinline__ cpx<R> cscale(cpx<R> a, R s) { return {a.re * s, a.im * s}; }
 template <typename R>
 __device__ __forceinline__ cpx<R> cneg(cpx<R> a) { return {-a.re, -a.im}; }
+// -a when `on`, else a: the sign bits flipped by an integer XOR (the same values as cneg)
+__device__ __forceinline__ cpx<double> cneg_if(cpx<double> a, bool on) {
+  const uint64_t m = (uint64_t)on << 63;
+  return {__longlong_as_double(__double_as_longlong(a.re) ^ (long long)m),
+          __longlong_as_double(__double_as_longlong(a.im) ^ (long long)m)};
+}
+__device__ __forceinline__ cpx<float> cneg_if(cpx<float> a, bool on) {
+  const uint32_t m = (uint32_t)on << 31;
+  return {__uint_as_float(__float_as_uint(a.re) ^ m), __uint_as_float(__float_as_uint(a.im) ^ m)};
+}
 
 inline size_t amp_bytes(int dtype) { return dtype == BRK_C128 ? 16 : 8; }
 
