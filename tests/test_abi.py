@@ -56,3 +56,17 @@ def test_argument_errors_without_gpu():
     assert _status("brk_pauli_x", DUMMY, 1 << 30, 0, many, nm, None, 0, None) == 4
     assert _status("brk_pauli_x", DUMMY, 16, 0, None, 0, None, 0, None) == 1  # no targets
     assert brk.load().brk_last_error() != b""
+
+
+def test_block_order_argument_errors():
+    circuit = brk.Circuit(4, [("H", (0,), (), ()), ("X", (1,), (0,), ())])
+    out = (brk.C.c_uint * 2)()
+    assert _status("brk_circuit_block_order", None, 12, 3, out) == 1  # null circuit
+    assert _status("brk_circuit_block_order", circuit.h, 12, 3, None) == 1  # null order, ops present
+    assert _status("brk_circuit_block_order", circuit.h, 0, 0, out) == 1  # no local qubits
+    assert _status("brk_circuit_block_order", circuit.h, 65, 3, out) == 1  # beyond 64 bits
+    assert _status("brk_circuit_block_order", circuit.h, 3, 4, out) == 1  # line bits above num_local
+    assert _status("brk_circuit_block_order", circuit.h, 12, 3, out) == 0
+    assert list(out) == [0, 1]
+    empty = brk.Circuit(3)
+    assert _status("brk_circuit_block_order", empty.h, 12, 3, None) == 0  # nothing to order
