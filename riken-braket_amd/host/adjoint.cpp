@@ -465,7 +465,17 @@ int brk_adjoint_ising_gradient(const brk_circuit* circuit, void* psi, void* work
     }
   }
   const unsigned L = dtype == BRK_C128 ? ADJ_BLOCK_LOCAL_C128 : ADJ_BLOCK_LOCAL_C64;
-  if (adjoint_blocks_enabled() && circuit->num_qubits >= L)
+  // blocks hold every op with the line qubits in L local qubits; an op on more qubits (a
+  // many-controlled gate) sends the whole gradient to the per-run sweep
+  const uint64_t line_mask = dtype == BRK_C128 ? 0x7ull : 0xfull;
+  bool all_fit = true;
+  for (const auto& o : ops) {
+    uint64_t m = line_mask;
+    for (unsigned q : o.targets) m |= 1ull << q;
+    for (unsigned q : o.controls) m |= 1ull << q;
+    if ((unsigned)__builtin_popcountll(m) > L) all_fit = false;
+  }
+  if (adjoint_blocks_enabled() && circuit->num_qubits >= L && all_fit)
     return blocked_gradient(circuit, pidx, nparams, psi, work, dtype, J, h, energy, gradient, stream);
   const uint64_t size = 1ull << circuit->num_qubits;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);

@@ -225,3 +225,30 @@ def test_rejects_local_swaps_and_controlled_parameters():
         adjoint(4, [("LOCAL_SWAP", (3, 0), (), ()), ("RY", (0,), (), (0.1,))], [0.1] * 3, [0.2] * 4)
     with pytest.raises(brk.BrkError):
         adjoint(4, [("RY", (0,), (1,), (0.1,))], [0.1] * 3, [0.2] * 4)
+
+
+def test_op_wider_than_a_block_uses_the_run_sweep():
+    """A gate on more qubits than a reverse block holds (X with 10 controls at 14 qubits:
+    11 qubits besides the 3 line qubits) sends the gradient to the per-run sweep: the result
+    equals the one with blocks switched off, and parameter shift."""
+    n = 14
+    rng = np.random.default_rng(31)
+    ops = mixed_ansatz(n, rng, layers=2)
+    ops.insert(len(ops) // 2, ("X", (3,), tuple(range(4, 14)), ()))
+    J, h = rng.uniform(-1, 1, n - 1), rng.uniform(-1, 1, n)
+    note("gpu adjoint: op wider than a block")
+    e1, g1, _ = adjoint(n, ops, J, h)
+    brk.tune("adjoint_blocks", 0)
+    try:
+        e0, g0, _ = adjoint(n, ops, J, h)
+    finally:
+        brk.tune("adjoint_blocks", 1)
+    assert e1 == e0 and np.array_equal(g1, g0)
+    theta = np.array(W.parameters(ops))
+    k = 1  # an RY (mixed_ansatz cycles RX, RY, RZ, ... over the qubits)
+    assert [op for op in ops if op[0] in W.PARAMETRIC][k][0] == "RY"
+    tp, tm = theta.copy(), theta.copy()
+    tp[k] += math.pi / 2
+    tm[k] -= math.pi / 2
+    ref = 0.5 * (plain_energy(n, W.with_parameters(ops, tp), J, h) - plain_energy(n, W.with_parameters(ops, tm), J, h))
+    assert abs(g1[k] - ref) < 1e-11
