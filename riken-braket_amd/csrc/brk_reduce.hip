@@ -215,9 +215,20 @@ __device__ inline double readlane_f64(double v, int lane) {
   return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
-// Coalesced load of one chunk's |a|^2 into LDS.
+// Coalesced load of one chunk's |a|^2 into LDS.  A whole chunk issues all its
+// loads before the first norm is stored (eight in flight per thread).
 template <typename R>
 __device__ inline void load_norms(const cpx<R>* __restrict__ a, uint64_t base, uint32_t chunk, double* nrm) {
+  constexpr int PER = SAMPLE_CHUNK / SAMPLE_THREADS;
+  if (chunk == (uint32_t)SAMPLE_CHUNK) {
+    cpx<R> x[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) x[k] = a[base + threadIdx.x + (uint32_t)k * SAMPLE_THREADS];
+#pragma unroll
+    for (int k = 0; k < PER; ++k)
+      nrm[threadIdx.x + (uint32_t)k * SAMPLE_THREADS] = (double)x[k].re * x[k].re + (double)x[k].im * x[k].im;
+    return;
+  }
   for (uint32_t j = threadIdx.x; j < chunk; j += SAMPLE_THREADS) {
     const cpx<R> x = a[base + j];
     nrm[j] = (double)x.re * x.re + (double)x.im * x.im;
