@@ -289,7 +289,7 @@ def sampling_leg(state, args, stream):
     rng = np.random.default_rng(4)
     out = {}
     for count in (1, 100000):
-        c = list(rng.random(count))
+        c = rng.random(count)  # float64 array: passed to the C ABI without conversion
         brk.generate_events(state.ptr, state.size, brk.C128, c, stream)  # warm-up
         t0 = time.perf_counter()
         ev = brk.generate_events(state.ptr, state.size, brk.C128, c, stream)

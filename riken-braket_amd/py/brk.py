@@ -338,12 +338,14 @@ def project_qubit(ptr, size, dtype, qubit, value, multiplier, stream=0):
 
 def generate_events(ptr, size, dtype, canonical, stream=0):
     """canonical: sequence of numbers in [0, 1); returns a list of basis indices."""
-    canonical = list(canonical)
-    n = len(canonical)
-    c = (C.c_double * max(n, 1))(*canonical)
-    out = (C.c_uint64 * max(n, 1))()
-    call("brk_generate_events", ptr, size, dtype, c, n, out, stream)
-    return list(out)[:n]
+    import numpy as np
+
+    c = np.ascontiguousarray(canonical, dtype=np.float64).reshape(-1)
+    n = int(c.size)
+    out = np.zeros(max(n, 1), dtype=np.uint64)
+    call("brk_generate_events", ptr, size, dtype, c.ctypes.data_as(_dp) if n else None, n,
+         out.ctypes.data_as(C.POINTER(C.c_uint64)), stream)
+    return out[:n].tolist()
 
 
 def expectation_pauli_string(ptr, size, dtype, pauli, qubits, stream=0):
