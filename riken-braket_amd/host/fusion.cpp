@@ -2,9 +2,12 @@
 // the C ABI.
 //
 // Planning (FP64 on the host):
-//  1. The circuit's ops are mapped to physical qubits by replaying the
-//     logical -> physical permutation; a local swap becomes a SWAP gate on its
-//     two physical qubits (same physical result as the plain runner).
+//  1. The circuit's ops are put in block order (block_order: a dependence-
+//     preserving order that a lookahead packs into few groups of `width`
+//     qubits; ops on disjoint qubits commute) and mapped to physical qubits by
+//     replaying the logical -> physical permutation; a local swap (which keeps
+//     its place relative to every op) becomes a SWAP gate on its two physical
+//     qubits (same physical result as the plain runner).
 //  2. Greedy fusion: a block starts at the first unplaced gate and scans the
 //     rest in order.  A gate joins the block when none of its qubits is
 //     "blocked" and the union of qubits stays within `width`; otherwise its
@@ -166,10 +169,11 @@ int brk_fused_plan_create(const brk_circuit* circuit, unsigned width, const unsi
   brk::permutation perm(n);
   if (permutation && !perm.load(permutation, n)) return fail(BRK_ERR_INVALID_ARGUMENT, "bad permutation");
 
-  // 1. physical gate list
+  // 1. physical gate list, the ops in block order for blocks of `width` qubits
   std::vector<physical_gate> gates;
   gates.reserve(circuit->ops.size());
-  for (const auto& o : circuit->ops) {
+  for (size_t oi : brk::block_order(circuit->ops, width, 0)) {
+    const auto& o = circuit->ops[oi];
     physical_gate g;
     if (o.code == BRK_OP_LOCAL_SWAP) {
       brk_circuit::op sw;
