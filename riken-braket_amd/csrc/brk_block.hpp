@@ -53,6 +53,7 @@ struct block_op {
   uint32_t ins[15];
   uint32_t group_flip, group_pair;  // register segments: flip and pair bit over the group index
   uint32_t negate;                   // diagonal with u = (-1, -1) (Z, CZ): an exact negation on cond
+  uint32_t group_zmask;              // register segments: zmask over the group index, ~0 if not inside
   double u[8];
 };
 

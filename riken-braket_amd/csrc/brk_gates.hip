@@ -111,8 +111,20 @@ __device__ __forceinline__ void pair_math(const U& u, uint64_t signmask, uint64_
       m01 = csel(flip, cpx<R>{-m01.re, -m01.im}, m01);
       m10 = csel(flip, cpx<R>{-m10.re, -m10.im}, m10);
     }
-    y0 = cadd(cmul(m00, x0), cmul(m01, x1));
-    y1 = cadd(cmul(m10, x0), cmul(m11, x1));
+    // uniform branches on the coefficients' exact zeros
+    if (u[1] == 0.0 && u[3] == 0.0 && u[5] == 0.0 && u[7] == 0.0) {
+      // a real 2x2 (RY, EY): real coefficients times amplitudes, the form of
+      // c a0 + s a1 in exponential_pauli_y.hpp:36-38, :84-87
+      y0 = {m00.re * x0.re + m01.re * x1.re, m00.re * x0.im + m01.re * x1.im};
+      y1 = {m10.re * x0.re + m11.re * x1.re, m10.re * x0.im + m11.re * x1.im};
+    } else if (u[1] == 0.0 && u[2] == 0.0 && u[4] == 0.0 && u[7] == 0.0) {
+      // [[r, i s], [i s', r']] (RX, EX): cos a0 + i sin a1 (exponential_pauli_x.hpp:37-38)
+      y0 = {m00.re * x0.re - m01.im * x1.im, m00.re * x0.im + m01.im * x1.re};
+      y1 = {m11.re * x1.re - m10.im * x0.im, m11.re * x1.im + m10.im * x0.re};
+    } else {
+      y0 = cadd(cmul(m00, x0), cmul(m01, x1));
+      y1 = cadd(cmul(m10, x0), cmul(m11, x1));
+    }
   }
 }
 
@@ -350,6 +362,16 @@ __device__ __forceinline__ void group_pairs(cpx<R> (&x)[PER], const uint32_t (&j
   }
 }
 
+// An unconditioned diagonal gate whose parity bits are group bits ZG (over the group index):
+// each member's coefficient is fixed at compile time.
+template <int ZG, typename R, int PER>
+__device__ __forceinline__ void group_diag(cpx<R> (&x)[PER], cpx<R> c0, cpx<R> c1) {
+  if constexpr (ZG < PER) {
+#pragma unroll
+    for (int m = 0; m < PER; ++m) x[m] = cmul(x[m], (__builtin_popcount(m & ZG) & 1) ? c1 : c0);
+  }
+}
+
 // A segment: each thread loads one group of PER amplitudes (local indices differing in
 // the group bits only), applies the segment's gates in registers and stores it back.
 template <typename R, int PER>
@@ -385,6 +407,25 @@ __device__ __forceinline__ void segment_pass(cpx<R>* lds, cprogram prog, uint32_
         continue;
       }
       const cpx<R> c0 = {(R)op.u[0], (R)op.u[1]}, c1 = {(R)op.u[2], (R)op.u[3]};
+      switch (op.group_zmask) {  // parity bits among the group bits: no per-amplitude select
+        case 0: group_diag<0>(x, c0, c1); continue;
+        case 1: group_diag<1>(x, c0, c1); continue;
+        case 2: group_diag<2>(x, c0, c1); continue;
+        case 3: group_diag<3>(x, c0, c1); continue;
+        case 4: group_diag<4>(x, c0, c1); continue;
+        case 5: group_diag<5>(x, c0, c1); continue;
+        case 6: group_diag<6>(x, c0, c1); continue;
+        case 7: group_diag<7>(x, c0, c1); continue;
+        case 8: group_diag<8>(x, c0, c1); continue;
+        case 9: group_diag<9>(x, c0, c1); continue;
+        case 10: group_diag<10>(x, c0, c1); continue;
+        case 11: group_diag<11>(x, c0, c1); continue;
+        case 12: group_diag<12>(x, c0, c1); continue;
+        case 13: group_diag<13>(x, c0, c1); continue;
+        case 14: group_diag<14>(x, c0, c1); continue;
+        case 15: group_diag<15>(x, c0, c1); continue;
+        default: break;
+      }
       const uint32_t zmask = op.zmask;
 #pragma unroll
       for (int m = 0; m < PER; ++m) x[m] = cmul(x[m], csel((__popc(j[m] & zmask) & 1) != 0, c1, c0));

@@ -288,6 +288,7 @@ static void make_adjoint_passes(adjoint_block_program& ap) {
       for (uint32_t i = first; i < first + count; ++i) {
         block_op& o = prog.ops[i];
         o.group_flip = o.group_pair = 0;
+        o.group_zmask = ~0u;  // the reverse segments select diagonal coefficients per amplitude
         for (unsigned g = 0; g < k; ++g) {
           if (o.mode != BLOCK_DIAG && (o.flip >> p.group[g] & 1u)) o.group_flip |= 1u << g;
           if ((o.mode != BLOCK_DIAG || ap.grad_slot[i] >= 0) && p.group[g] == o.pair_bit) o.group_pair = 1u << g;

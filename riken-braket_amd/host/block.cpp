@@ -89,13 +89,19 @@ static void make_passes(block_program& prog) {
     unsigned k = 0;
     for (unsigned b = 0; b < prog.num_local; ++b)
       if (bits >> b & 1u) p.group[k++] = b;
-    for (uint32_t i = first; i < first + count; ++i) {  // flip / pair bit over the group index
+    for (uint32_t i = first; i < first + count; ++i) {  // flip / pair bit / zmask over the group index
       block_op& o = prog.ops[i];
       o.group_flip = o.group_pair = 0;
+      uint32_t zg = 0, zrest = o.zmask;
       for (unsigned g = 0; g < k; ++g) {
         if (o.mode != BLOCK_DIAG && (o.flip >> p.group[g] & 1u)) o.group_flip |= 1u << g;
         if (o.mode != BLOCK_DIAG && p.group[g] == o.pair_bit) o.group_pair = 1u << g;
+        if (o.zmask >> p.group[g] & 1u) {
+          zg |= 1u << g;
+          zrest &= ~(1u << p.group[g]);
+        }
       }
+      o.group_zmask = zrest == 0 ? zg : ~0u;
     }
   };
   uint32_t first = 0, count = 0;
