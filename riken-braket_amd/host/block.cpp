@@ -5,8 +5,9 @@
 // the union of their qubits, together with the qubits that address one
 // 128-byte line (so every chunk access is whole lines), fits in the local
 // bits of a chunk; the local set is then filled up with the lowest remaining
-// qubits.  A group of two or more ops becomes one block launch; a single op
-// and every local swap run on the plain kernels.  This is the role of bra's
+// qubits.  A local swap joins a block as the SWAP gate of its two physical
+// qubits (the map changes for the ops after it).  A group of two or more ops
+// becomes one block launch; a single op runs on the plain kernels.  This is the role of bra's
 // BEGIN/END FUSION (bra/src/state.cpp:2193-2217, nompi_state.cpp:1282-1357)
 // and of ket's cache-aware gate functions (gate_bitmasks.hpp:587-2134): a
 // run of gates applied per group of amplitudes held near the core, with the
@@ -347,22 +348,21 @@ static int build_plan(const brk_circuit* circuit, int dtype, const unsigned* per
   b.line_bits = block_run_qubits(dtype);  // default: one 128-byte line
   for (unsigned i = 0; i < circuit->ops.size(); ++i) {
     const auto& o = circuit->ops[i];
-    if (o.code == BRK_OP_LOCAL_SWAP) {
-      flush(b, *plan);
-      brk_block_plan::step s;
-      s.first_op = i;
-      s.num_ops = 1;
-      s.physical = o;
-      plan->steps.push_back(std::move(s));
-      perm.permutate(perm.inverse[o.targets[0]], perm.inverse[o.targets[1]]);
-      continue;
-    }
-    const brk_circuit::op m = map_op(o, perm);
     gate_desc d;
-    if (int st = describe_physical_op(m.code, m.targets.data(), (unsigned)m.targets.size(),
-                                      m.controls.empty() ? nullptr : m.controls.data(), (unsigned)m.controls.size(),
-                                      m.params.data(), &d))
-      return st;
+    brk_circuit::op m;
+    if (o.code == BRK_OP_LOCAL_SWAP) {
+      // the data of a local swap is a SWAP gate on its two physical qubits
+      // (swap_permutated_local_qubits.hpp:64-77); the map changes for the ops after it
+      m = o;
+      if (int st = describe_physical_op(BRK_OP_SWAP, o.targets.data(), 2, nullptr, 0, nullptr, &d)) return st;
+      perm.permutate(perm.inverse[o.targets[0]], perm.inverse[o.targets[1]]);
+    } else {
+      m = map_op(o, perm);
+      if (int st = describe_physical_op(m.code, m.targets.data(), (unsigned)m.targets.size(),
+                                        m.controls.empty() ? nullptr : m.controls.data(),
+                                        (unsigned)m.controls.size(), m.params.data(), &d))
+        return st;
+    }
     if (!b.fits(d)) flush(b, *plan);
     if (b.ops.empty()) b.first_op = i;
     b.qubits |= d.qubits;

@@ -52,7 +52,7 @@ def test_plan_invariants(n, dtype, seed, swaps):
         assert list(local) == sorted(local) and len(local) == L
         assert line <= set(local)
         for k in range(first, first + count):
-            assert ops[k][0] != "LOCAL_SWAP" and phys[k] <= set(local)
+            assert phys[k] <= set(local)  # a local swap joins as the SWAP of its physical qubits
     assert nxt == len(ops) and (blocks > 0) == (L >= 9)  # a chunk holds at least 2^9 amplitudes
     assert plan.final_permutation == final
 
@@ -68,10 +68,20 @@ def test_blocks_are_maximal_runs():
     assert {18, 19} <= set(plan.steps[1][2])
 
 
-def test_single_ops_and_local_swaps_are_plain():
+def test_local_swaps_join_blocks_and_single_ops_are_plain():
+    # all 10 qubits fit one chunk: the swap of physical 1 and 9 runs inside the block, and the
+    # H on logical 1 after it acts on physical 9
     ops = [("H", (0,), (), ()), ("LOCAL_SWAP", (1, 9), (), ()), ("H", (1,), (), ())]
     plan = brk.BlockPlan(brk.Circuit(10, ops))
-    assert plan.steps == [(0, 1, ()), (1, 1, ()), (2, 1, ())]
+    assert plan.steps == [(0, 3, tuple(range(10)))]
+    assert plan.final_permutation == [0, 9, 2, 3, 4, 5, 6, 7, 8, 1]
+    # the swap moves logical 1 to physical 18, so the H after it joins the gates on 18 and 19
+    ops = [("H", (19,), (), ()), ("LOCAL_SWAP", (1, 18), (), ()), ("H", (1,), (), ())]
+    plan = brk.BlockPlan(brk.Circuit(20, ops))
+    assert plan.steps == [(0, 3, (0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 18, 19))]
+    # a lone op, and a lone local swap, are plain steps
+    assert brk.BlockPlan(brk.Circuit(20, [("LOCAL_SWAP", (1, 18), (), ())])).steps == [(0, 1, ())]
+    assert brk.BlockPlan(brk.Circuit(20, [("H", (19,), (), ())])).steps == [(0, 1, ())]
 
 
 def _shares(a, b):
