@@ -374,7 +374,8 @@ typedef struct brk_block_plan brk_block_plan;
 
 /* Groups the circuit's ops (greedily, in order) into blocks whose qubits,
    with the 3 (c128) or 4 (c64) lowest qubits, fit in the local qubits of a
-   chunk; single ops and local swaps stay plain steps.  `permutation` (may be
+   chunk; a local swap joins a block as the SWAP of its two physical qubits,
+   a single op stays a plain step.  `permutation` (may be
    NULL = identity) is the logical->physical map at the start, as for
    brk_circuit_run.  Planning needs no device. */
 int brk_block_plan_create(const brk_circuit* circuit, int dtype, const unsigned* permutation,
