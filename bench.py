@@ -301,9 +301,9 @@ def sampling_leg(state, args, stream):
     t0 = time.perf_counter()
     p = brk.zero_one_probabilities(state.ptr, state.size, brk.C128, n // 2, stream)
     brk.project_qubit(state.ptr, state.size, brk.C128, n // 2, 0, p[0] ** -0.5, stream)
-    norm = brk.norm2(state.ptr, state.size, brk.C128, stream)
+    brk.call("brk_stream_synchronize", stream)
     out["projective_measurement_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
-    out["norm_after_projection_error"] = abs(norm - 1.0)
+    out["norm_after_projection_error"] = abs(brk.norm2(state.ptr, state.size, brk.C128, stream) - 1.0)
     t0 = time.perf_counter()
     brk.shor_box(state.ptr, state.size, brk.C128, n // 2, 65521, 7, stream)
     out["shor_box_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
