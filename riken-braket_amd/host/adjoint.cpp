@@ -332,21 +332,24 @@ static int blocked_gradient(const brk_circuit* given, const std::vector<int>& gi
   const unsigned line_bits = dtype == BRK_C128 ? 3 : 4;  // one 128-byte line
   const uint64_t line = (1ull << line_bits) - 1ull;
 
-  // the ops reordered into few blocks of L qubits (ops on disjoint qubits commute), with
-  // their parameter indices
-  brk_circuit ordered;
-  ordered.num_qubits = n;
+  // The ops reordered into few blocks (ops on disjoint qubits commute): for the forward
+  // blocks of block_max_local qubits, and for the reverse blocks of L, with their parameter
+  // indices.  Any order of the same operator is a valid sequence for the adjoint method: the
+  // reverse sweep un-applies its own order from the same final state.
+  brk_circuit forward_order, ordered;
+  forward_order.num_qubits = ordered.num_qubits = n;
+  for (size_t i : block_order(given->ops, block_max_local(dtype), line_bits))
+    forward_order.ops.push_back(given->ops[i]);
   std::vector<int> pidx;
   for (size_t i : block_order(given->ops, L, line_bits)) {
     ordered.ops.push_back(given->ops[i]);
     pidx.push_back(given_pidx[i]);
   }
-  const brk_circuit* circuit = &ordered;
   const auto& ops = ordered.ops;
 
   // forward pass in gate blocks, lam = H psi, E = <psi|lam>
   brk_block_plan* plan = nullptr;
-  if (int s = brk_block_plan_create(circuit, dtype, nullptr, &plan)) return s;
+  if (int s = brk_block_plan_create(&forward_order, dtype, nullptr, &plan)) return s;
   int status = brk_block_plan_run(plan, psi, stream);
   if (!status) status = brk_apply_ising(lam, psi, size, dtype, J, h, stream);
   double ip[2] = {0.0, 0.0};
