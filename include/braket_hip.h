@@ -301,7 +301,7 @@ int brk_project_qubit(void* first, uint64_t size, int dtype, unsigned qubit, int
    (ket/include/ket/generate_events.hpp:15-48) and the sampling step of
    ket::measure (ket/include/ket/measure.hpp:14-40).  The reference overwrites
    the state with the scan; this call leaves it unchanged.  The scan is summed
-   per chunk of 2^14 amplitudes and then across chunks, so an event can differ
+   per chunk of 2^13 amplitudes and then across chunks, so an event can differ
    from a sequential scan only where canonical[e] * total lies within rounding
    of a cumulative boundary.  Synchronises the stream. */
 int brk_generate_events(const void* first, uint64_t size, int dtype, const double* canonical, unsigned num_events,

@@ -345,6 +345,29 @@ static int spin_raw(const void* first, uint64_t size, int dtype, unsigned qubit,
       out);
 }
 
+// Per event: target r = u * total, its chunk c = the first with S[c] > r (the running
+// sums S over whole chunks, computed on the host left to right), or the last chunk
+// holding probability when none is (r at the total), and the running sum before c.
+__global__ __launch_bounds__(256) void k_locate_events(const double* __restrict__ S, uint64_t nchunks,
+                                                       double total, uint64_t last_chunk,
+                                                       const double* __restrict__ canonical, unsigned num_events,
+                                                       uint64_t* __restrict__ chunk_of,
+                                                       double* __restrict__ chunk_start, double* __restrict__ target) {
+  const unsigned e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= num_events) return;
+  const double r = canonical[e] * total;
+  uint64_t lo = 0, hi = nchunks;
+  while (lo < hi) {  // upper bound: the first index with S > r
+    const uint64_t mid = lo + (hi - lo) / 2;
+    if (S[mid] > r) hi = mid;
+    else lo = mid + 1;
+  }
+  const uint64_t c = lo >= nchunks ? last_chunk : lo;
+  chunk_of[e] = c;
+  chunk_start[e] = c ? S[c - 1] : 0.0;
+  target[e] = r;
+}
+
 constexpr int ROW_THREADS = 256;
 
 __global__ __launch_bounds__(ROW_THREADS) void k_row_sums(const double* __restrict__ in, uint64_t cols,
@@ -596,12 +619,14 @@ int brk_generate_events(const void* first, uint64_t size, int dtype, const doubl
   const uint64_t nchunks = size / chunk;
   if (nchunks > 0x7fffffffull) return fail(BRK_ERR_INVALID_ARGUMENT, "state too large");
   hipStream_t s = as_stream(stream);
-  // device buffers: chunk sums | per event: chunk, chunk start, target, result
-  const size_t bytes = sizeof(double) * nchunks + (sizeof(uint64_t) * 2 + sizeof(double) * 2) * num_events;
+  // device buffers: chunk sums, then their running sums | per event: draw, chunk, chunk start,
+  // target, result
+  const size_t bytes = sizeof(double) * nchunks + (sizeof(uint64_t) * 2 + sizeof(double) * 3) * num_events;
   void* scratch = nullptr;
   BRK_HIP_CHECK(hipMalloc(&scratch, bytes));
   double* d_sums = static_cast<double*>(scratch);
-  uint64_t* d_chunk = reinterpret_cast<uint64_t*>(d_sums + nchunks);
+  double* d_canonical = d_sums + nchunks;
+  uint64_t* d_chunk = reinterpret_cast<uint64_t*>(d_canonical + num_events);
   double* d_start = reinterpret_cast<double*>(d_chunk + num_events);
   double* d_target = d_start + num_events;
   uint64_t* d_events = reinterpret_cast<uint64_t*>(d_target + num_events);
@@ -627,23 +652,16 @@ int brk_generate_events(const void* first, uint64_t size, int dtype, const doubl
     }
   if (!status && last_chunk == nchunks) status = fail(BRK_ERR_INVALID_ARGUMENT, "the state has zero norm");
   if (!status) {
-    const double total = S[nchunks - 1];
-    std::vector<uint64_t> chunk_of(num_events);
-    std::vector<double> start(num_events), target(num_events);
-    for (unsigned e = 0; e < num_events; ++e) {
-      target[e] = canonical[e] * total;
-      uint64_t c = (uint64_t)(std::upper_bound(S.begin(), S.end(), target[e]) - S.begin());
-      if (c >= nchunks) c = last_chunk;
-      chunk_of[e] = c;
-      start[e] = c ? S[c - 1] : 0.0;
+    // the running sums replace the chunk sums on the device; each event finds its chunk there
+    if (hipMemcpyAsync(d_sums, S.data(), sizeof(double) * nchunks, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(d_canonical, canonical, sizeof(double) * num_events, hipMemcpyHostToDevice, s) !=
+            hipSuccess)
+      status = fail(BRK_ERR_HIP, "uploading the running sums failed");
+    if (!status) {
+      hipLaunchKernelGGL(k_locate_events, dim3((num_events + 255) / 256), dim3(256), 0, s, d_sums, nchunks,
+                         S[nchunks - 1], last_chunk, d_canonical, num_events, d_chunk, d_start, d_target);
+      status = check_launch("locate events");
     }
-    if (hipMemcpyAsync(d_chunk, chunk_of.data(), sizeof(uint64_t) * num_events, hipMemcpyHostToDevice, s) !=
-            hipSuccess ||
-        hipMemcpyAsync(d_start, start.data(), sizeof(double) * num_events, hipMemcpyHostToDevice, s) != hipSuccess ||
-        hipMemcpyAsync(d_target, target.data(), sizeof(double) * num_events, hipMemcpyHostToDevice, s) != hipSuccess)
-      status = fail(BRK_ERR_HIP, "uploading the event chunks failed");
-    // the host vectors must outlive the copies
-    if (!status && hipStreamSynchronize(s) != hipSuccess) status = fail(BRK_ERR_HIP, "upload failed");
   }
   if (!status) {
     if (dtype == BRK_C128)
