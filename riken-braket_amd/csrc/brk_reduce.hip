@@ -180,16 +180,16 @@ __global__ __launch_bounds__(RED_THREADS) void k_zero_one(const cpx<R>* __restri
 // Keeps the half of the pairs where the qubit equals `value`, scaled by m, and
 // sets the other half to +0 (change_state_after_measuring_zero/one,
 // projective_measurement.hpp:74-150; clear.hpp; set.hpp).
-template <typename R>
+template <typename R, bool NT>
 __global__ __launch_bounds__(256) void k_project(cpx<R>* __restrict__ a, uint64_t npairs, uint64_t lower,
                                                  uint64_t qmask, int value, R m) {
   const uint64_t g = (uint64_t)blockIdx.x * 256 + threadIdx.x;
   if (g >= npairs) return;
   const uint64_t i0 = ((g & ~lower) << 1) | (g & lower);
   const uint64_t keep = value ? (i0 | qmask) : i0, drop = value ? i0 : (i0 | qmask);
-  const cpx<R> x = a[keep];
-  a[keep] = cpx<R>{x.re * m, x.im * m};
-  a[drop] = cpx<R>{R(0), R(0)};
+  const cpx<R> x = load_amp<NT>(a + keep);  // NT: large states stream through once
+  store_amp<NT>(a + keep, cpx<R>{x.re * m, x.im * m});
+  store_amp<NT>(a + drop, cpx<R>{R(0), R(0)});
 }
 
 // Sampling (generate_events.hpp:15-48).  The cumulative probability of basis
@@ -597,12 +597,23 @@ int brk_project_qubit(void* first, uint64_t size, int dtype, unsigned qubit, int
   const uint64_t qmask = 1ull << qubit, lower = qmask - 1ull, npairs = size >> 1;
   const uint64_t blocks = (npairs + 255) / 256;
   if (blocks > 0xffffffffull) return fail(BRK_ERR_INVALID_ARGUMENT, "state too large");
-  if (dtype == BRK_C128)
-    hipLaunchKernelGGL(k_project<double>, dim3((unsigned)blocks), dim3(256), 0, as_stream(stream), (cpx<double>*)first,
-                       npairs, lower, qmask, value, multiplier);
-  else
-    hipLaunchKernelGGL(k_project<float>, dim3((unsigned)blocks), dim3(256), 0, as_stream(stream), (cpx<float>*)first,
-                       npairs, lower, qmask, value, (float)multiplier);
+  const bool nt = size * amp_bytes(dtype) >= get_tuning().nt_min_bytes;
+  hipStream_t s = as_stream(stream);
+  if (dtype == BRK_C128) {
+    if (nt)
+      hipLaunchKernelGGL((k_project<double, true>), dim3((unsigned)blocks), dim3(256), 0, s, (cpx<double>*)first,
+                         npairs, lower, qmask, value, multiplier);
+    else
+      hipLaunchKernelGGL((k_project<double, false>), dim3((unsigned)blocks), dim3(256), 0, s, (cpx<double>*)first,
+                         npairs, lower, qmask, value, multiplier);
+  } else {
+    if (nt)
+      hipLaunchKernelGGL((k_project<float, true>), dim3((unsigned)blocks), dim3(256), 0, s, (cpx<float>*)first,
+                         npairs, lower, qmask, value, (float)multiplier);
+    else
+      hipLaunchKernelGGL((k_project<float, false>), dim3((unsigned)blocks), dim3(256), 0, s, (cpx<float>*)first,
+                         npairs, lower, qmask, value, (float)multiplier);
+  }
   return check_launch("project");
 }
 
