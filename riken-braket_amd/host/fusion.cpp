@@ -7,7 +7,9 @@
 //     qubits; ops on disjoint qubits commute) and mapped to physical qubits by
 //     replaying the logical -> physical permutation; a local swap (which keeps
 //     its place relative to every op) becomes a SWAP gate on its two physical
-//     qubits (same physical result as the plain runner).
+//     qubits (same physical result as the plain runner).  The block order of
+//     FUSION_ORDER_TRIES - 1 random topological orders (fixed seeds) is fused
+//     too, and the candidate with the fewest blocks is kept.
 //  2. Greedy fusion: a block starts at the first unplaced gate and scans the
 //     rest in order.  A gate joins the block when none of its qubits is
 //     "blocked" and the union of qubits stays within `width`; otherwise its
@@ -30,6 +32,7 @@
 #include <algorithm>
 #include <complex>
 #include <new>
+#include <random>
 #include <set>
 
 namespace brk {
@@ -158,22 +161,55 @@ static void embed(const std::vector<cd>& u, unsigned w, float* out) {
 
 using namespace brk;
 
-extern "C" {
+// The ops in a random topological order of their dependences (ops sharing a qubit keep their
+// order; a local swap keeps its place relative to every op), drawn from `gen`.
+static std::vector<size_t> random_topological_order(const std::vector<brk_circuit::op>& ops, std::mt19937_64& gen) {
+  const size_t count = ops.size();
+  std::vector<std::vector<size_t>> succ(count);
+  std::vector<unsigned> indeg(count, 0);
+  std::vector<long> last(64, -1);
+  for (size_t i = 0; i < count; ++i) {
+    const auto& o = ops[i];
+    uint64_t m = 0;
+    for (unsigned q : o.targets) m |= 1ull << q;
+    for (unsigned q : o.controls) m |= 1ull << q;
+    if (o.code == BRK_OP_LOCAL_SWAP) m = ~0ull;
+    std::vector<long> prev;
+    for (unsigned q = 0; q < 64; ++q)
+      if (m >> q & 1ull) prev.push_back(last[q]);
+    std::sort(prev.begin(), prev.end());
+    prev.erase(std::unique(prev.begin(), prev.end()), prev.end());
+    for (long f : prev)
+      if (f >= 0) {
+        succ[(size_t)f].push_back(i);
+        ++indeg[i];
+      }
+    for (unsigned q = 0; q < 64; ++q)
+      if (m >> q & 1ull) last[q] = (long)i;
+  }
+  std::vector<size_t> ready, order;
+  for (size_t i = 0; i < count; ++i)
+    if (!indeg[i]) ready.push_back(i);
+  while (!ready.empty()) {
+    const size_t k = (size_t)(gen() % ready.size());
+    const size_t i = ready[k];
+    ready[k] = ready.back();
+    ready.pop_back();
+    order.push_back(i);
+    for (size_t j : succ[i])
+      if (--indeg[j] == 0) ready.push_back(j);
+  }
+  return order;
+}
 
-int brk_fused_plan_create(const brk_circuit* circuit, unsigned width, const unsigned* permutation,
-                          brk_fused_plan** out) {
-  if (!circuit || !out) return fail(BRK_ERR_INVALID_ARGUMENT, "null argument");
-  if (width < 4 || width > 6) return fail(BRK_ERR_UNSUPPORTED, "fusion width %u not in 4..6", width);
-  const unsigned n = circuit->num_qubits;
-  if (n < width + 5) return fail(BRK_ERR_UNSUPPORTED, "fusion width %u needs at least %u qubits", width, width + 5);
-  brk::permutation perm(n);
-  if (permutation && !perm.load(permutation, n)) return fail(BRK_ERR_INVALID_ARGUMENT, "bad permutation");
-
-  // 1. physical gate list, the ops in block order for blocks of `width` qubits
-  std::vector<physical_gate> gates;
-  gates.reserve(circuit->ops.size());
-  for (size_t oi : brk::block_order(circuit->ops, width, 0)) {
-    const auto& o = circuit->ops[oi];
+// The physical gate list of the ops in `order` (replaying the permutation from `perm`; a local
+// swap becomes a SWAP gate on its two physical qubits).
+static int physical_gates(const brk_circuit& circuit, const std::vector<size_t>& order, brk::permutation perm,
+                          unsigned width, std::vector<physical_gate>& gates, std::vector<unsigned>& final_perm) {
+  gates.clear();
+  gates.reserve(order.size());
+  for (size_t oi : order) {
+    const auto& o = circuit.ops[oi];
     physical_gate g;
     if (o.code == BRK_OP_LOCAL_SWAP) {
       brk_circuit::op sw;
@@ -192,14 +228,19 @@ int brk_fused_plan_create(const brk_circuit* circuit, unsigned width, const unsi
     }
     gates.push_back(std::move(g));
   }
+  final_perm = perm.perm;
+  return BRK_OK;
+}
 
-  auto* plan = new (std::nothrow) brk_fused_plan;
-  if (!plan) return fail(BRK_ERR_OUT_OF_MEMORY, "cannot allocate plan");
-  plan->num_qubits = n;
-  plan->width = width;
-  plan->final_perm = perm.perm;
+struct fused_group {
+  std::set<unsigned> qubits;
+  std::vector<size_t> members;  // gate indices, in order
+};
 
-  // 2. greedy fusion
+// Greedy fusion: a block starts at the first unplaced gate and takes every later gate whose
+// qubits keep the union within `width` and none of which is blocked by a gate left out.
+static std::vector<fused_group> group_gates(const std::vector<physical_gate>& gates, unsigned width) {
+  std::vector<fused_group> groups;
   std::vector<char> placed(gates.size(), 0);
   size_t first = 0;
   while (first < gates.size()) {
@@ -224,6 +265,60 @@ int brk_fused_plan_create(const brk_circuit* circuit, unsigned width, const unsi
       }
     }
     while (first < gates.size() && placed[first]) ++first;
+    groups.push_back({std::move(S), std::move(members)});
+  }
+  return groups;
+}
+
+// Block orders tried per plan: the block order of the given sequence, then that of random
+// topological orders (fixed seeds: the plan is reproducible); the fewest blocks win.
+constexpr int FUSION_ORDER_TRIES = 32;
+
+extern "C" {
+
+int brk_fused_plan_create(const brk_circuit* circuit, unsigned width, const unsigned* permutation,
+                          brk_fused_plan** out) {
+  if (!circuit || !out) return fail(BRK_ERR_INVALID_ARGUMENT, "null argument");
+  if (width < 4 || width > 6) return fail(BRK_ERR_UNSUPPORTED, "fusion width %u not in 4..6", width);
+  const unsigned n = circuit->num_qubits;
+  if (n < width + 5) return fail(BRK_ERR_UNSUPPORTED, "fusion width %u needs at least %u qubits", width, width + 5);
+  brk::permutation perm(n);
+  if (permutation && !perm.load(permutation, n)) return fail(BRK_ERR_INVALID_ARGUMENT, "bad permutation");
+
+  // 1-2. physical gate lists of candidate orders, greedy fusion of each; the fewest blocks win
+  std::vector<physical_gate> gates, trial;
+  std::vector<fused_group> groups;
+  std::vector<unsigned> final_perm, trial_perm;
+  for (int t = 0; t < FUSION_ORDER_TRIES; ++t) {
+    std::vector<size_t> order;
+    if (t == 0) {
+      order = brk::block_order(circuit->ops, width, 0);
+    } else {
+      std::mt19937_64 gen((uint64_t)t);
+      const std::vector<size_t> shuffled = random_topological_order(circuit->ops, gen);
+      std::vector<brk_circuit::op> ops;
+      ops.reserve(shuffled.size());
+      for (size_t i : shuffled) ops.push_back(circuit->ops[i]);
+      for (size_t i : brk::block_order(ops, width, 0)) order.push_back(shuffled[i]);
+    }
+    if (int st = physical_gates(*circuit, order, perm, width, trial, trial_perm)) return st;
+    std::vector<fused_group> g = group_gates(trial, width);
+    if (t == 0 || g.size() < groups.size()) {
+      groups = std::move(g);
+      gates.swap(trial);
+      final_perm = trial_perm;
+    }
+  }
+
+  auto* plan = new (std::nothrow) brk_fused_plan;
+  if (!plan) return fail(BRK_ERR_OUT_OF_MEMORY, "cannot allocate plan");
+  plan->num_qubits = n;
+  plan->width = width;
+  plan->final_perm = final_perm;
+
+  for (auto& grp : groups) {
+    std::set<unsigned>& S = grp.qubits;
+    const std::vector<size_t>& members = grp.members;
     // widen to exactly `width` qubits with the lowest unused ones
     for (unsigned q = 0; S.size() < width; ++q) S.insert(q);
     brk_fused_plan::block blk;
