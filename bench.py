@@ -446,7 +446,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
-              f"(the state-vector path is single-device; see DESIGN.md for how it would shard)", file=sys.stderr)
+              f"(the state-vector path runs on one device)", file=sys.stderr)
         return 2
 
     import brk
