@@ -8,8 +8,8 @@
 //     replaying the logical -> physical permutation; a local swap (which keeps
 //     its place relative to every op) becomes a SWAP gate on its two physical
 //     qubits (same physical result as the plain runner).  The block order of
-//     FUSION_ORDER_TRIES - 1 random topological orders (fixed seeds) is fused
-//     too, and the candidate with the fewest blocks is kept.
+//     up to FUSION_ORDER_TRIES - 1 random topological orders (fixed seeds) is
+//     fused too, and the candidate with the fewest blocks is kept.
 //  2. Greedy fusion: a block starts at the first unplaced gate and scans the
 //     rest in order.  A gate joins the block when none of its qubits is
 //     "blocked" and the union of qubits stays within `width`; otherwise its
@@ -271,8 +271,12 @@ static std::vector<fused_group> group_gates(const std::vector<physical_gate>& ga
 }
 
 // Block orders tried per plan: the block order of the given sequence, then that of random
-// topological orders (fixed seeds: the plan is reproducible); the fewest blocks win.
+// topological orders (fixed seeds: the plan is reproducible); the fewest blocks win.  Long
+// circuits try fewer (planning time grows with ops x tries).
 constexpr int FUSION_ORDER_TRIES = 32;
+static int fusion_order_tries(size_t num_ops) {
+  return num_ops <= 4096 ? FUSION_ORDER_TRIES : num_ops <= 65536 ? 4 : 1;
+}
 
 extern "C" {
 
@@ -289,7 +293,8 @@ int brk_fused_plan_create(const brk_circuit* circuit, unsigned width, const unsi
   std::vector<physical_gate> gates, trial;
   std::vector<fused_group> groups;
   std::vector<unsigned> final_perm, trial_perm;
-  for (int t = 0; t < FUSION_ORDER_TRIES; ++t) {
+  const int tries = fusion_order_tries(circuit->ops.size());
+  for (int t = 0; t < tries; ++t) {
     std::vector<size_t> order;
     if (t == 0) {
       order = brk::block_order(circuit->ops, width, 0);
