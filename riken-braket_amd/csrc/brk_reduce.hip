@@ -18,6 +18,10 @@
 
 namespace brk {
 
+// Grid-stride loops load RED_UNROLL strides ahead before adding them in stride order (the
+// same sums, more loads in flight per thread).
+constexpr int RED_UNROLL = 4;
+
 template <int NV>
 struct dvec {
   double v[NV];
@@ -53,14 +57,25 @@ __global__ __launch_bounds__(RED_THREADS) void k_inner(const cpx<R1>* __restrict
                                                       uint64_t size, double* partial) {
   dvec<2> acc{{0.0, 0.0}};
   const uint64_t stride = (uint64_t)RED_BLOCKS * RED_THREADS;
-  for (uint64_t i = (uint64_t)blockIdx.x * RED_THREADS + threadIdx.x; i < size; i += stride) {
-    cpx<R1> x = a[i];
-    cpx<R2> y = b[i];
+  auto add = [&](cpx<R1> x, cpx<R2> y) {
     const double xr = x.re, xi = x.im, yr = y.re, yi = y.im;
     // conj(y) * x
     acc.v[0] += yr * xr + yi * xi;
     acc.v[1] += yr * xi - yi * xr;
+  };
+  uint64_t i = (uint64_t)blockIdx.x * RED_THREADS + threadIdx.x;
+  for (; i + (RED_UNROLL - 1) * stride < size; i += RED_UNROLL * stride) {
+    cpx<R1> x[RED_UNROLL];
+    cpx<R2> y[RED_UNROLL];
+#pragma unroll
+    for (int u = 0; u < RED_UNROLL; ++u) {
+      x[u] = a[i + u * stride];
+      y[u] = b[i + u * stride];
+    }
+#pragma unroll
+    for (int u = 0; u < RED_UNROLL; ++u) add(x[u], y[u]);
   }
+  for (; i < size; i += stride) add(a[i], b[i]);
   block_store<2>(acc, partial);
 }
 
@@ -68,11 +83,19 @@ template <typename R>
 __global__ __launch_bounds__(RED_THREADS) void k_norm(const cpx<R>* __restrict__ a, uint64_t size, double* partial) {
   dvec<1> acc{{0.0}};
   const uint64_t stride = (uint64_t)RED_BLOCKS * RED_THREADS;
-  for (uint64_t i = (uint64_t)blockIdx.x * RED_THREADS + threadIdx.x; i < size; i += stride) {
-    cpx<R> x = a[i];
+  auto add = [&](cpx<R> x) {
     const double xr = x.re, xi = x.im;
     acc.v[0] += xr * xr + xi * xi;
+  };
+  uint64_t i = (uint64_t)blockIdx.x * RED_THREADS + threadIdx.x;
+  for (; i + (RED_UNROLL - 1) * stride < size; i += RED_UNROLL * stride) {
+    cpx<R> x[RED_UNROLL];
+#pragma unroll
+    for (int u = 0; u < RED_UNROLL; ++u) x[u] = a[i + u * stride];
+#pragma unroll
+    for (int u = 0; u < RED_UNROLL; ++u) add(x[u]);
   }
+  for (; i < size; i += stride) add(a[i]);
   block_store<1>(acc, partial);
 }
 
@@ -168,11 +191,26 @@ __global__ __launch_bounds__(RED_THREADS) void k_zero_one(const cpx<R>* __restri
                                                          uint64_t qmask, double* partial) {
   dvec<2> acc{{0.0, 0.0}};
   const uint64_t stride = (uint64_t)RED_BLOCKS * RED_THREADS;
-  for (uint64_t g = (uint64_t)blockIdx.x * RED_THREADS + threadIdx.x; g < npairs; g += stride) {
-    const uint64_t i0 = ((g & ~lower) << 1) | (g & lower);
-    const cpx<R> x0 = a[i0], x1 = a[i0 | qmask];
+  auto add = [&](cpx<R> x0, cpx<R> x1) {
     acc.v[0] += (double)x0.re * x0.re + (double)x0.im * x0.im;
     acc.v[1] += (double)x1.re * x1.re + (double)x1.im * x1.im;
+  };
+  auto first = [&](uint64_t g) { return ((g & ~lower) << 1) | (g & lower); };
+  uint64_t g = (uint64_t)blockIdx.x * RED_THREADS + threadIdx.x;
+  for (; g + (RED_UNROLL - 1) * stride < npairs; g += RED_UNROLL * stride) {
+    cpx<R> x0[RED_UNROLL], x1[RED_UNROLL];
+#pragma unroll
+    for (int u = 0; u < RED_UNROLL; ++u) {
+      const uint64_t i0 = first(g + u * stride);
+      x0[u] = a[i0];
+      x1[u] = a[i0 | qmask];
+    }
+#pragma unroll
+    for (int u = 0; u < RED_UNROLL; ++u) add(x0[u], x1[u]);
+  }
+  for (; g < npairs; g += stride) {
+    const uint64_t i0 = first(g);
+    add(a[i0], a[i0 | qmask]);
   }
   block_store<2>(acc, partial);
 }
