@@ -200,42 +200,41 @@ block_program make_block_program(const gate_desc* descs, unsigned count, unsigne
   return make_program(b);
 }
 
-std::vector<size_t> block_order(const std::vector<brk_circuit::op>& ops, unsigned num_local, unsigned line_bits) {
+op_dependences dependences_of(const std::vector<brk_circuit::op>& ops) {
   const size_t count = ops.size();
-  std::vector<uint64_t> mask(count, 0);
-  std::vector<std::vector<size_t>> succ(count);
-  std::vector<unsigned> indeg(count, 0);
+  op_dependences d;
+  d.mask.assign(count, 0);
+  d.succ.assign(count, {});
+  d.indeg.assign(count, 0);
   std::vector<long> last(64, -1);
-  auto edge = [&](long from, size_t to) {
-    if (from < 0) return;
-    auto& out = succ[(size_t)from];
-    if (out.empty() || out.back() != to) {  // edges into `to` are added in one go: no duplicates
-      out.push_back(to);
-      ++indeg[to];
-    }
-  };
+  std::vector<long> prev;
   for (size_t i = 0; i < count; ++i) {
     const auto& o = ops[i];
-    for (unsigned q : o.targets) mask[i] |= 1ull << q;
-    for (unsigned q : o.controls) mask[i] |= 1ull << q;
-    if (o.code == BRK_OP_LOCAL_SWAP) {  // a barrier: it moves qubits between slots
-      std::vector<long> prev(last);
-      std::sort(prev.begin(), prev.end());
-      prev.erase(std::unique(prev.begin(), prev.end()), prev.end());
-      for (long f : prev) edge(f, i);
-      std::fill(last.begin(), last.end(), (long)i);
-      mask[i] = ~0ull;
-      continue;
-    }
-    std::vector<long> prev;
+    for (unsigned q : o.targets) d.mask[i] |= 1ull << q;
+    for (unsigned q : o.controls) d.mask[i] |= 1ull << q;
+    if (o.code == BRK_OP_LOCAL_SWAP) d.mask[i] = ~0ull;  // a barrier: it moves qubits between slots
+    prev.clear();
     for (unsigned q = 0; q < 64; ++q)
-      if (mask[i] >> q & 1ull) prev.push_back(last[q]);
+      if (d.mask[i] >> q & 1ull) prev.push_back(last[q]);
     std::sort(prev.begin(), prev.end());
     prev.erase(std::unique(prev.begin(), prev.end()), prev.end());
-    for (long f : prev) edge(f, i);
+    for (long f : prev)
+      if (f >= 0) {
+        d.succ[(size_t)f].push_back(i);
+        ++d.indeg[i];
+      }
     for (unsigned q = 0; q < 64; ++q)
-      if (mask[i] >> q & 1ull) last[q] = (long)i;
+      if (d.mask[i] >> q & 1ull) last[q] = (long)i;
   }
+  return d;
+}
+
+std::vector<size_t> block_order(const std::vector<brk_circuit::op>& ops, unsigned num_local, unsigned line_bits) {
+  const size_t count = ops.size();
+  op_dependences dep = dependences_of(ops);
+  const std::vector<uint64_t>& mask = dep.mask;
+  const std::vector<std::vector<size_t>>& succ = dep.succ;
+  std::vector<unsigned>& indeg = dep.indeg;
   std::vector<size_t> ready, order;
   order.reserve(count);
   for (size_t i = 0; i < count; ++i)

@@ -46,6 +46,17 @@ int apply_physical_op(const brk_circuit::op& o, void* first, uint64_t size, int 
 // ops without a dense form (LOCAL_SWAP).
 bool op_unitary(const brk_circuit::op& o, std::vector<unsigned>& qubits, std::vector<std::array<double, 2>>& u);
 
+// The dependences between ops: op j depends on op i < j when they share a qubit (target or
+// control) and no op between them does; a local swap depends on, and is depended on by, the
+// ops before and after it (it moves qubits between slots).  mask[i] is op i's qubit mask
+// (all ones for a local swap).
+struct op_dependences {
+  std::vector<uint64_t> mask;
+  std::vector<std::vector<size_t>> succ;
+  std::vector<unsigned> indeg;
+};
+op_dependences dependences_of(const std::vector<brk_circuit::op>& ops);
+
 // An order of the ops for gate blocks: a topological order of their dependences (ops that
 // share a qubit keep their relative order; a local swap keeps its place relative to every
 // op) that a greedy scheduler packs into few blocks of num_local qubits, the line_bits

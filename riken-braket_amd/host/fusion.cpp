@@ -165,28 +165,9 @@ using namespace brk;
 // order; a local swap keeps its place relative to every op), drawn from `gen`.
 static std::vector<size_t> random_topological_order(const std::vector<brk_circuit::op>& ops, std::mt19937_64& gen) {
   const size_t count = ops.size();
-  std::vector<std::vector<size_t>> succ(count);
-  std::vector<unsigned> indeg(count, 0);
-  std::vector<long> last(64, -1);
-  for (size_t i = 0; i < count; ++i) {
-    const auto& o = ops[i];
-    uint64_t m = 0;
-    for (unsigned q : o.targets) m |= 1ull << q;
-    for (unsigned q : o.controls) m |= 1ull << q;
-    if (o.code == BRK_OP_LOCAL_SWAP) m = ~0ull;
-    std::vector<long> prev;
-    for (unsigned q = 0; q < 64; ++q)
-      if (m >> q & 1ull) prev.push_back(last[q]);
-    std::sort(prev.begin(), prev.end());
-    prev.erase(std::unique(prev.begin(), prev.end()), prev.end());
-    for (long f : prev)
-      if (f >= 0) {
-        succ[(size_t)f].push_back(i);
-        ++indeg[i];
-      }
-    for (unsigned q = 0; q < 64; ++q)
-      if (m >> q & 1ull) last[q] = (long)i;
-  }
+  brk::op_dependences dep = brk::dependences_of(ops);
+  const std::vector<std::vector<size_t>>& succ = dep.succ;
+  std::vector<unsigned>& indeg = dep.indeg;
   std::vector<size_t> ready, order;
   for (size_t i = 0; i < count; ++i)
     if (!indeg[i]) ready.push_back(i);
