@@ -95,6 +95,23 @@ int brk_event_elapsed_ms(void* start, void* end, float* ms);
 
 /* state := |basis_index> */
 int brk_set_basis_state(void* first, uint64_t size, int dtype, uint64_t basis_index, void* stream);
+
+/* A device state with its stream: the owned counterpart of the range the
+   reference's gate functions take (ket's std::vector, bra::nompi_state's
+   data_, bra/include/bra/nompi_state.hpp:155-165).  create allocates 2^n
+   amplitudes of dtype and sets |0..0>; brk_state_data gives the
+   (first, size, dtype) triple and the stream every other call takes.
+   upload / download copy the whole state to or from host memory (2^n
+   amplitudes of the state's dtype) and synchronise the stream; norm2 is
+   brk_norm2 on the state. */
+typedef struct brk_state brk_state;
+int brk_state_create(unsigned num_qubits, int dtype, void* stream, brk_state** out);
+int brk_state_destroy(brk_state* state);
+int brk_state_data(const brk_state* state, void** first, uint64_t* size, int* dtype, void** stream);
+int brk_state_set_basis(brk_state* state, uint64_t basis_index);
+int brk_state_upload(brk_state* state, const void* host);
+int brk_state_download(const brk_state* state, void* host);
+int brk_state_norm2(const brk_state* state, double* result);
 /* Deterministic, index-only fills used to build states of any size on the
    device (test states of the reference's ket/test/ *.cpp are of this form):
      re[i] = scales[0] * ((params[0]*i + params[1]) mod params[2])

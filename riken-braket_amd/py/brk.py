@@ -54,6 +54,13 @@ _SIGS = {
     "brk_memcpy_dtod": [_p, _p, C.c_size_t, _p],
     "brk_stream_synchronize": [_p],
     "brk_set_basis_state": [_p, _u64, _i, _u64, _p],
+    "brk_state_create": [_u, _i, _p, C.POINTER(C.c_void_p)],
+    "brk_state_destroy": [_p],
+    "brk_state_data": [_p, C.POINTER(C.c_void_p), C.POINTER(C.c_uint64), C.POINTER(C.c_int), C.POINTER(C.c_void_p)],
+    "brk_state_set_basis": [_p, _u64],
+    "brk_state_upload": [_p, _p],
+    "brk_state_download": [_p, _p],
+    "brk_state_norm2": [_p, _dp],
     "brk_fill_modular": [_p, _u64, _i, _dp, C.POINTER(C.c_uint64), _p],
     "brk_fill_hash": [_p, _u64, _i, _u64, _d, _p],
     "brk_convert": [_p, _i, _p, _i, _u64, _p],

@@ -70,3 +70,14 @@ def test_block_order_argument_errors():
     assert list(out) == [0, 1]
     empty = brk.Circuit(3)
     assert _status("brk_circuit_block_order", empty.h, 12, 3, None) == 0  # nothing to order
+
+
+def test_state_handle_argument_errors():
+    h = C.c_void_p()
+    assert _status("brk_state_create", 10, 7, None, C.byref(h)) == 5  # unknown dtype
+    assert _status("brk_state_create", 41, 0, None, C.byref(h)) == 1  # beyond one device
+    assert _status("brk_state_create", 10, 0, None, None) == 1
+    assert _status("brk_state_data", None, None, None, None, None) == 1
+    assert _status("brk_state_set_basis", None, 0) == 1
+    assert _status("brk_state_norm2", None, None) == 1
+    assert _status("brk_state_destroy", None) == 0
