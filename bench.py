@@ -91,6 +91,29 @@ def physical_kernel_names(ops, n):
     return names
 
 
+def kind_and_band(ops, n, line_qubits=3):
+    """(gate kind, target band) for every op, replaying the logical->physical permutation.
+    Kinds: the op name, CNOT for one-control X, CZ for Z on two qubits; band "low" when a physical qubit the
+    op touches is one of the line_qubits lowest (they share 128-byte lines), else "high"."""
+    perm, inv = list(range(n)), list(range(n))
+    out = []
+    for name, t, c, _ in ops:
+        if name == "LOCAL_SWAP":
+            phys = list(t)
+            a, b = inv[t[0]], inv[t[1]]
+            inv[perm[a]], inv[perm[b]] = inv[perm[b]], inv[perm[a]]
+            perm[a], perm[b] = perm[b], perm[a]
+        else:
+            phys = [perm[q] for q in list(t) + list(c)]
+        kind = name
+        if name == "X" and len(t) == 1 and len(c) == 1:
+            kind = "CNOT"
+        elif name == "Z" and len(t) + len(c) == 2:  # the two-qubit phase flip, CZ
+            kind = "CZ"
+        out.append((kind, "low" if min(phys) < line_qubits else "high"))
+    return out
+
+
 def cpu_baseline(n_cpu=30, budget_s=25.0, seed=2):
     import numpy as np
 
@@ -526,6 +549,16 @@ def main():
             "GB/s": round(d["bytes"] / (d["ms"] * 1e-3) / 1e9, 1),
             "share_of_time": round(d["ms"] / (ms_per_step * args.steps), 4),
         }
+    # per gate kind and target band (qubits 0..2 share 128-byte lines)
+    kinds = {}
+    for (kind, band), b, ms in zip(kind_and_band(ops, n), op_bytes, per_op):
+        d = kinds.setdefault(f"{kind}/{band}", {"gates": 0, "bytes": 0.0, "ms": 0.0})
+        d["gates"] += 1
+        d["bytes"] += b * len(ms)
+        d["ms"] += sum(ms)
+    by_kind = {k: {"gates_per_step": d["gates"], "GB/s": round(d["bytes"] / (d["ms"] * 1e-3) / 1e9, 1),
+                   "hbm_fraction": round(d["bytes"] / (d["ms"] * 1e-3) / 8e12, 4)}
+               for k, d in sorted(kinds.items())}
     uniform = {kn: d for kn, d in kernels.items() if len(d["bytes_per_launch"]) == 1}
     dom_name = max(uniform, key=lambda kn: uniform[kn]["ms"])
     dom = uniform[dom_name]
@@ -568,6 +601,7 @@ def main():
         },
         "device": brk.device_name(),
         "kernels": table,
+        "gate_kinds": by_kind,
         "norm_error": abs(norm - 1.0),
     }
     if fused is not None:
