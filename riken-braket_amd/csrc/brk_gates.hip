@@ -458,7 +458,8 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_block(cpx<R>* __restrict__ s,
   __shared__ cpx<R> lds[CAP];
   const cprogram prog = (cprogram)program;  // uniform: scalar loads
   const uint32_t L = prog->num_local;
-  const uint64_t base = chunk_base(blockIdx.x, prog, L);
+  // XCD-contiguous chunk order: consecutive chunks (neighbouring runs of memory) on one XCD
+  const uint64_t base = chunk_base(xcd_tile(blockIdx.x, gridDim.x, 1), prog, L);
   // local index j = t + BLOCK_THREADS * k: the bits of t go to pos[0 .. 8]
   const uint32_t t = threadIdx.x;
   uint64_t low = 0;
@@ -807,7 +808,8 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_adjoint_block(cpx<R>* __restr
   const cadjoint aprog = (cadjoint)program;
   const cprogram prog = (cprogram)&program->base;
   const uint32_t L = prog->num_local;
-  const uint64_t base = chunk_base(blockIdx.x, prog, L);
+  const uint32_t chunk = (uint32_t)xcd_tile(blockIdx.x, gridDim.x, 1);  // XCD-contiguous chunk order
+  const uint64_t base = chunk_base(chunk, prog, L);
   const uint32_t t = threadIdx.x;
   uint64_t low = 0;
   for (uint32_t r = 0; r < (uint32_t)BLOCK_THREAD_BITS; ++r) low |= (uint64_t)((t >> r) & 1u) << prog->pos[r];
