@@ -491,6 +491,30 @@ typedef const __attribute__((address_space(4))) adjoint_block_program* cadjoint;
 
 typedef const __attribute__((address_space(4))) double* cdouble;
 
+// A lane's value from another lane of its row by a DPP control (both 32-bit halves).  Every
+// control used below selects a lane inside the row, so all lanes read a valid source.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  return __hiloint2double(__builtin_amdgcn_update_dpp(0, hi, CTRL, 0xF, 0xF, false),
+                          __builtin_amdgcn_update_dpp(0, lo, CTRL, 0xF, 0xF, false));
+}
+
+// The sum of v over the 64 lanes of the wave, in a fixed order, without LDS traffic: pairs and
+// quads (quad permutations), 8 and 16 lanes (half-row and row mirrors; every lane then holds
+// its row's sum), then the four row sums added in row order.  Uniform result.
+__device__ __forceinline__ double wave_sum_dpp(double v) {
+  v += dpp_f64<0xB1>(v);   // quad_perm [1, 0, 3, 2]
+  v += dpp_f64<0x4E>(v);   // quad_perm [2, 3, 0, 1]
+  v += dpp_f64<0x141>(v);  // row_half_mirror
+  v += dpp_f64<0x140>(v);  // row_mirror
+  auto lane = [&](int l) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                            __builtin_amdgcn_readlane(__double2loint(v), l));
+  };
+  return ((lane(0) + lane(16)) + lane(32)) + lane(48);
+}
+
 __device__ __forceinline__ float fma_r(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fma_r(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
@@ -739,7 +763,7 @@ __device__ __forceinline__ void adjoint_segment(cpx<R>* lp, cpx<R>* ll, cadjoint
         default: g = group_param_any<ADJ_GRAD_GENERAL>(xp, xl, j, op, aprog->d[o]); break;
       }
       // the term's wave sum now (fixed order), kept per wave until the segment ends
-      for (int off = 32; off > 0; off >>= 1) g += __shfl_xor(g, off, 64);
+      g = wave_sum_dpp(g);
       if ((t & 63) == 0) wave_sums[ngrad][t >> 6] = g;
       ++ngrad;
       continue;
@@ -845,7 +869,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_adjoint_block(cpx<R>* __restr
       case ADJ_GRAD_X: acc = param_pass<ADJ_GRAD_X, R, PER>(lp, ll, op, aprog->d[o], t); break;
       default: acc = param_pass<ADJ_GRAD_GENERAL, R, PER>(lp, ll, op, aprog->d[o], t); break;
     }
-    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    acc = wave_sum_dpp(acc);
     if ((t & 63) == 0) wave_sums[0][t >> 6] = acc;
     __syncthreads();
     if (t == 0) {
