@@ -65,7 +65,17 @@ int brk_device_name(char* buffer, size_t size);
    "pair_lds", "diag_lds" (bytes of LDS reserved per block, 0..65536: caps
    resident blocks per CU), "pair_remap", "diag_remap" (0/1: XCD-contiguous
    tile order), "nt_min_bytes" (states at least this large are streamed with
-   non-temporal loads/stores).  Process-wide; set before launching. */
+   non-temporal loads/stores), "max_blocks" (grid cap of the gate kernels),
+   "line_bytes" (write granularity for control bits), "fused_groups",
+   "fused_staged_groups" (grid caps of the fused kernels), "fused_staged"
+   (0/1: LDS-staged or register-direct fused tiles), "fused_occupancy" (2 or
+   4), "fused_chunked" (0/1), "block_lds" (65536, 32768 or 16384: gate-block
+   chunk bytes), "block_run_bytes" (power of two in [16, 4096]: the lowest
+   qubits covering this many bytes stay local in every gate block),
+   "adjoint_blocks" (0/1: blocked or per-run adjoint sweep).  Each key can
+   also be set by the environment variable BRK_TUNE_<KEY> read at first use.
+   Process-wide and not synchronised with launches: set before launching, not
+   while another thread launches.  brk_get_tune reads a key back. */
 int brk_tune(const char* key, int64_t value);
 int brk_get_tune(const char* key, int64_t* value);
 

@@ -35,6 +35,10 @@
 // MFMA 32x32x2 f32 operand layout (lane l, accumulator register v):
 //   A[i][k] = lane (i + 32k), B[k][j] = lane (j + 32k),
 //   D[8(v/4) + 4(l/32) + v%4][l%32] = acc[v] of lane l.
+#include <mutex>
+#include <set>
+#include <utility>
+
 #include "brk_internal.hpp"
 #include "brk_fused.hpp"
 
@@ -305,15 +309,25 @@ static size_t staged_lds_bytes() {
          128 * sizeof(uint32_t) + 128 * sizeof(uint64_t);
 }
 
+// Raises a kernel's dynamic-LDS limit (above 64 KiB a kernel has to opt in) once per device:
+// the attribute belongs to the device the call is made on, and launches may come from several
+// host threads.
+static int allow_dynamic_lds(const void* kernel, size_t bytes) {
+  static std::mutex m;
+  static std::set<std::pair<const void*, int>> done;
+  int dev = 0;
+  BRK_HIP_CHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(m);
+  if (done.count({kernel, dev})) return BRK_OK;
+  BRK_HIP_CHECK(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  done.insert({kernel, dev});
+  return BRK_OK;
+}
+
 template <int KB>
 static int launch_staged(const fused_launch& L, void* state, void* stream) {
   const size_t lds = staged_lds_bytes<KB>();
-  static bool configured = false;
-  if (!configured) {
-    BRK_HIP_CHECK(hipFuncSetAttribute((const void*)k_fused_staged<KB>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)lds));
-    configured = true;
-  }
+  if (const int rc = allow_dynamic_lds((const void*)k_fused_staged<KB>, lds)) return rc;
   const unsigned grid = (unsigned)std::min<uint64_t>(L.sntiles, (uint64_t)get_tuning().fused_staged_groups);
   hipLaunchKernelGGL(k_fused_staged<KB>, dim3(grid), dim3(STAGED_THREADS), lds, as_stream(stream),
                      (cpx<float>*)state, L);
@@ -324,13 +338,7 @@ template <int KB, int OCC>
 static int launch_k(const fused_launch& L, void* state, void* stream) {
   constexpr int W = 2 << KB;
   const size_t lds = sizeof(float) * (size_t)W * W;
-  static bool configured = false;
-  if (!configured) {
-    // above 64 KiB of dynamic LDS a kernel has to opt in
-    BRK_HIP_CHECK(hipFuncSetAttribute((const void*)k_fused<KB, OCC>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)lds));
-    configured = true;
-  }
+  if (const int rc = allow_dynamic_lds((const void*)k_fused<KB, OCC>, lds)) return rc;
   const uint64_t groups = (L.ntiles + 3) / 4;
   const unsigned grid = (unsigned)std::min<uint64_t>(groups, (uint64_t)get_tuning().fused_groups);
   fused_launch Lc = L;
