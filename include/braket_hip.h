@@ -17,7 +17,10 @@
  * and brk_last_error() gives a message for the calling thread.
  *
  * Two calls on the same state must not run concurrently on different streams
- * (the same rule the reference states for its parallel policies).
+ * (the same rule the reference states for its parallel policies).  Calls on
+ * different states may come from several host threads: the library's
+ * per-device buffers (reduction partials, adjoint-gradient partials) are held
+ * by one call at a time, from its launch until it has read its result back.
  */
 #ifndef BRAKET_HIP_H
 #define BRAKET_HIP_H
@@ -172,6 +175,24 @@ int brk_pauli_y(void* first, uint64_t size, int dtype, const unsigned* targets, 
    (ket/include/ket/gate/pauli_z.hpp:55-88 Z, :90-137 CZ, :139-210 CnZ). */
 int brk_pauli_z(void* first, uint64_t size, int dtype, const unsigned* qubits, unsigned num_qubits,
                 void* stream);
+
+/* Z on every qubit of `targets` (Z...Z strings) where every control bit is 1:
+   the amplitude is negated where an odd number of the target bits is 1.
+   Replaces ket::gate::pauli_z, case 2 (ket/include/ket/gate/pauli_z.hpp:265-303
+   Z and ZZ on ket::qubit targets, :305-350 CZ with a target, :392-470 C...CZ...Z). */
+int brk_pauli_zn(void* first, uint64_t size, int dtype, const unsigned* targets, unsigned num_targets,
+                 const unsigned* controls, unsigned num_controls, void* stream);
+
+/* sqrt(Z) on every qubit of `targets` (sZ...sZ strings) where every control bit
+   is 1: the amplitude is multiplied by i^f, f = number of target bits that are 1;
+   the adjoint by (-i)^f.  Replaces ket::gate::sqrt_pauli_z and
+   adj_sqrt_pauli_z, case 2 (ket/include/ket/gate/sqrt_pauli_z.hpp:428-700,
+   :710-1000).  The control-qubit-only forms (case 1, :35-420) are
+   brk_phase_shift_coeff with coeff = i or -i. */
+int brk_sqrt_pauli_zn(void* first, uint64_t size, int dtype, const unsigned* targets, unsigned num_targets,
+                      const unsigned* controls, unsigned num_controls, void* stream);
+int brk_adj_sqrt_pauli_zn(void* first, uint64_t size, int dtype, const unsigned* targets, unsigned num_targets,
+                          const unsigned* controls, unsigned num_controls, void* stream);
 
 /* Multiply the amplitudes whose listed bits are all 1 by `coeff`
    (U1 / CU1 / CnU1 given exp(i theta)).  Replaces ket::gate::phase_shift_coeff

@@ -335,6 +335,48 @@ void or_pauli_z(double* s, uint64_t size, const unsigned* qubits, unsigned nq) {
   }
 }
 
+/* Z on ket::qubit targets under controls (case 2) -- pauli_z.hpp:265-303 (Z, ZZ), :392-470
+   (C...CZ...Z): per group of the operated qubits (targets, then controls), every local value
+   with all control bits set whose target part has an odd number of ones is multiplied by -1. */
+void or_pauli_zn(double* s, uint64_t size, const unsigned* targets, unsigned nt, const unsigned* controls,
+                 unsigned nc) {
+  unsigned q[128];
+  const unsigned k = join_qubits(q, targets, nt, controls, nc);
+  uint64_t im[65];
+  or_make_index_masks(q, k, im);
+  const uint64_t base = (((uint64_t)1 << nc) - 1u) << nt;
+  const int64_t n = (int64_t)(size >> k);
+#pragma omp parallel for schedule(static)
+  for (int64_t g = 0; g < n; ++g)
+    for (uint64_t v = 0; v < ((uint64_t)1 << nt); ++v) {
+      if ((popcount64(v) & 1) == 0) continue;
+      const uint64_t i = or_index_with_qubits((uint64_t)g, base + v, q, k, im);
+      const cd a = cd_load(s, i);
+      cd_store(s, i, cd_make(-a.re, -a.im));
+    }
+}
+
+/* sqrt(Z) (adjoint: sqrt(Z)^dagger) on ket::qubit targets under controls -- sqrt_pauli_z.hpp:
+   565-700 (:846-1000): every local value with all control bits set is multiplied by
+   1, i, -1, -i (1, -i, -1, i) as its target part has 0, 1, 2, 3 ones mod 4. */
+void or_sqrt_pauli_zn(double* s, uint64_t size, const unsigned* targets, unsigned nt, const unsigned* controls,
+                      unsigned nc, int adjoint) {
+  unsigned q[128];
+  const unsigned k = join_qubits(q, targets, nt, controls, nc);
+  uint64_t im[65];
+  or_make_index_masks(q, k, im);
+  const uint64_t base = (((uint64_t)1 << nc) - 1u) << nt;
+  const double sgn = adjoint ? -1.0 : 1.0;
+  const cd coefficient[4] = {cd_make(1.0, 0.0), cd_make(0.0, sgn), cd_make(-1.0, 0.0), cd_make(0.0, -sgn)};
+  const int64_t n = (int64_t)(size >> k);
+#pragma omp parallel for schedule(static)
+  for (int64_t g = 0; g < n; ++g)
+    for (uint64_t v = 0; v < ((uint64_t)1 << nt); ++v) {
+      const uint64_t i = or_index_with_qubits((uint64_t)g, base + v, q, k, im);
+      cd_store(s, i, cd_mul(cd_load(s, i), coefficient[popcount64(v) & 3]));
+    }
+}
+
 /* U1 via coefficient on control qubits -- phase_shift.hpp:37-93 (`*it *= coeff`) */
 void or_phase_shift_coeff(double* s, uint64_t size, const double coeff[2], const unsigned* qubits, unsigned nq) {
   uint64_t im[65];

@@ -40,6 +40,10 @@ void or_pauli_x(double* s, uint64_t size, const unsigned* targets, unsigned nt, 
 void or_pauli_y(double* s, uint64_t size, const unsigned* targets, unsigned nt, const unsigned* controls,
                 unsigned nc);
 void or_pauli_z(double* s, uint64_t size, const unsigned* qubits, unsigned nq);
+void or_pauli_zn(double* s, uint64_t size, const unsigned* targets, unsigned nt, const unsigned* controls,
+                 unsigned nc);
+void or_sqrt_pauli_zn(double* s, uint64_t size, const unsigned* targets, unsigned nt, const unsigned* controls,
+                      unsigned nc, int adjoint);
 void or_phase_shift_coeff(double* s, uint64_t size, const double coeff[2], const unsigned* qubits, unsigned nq);
 void or_phase_shift2(double* s, uint64_t size, double phase1, double phase2, unsigned target,
                      const unsigned* controls, unsigned nc);

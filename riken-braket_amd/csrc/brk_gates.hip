@@ -15,8 +15,10 @@
 //  * k_lane_swap -- SWAP / local swap of a low qubit (< 6) with a high one:
 //    the two halves of the high qubit stream contiguously and the low-qubit
 //    exchange happens across lanes.
-//  * k_diag  -- multiplies the amplitudes of a subspace by one of two
-//    coefficients chosen by the parity of the target bits (EZ...Z, U1, Z, CZ).
+//  * k_diag  -- multiplies the amplitudes of a subspace by a coefficient
+//    chosen by the number of set target bits: one coefficient (U1, Z, CZ),
+//    two by its parity (EZ...Z, Z...Z strings), or four by its value mod 4
+//    (sqrt(Z)...sqrt(Z) strings).
 //
 // Controls: a control bit at or above the line boundary (the bits addressing
 // amplitudes inside one 64-byte line) is removed from the iteration space, so
@@ -44,11 +46,14 @@ struct gate_args {
   uint64_t flipmask;  // pair kernels: i1 = i0 ^ flipmask
   uint64_t signmask;  // matrix mode: off-diagonal entries change sign when parity(i0 & signmask) is odd
   uint64_t condmask;  // the op acts only where all these bits are 1 (controls kept in the iteration)
-  uint64_t zmask;     // diagonal kernel: parity mask selecting the second coefficient
+  uint64_t zmask;     // diagonal kernel: target mask whose set-bit count selects the coefficient
   uint64_t swapmask;  // k_lane swap mode on two low qubits: acts where parity(i & swapmask) is odd
   int lane_bit;       // lane kernels: the low qubit
-  double u[8];        // matrix (m00, m01, m10, m11) or diagonal coefficients (d0, d1) as (re, im)
+  double u[8];        // matrix (m00, m01, m10, m11) or diagonal coefficients (d0, d1[, d2, d3]) as (re, im)
 };
+
+// How k_diag picks the coefficient from popcount(i & zmask).
+enum diag_select { DIAG_ONE = 0, DIAG_PARITY = 1, DIAG_MOD4 = 2 };
 
 constexpr int GATE_THREADS = 256;
 constexpr unsigned LANE_BITS = 6;  // log2(wavefront size)
@@ -209,21 +214,27 @@ __global__ __launch_bounds__(GATE_THREADS) void k_lane_swap(cpx<R>* __restrict__
   }
 }
 
-template <typename R, int K, bool PARITY, bool NT, bool COND>
+template <typename R, int K, int SELECT, bool NT, bool COND>
 __global__ __launch_bounds__(GATE_THREADS) void k_diag(cpx<R>* __restrict__ s, uint64_t nitems, gate_args a,
                                                       int remap) {
   if (a.k < 0) occupancy_lds[threadIdx.x] = 0;
   const cpx<R> c0 = {(R)a.u[0], (R)a.u[1]};
   const cpx<R> c1 = {(R)a.u[2], (R)a.u[3]};
+  const cpx<R> c2 = {(R)a.u[4], (R)a.u[5]};
+  const cpx<R> c3 = {(R)a.u[6], (R)a.u[7]};
   // Z and CZ: both coefficients -1, an exact negation
-  const bool negate = a.u[0] == -1.0 && a.u[1] == 0.0 && a.u[2] == -1.0 && a.u[3] == 0.0;
+  const bool negate = SELECT != DIAG_MOD4 && a.u[0] == -1.0 && a.u[1] == 0.0 && a.u[2] == -1.0 && a.u[3] == 0.0;
   const uint64_t stride = (uint64_t)gridDim.x * GATE_THREADS;
   for (uint64_t g = xcd_tile(blockIdx.x, gridDim.x, remap) * GATE_THREADS + threadIdx.x; g < nitems;
        g += stride) {
     const uint64_t i = expand_index<K>(g, a.m, a.k) | a.setmask;
     const cpx<R> x = load_amp<NT>(s + i);
     cpx<R> c = c0;
-    if constexpr (PARITY) c = csel((__popcll(i & a.zmask) & 1) != 0, c1, c0);
+    if constexpr (SELECT == DIAG_PARITY) c = csel((__popcll(i & a.zmask) & 1) != 0, c1, c0);
+    if constexpr (SELECT == DIAG_MOD4) {
+      const unsigned n = (unsigned)__popcll(i & a.zmask);
+      c = csel((n & 2u) != 0, csel((n & 1u) != 0, c3, c2), csel((n & 1u) != 0, c1, c0));
+    }
     cpx<R> y = negate ? cneg(x) : cmul(x, c);
     if (COND) y = csel((i & a.condmask) == a.condmask, y, x);
     store_amp<NT>(s + i, y);
@@ -1014,9 +1025,9 @@ static void lane_swap_k(const launch_ctx& c, cpx<R>* s, uint64_t n, const gate_a
 #undef L
 }
 
-template <typename R, bool PARITY, bool NT, bool COND>
+template <typename R, int SELECT, bool NT, bool COND>
 static void diag_k(const launch_ctx& c, cpx<R>* s, uint64_t n, const gate_args& a) {
-#define L(K, ...) hipLaunchKernelGGL((k_diag<R, K, PARITY, NT, COND>), c.grid, c.block, c.lds, c.st, s, n, a, c.remap)
+#define L(K, ...) hipLaunchKernelGGL((k_diag<R, K, SELECT, NT, COND>), c.grid, c.block, c.lds, c.st, s, n, a, c.remap)
   BRK_DISPATCH_K(a.k, L, 0)
 #undef L
 }
@@ -1041,8 +1052,9 @@ static void launch_family(int family, int mode, const launch_ctx& c, cpx<R>* s, 
     case FAM_LANE_SWAP2: lane_k<R, MODE_SWAP, NT, COND, true>(c, s, n, a); break;
     case FAM_LANE_SWAP: lane_swap_k<R, NT, COND>(c, s, n, a); break;
     default:
-      if (a.zmask) diag_k<R, true, NT, COND>(c, s, n, a);
-      else diag_k<R, false, NT, COND>(c, s, n, a);
+      if (mode == DIAG_MOD4) diag_k<R, DIAG_MOD4, NT, COND>(c, s, n, a);
+      else if (a.zmask) diag_k<R, DIAG_PARITY, NT, COND>(c, s, n, a);
+      else diag_k<R, DIAG_ONE, NT, COND>(c, s, n, a);
       break;
   }
 }
@@ -1081,6 +1093,10 @@ static unsigned line_bits(int dtype) {
   const uint64_t per_line = std::max<uint64_t>(1, get_tuning().line_bytes / amp_bytes(dtype));
   return (unsigned)__builtin_ctzll(per_line);
 }
+
+// Low qubits whose pair partner the lane kernels fetch across the wavefront: inside a line,
+// and below the wavefront width (a shuffle mask of 2^6 or more would return the lane's own value).
+static unsigned lane_target_bits(int dtype) { return std::min(line_bits(dtype), LANE_BITS); }
 
 struct plan {
   unsigned pos[BRK_MAX_OPERATED_QUBITS];  // inserted positions
@@ -1129,7 +1145,7 @@ static int single_target(void* first, uint64_t size, int dtype, unsigned target,
   gate_args a;
   // Below the line boundary the two amplitudes of a pair share a line: the lane
   // kernel keeps every access a whole contiguous wavefront.
-  if (v.nq >= LANE_BITS && target < line_bits(dtype)) {
+  if (v.nq >= LANE_BITS && target < lane_target_bits(dtype)) {
     for (unsigned i = 0; i < nc; ++i) p.control(controls[i], LANE_BITS);
     a = make_args(p);
     a.lane_bit = (int)target;
@@ -1176,7 +1192,21 @@ static int diagonal(void* first, uint64_t size, int dtype, const unsigned* contr
   a.u[1] = d0.imag();
   a.u[2] = d1.real();
   a.u[3] = d1.imag();
-  return launch(dtype, FAM_DIAG, 0, first, size, size >> p.np, a, stream);
+  return launch(dtype, FAM_DIAG, zmask ? DIAG_PARITY : DIAG_ONE, first, size, size >> p.np, a, stream);
+}
+
+// d[popcount(i & zmask) mod 4] where every control bit of i is 1.
+static int diagonal_mod4(void* first, uint64_t size, int dtype, const unsigned* controls, unsigned nc,
+                         uint64_t zmask, const std::complex<double> (&d)[4], void* stream) {
+  plan p;
+  for (unsigned i = 0; i < nc; ++i) p.control(controls[i], line_bits(dtype));
+  gate_args a = make_args(p);
+  a.zmask = zmask;
+  for (int k = 0; k < 4; ++k) {
+    a.u[2 * k] = d[k].real();
+    a.u[2 * k + 1] = d[k].imag();
+  }
+  return launch(dtype, FAM_DIAG, DIAG_MOD4, first, size, size >> p.np, a, stream);
 }
 
 // (-i)^m as a complex number.
@@ -1376,6 +1406,67 @@ int brk_pauli_z(void* first, uint64_t size, int dtype, const unsigned* qubits, u
   return diagonal(first, size, dtype, qubits, num_qubits, 0, {-1.0, 0.0}, {-1.0, 0.0}, stream);
 }
 
+}  // extern "C"
+
+namespace brk {
+
+// Targets then controls in one list (the qubits of a diagonal acting where all are 1).
+static unsigned join_qubits(const unsigned* t, unsigned nt, const unsigned* c, unsigned nc,
+                            unsigned (&all)[2 * BRK_MAX_OPERATED_QUBITS]) {
+  unsigned k = 0;
+  for (unsigned i = 0; i < nt; ++i) all[k++] = t[i];
+  for (unsigned i = 0; i < nc; ++i) all[k++] = c[i];
+  return k;
+}
+
+// C..C(sZ_t1 ... sZ_tm) and its adjoint: i^f (or (-i)^f), f = number of set target bits, where
+// every control is 1 (ket/include/ket/gate/sqrt_pauli_z.hpp:565-700 and :846-1000: the
+// coefficient by f mod 4; one target: i on the all-ones subspace, :428-460).
+static int sqrt_z_string(void* first, uint64_t size, int dtype, const unsigned* targets, unsigned nt,
+                         const unsigned* controls, unsigned nc, bool adjoint, void* stream) {
+  if (nt == 0) return fail(BRK_ERR_INVALID_ARGUMENT, "no target qubits");
+  if (!targets) return fail(BRK_ERR_INVALID_ARGUMENT, "target list is null");
+  validated v;
+  if (int st = validate(first, size, dtype, targets, nt, controls, nc, &v)) return st;
+  const std::complex<double> i{0.0, adjoint ? -1.0 : 1.0};
+  if (nt == 1) {
+    unsigned all[2 * BRK_MAX_OPERATED_QUBITS];
+    return diagonal(first, size, dtype, all, join_qubits(targets, nt, controls, nc, all), 0, i, i, stream);
+  }
+  const std::complex<double> d[4] = {{1.0, 0.0}, i, {-1.0, 0.0}, -i};
+  return diagonal_mod4(first, size, dtype, controls, nc, mask_of(targets, nt), d, stream);
+}
+
+}  // namespace brk
+
+extern "C" {
+
+int brk_pauli_zn(void* first, uint64_t size, int dtype, const unsigned* targets, unsigned num_targets,
+                 const unsigned* controls, unsigned num_controls, void* stream) {
+  if (num_targets == 0) return fail(BRK_ERR_INVALID_ARGUMENT, "no target qubits");
+  if (!targets) return fail(BRK_ERR_INVALID_ARGUMENT, "target list is null");
+  validated v;
+  if (int st = validate(first, size, dtype, targets, num_targets, controls, num_controls, &v)) return st;
+  if (num_targets == 1) {  // CnZ: -1 where the target and every control are 1 (pauli_z.hpp:352-390)
+    unsigned all[2 * BRK_MAX_OPERATED_QUBITS];
+    const unsigned k = join_qubits(targets, num_targets, controls, num_controls, all);
+    return diagonal(first, size, dtype, all, k, 0, {-1.0, 0.0}, {-1.0, 0.0}, stream);
+  }
+  // C..C(Z_t1 ... Z_tm): -1 on odd parity of the targets (pauli_z.hpp:392-470)
+  return diagonal(first, size, dtype, controls, num_controls, mask_of(targets, num_targets), {1.0, 0.0},
+                  {-1.0, 0.0}, stream);
+}
+
+int brk_sqrt_pauli_zn(void* first, uint64_t size, int dtype, const unsigned* targets, unsigned num_targets,
+                      const unsigned* controls, unsigned num_controls, void* stream) {
+  return sqrt_z_string(first, size, dtype, targets, num_targets, controls, num_controls, false, stream);
+}
+
+int brk_adj_sqrt_pauli_zn(void* first, uint64_t size, int dtype, const unsigned* targets, unsigned num_targets,
+                          const unsigned* controls, unsigned num_controls, void* stream) {
+  return sqrt_z_string(first, size, dtype, targets, num_targets, controls, num_controls, true, stream);
+}
+
 int brk_phase_shift_coeff(void* first, uint64_t size, int dtype, const double coeff[2], const unsigned* qubits,
                           unsigned num_qubits, void* stream) {
   if (!coeff) return fail(BRK_ERR_INVALID_ARGUMENT, "coeff is null");
@@ -1504,13 +1595,13 @@ int brk_swap(void* first, uint64_t size, int dtype, unsigned qubit1, unsigned qu
   const unsigned lo = std::min(qubit1, qubit2), hi = std::max(qubit1, qubit2);
   plan p;
   gate_args a;
-  if (v.nq >= LANE_BITS && hi < LANE_BITS && lo < line_bits(dtype)) {
+  if (v.nq >= LANE_BITS && hi < LANE_BITS && lo < lane_target_bits(dtype)) {
     for (unsigned i = 0; i < num_controls; ++i) p.control(controls[i], LANE_BITS);
     a = make_args(p);
     a.swapmask = (1ull << lo) | (1ull << hi);
     return launch(dtype, FAM_LANE_SWAP2, MODE_SWAP, first, size, size >> p.np, a, stream);
   }
-  if (v.nq > LANE_BITS && lo < line_bits(dtype) && hi >= LANE_BITS) {
+  if (v.nq > LANE_BITS && lo < lane_target_bits(dtype) && hi >= LANE_BITS) {
     p.insert(hi);
     for (unsigned i = 0; i < num_controls; ++i) p.control(controls[i], LANE_BITS);
     a = make_args(p);

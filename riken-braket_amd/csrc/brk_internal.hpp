@@ -3,6 +3,8 @@
 // map, and the device complex type.
 #pragma once
 
+#include <mutex>
+
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -169,10 +171,13 @@ inline uint64_t mask_of(const unsigned* q, unsigned n) {
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
 // Per-device scratch used by the reductions (partial sums).  Allocated on the
-// first reduction on a device and kept until the process ends.
+// first reduction on a device and kept until the process ends.  A reduction
+// holds `busy` from its launch until it has read the partials back, so calls
+// from several host threads on one device take turns.
 struct reduce_scratch {
   double* device = nullptr;   // RED_MAX_BLOCKS * RED_MAX_VALUES doubles
   double* host = nullptr;     // pinned mirror
+  std::mutex busy;
 };
 int get_reduce_scratch(reduce_scratch** out);
 

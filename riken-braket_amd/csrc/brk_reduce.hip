@@ -357,11 +357,16 @@ template <int NV, typename Launch>
 static int run_reduction(void* stream, Launch launch, double* out) {
   reduce_scratch* sc;
   if (int st = get_reduce_scratch(&sc)) return st;
+  // the device's scratch is ours from the launch until the partials are read back
+  std::lock_guard<std::mutex> hold(sc->busy);
   hipStream_t s = as_stream(stream);
   launch(sc->device, s);
   if (int st = check_launch("reduction")) return st;
-  BRK_HIP_CHECK(hipMemcpyAsync(sc->host, sc->device, sizeof(double) * RED_BLOCKS * NV, hipMemcpyDeviceToHost, s));
-  BRK_HIP_CHECK(hipStreamSynchronize(s));
+  const hipError_t copied =
+      hipMemcpyAsync(sc->host, sc->device, sizeof(double) * RED_BLOCKS * NV, hipMemcpyDeviceToHost, s);
+  const hipError_t synced = hipStreamSynchronize(s);
+  if (copied != hipSuccess) return fail(BRK_ERR_HIP, "reduction read-back: %s", hipGetErrorString(copied));
+  if (synced != hipSuccess) return fail(BRK_ERR_HIP, "reduction: %s", hipGetErrorString(synced));
   for (int v = 0; v < NV; ++v) out[v] = 0.0;
   for (int b = 0; b < RED_BLOCKS; ++b)
     for (int v = 0; v < NV; ++v) out[v] += sc->host[b * NV + v];

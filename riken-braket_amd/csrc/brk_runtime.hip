@@ -3,10 +3,14 @@
 // dtype conversion and the streaming copy kernel.
 #include "brk_internal.hpp"
 
+#include <cctype>
+#include <cerrno>
 #include <cstdio>
 
+#include <memory>
 #include <mutex>
 #include <cstdlib>
+#include <string>
 #include <type_traits>
 #include <vector>
 
@@ -62,24 +66,27 @@ static tuning g_tuning = {
 };
 static std::once_flag g_tuning_once;
 
+static int set_tuning(tuning& t, const std::string& k, int64_t value);
+
+// BRK_TUNE_<KEY> environment values go through the same checks as brk_tune; a rejected
+// value keeps the default and is reported on stderr.
 static void read_tuning_env() {
-  auto get = [](const char* key, auto* field) {
-    if (const char* v = std::getenv(key)) *field = (std::remove_pointer_t<decltype(field)>)std::strtoll(v, nullptr, 0);
-  };
-  get("BRK_TUNE_PAIR_LDS", &g_tuning.pair_lds);
-  get("BRK_TUNE_DIAG_LDS", &g_tuning.diag_lds);
-  get("BRK_TUNE_PAIR_REMAP", &g_tuning.pair_remap);
-  get("BRK_TUNE_DIAG_REMAP", &g_tuning.diag_remap);
-  get("BRK_TUNE_NT_MIN_BYTES", &g_tuning.nt_min_bytes);
-  get("BRK_TUNE_MAX_BLOCKS", &g_tuning.max_blocks);
-  get("BRK_TUNE_LINE_BYTES", &g_tuning.line_bytes);
-  get("BRK_TUNE_FUSED_GROUPS", &g_tuning.fused_groups);
-  get("BRK_TUNE_FUSED_OCCUPANCY", &g_tuning.fused_occupancy);
-  get("BRK_TUNE_FUSED_STAGED", &g_tuning.fused_staged);
-  get("BRK_TUNE_FUSED_STAGED_GROUPS", &g_tuning.fused_staged_groups);
-  get("BRK_TUNE_BLOCK_LDS", &g_tuning.block_lds);
-  get("BRK_TUNE_BLOCK_RUN_BYTES", &g_tuning.block_run_bytes);
-  get("BRK_TUNE_ADJOINT_BLOCKS", &g_tuning.adjoint_blocks);
+  static const char* const keys[] = {"pair_lds",     "diag_lds",          "pair_remap",          "diag_remap",
+                                     "nt_min_bytes", "max_blocks",        "line_bytes",          "fused_groups",
+                                     "fused_occupancy", "fused_staged",   "fused_staged_groups", "fused_chunked",
+                                     "block_lds",    "block_run_bytes",   "adjoint_blocks"};
+  for (const char* key : keys) {
+    std::string env = "BRK_TUNE_";
+    for (const char* c = key; *c; ++c) env.push_back((char)std::toupper((unsigned char)*c));
+    const char* v = std::getenv(env.c_str());
+    if (!v) continue;
+    char* end = nullptr;
+    errno = 0;
+    const long long value = std::strtoll(v, &end, 0);
+    if (errno || end == v || *end != '\0' || set_tuning(g_tuning, key, value) != BRK_OK)
+      std::fprintf(stderr, "braket-mi355x: ignoring %s=%s (%s)\n", env.c_str(), v,
+                   errno || end == v || *end ? "not an integer" : g_last_error.c_str());
+  }
 }
 
 tuning& get_tuning() {
@@ -103,14 +110,15 @@ unsigned device_cus() {
 }
 
 static std::mutex g_scratch_mutex;
-static std::vector<reduce_scratch> g_scratch;
+static std::vector<std::unique_ptr<reduce_scratch>> g_scratch;  // stable addresses: each holds a mutex
 
 int get_reduce_scratch(reduce_scratch** out) {
   int dev = 0;
   BRK_HIP_CHECK(hipGetDevice(&dev));
   std::lock_guard<std::mutex> lock(g_scratch_mutex);
   if ((int)g_scratch.size() <= dev) g_scratch.resize(dev + 1);
-  reduce_scratch& s = g_scratch[dev];
+  if (!g_scratch[dev]) g_scratch[dev] = std::make_unique<reduce_scratch>();
+  reduce_scratch& s = *g_scratch[dev];
   if (s.device == nullptr) {
     size_t bytes = sizeof(double) * RED_BLOCKS * RED_MAX_VALUES;
     if (hipMalloc(&s.device, bytes) != hipSuccess) {
@@ -216,31 +224,10 @@ static unsigned grid_for(uint64_t n, unsigned threads, unsigned cap = 65536u) {
 
 using namespace brk;
 
-extern "C" {
+namespace brk {
 
-const char* brk_status_string(int status) {
-  switch (status) {
-    case BRK_OK: return "ok";
-    case BRK_ERR_INVALID_ARGUMENT: return "invalid argument";
-    case BRK_ERR_QUBIT_OUT_OF_RANGE: return "qubit out of range";
-    case BRK_ERR_DUPLICATE_QUBIT: return "duplicate qubit";
-    case BRK_ERR_TOO_MANY_QUBITS: return "too many operated qubits";
-    case BRK_ERR_UNSUPPORTED_DTYPE: return "unsupported dtype";
-    case BRK_ERR_OUT_OF_MEMORY: return "out of memory";
-    case BRK_ERR_HIP: return "HIP error";
-    case BRK_ERR_UNSUPPORTED: return "unsupported";
-    default: return "unknown status";
-  }
-}
-
-const char* brk_last_error(void) { return g_last_error.c_str(); }
-
-const char* brk_version(void) { return "braket-mi355x 0.1 (gfx950)"; }
-
-int brk_tune(const char* key, int64_t value) {
-  if (!key) return fail(BRK_ERR_INVALID_ARGUMENT, "key is null");
-  tuning& t = get_tuning();
-  const std::string k = key;
+static int set_tuning(tuning& t, const std::string& k, int64_t value) {
+  const char* key = k.c_str();
   if (k == "pair_lds" || k == "diag_lds") {
     if (value < 0 || value > 65536) return fail(BRK_ERR_INVALID_ARGUMENT, "%s must be in [0, 65536]", key);
     (k == "pair_lds" ? t.pair_lds : t.diag_lds) = (int)value;
@@ -280,6 +267,34 @@ int brk_tune(const char* key, int64_t value) {
     t.max_blocks = (uint64_t)value;
   } else return fail(BRK_ERR_INVALID_ARGUMENT, "unknown tuning key '%s'", key);
   return BRK_OK;
+}
+
+}  // namespace brk
+
+extern "C" {
+
+const char* brk_status_string(int status) {
+  switch (status) {
+    case BRK_OK: return "ok";
+    case BRK_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case BRK_ERR_QUBIT_OUT_OF_RANGE: return "qubit out of range";
+    case BRK_ERR_DUPLICATE_QUBIT: return "duplicate qubit";
+    case BRK_ERR_TOO_MANY_QUBITS: return "too many operated qubits";
+    case BRK_ERR_UNSUPPORTED_DTYPE: return "unsupported dtype";
+    case BRK_ERR_OUT_OF_MEMORY: return "out of memory";
+    case BRK_ERR_HIP: return "HIP error";
+    case BRK_ERR_UNSUPPORTED: return "unsupported";
+    default: return "unknown status";
+  }
+}
+
+const char* brk_last_error(void) { return g_last_error.c_str(); }
+
+const char* brk_version(void) { return "braket-mi355x 0.1 (gfx950)"; }
+
+int brk_tune(const char* key, int64_t value) {
+  if (!key) return fail(BRK_ERR_INVALID_ARGUMENT, "key is null");
+  return set_tuning(get_tuning(), key, value);
 }
 
 int brk_get_tune(const char* key, int64_t* value) {

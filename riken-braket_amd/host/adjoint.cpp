@@ -33,6 +33,7 @@
 #include <algorithm>
 #include <array>
 #include <complex>
+#include <memory>
 #include <mutex>
 #include <vector>
 
@@ -212,26 +213,52 @@ static int forward(const std::vector<brk_circuit::op>& ops, void* psi, uint64_t 
   return BRK_OK;
 }
 
-// Per-device FP64 partial-sum buffer, grown as needed and kept until exit.
+// Per-device FP64 partial-sum buffer, grown as needed and kept until exit.  A gradient
+// holds the device's buffer (a lease) from its first launch until it has read the
+// result back: calls from several host threads on one device take turns, and the
+// buffer is never freed or regrown while another call's kernels still write it.
+struct partials_slot {
+  std::mutex busy;
+  double* ptr = nullptr;
+  size_t doubles = 0;
+};
 static std::mutex g_partials_mutex;
-static std::vector<std::pair<double*, size_t>> g_partials;
+static std::vector<std::unique_ptr<partials_slot>> g_partials;
 
-static int get_partials(size_t doubles, double** out) {
+struct partials_lease {
+  std::unique_lock<std::mutex> hold;
+  double* ptr = nullptr;
+  hipStream_t stream = nullptr;
+  // an early return may leave kernels that write the buffer in flight: drain them first
+  ~partials_lease() {
+    if (hold.owns_lock()) (void)hipStreamSynchronize(stream);
+  }
+};
+
+static int get_partials(size_t doubles, hipStream_t stream, partials_lease& lease) {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return fail(BRK_ERR_HIP, "hipGetDevice failed");
-  std::lock_guard<std::mutex> lock(g_partials_mutex);
-  if ((int)g_partials.size() <= dev) g_partials.resize(dev + 1, {nullptr, 0});
-  auto& slot = g_partials[dev];
-  if (slot.second < doubles) {
-    if (slot.first) (void)hipFree(slot.first);
-    slot = {nullptr, 0};
-    if (hipMalloc(&slot.first, doubles * sizeof(double)) != hipSuccess) {
-      slot.first = nullptr;
+  partials_slot* slot = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_partials_mutex);
+    if ((int)g_partials.size() <= dev) g_partials.resize(dev + 1);
+    if (!g_partials[dev]) g_partials[dev] = std::make_unique<partials_slot>();
+    slot = g_partials[dev].get();
+  }
+  std::unique_lock<std::mutex> hold(slot->busy);
+  if (slot->doubles < doubles) {
+    if (slot->ptr) (void)hipFree(slot->ptr);
+    slot->ptr = nullptr;
+    slot->doubles = 0;
+    if (hipMalloc(&slot->ptr, doubles * sizeof(double)) != hipSuccess) {
+      slot->ptr = nullptr;
       return fail(BRK_ERR_OUT_OF_MEMORY, "cannot allocate %zu gradient partials", doubles);
     }
-    slot.second = doubles;
+    slot->doubles = doubles;
   }
-  *out = slot.first;
+  lease.hold = std::move(hold);
+  lease.ptr = slot->ptr;
+  lease.stream = stream;
   return BRK_OK;
 }
 
@@ -410,9 +437,10 @@ static int blocked_gradient(const brk_circuit* given, const std::vector<int>& gi
   flush();
 
   const uint64_t chunks = size >> L;
-  double* partials = nullptr;
   // nparams x chunks partials, then the nparams gradients
-  if (int s = get_partials((size_t)nparams * chunks + std::max(1u, nparams), &partials)) return s;
+  partials_lease lease;
+  if (int s = get_partials((size_t)nparams * chunks + std::max(1u, nparams), st, lease)) return s;
+  double* partials = lease.ptr;
   double* sums = partials + (size_t)nparams * chunks;
   adjoint_block_program* device = nullptr;
   const size_t bytes = sizeof(adjoint_block_program) * programs.size();
@@ -494,8 +522,9 @@ int brk_adjoint_ising_gradient(const brk_circuit* circuit, void* psi, void* work
   // reverse sweep; one partial region of ADJ_MAX_CHAIN * ADJ_BLOCKS doubles per chain
   size_t nchains = 0;
   for (size_t i = 0; i < ops.size(); ++i) nchains += chainable(ops[i]) ? 1 : 0;  // upper bound
-  double* partials = nullptr;
-  if (int s = get_partials(std::max<size_t>(1, nchains) * ADJ_MAX_CHAIN * ADJ_BLOCKS, &partials)) return s;
+  partials_lease lease;
+  if (int s = get_partials(std::max<size_t>(1, nchains) * ADJ_MAX_CHAIN * ADJ_BLOCKS, st, lease)) return s;
+  double* partials = lease.ptr;
   struct slot {
     int param;
     size_t offset;

@@ -279,9 +279,16 @@ inline void pauli_y(hip::policy p, iter<Complex> first, iter<Complex> last, cons
   hip::check(brk_pauli_y(first.get(), span(first, last), dtype<Complex>(), o.targets, o.nt,
                          ptr_or_null(o.controls, o.nc), o.nc, p.stream));
 }
-// Z on control qubits (Z, CZ, CnZ) -- pauli_z.hpp:55-210; targets and controls are treated alike
+// Z on control qubits (Z, CZ, CnZ) -- pauli_z.hpp:55-210 -- and with one ket::qubit target
+// (CZ_{tc}, :305-390): -1 where every listed bit is 1.  Two or more targets (ZZ, C...CZ...Z,
+// :265-303, :392-470) are a product of Z's: -1 on odd parity of the targets under the controls.
 template <typename Complex>
 inline void pauli_z(hip::policy p, iter<Complex> first, iter<Complex> last, const operands& o) {
+  if (o.nt >= 2) {
+    hip::check(brk_pauli_zn(first.get(), span(first, last), dtype<Complex>(), o.targets, o.nt,
+                            ptr_or_null(o.controls, o.nc), o.nc, p.stream));
+    return;
+  }
   unsigned all[2 * BRK_MAX_OPERATED_QUBITS], n = 0;
   hip::detail::all_qubits(o, all, &n);
   hip::check(brk_pauli_z(first.get(), span(first, last), dtype<Complex>(), all, n, p.stream));
@@ -312,8 +319,15 @@ inline void controlled_phase_shift(hip::policy p, iter<Complex> first, iter<Comp
                                    const operands& o) {
   phase_shift(p, first, last, phase, o);
 }
+// sqrt(Z): i where every listed bit is 1 (sqrt_pauli_z.hpp:35-420, one target :428-460);
+// two or more targets (sZZ, C...CsZ...Z, :462-700): i^f, f = number of set target bits.
 template <typename Complex>
 inline void sqrt_pauli_z(hip::policy p, iter<Complex> first, iter<Complex> last, const operands& o) {
+  if (o.nt >= 2) {
+    hip::check(brk_sqrt_pauli_zn(first.get(), span(first, last), dtype<Complex>(), o.targets, o.nt,
+                                 ptr_or_null(o.controls, o.nc), o.nc, p.stream));
+    return;
+  }
   phase_shift_coeff(p, first, last, {0.0, 1.0}, o);
 }
 // U2, U3 -- phase_shift.hpp:653-709, 1195-1254
@@ -483,6 +497,11 @@ KET_HIP_COEFF_ADJOINT(controlled_v_coeff)
 #undef KET_HIP_COEFF_ADJOINT
 template <typename Complex>
 inline void adj_sqrt_pauli_z(hip::policy p, iter<Complex> first, iter<Complex> last, const operands& o) {
+  if (o.nt >= 2) {
+    hip::check(brk_adj_sqrt_pauli_zn(first.get(), span(first, last), dtype<Complex>(), o.targets, o.nt,
+                                     ptr_or_null(o.controls, o.nc), o.nc, p.stream));
+    return;
+  }
   phase_shift_coeff(p, first, last, {0.0, -1.0}, o);
 }
 template <typename Complex>

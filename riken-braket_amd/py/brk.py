@@ -70,6 +70,9 @@ _SIGS = {
     "brk_controlled_not": [_p, _u64, _i, _u, _up, _u, _p],
     "brk_pauli_y": [_p, _u64, _i, _up, _u, _up, _u, _p],
     "brk_pauli_z": [_p, _u64, _i, _up, _u, _p],
+    "brk_pauli_zn": [_p, _u64, _i, _up, _u, _up, _u, _p],
+    "brk_sqrt_pauli_zn": [_p, _u64, _i, _up, _u, _up, _u, _p],
+    "brk_adj_sqrt_pauli_zn": [_p, _u64, _i, _up, _u, _up, _u, _p],
     "brk_phase_shift_coeff": [_p, _u64, _i, _dp, _up, _u, _p],
     "brk_phase_shift": [_p, _u64, _i, _d, _up, _u, _p],
     "brk_phase_shift2": [_p, _u64, _i, _d, _d, _u, _up, _u, _p],
@@ -229,6 +232,24 @@ def pauli_y(ptr, size, dtype, targets, controls=(), stream=0):
 def pauli_z(ptr, size, dtype, qubits, stream=0):
     q, nq = uarr(qubits)
     call("brk_pauli_z", ptr, size, dtype, q, nq, stream)
+
+
+def pauli_zn(ptr, size, dtype, targets, controls=(), stream=0):
+    t, nt = uarr(targets)
+    c, nc = uarr(controls)
+    call("brk_pauli_zn", ptr, size, dtype, t, nt, c, nc, stream)
+
+
+def sqrt_pauli_zn(ptr, size, dtype, targets, controls=(), stream=0):
+    t, nt = uarr(targets)
+    c, nc = uarr(controls)
+    call("brk_sqrt_pauli_zn", ptr, size, dtype, t, nt, c, nc, stream)
+
+
+def adj_sqrt_pauli_zn(ptr, size, dtype, targets, controls=(), stream=0):
+    t, nt = uarr(targets)
+    c, nc = uarr(controls)
+    call("brk_adj_sqrt_pauli_zn", ptr, size, dtype, t, nt, c, nc, stream)
 
 
 def phase_shift_coeff(ptr, size, dtype, coeff, qubits, stream=0):

@@ -73,6 +73,12 @@ int run(const std::string& prefix, const char* tag) {
   r::sqrt_pauli_x(p, s, 4_q);
   r::adj_sqrt_pauli_y(p, s, 2_q, 5_cq);
   r::sqrt_pauli_z(p, s, 1_cq, 3_cq);
+  // several ket::qubit targets: products over the targets (ZZ, C(ZZZ), sZZ, C(sZ+ sZ+ sZ+))
+  r::pauli_z(p, s, 1_q, 3_q);
+  r::pauli_z(p, s, 0_q, 5_q, 6_q, 2_cq);
+  r::sqrt_pauli_z(p, s, 1_q, 3_q);
+  r::adj_sqrt_pauli_z(p, s, 0_q, 4_q, 7_q, 2_cq);
+  r::sqrt_pauli_z(p, s, 6_q, 5_cq);
   r::x_rotation_half_pi(p, s, 6_q);
   r::adj_y_rotation_half_pi(p, s, 0_q, 7_cq);
   r::controlled_v(p, s, 0.25, 3_q, 0_cq, 1_cq);

@@ -45,6 +45,11 @@ class Case:
         if k == "Z":  # Z on "control" qubits: -1 where all bits are 1
             q = t + c
             return np.diag([1.0] * (2 ** len(q) - 1) + [-1.0]).astype(np.complex128), q, []
+        if k == "ZN":  # Z on every target under the controls
+            return D.kron_all([D.Z] * m), t, c
+        if k in ("SZN", "SZN+"):  # sqrt(Z) = diag(1, i) (adjoint diag(1, -i)) on every target
+            sz = np.diag([1.0, 1j if k == "SZN" else -1j]).astype(np.complex128)
+            return D.kron_all([sz] * m), t, c
         if k == "U1coeff":
             q = t + c
             return np.diag([1.0] * (2 ** len(q) - 1) + [p[0]]).astype(np.complex128), q, []
@@ -80,6 +85,10 @@ class Case:
             o.pauli_y(s, t, c)
         elif k == "Z":
             o.pauli_z(s, t + c)
+        elif k == "ZN":
+            o.pauli_zn(s, t, c)
+        elif k in ("SZN", "SZN+"):
+            o.sqrt_pauli_zn(s, t, c, adjoint=k == "SZN+")
         elif k == "U1coeff":
             o.phase_shift_coeff(s, p[0], t + c)
         elif k == "U1":
@@ -118,6 +127,12 @@ class Case:
             brk.pauli_y(ptr, size, dtype, t, c, stream)
         elif k == "Z":
             brk.pauli_z(ptr, size, dtype, t + c, stream)
+        elif k == "ZN":
+            brk.pauli_zn(ptr, size, dtype, t, c, stream)
+        elif k == "SZN":
+            brk.sqrt_pauli_zn(ptr, size, dtype, t, c, stream)
+        elif k == "SZN+":
+            brk.adj_sqrt_pauli_zn(ptr, size, dtype, t, c, stream)
         elif k == "U1coeff":
             brk.phase_shift_coeff(ptr, size, dtype, p[0], t + c, stream)
         elif k == "U1":
@@ -167,7 +182,7 @@ REFERENCE_CASES = [
 ]
 
 KINDS_1T = ["H", "U2", "U3", "M1"]  # single target with controls
-KINDS_MT = ["X", "Y", "EX", "EY", "EZ"]  # any number of targets with controls
+KINDS_MT = ["X", "Y", "EX", "EY", "EZ", "ZN", "SZN", "SZN+"]  # any number of targets with controls
 KINDS_Q = ["Z", "U1", "U1coeff"]  # "control-type" qubits only
 
 

@@ -29,6 +29,8 @@ _SIGS = {
     "or_pauli_x": ([_p, _u64, _up, _u, _up, _u], None),
     "or_pauli_y": ([_p, _u64, _up, _u, _up, _u], None),
     "or_pauli_z": ([_p, _u64, _up, _u], None),
+    "or_pauli_zn": ([_p, _u64, _up, _u, _up, _u], None),
+    "or_sqrt_pauli_zn": ([_p, _u64, _up, _u, _up, _u, C.c_int], None),
     "or_phase_shift_coeff": ([_p, _u64, C.POINTER(C.c_double), _up, _u], None),
     "or_phase_shift2": ([_p, _u64, _d, _d, _u, _up, _u], None),
     "or_phase_shift3": ([_p, _u64, _d, _d, _d, _u, _up, _u], None),
@@ -132,6 +134,16 @@ class Oracle:
     def pauli_z(self, s, qubits):
         q, nq = _ua(qubits)
         self.lib.or_pauli_z(_ptr(s), s.size, q, nq)
+
+    def pauli_zn(self, s, targets, controls=()):
+        t, nt = _ua(targets)
+        c, nc = _ua(controls)
+        self.lib.or_pauli_zn(_ptr(s), s.size, t, nt, c, nc)
+
+    def sqrt_pauli_zn(self, s, targets, controls=(), adjoint=False):
+        t, nt = _ua(targets)
+        c, nc = _ua(controls)
+        self.lib.or_sqrt_pauli_zn(_ptr(s), s.size, t, nt, c, nc, 1 if adjoint else 0)
 
     def phase_shift_coeff(self, s, coeff, qubits):
         q, nq = _ua(qubits)

@@ -119,13 +119,27 @@ def mixed_ansatz(n, rng, layers=3):
     return ops
 
 
+@pytest.fixture(params=["temporal", "nontemporal"])
+def access(request):
+    """Default access policy, then the non-temporal instantiations large states take."""
+    if request.param == "temporal":
+        yield request.param
+        return
+    old = brk.get_tune("nt_min_bytes")
+    brk.tune("nt_min_bytes", 0)
+    try:
+        yield request.param
+    finally:
+        brk.tune("nt_min_bytes", old)
+
+
 @pytest.mark.parametrize("n,dtype", [(14, brk.C128), (13, brk.C64)])
-def test_blocked_sweep_matches_chain_sweep(n, dtype):
+def test_blocked_sweep_matches_chain_sweep(n, dtype, access):
     """The blocked reverse sweep (default) against the per-run passes (tuning adjoint_blocks = 0)."""
     rng = np.random.default_rng(20 + n)
     ops = mixed_ansatz(n, rng)
     J, h = rng.uniform(-1, 1, n - 1), rng.uniform(-1, 1, n)
-    note(f"gpu adjoint blocked vs chain n={n} dtype={dtype}")
+    note(f"gpu adjoint blocked vs chain n={n} dtype={dtype} ({access})")
     e1, g1, back1 = adjoint(n, ops, J, h, dtype)
     brk.tune("adjoint_blocks", 0)
     try:
@@ -182,6 +196,42 @@ def test_24q_12_layers_tfim():
         fd = (plain_energy(n, W.with_parameters(ops, tp), J, h) -
               plain_energy(n, W.with_parameters(ops, tm), J, h)) / 2e-5
         assert abs(g[k] - fd) < 1e-7
+
+
+@pytest.mark.slow
+@pytest.mark.timeout(600)
+def test_24q_12_layers_full_gradient_vs_parameter_shift():
+    """All 576 parameters of the differentiation workload against the parameter-shift rule
+    on the FP64 path: each shifted energy runs the circuit in gate blocks, which give the
+    per-gate path's amplitudes bit for bit."""
+    n, layers = 24, 12
+    ops, theta = W.adjoint_ansatz(n, layers, seed=5)
+    theta = np.array(theta)
+    J, h = W.tfim(n)
+    note("gpu adjoint 24q full gradient: adjoint sweep")
+    e, g, _ = adjoint(n, ops, J, h)
+    st = DeviceState(n)
+
+    def energy(values):
+        brk.set_basis_state(st.ptr, st.size, brk.C128, 0)
+        brk.Circuit(n, W.with_parameters(ops, values)).run_blocked(st.ptr)
+        return brk.expectation_ising(st.ptr, st.size, brk.C128, J, h)
+
+    try:
+        assert abs(e - energy(theta)) < 1e-10
+        shift = np.empty_like(theta)
+        for k in range(theta.size):
+            if k % 64 == 0:
+                note(f"gpu adjoint 24q full gradient: parameters {k}..{min(k + 63, theta.size - 1)}")
+            tp, tm = theta.copy(), theta.copy()
+            tp[k] += math.pi / 2
+            tm[k] -= math.pi / 2
+            shift[k] = 0.5 * (energy(tp) - energy(tm))
+    finally:
+        st.free()
+    err = float(np.max(np.abs(g - shift)))
+    note(f"gpu adjoint 24q full gradient: max |adjoint - parameter shift| over 576 = {err:.3e}")
+    assert err <= 1e-10
 
 
 def test_torch_autograd_wrapper():

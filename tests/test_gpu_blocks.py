@@ -44,11 +44,25 @@ def same_bits(x, y):
     return np.array_equal(x.view(w), y.view(w))
 
 
+@pytest.fixture(params=["temporal", "nontemporal"])
+def access(request):
+    """Default access policy, then the non-temporal instantiations large states take."""
+    if request.param == "temporal":
+        yield request.param
+        return
+    old = brk.get_tune("nt_min_bytes")
+    brk.tune("nt_min_bytes", 0)
+    try:
+        yield request.param
+    finally:
+        brk.tune("nt_min_bytes", old)
+
+
 @pytest.mark.parametrize("n,dtype,seed", [(3, brk.C128, 1), (9, brk.C128, 2), (12, brk.C128, 3), (13, brk.C128, 4),
                                           (16, brk.C128, 5), (20, brk.C128, 6), (12, brk.C64, 7), (14, brk.C64, 8),
                                           (20, brk.C64, 9)])
-def test_blocked_equals_plain_bitwise(oracle, n, dtype, seed):
-    note(f"gpu blocks n={n} dtype={dtype} seed={seed}")
+def test_blocked_equals_plain_bitwise(oracle, n, dtype, seed, access):
+    note(f"gpu blocks n={n} dtype={dtype} seed={seed} ({access})")
     ops = random_ops(n, 160, seed, local_swaps=3 if n >= 4 else 0)
     perm0 = list(np.random.default_rng(seed).permutation(n))
     s, plain, blocked, pa, pb = run_both(n, ops, dtype, perm0, seed)

@@ -31,8 +31,24 @@ def note(msg):
     sys.stdout.flush()
 
 
-def test_layered_rz_cnot_20q(oracle):
-    note("gpu circuit: layered RZ/CNOT 20q")
+@pytest.fixture(params=["temporal", "nontemporal"])
+def access(request):
+    """Runs a test twice: with the default access policy (small states use ordinary
+    loads and stores) and with the non-temporal kernel instantiations that large
+    states take (tuning nt_min_bytes = 0)."""
+    if request.param == "temporal":
+        yield request.param
+        return
+    old = brk.get_tune("nt_min_bytes")
+    brk.tune("nt_min_bytes", 0)
+    try:
+        yield request.param
+    finally:
+        brk.tune("nt_min_bytes", old)
+
+
+def test_layered_rz_cnot_20q(oracle, access):
+    note(f"gpu circuit: layered RZ/CNOT 20q ({access})")
     ops = W.layered_rz_cnot(20, 10, seed=1)
     assert len(ops) == 315
     s = np.zeros(1 << 20, dtype=np.complex128)
@@ -49,8 +65,8 @@ def test_layered_rz_cnot_20q(oracle):
 
 
 @pytest.mark.parametrize("n,seed", [(20, 2), (22, 3)])
-def test_random_circuit_with_local_swaps(oracle, n, seed):
-    note(f"gpu circuit: random n={n} seed={seed}")
+def test_random_circuit_with_local_swaps(oracle, n, seed, access):
+    note(f"gpu circuit: random n={n} seed={seed} ({access})")
     ops = W.random_circuit(n, 200, 8, seed=seed)
     rng = np.random.default_rng(seed)
     s = D.random_state(n, rng)

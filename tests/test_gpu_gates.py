@@ -137,6 +137,10 @@ def _band_cases(n):
             out.append(Case(f"X t{t} c{ctrl}", "X", [t], list(ctrl)))
             out.append(Case(f"Z t{t} c{ctrl}", "Z", [t], list(ctrl)))
             out.append(Case(f"EZ t{t} c{ctrl}", "EZ", [t], list(ctrl), (0.4,)))
+            t2 = [q for q in qs if q != t and q not in ctrl][-1:]
+            out.append(Case(f"ZN t{[t] + t2} c{ctrl}", "ZN", [t] + t2, list(ctrl)))
+            out.append(Case(f"SZN t{[t] + t2} c{ctrl}", "SZN", [t] + t2, list(ctrl)))
+            out.append(Case(f"SZN+ t{t2 + [t]} c{ctrl}", "SZN+", t2 + [t], list(ctrl)))
     for a in qs:
         for b in qs:
             if a < b:
@@ -173,3 +177,29 @@ def test_kernel_bands_c64(oracle, n):
         case.oracle(oracle, expected)
         got = run_gpu(case, s.astype(np.complex64), brk.C64).astype(np.complex128)
         assert np.max(np.abs(got - expected)) < TOL64, str(case)
+
+
+@pytest.mark.parametrize("line_bytes", [1024, 4096])
+def test_wide_line_setting_keeps_lane_kernels_in_range(oracle, line_bytes):
+    # With lines wider than a wavefront's amplitudes, targets 6.. must not reach the
+    # lane kernels (a shuffle mask of 2^6 or more would read the lane's own value).
+    n = 12
+    rng = np.random.default_rng(line_bytes)
+    s = D.random_state(n, rng)
+    expected = s.copy()
+    st = DeviceState.from_host(s)
+    brk.tune("line_bytes", line_bytes)
+    try:
+        for q in range(10):
+            note(f"gpu line_bytes={line_bytes} q={q}")
+            brk.hadamard(st.ptr, st.size, brk.C128, q)
+            oracle.hadamard(expected, q)
+            brk.phase_shift3(st.ptr, st.size, brk.C128, 0.2 * q, 0.5, -1.1, q, [(q + 5) % n])
+            oracle.phase_shift3(expected, 0.2 * q, 0.5, -1.1, q, [(q + 5) % n])
+            brk.swap(st.ptr, st.size, brk.C128, q, (q + 7) % n)
+            oracle.swap(expected, q, (q + 7) % n)
+        got = st.download()
+    finally:
+        brk.tune("line_bytes", 64)
+        st.free()
+    assert np.max(np.abs(got - expected)) < TOL128

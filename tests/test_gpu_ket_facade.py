@@ -51,6 +51,11 @@ def oracle_sequence(o):
     o.gate_1q(s, 4, [a, b, b, a])                         # sqrt(X)
     o.gate_1q(s, 2, [b, b, -b, b], [5])                   # sqrt(Y)^+
     o.phase_shift_coeff(s, 1j, [1, 3])                    # sqrt(Z) on control qubits
+    o.pauli_zn(s, [1, 3])                                 # ZZ on ket::qubit targets
+    o.pauli_zn(s, [0, 5, 6], [2])
+    o.sqrt_pauli_zn(s, [1, 3])
+    o.sqrt_pauli_zn(s, [0, 4, 7], [2], adjoint=True)
+    o.phase_shift_coeff(s, 1j, [6, 5])                    # CsZ with a target: i on |11>
     o.gate_1q(s, 6, [r, 1j * r, 1j * r, r])               # +X
     o.gate_1q(s, 0, [r, -r, r, r], [7])                   # (+Y)^+ = -Y
     o.gate_1q(s, 3, [(1 + e) / 2, (1 - e) / 2, (1 - e) / 2, (1 + e) / 2], [0, 1])  # controlled V
