@@ -382,7 +382,9 @@ typedef enum brk_op {
   BRK_OP_LOCAL_SWAP = 14,/* data swap of PHYSICAL qubits targets[0], targets[1] and
                             permutate(inverse[t0], inverse[t1]) of the permutation  */
   BRK_OP_MATRIX = 15,    /* 2x2 matrix p0..p7 (as brk_gate_1q) on targets[0]       */
-  BRK_OP_ESWAP = 16      /* exp(i p0 SWAP) on targets[0], targets[1], controls      */
+  BRK_OP_ESWAP = 16,     /* exp(i p0 SWAP) on targets[0], targets[1], controls      */
+  BRK_OP_PHASE_COEFF = 17/* p0 + i p1 where every bit of targets and controls is 1
+                            (ket::gate::phase_shift_coeff: S = sqrt(Z) with i, ...)  */
 } brk_op;
 
 int brk_circuit_create(unsigned num_qubits, brk_circuit** out);
@@ -567,9 +569,12 @@ int brk_batch_run_timed(brk_batch* batch, double* energies, double* elapsed_ms, 
 /*   mode does not permute), comments after '!', case-insensitive mnemonics;*/
 /*   gates I IC In, H, NOT, X, Y, Z, Xn/XX.., Yn/YY.., Zn/ZZ.., SWAP, ESWAP  */
 /*   ESWAP+, S S+,                                                          */
-/*   SZ SZ+, T T+, U1 U1+, U2 U2+, U3 U3+, R R+, U U+, +X -X +Y -Y, SX SX+,  */
+/*   SZ SZ+, SZZ.. SZn (+: sqrt(Z) strings), T T+, U1 U1+, U2 U2+, U3 U3+,  */
+/*   R R+, U U+, +X -X +Y -Y, SX SX+,                                       */
 /*   SY SY+, EX EY EZ (+ adjoints, EXX../EXn forms), TOFFOLI, and every one */
 /*   of them behind a C..C / Cn control prefix (controls come first);       */
+/*   DEPOLARIZING CHANNEL P_X=..,P_Y=..,P_Z=..,SEED=.. (brk_qcx_depolarizing */
+/*   and brk_qcx_gate give what a runner needs to insert the noise);        */
 /*   DO MEASUREMENT [p], BEGIN MEASUREMENT, DO AMPLITUDES [i..],            */
 /*   BEGIN FUSION / END FUSION (gates in between run in order), M, CLEAR,   */
 /*   SET, SHORBOX m divisor base, GENERATE EVENTS n seed and EXIT (reading */
@@ -596,6 +601,21 @@ typedef enum brk_qcx_kind {
 } brk_qcx_kind;
 
 int brk_qcx_parse(const char* text, brk_qcx** out);
+/* The program's DEPOLARIZING CHANNEL statement (bra/src/interpreter.cpp:431-445,
+   :1890-1975): enabled 0/1, probabilities {P_X, P_Y, P_Z} (0 when not given),
+   seed (-1 when not given; bra seeds a generator of its own only when seed > 0,
+   bra/src/bra.cpp:317-320, else the noise draws come from the state's
+   generator). */
+int brk_qcx_depolarizing(const brk_qcx* program, int* enabled, double probabilities[3], int64_t* seed);
+/* The program's gates inside gate statement `statement` (BRK_QCX_GATES): their
+   number, and for gate `gate` the ops [first_op, first_op + num_ops) of the
+   statement's circuit it became (0 ops for identity gates) and the qubits after
+   which bra::state draws a depolarizing noise (BRK_MAX_OPERATED_QUBITS entries
+   at most), in bra's order (bra/src/state.cpp:1000-3330: targets then controls;
+   operand order for the gates bra reads on "control" qubits only). */
+int brk_qcx_num_gates(const brk_qcx* program, unsigned statement, unsigned* num_gates);
+int brk_qcx_gate(const brk_qcx* program, unsigned statement, unsigned gate, unsigned* first_op, unsigned* num_ops,
+                 unsigned* noise_qubits, unsigned* num_noise_qubits);
 int brk_qcx_destroy(brk_qcx* program);
 int brk_qcx_num_qubits(const brk_qcx* program, unsigned* num_qubits);
 /* the INITIAL STATE value (0 if absent): the run starts from that basis state */

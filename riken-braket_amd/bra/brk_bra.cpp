@@ -1,6 +1,6 @@
 // brk-bra: bra's nompi front end on the MI355X engine.
 //
-//   brk-bra [--file <path>] [--threads <n>] [--seed <n>]
+//   brk-bra [--file <path>] [--threads <n>] [--seed <n>] [--amplitude-precision <p>]
 //
 // Reads a quantum-assembler program (from --file or standard input, as
 // bra/src/bra.cpp does), runs it on one GPU in complex<double> and prints what
@@ -12,10 +12,14 @@
 // each framed by the "... finished: <seconds since start> (<seconds since last>)"
 // lines; M, CLEAR and SET print nothing.  Runs of gates (inside BEGIN/END
 // FUSION or not) are applied in LDS-resident blocks, which leaves every
-// amplitude bit-identical to gate-by-gate application.  --seed seeds the state's
+// amplitude bit-identical to gate-by-gate application.  A DEPOLARIZING CHANNEL
+// statement inserts X, Y or Z after every gate (and CLEAR / SET) with bra's draws
+// (bra::state::apply_noise, bra/src/state.cpp:3331-3362).  --seed seeds the state's
 // std::mt19937_64 (default 1, bra/src/bra.cpp:73); --threads is accepted for
-// command-line compatibility.
+// command-line compatibility.  --amplitude-precision (not a bra option) sets the
+// significant digits DO AMPLITUDES prints (default: the stream's 6, as bra).
 #include <chrono>
+#include <cstdlib>
 #include <cstdio>
 #include <fstream>
 #include <iostream>
@@ -81,10 +85,11 @@ void measurement(brk::bra::hip_state& st, int precision, timer& t) {
   std::cout << oss.str() << t.lap("Expectation values") << std::flush;
 }
 
-void amplitudes(brk::bra::hip_state& st, const std::vector<std::uint64_t>& indices, timer& t) {
+void amplitudes(brk::bra::hip_state& st, const std::vector<std::uint64_t>& indices, int precision, timer& t) {
   std::cout << t.lap("Operations") << std::flush;
   const auto a = st.amplitudes(indices);
   std::ostringstream oss;
+  if (precision > 0) oss.precision(precision);
   for (size_t i = 0; i < a.size(); ++i) {
     const std::uint64_t index = indices.empty() ? i : indices[i];
     oss << bits_string(index, st.num_qubits()) << " => " << a[i].real() << " + " << a[i].imag() << " i\n";
@@ -109,7 +114,7 @@ void exit_statement(brk::bra::hip_state& st, timer& t) {
 }
 
 int usage() {
-  std::cerr << "usage: brk-bra [--file <path>] [--threads <n>] [--seed <n>]\n";
+  std::cerr << "usage: brk-bra [--file <path>] [--threads <n>] [--seed <n>] [--amplitude-precision <p>]\n";
   return 2;
 }
 
@@ -118,10 +123,15 @@ int usage() {
 int main(int argc, char** argv) {
   std::string file;
   brk::bra::hip_state::seed_type seed = 1;
+  int amplitude_precision = 0;
   for (int i = 1; i < argc; ++i) {
     const std::string a = argv[i];
     if ((a == "--file" || a == "-f") && i + 1 < argc) file = argv[++i];
     else if (a == "--threads" && i + 1 < argc) ++i;
+    else if (a == "--amplitude-precision" && i + 1 < argc) {
+      amplitude_precision = std::atoi(argv[++i]);
+      if (amplitude_precision < 1 || amplitude_precision > 17) return usage();
+    }
     else if (a == "--seed" && i + 1 < argc) {
       try {
         size_t used = 0;
@@ -158,6 +168,7 @@ int main(int argc, char** argv) {
     brk::bra::check(brk_qcx_num_statements(prog, &count));
     timer t;
     brk::bra::hip_state st(n, initial, seed);
+    st.set_depolarizing_channel(prog);
     for (unsigned i = 0; i < count; ++i) {
       int kind = 0, precision = 3;
       const brk_circuit* gates = nullptr;
@@ -165,12 +176,20 @@ int main(int argc, char** argv) {
       const std::uint64_t* idx = nullptr;
       brk::bra::check(brk_qcx_statement(prog, i, &kind, &gates, &precision, &nidx, &idx));
       switch (kind) {
-        case BRK_QCX_GATES: st.apply_blocked(gates); break;
+        case BRK_QCX_GATES: st.apply_blocked(prog, i); break;
         case BRK_QCX_MEASUREMENT: measurement(st, precision, t); break;
-        case BRK_QCX_AMPLITUDES: amplitudes(st, std::vector<std::uint64_t>(idx, idx + nidx), t); break;
+        case BRK_QCX_AMPLITUDES:
+          amplitudes(st, std::vector<std::uint64_t>(idx, idx + nidx), amplitude_precision, t);
+          break;
         case BRK_QCX_PROJECTIVE_MEASUREMENT: st.do_projective_measurement((unsigned)idx[0]); break;
-        case BRK_QCX_CLEAR: st.do_clear((unsigned)idx[0]); break;
-        case BRK_QCX_SET: st.do_set((unsigned)idx[0]); break;
+        case BRK_QCX_CLEAR:
+          st.do_clear((unsigned)idx[0]);
+          st.apply_noise((unsigned)idx[0]);
+          break;
+        case BRK_QCX_SET:
+          st.do_set((unsigned)idx[0]);
+          st.apply_noise((unsigned)idx[0]);
+          break;
         case BRK_QCX_GENERATE_EVENTS: generate_events(st, (int)idx[0], precision, t); break;
         case BRK_QCX_EXIT: exit_statement(st, t); break;
         case BRK_QCX_SHOR_BOX: st.do_shor_box((unsigned)idx[0], idx[1], idx[2]); break;

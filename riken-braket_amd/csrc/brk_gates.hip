@@ -1329,6 +1329,7 @@ int describe_physical_op(int code, const unsigned* t, unsigned nt, const unsigne
       diag(tmask | cmask, 0, e, e);
       break;
     }
+    case BRK_OP_PHASE_COEFF: diag(tmask | cmask, 0, {p[0], p[1]}, {p[0], p[1]}); break;
     case BRK_OP_U2: pair(BLOCK_MATRIX); coeffs_u2(p[0], p[1], d->u); break;
     case BRK_OP_U3: pair(BLOCK_MATRIX); coeffs_u3(p[0], p[1], p[2], d->u); break;
     case BRK_OP_MATRIX: pair(BLOCK_MATRIX); std::copy(p, p + 8, d->u); break;

@@ -108,6 +108,7 @@ static brk_circuit::op inverse_op(const brk_circuit::op& o) {
       r.params[1] = -o.params[1];
       r.params[2] = -o.params[0];
       break;
+    case BRK_OP_PHASE_COEFF: r.params[1] = -o.params[1]; break;  // the conjugate coefficient
     case BRK_OP_MATRIX:  // conjugate transpose
       for (int rr = 0; rr < 2; ++rr)
         for (int c = 0; c < 2; ++c) {

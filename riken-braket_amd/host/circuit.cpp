@@ -40,7 +40,7 @@ static unsigned expected_params(int code) {
   switch (code) {
     case BRK_OP_U1: case BRK_OP_EX: case BRK_OP_EY: case BRK_OP_EZ:
     case BRK_OP_RX: case BRK_OP_RY: case BRK_OP_RZ: case BRK_OP_ESWAP: return 1;
-    case BRK_OP_U2: return 2;
+    case BRK_OP_U2: case BRK_OP_PHASE_COEFF: return 2;
     case BRK_OP_U3: return 3;
     case BRK_OP_MATRIX: return 8;
     default: return 0;
@@ -58,12 +58,14 @@ int apply_physical_op(const brk_circuit::op& o, void* first, uint64_t size, int 
     case BRK_OP_X: return brk_pauli_x(first, size, dtype, t, nt, c, nc, stream);
     case BRK_OP_Y: return brk_pauli_y(first, size, dtype, t, nt, c, nc, stream);
     case BRK_OP_Z:
-    case BRK_OP_U1: {
+    case BRK_OP_U1:
+    case BRK_OP_PHASE_COEFF: {
       unsigned all[BRK_MAX_OPERATED_QUBITS];
       unsigned k = 0;
       for (unsigned i = 0; i < nt && k < BRK_MAX_OPERATED_QUBITS; ++i) all[k++] = t[i];
       for (unsigned i = 0; i < nc && k < BRK_MAX_OPERATED_QUBITS; ++i) all[k++] = c[i];
       if (o.code == BRK_OP_Z) return brk_pauli_z(first, size, dtype, all, k, stream);
+      if (o.code == BRK_OP_PHASE_COEFF) return brk_phase_shift_coeff(first, size, dtype, p, all, k, stream);
       return brk_phase_shift(first, size, dtype, p[0], all, k, stream);
     }
     case BRK_OP_U2: return brk_phase_shift2(first, size, dtype, p[0], p[1], t[0], c, nc, stream);
@@ -123,13 +125,15 @@ bool op_unitary(const brk_circuit::op& o, std::vector<unsigned>& qubits, std::ve
     case BRK_OP_X: pauli_string_matrix(X, m, core); break;
     case BRK_OP_Y: pauli_string_matrix(Y, m, core); break;
     case BRK_OP_Z:
-    case BRK_OP_U1: {
+    case BRK_OP_U1:
+    case BRK_OP_PHASE_COEFF: {
       // acts on targets ++ controls jointly
       core_bits = m + (unsigned)o.controls.size();
       const unsigned d = 1u << core_bits;
       core.assign((size_t)d * d, cd{0, 0});
       for (unsigned r = 0; r < d; ++r) core[(size_t)r * d + r] = 1.0;
-      core[(size_t)(d - 1) * d + (d - 1)] = o.code == BRK_OP_Z ? cd(-1, 0) : std::exp(I * p[0]);
+      core[(size_t)(d - 1) * d + (d - 1)] =
+          o.code == BRK_OP_Z ? cd(-1, 0) : o.code == BRK_OP_U1 ? std::exp(I * p[0]) : cd(p[0], p[1]);
       break;
     }
     case BRK_OP_U2: {
@@ -205,7 +209,7 @@ int brk_circuit_destroy(brk_circuit* circuit) {
 int brk_circuit_add(brk_circuit* circuit, int op, const unsigned* targets, unsigned num_targets,
                     const unsigned* controls, unsigned num_controls, const double* params, unsigned num_params) {
   if (!circuit) return fail(BRK_ERR_INVALID_ARGUMENT, "circuit is null");
-  if (op < BRK_OP_H || op > BRK_OP_ESWAP) return fail(BRK_ERR_INVALID_ARGUMENT, "unknown op code %d", op);
+  if (op < BRK_OP_H || op > BRK_OP_PHASE_COEFF) return fail(BRK_ERR_INVALID_ARGUMENT, "unknown op code %d", op);
   if (num_targets == 0 || !targets) return fail(BRK_ERR_INVALID_ARGUMENT, "op %d has no targets", op);
   if (num_controls && !controls) return fail(BRK_ERR_INVALID_ARGUMENT, "controls is null");
   const unsigned need = expected_params(op);

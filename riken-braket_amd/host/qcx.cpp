@@ -12,6 +12,7 @@
 // brk_circuit, in program order.
 #include "circuit.hpp"
 
+#include <algorithm>
 #include <cctype>
 #include <cerrno>
 #include <cmath>
@@ -26,15 +27,26 @@ int fail(int status, const char* fmt, ...) __attribute__((format(printf, 2, 3)))
 }
 
 struct brk_qcx {
+  // one gate of the program: the ops it became in its statement's circuit and the
+  // qubits bra draws depolarizing noise for after it, in bra's order
+  struct gate_record {
+    unsigned first_op = 0, num_ops = 0;
+    std::vector<unsigned> noise;
+  };
   struct statement {
     int kind = BRK_QCX_GATES;
     brk_circuit* gates = nullptr;
     int precision = 3;
     std::vector<uint64_t> indices;
+    std::vector<gate_record> records;  // kind BRK_QCX_GATES
   };
   unsigned num_qubits = 0;
   uint64_t initial_state = 0;
   std::vector<statement> statements;
+  // DEPOLARIZING CHANNEL (bra/src/interpreter.cpp:431-445, :1890-1975)
+  bool depolarizing = false;
+  double px = 0.0, py = 0.0, pz = 0.0;
+  long long depolarizing_seed = -1;
   ~brk_qcx() {
     for (auto& s : statements)
       if (s.gates) brk_circuit_destroy(s.gates);
@@ -93,6 +105,7 @@ class parser {
       return;
     }
     if (m == "BIT") return bit_assignment();
+    if (m == "DEPOLARIZING") return depolarizing_statement();
     if (m == "DO") return do_statement();
     if (m == "BEGIN") return begin_statement();
     if (m == "END") return end_statement();
@@ -117,6 +130,8 @@ class parser {
   size_t line_ = 0;
   std::vector<std::string> cols_;
   brk_circuit* current_ = nullptr;
+  std::vector<brk_qcx::gate_record> records_;  // of current_
+  std::vector<unsigned> noise_;                // the gate being parsed: its noise qubits
   bool in_fusion_ = false, stopped_ = false;
 
   [[noreturn]] void error(const std::string& what) const {
@@ -178,9 +193,53 @@ class parser {
       brk_qcx::statement s;
       s.kind = BRK_QCX_GATES;
       s.gates = current_;
+      s.records = std::move(records_);
       prog_.statements.push_back(std::move(s));
       current_ = nullptr;
+      records_.clear();
     }
+  }
+
+  // DEPOLARIZING CHANNEL P_X=px, P_Y=py, P_Z=pz, SEED=seed: keys in any order and case, each
+  // at most once, '=' and ',' with or without spaces around them (interpreter.hpp:518-600);
+  // probabilities in [0, 1] with px + py + pz <= 1; missing ones are 0, a missing seed -1.
+  void depolarizing_statement() {
+    if (cols_.size() <= 2 || upper(cols_[1]) != "CHANNEL") error("expected DEPOLARIZING CHANNEL P_X=..,P_Y=..,P_Z=..,SEED=..");
+    std::string text;
+    for (size_t i = 2; i < cols_.size(); ++i) text += cols_[i];
+    double p[3] = {0.0, 0.0, 0.0};
+    long long seed = -1;
+    bool seen[4] = {false, false, false, false};
+    size_t at = 0;
+    while (true) {
+      const size_t comma = text.find(',', at);
+      const std::string item = text.substr(at, comma == std::string::npos ? std::string::npos : comma - at);
+      const size_t eq = item.find('=');
+      if (eq == std::string::npos || eq + 1 >= item.size()) error("DEPOLARIZING CHANNEL: expected KEY=value, got '" + item + "'");
+      const std::string key = upper(item.substr(0, eq)), value = item.substr(eq + 1);
+      const int slot = key == "P_X" ? 0 : key == "P_Y" ? 1 : key == "P_Z" ? 2 : key == "SEED" ? 3 : -1;
+      if (slot < 0) error("DEPOLARIZING CHANNEL: unknown key '" + key + "'");
+      if (seen[slot]) error("DEPOLARIZING CHANNEL: " + key + " given twice");
+      seen[slot] = true;
+      char* end = nullptr;
+      errno = 0;
+      if (slot == 3) {
+        seed = std::strtoll(value.c_str(), &end, 10);
+        if (errno || *end || seed < -0x80000000ll || seed > 0x7fffffffll) error("DEPOLARIZING CHANNEL: bad SEED '" + value + "'");
+      } else {
+        p[slot] = std::strtod(value.c_str(), &end);
+        if (errno || *end || !(p[slot] >= 0.0 && p[slot] <= 1.0))
+          error("DEPOLARIZING CHANNEL: " + key + " must be a probability, got '" + value + "'");
+      }
+      if (comma == std::string::npos) break;
+      at = comma + 1;
+    }
+    if (p[0] + p[1] + p[2] > 1.0) error("DEPOLARIZING CHANNEL: P_X + P_Y + P_Z exceeds 1");
+    prog_.depolarizing = true;
+    prog_.px = p[0];
+    prog_.py = p[1];
+    prog_.pz = p[2];
+    prog_.depolarizing_seed = seed;
   }
   void directive(int kind, int precision = 3, std::vector<uint64_t> indices = {}) {
     flush();
@@ -315,7 +374,40 @@ class parser {
       }
       if (nc == 0 || base.empty()) unsupported("mnemonic " + m);
     }
+    if (!current_ && brk_circuit_create(prog_.num_qubits, &current_) != BRK_OK) error("cannot create a circuit");
+    unsigned before = 0;
+    brk_circuit_num_ops(current_, &before);
+    noise_.clear();
     apply_base(m, base, nc);
+    unsigned after = 0;
+    brk_circuit_num_ops(current_, &after);
+    brk_qcx::gate_record r;
+    r.first_op = before;
+    r.num_ops = after - before;
+    r.noise = noise_;
+    records_.push_back(std::move(r));
+  }
+
+  // Noise qubits of a gate, in the order bra::state draws them (bra/src/state.cpp:1000-3330,
+  // apply_noises(targets..., controls...)): targets then controls, or, for the gates bra
+  // reads as "control" qubits only (Z, CZ, S, T, U1, R, U and their controlled forms), all
+  // operands in program order.
+  void noise_targets_then_controls(const std::vector<unsigned>& t, const std::vector<unsigned>& c) {
+    noise_ = t;
+    noise_.insert(noise_.end(), c.begin(), c.end());
+  }
+  void noise_operand_order(const std::vector<unsigned>& c, const std::vector<unsigned>& t) {
+    noise_ = c;
+    noise_.insert(noise_.end(), t.begin(), t.end());
+  }
+
+  // sqrt(Z) strings: S followed by k >= 2 Z's or by Z<k>, optionally '+' (the adjoint).
+  static size_t sqrt_z_targets(const std::string& base, bool* adjoint) {
+    std::string core = base;
+    *adjoint = !core.empty() && core.back() == '+';
+    if (*adjoint) core.pop_back();
+    if (core.size() < 2 || core[0] != 'S' || core[1] != 'Z') return 0;
+    return multiplicity(core, 1, 'Z');
   }
 
   void apply_base(const std::string& m, const std::string& base, size_t nc) {
@@ -325,16 +417,20 @@ class parser {
     if (base == "I" || base == "IC") {
       expect_columns(2 + nc);
       operands(nc, 1, c, t);
+      if (base == "I") noise_targets_then_controls(t, c);
+      else noise_operand_order(c, t);
       return;
     }
     if (size_t k = multiplicity(base, 0, 'I'); k >= 2) {
       expect_columns(1 + nc + k);
       operands(nc, k, c, t);
+      noise_targets_then_controls(t, c);
       return;
     }
     if (base == "H") {
       expect_columns(2 + nc);
       operands(nc, 1, c, t);
+      noise_targets_then_controls(t, c);
       return add(BRK_OP_H, t, c);
     }
     if (base == "NOT" || base == "TOFFOLI") {
@@ -342,12 +438,14 @@ class parser {
       if (base == "TOFFOLI" && nc) unsupported("controlled TOFFOLI");
       expect_columns(2 + nc + extra);
       operands(nc + extra, 1, c, t);
+      noise_targets_then_controls(t, c);
       return add(BRK_OP_X, t, c);
     }
     for (const char letter : {'X', 'Y'}) {
       if (size_t k = multiplicity(base, 0, letter); k >= 1) {
         expect_columns(1 + nc + k);
         operands(nc, k, c, t);
+        noise_targets_then_controls(t, c);
         return add(letter == 'X' ? BRK_OP_X : BRK_OP_Y, t, c);
       }
     }
@@ -357,42 +455,69 @@ class parser {
       const size_t q = nc == 0 ? 1 : nc + 1;
       expect_columns(1 + q);
       operands(0, q, c, t);
+      noise_operand_order({}, t);
       return add(BRK_OP_Z, t, {});
     }
     if (size_t k = multiplicity(base, 0, 'Z'); k >= 2) {
       expect_columns(1 + nc + k);
       operands(nc, k, c, t);
+      noise_targets_then_controls(t, c);
       for (unsigned q : t) add(BRK_OP_Z, {q}, c);  // Z..Z = product of single-qubit Z's
       return;
+    }
+    // sqrt(Z) strings SZZ, SZZZ.., SZ<n> and their adjoints (interpreter.cpp:808-819,
+    // 2553-2650; controlled: 3494-3505, 4754-4870; docs/bra.md:133): i (or -i) per target
+    // that is 1, where every control is 1 -- a product of per-target phases, each exactly
+    // the coefficient i (ket::utility::imaginary_unit, sqrt_pauli_z.hpp:462-700).  The
+    // reference's SZ<n> readers check their digits from the 'Z' on and so reject every such
+    // mnemonic; this parser takes them as docs/bra.md:133 describes them (C3SZ3 c1 c2 c3 t1 t2 t3).
+    {
+      bool adjoint = false;
+      if (size_t k = sqrt_z_targets(base, &adjoint); k >= 2) {
+        expect_columns(1 + nc + k);
+        operands(nc, k, c, t);
+        noise_targets_then_controls(t, c);
+        for (unsigned q : t) add(BRK_OP_PHASE_COEFF, {q}, c, {0.0, adjoint ? -1.0 : 1.0});
+        return;
+      }
     }
     if (base == "SWAP") {
       expect_columns(3 + nc);
       operands(nc, 2, c, t);
+      noise_targets_then_controls(t, c);
       return add(BRK_OP_SWAP, t, c);
     }
     // ESWAP t1 t2 theta = exp(i theta SWAP); ESWAP+ its adjoint (interpreter.cpp:3169-3187)
     if (base == "ESWAP" || base == "ESWAP+") {
       expect_columns(4 + nc);
       operands(nc, 2, c, t);
+      noise_targets_then_controls(t, c);
       const double theta = real(3 + nc);
       return add(BRK_OP_ESWAP, t, c, {base == "ESWAP" ? theta : -theta});
     }
-    // phases on control qubits: the controls of the prefix and the operand are all "control" qubits
-    struct phase_gate {
-      const char* name;
-      double phase;
-    };
-    static const phase_gate phases[] = {{"S", kPi / 2}, {"SZ", kPi / 2}, {"S+", -kPi / 2}, {"SZ+", -kPi / 2},
-                                        {"T", kPi / 4}, {"T+", -kPi / 4}};
-    for (const auto& g : phases)
-      if (base == g.name) {
+    // sqrt(Z) and its adjoint on control qubits: S, SZ, SZ1 (+) -- exactly i / -i
+    // (ket::gate::sqrt_pauli_z, sqrt_pauli_z.hpp:35-420)
+    {
+      bool adjoint = false;
+      if (base == "S" || base == "S+" || sqrt_z_targets(base, &adjoint) == 1) {
+        adjoint = base.back() == '+';
         expect_columns(2 + nc);
         operands(nc, 1, c, t);
-        return add(BRK_OP_U1, t, c, {g.phase});
+        noise_operand_order(c, t);
+        return add(BRK_OP_PHASE_COEFF, t, c, {0.0, adjoint ? -1.0 : 1.0});
       }
+    }
+    // T = phase_shift(3): exp(2 pi i / 2^3) on control qubits (bra/src/t_gate.cpp, state.cpp:1405-1421)
+    if (base == "T" || base == "T+") {
+      expect_columns(2 + nc);
+      operands(nc, 1, c, t);
+      noise_operand_order(c, t);
+      return add(BRK_OP_U1, t, c, {base == "T" ? kPi / 4 : -kPi / 4});
+    }
     if (base == "U1" || base == "U1+") {
       expect_columns(3 + nc);
       operands(nc, 1, c, t);
+      noise_operand_order(c, t);
       const double lambda = real(2 + nc);
       return add(BRK_OP_U1, t, c, {base == "U1" ? lambda : -lambda});
     }
@@ -403,6 +528,7 @@ class parser {
       operands(0, q, c, t);
       const long long k = integer(1 + q);
       if (k < -62 || k > 62) error("phase exponent out of range");
+      noise_operand_order({}, t);
       const unsigned long long kk = (unsigned long long)(k < 0 ? -k : k);
       double phase = 2.0 * kPi / (double)(1ull << kk);
       if ((k < 0) != (base.back() == '+')) phase = -phase;
@@ -411,6 +537,7 @@ class parser {
     if (base == "U2" || base == "U2+") {
       expect_columns(4 + nc);
       operands(nc, 1, c, t);
+      noise_targets_then_controls(t, c);
       const double phi = real(2 + nc), lambda = real(3 + nc);
       if (base == "U2") return add(BRK_OP_U2, t, c, {phi, lambda});
       return add(BRK_OP_U3, t, c, {-kPi / 2, -lambda, -phi});  // U2(phi, lambda)^dagger
@@ -418,6 +545,7 @@ class parser {
     if (base == "U3" || base == "U3+") {
       expect_columns(5 + nc);
       operands(nc, 1, c, t);
+      noise_targets_then_controls(t, c);
       const double theta = real(2 + nc), phi = real(3 + nc), lambda = real(4 + nc);
       if (base == "U3") return add(BRK_OP_U3, t, c, {theta, phi, lambda});
       return add(BRK_OP_U3, t, c, {-theta, -lambda, -phi});
@@ -438,6 +566,7 @@ class parser {
       if (base == g.name) {
         expect_columns(2 + nc);
         operands(nc, 1, c, t);
+        noise_targets_then_controls(t, c);
         return add_matrix(t[0], c, g.m);
       }
     // exponentials of Pauli strings: EX t theta, EXX t1 t2 theta, EX3 ..., and their adjoints
@@ -449,6 +578,7 @@ class parser {
         if (k >= 1) {
           expect_columns(2 + nc + k);
           operands(nc, k, c, t);
+          noise_targets_then_controls(t, c);
           const double theta = real(1 + nc + k);
           const int op = letter == 'X' ? BRK_OP_EX : letter == 'Y' ? BRK_OP_EY : BRK_OP_EZ;
           return add(op, t, c, {adjoint ? -theta : theta});
@@ -518,6 +648,41 @@ int brk_qcx_statement(const brk_qcx* program, unsigned i, int* kind, const brk_c
   if (precision) *precision = s.precision;
   if (num_indices) *num_indices = (unsigned)s.indices.size();
   if (indices) *indices = s.indices.empty() ? nullptr : s.indices.data();
+  return BRK_OK;
+}
+
+int brk_qcx_depolarizing(const brk_qcx* program, int* enabled, double probabilities[3], int64_t* seed) {
+  if (!program) return fail(BRK_ERR_INVALID_ARGUMENT, "program is null");
+  if (enabled) *enabled = program->depolarizing ? 1 : 0;
+  if (probabilities) {
+    probabilities[0] = program->px;
+    probabilities[1] = program->py;
+    probabilities[2] = program->pz;
+  }
+  if (seed) *seed = program->depolarizing_seed;
+  return BRK_OK;
+}
+
+int brk_qcx_num_gates(const brk_qcx* program, unsigned statement, unsigned* num_gates) {
+  if (!program || !num_gates) return fail(BRK_ERR_INVALID_ARGUMENT, "null argument");
+  if (statement >= program->statements.size())
+    return fail(BRK_ERR_INVALID_ARGUMENT, "statement %u out of range", statement);
+  *num_gates = (unsigned)program->statements[statement].records.size();
+  return BRK_OK;
+}
+
+int brk_qcx_gate(const brk_qcx* program, unsigned statement, unsigned gate, unsigned* first_op, unsigned* num_ops,
+                 unsigned* noise_qubits, unsigned* num_noise_qubits) {
+  if (!program) return fail(BRK_ERR_INVALID_ARGUMENT, "program is null");
+  if (statement >= program->statements.size())
+    return fail(BRK_ERR_INVALID_ARGUMENT, "statement %u out of range", statement);
+  const auto& records = program->statements[statement].records;
+  if (gate >= records.size()) return fail(BRK_ERR_INVALID_ARGUMENT, "gate %u out of range", gate);
+  const auto& r = records[gate];
+  if (first_op) *first_op = r.first_op;
+  if (num_ops) *num_ops = r.num_ops;
+  if (num_noise_qubits) *num_noise_qubits = (unsigned)r.noise.size();
+  if (noise_qubits) std::copy(r.noise.begin(), r.noise.end(), noise_qubits);
   return BRK_OK;
 }
 

@@ -130,6 +130,83 @@ class hip_state {
     check(brk_circuit_run_blocked(circuit, data_, BRK_C128, nullptr, stream_));
   }
 
+  // --- depolarizing noise (bra::state::apply_noise, bra/src/state.cpp:3331-3362) ---------
+  // After every gate bra draws u from uniform_real_distribution<double>{0, 1} once per
+  // operated qubit and inserts X if u < px, Y if u < px + py, Z (on the qubit as a control)
+  // if u < px + py + pz.  The draws take a generator of their own when the program's SEED is
+  // positive, else the state's generator, which measurements share
+  // (bra/src/bra.cpp:317-320, bra/src/nompi_state.cpp:168-177).
+  void set_depolarizing_channel(double px, double py, double pz, bool uses_seed, seed_type seed) {
+    noisy_ = true;
+    px_ = px;
+    py_ = py;
+    pz_ = pz;
+    uses_depolarizing_seed_ = uses_seed;
+    depolarizing_rng_.seed(uses_seed ? seed : seed_type{});
+  }
+  // the program's DEPOLARIZING CHANNEL statement, if it has one
+  void set_depolarizing_channel(const brk_qcx* program) {
+    int enabled = 0;
+    double p[3];
+    std::int64_t seed = -1;
+    check(brk_qcx_depolarizing(program, &enabled, p, &seed));
+    if (enabled) set_depolarizing_channel(p[0], p[1], p[2], seed > 0, static_cast<seed_type>(seed));
+  }
+  // one noise draw: BRK_OP_X, BRK_OP_Y, BRK_OP_Z or -1 for none
+  int draw_noise() {
+    std::uniform_real_distribution<double> distribution{0.0, 1.0};
+    const double u = uses_depolarizing_seed_ ? distribution(depolarizing_rng_) : distribution(rng_);
+    if (u < px_) return BRK_OP_X;
+    if (u < px_ + py_) return BRK_OP_Y;
+    if (u < px_ + py_ + pz_) return BRK_OP_Z;
+    return -1;
+  }
+  // the noise after a gate on `qubit`, applied now (a CLEAR or SET statement)
+  void apply_noise(unsigned qubit) {
+    if (!noisy_) return;
+    switch (draw_noise()) {
+      case BRK_OP_X: check(brk_pauli_x(data_, size_, BRK_C128, &qubit, 1, nullptr, 0, stream_)); break;
+      case BRK_OP_Y: check(brk_pauli_y(data_, size_, BRK_C128, &qubit, 1, nullptr, 0, stream_)); break;
+      case BRK_OP_Z: check(brk_pauli_z(data_, size_, BRK_C128, &qubit, 1, stream_)); break;
+      default: break;
+    }
+  }
+  // Gate statement `statement` of a parsed program in blocks.  With a depolarizing channel
+  // the noise gates bra would insert after each gate are drawn now, in bra's order, and join
+  // the circuit, so the run keeps its gate blocks.
+  void apply_blocked(const brk_qcx* program, unsigned statement) {
+    int kind = 0;
+    const brk_circuit* gates = nullptr;
+    check(brk_qcx_statement(program, statement, &kind, &gates, nullptr, nullptr, nullptr));
+    if (kind != BRK_QCX_GATES) throw error(BRK_ERR_INVALID_ARGUMENT, "statement is not a run of gates");
+    if (!noisy_) return apply_blocked(gates);
+    unsigned num_gates = 0, nq = 0;
+    check(brk_qcx_num_gates(program, statement, &num_gates));
+    check(brk_circuit_num_qubits(gates, &nq));
+    brk_circuit* noisy = nullptr;
+    check(brk_circuit_create(nq, &noisy));
+    struct destroy {
+      brk_circuit* c;
+      ~destroy() { brk_circuit_destroy(c); }
+    } guard{noisy};
+    for (unsigned g = 0; g < num_gates; ++g) {
+      unsigned first = 0, count = 0, noise[BRK_MAX_OPERATED_QUBITS], num_noise = 0;
+      check(brk_qcx_gate(program, statement, g, &first, &count, noise, &num_noise));
+      for (unsigned i = first; i < first + count; ++i) {
+        int op = 0;
+        unsigned t[BRK_MAX_OPERATED_QUBITS], c[BRK_MAX_OPERATED_QUBITS], nt = 0, nc = 0, np = 0;
+        double p[8];
+        check(brk_circuit_op(gates, i, &op, t, &nt, c, &nc, p, &np));
+        check(brk_circuit_add(noisy, op, t, nt, nc ? c : nullptr, nc, np ? p : nullptr, np));
+      }
+      for (unsigned k = 0; k < num_noise; ++k) {
+        const int op = draw_noise();
+        if (op >= 0) check(brk_circuit_add(noisy, op, &noise[k], 1, nullptr, 0, nullptr, 0));
+      }
+    }
+    apply_blocked(noisy);
+  }
+
   // bra::state::shor_box: exponent qubits are the top m, the rest hold base^x mod divisor
   // (bra/src/state.cpp:2173-2191, bra/src/nompi_state.cpp:1276-1280)
   void do_shor_box(unsigned num_exponent_qubits, std::uint64_t divisor, std::uint64_t base) {
@@ -220,6 +297,9 @@ class hip_state {
   void* stream_;
   random_number_generator_type rng_;
   void* data_ = nullptr;
+  bool noisy_ = false, uses_depolarizing_seed_ = false;
+  double px_ = 0.0, py_ = 0.0, pz_ = 0.0;
+  random_number_generator_type depolarizing_rng_;
 };
 
 }  // namespace bra

@@ -129,6 +129,9 @@ _SIGS = {
     "brk_qcx_num_qubits": [_p, _up],
     "brk_qcx_initial_state": [_p, C.POINTER(_u64)],
     "brk_qcx_num_statements": [_p, _up],
+    "brk_qcx_depolarizing": [_p, C.POINTER(C.c_int), _dp, C.POINTER(C.c_int64)],
+    "brk_qcx_num_gates": [_p, _u, _up],
+    "brk_qcx_gate": [_p, _u, _u, _up, _up, _up, _up],
     "brk_qcx_statement": [_p, _u, C.POINTER(C.c_int), C.POINTER(C.c_void_p), C.POINTER(C.c_int), _up,
                           C.POINTER(C.POINTER(_u64))],
     "brk_batch_create": [C.POINTER(C.c_void_p), _u, C.POINTER(_dp), C.POINTER(_dp), C.POINTER(C.c_void_p)],
@@ -406,7 +409,7 @@ def convert(dst, dt_dst, src, dt_src, size, stream=0):
 # ---------------------------------------------------------------------------
 
 OP = dict(H=0, X=1, Y=2, Z=3, U1=4, U2=5, U3=6, EX=7, EY=8, EZ=9, RX=10, RY=11, RZ=12, SWAP=13,
-          LOCAL_SWAP=14, MATRIX=15, ESWAP=16)
+          LOCAL_SWAP=14, MATRIX=15, ESWAP=16, PHASE=17)
 
 
 class BlockPlan:
@@ -636,7 +639,7 @@ class Batch:
 
 
 OP_NAMES = ("H", "X", "Y", "Z", "U1", "U2", "U3", "EX", "EY", "EZ", "RX", "RY", "RZ", "SWAP", "LOCAL_SWAP", "MATRIX",
-            "ESWAP")
+            "ESWAP", "PHASE")
 
 
 def circuit_ops(handle):
@@ -670,6 +673,10 @@ class Qcx:
         call("brk_qcx_num_statements", self.h, C.byref(cnt))
         self.num_qubits, self.initial_state = n.value, init.value
         self.statements = []
+        self.gate_records = {}  # statement -> [(first_op, num_ops, noise qubits)] per program gate
+        en, probs, seed = C.c_int(), (C.c_double * 3)(), C.c_int64()
+        call("brk_qcx_depolarizing", self.h, C.byref(en), probs, C.byref(seed))
+        self.depolarizing = (list(probs), seed.value) if en.value else None
         for i in range(cnt.value):
             kind, prec, nidx = C.c_int(), C.c_int(), C.c_uint()
             gates = C.c_void_p()
@@ -678,6 +685,15 @@ class Qcx:
                  C.byref(idx))
             if kind.value == QCX_GATES:
                 self.statements.append((kind.value, circuit_ops(gates.value)))
+                ng = C.c_uint()
+                call("brk_qcx_num_gates", self.h, i, C.byref(ng))
+                records = []
+                for g in range(ng.value):
+                    first, count, nn = C.c_uint(), C.c_uint(), C.c_uint()
+                    noise = (C.c_uint * 16)()
+                    call("brk_qcx_gate", self.h, i, g, C.byref(first), C.byref(count), noise, C.byref(nn))
+                    records.append((first.value, count.value, list(noise[: nn.value])))
+                self.gate_records[i] = records
             elif kind.value == QCX_MEASUREMENT:
                 self.statements.append((kind.value, prec.value))
             elif kind.value == QCX_AMPLITUDES:

@@ -27,6 +27,8 @@ def apply_op(o, s, op, perm):
         o.pauli_z(s, pt + pc)
     elif name == "U1":
         o.phase_shift(s, p[0], pt + pc)
+    elif name == "PHASE":
+        o.phase_shift_coeff(s, complex(p[0], p[1]), pt + pc)
     elif name == "U2":
         o.phase_shift2(s, p[0], p[1], pt[0], pc)
     elif name == "U3":

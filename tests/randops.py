@@ -9,7 +9,7 @@ import random
 
 ONE_TARGET = ("H", "U2", "U3", "MATRIX")
 MULTI_TARGET = ("X", "Y", "EX", "EY", "EZ", "RX", "RY", "RZ")
-QUBITS_ONLY = ("Z", "U1")
+QUBITS_ONLY = ("Z", "U1", "PHASE")
 PAIR = ("SWAP", "ESWAP")
 KINDS = ONE_TARGET + MULTI_TARGET + QUBITS_ONLY + PAIR
 
@@ -24,6 +24,9 @@ def _params(rng, name):
         return (a(), a(), a())
     if name == "MATRIX":
         return tuple(rng.gauss(0, 0.7) for _ in range(8))
+    if name == "PHASE":  # a unit coefficient, exactly i or -i half of the time (S, S+)
+        t = rng.choice([math.pi / 2, -math.pi / 2, a()])
+        return (0.0, math.copysign(1.0, t)) if abs(abs(t) - math.pi / 2) < 1e-15 else (math.cos(t), math.sin(t))
     return ()
 
 

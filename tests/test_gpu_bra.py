@@ -1,7 +1,9 @@
-"""GPU checks of brk-bra, bra's nompi front end on the engine: the reference's
-sample programs and synthetic ones are run by the binary and its printed
-expectation values / amplitudes are compared with the CPU oracle running the
-same parsed gates (and with closed forms where they exist)."""
+"""GPU checks of brk-bra, bra's nompi front end on the engine: this repository's
+test programs (tests/golden/qcx/*.qcx) and synthetic ones are run by the binary
+and its printed expectation values / amplitudes (17 significant digits:
+DO MEASUREMENT -17, --amplitude-precision 17) are compared with the CPU oracle
+running the same parsed gates at 1e-12, with bra's random draws replayed for
+measurements and depolarizing noise (and with closed forms where they exist)."""
 import os
 import re
 import subprocess
@@ -19,14 +21,15 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "riken-braket_amd", "brk-bra")
 GOLDEN = os.path.join(ROOT, "tests", "golden", "qcx")
 TIMING = re.compile(r"^(Operations|Expectation values|Amplitudes|Events|Measurement) finished: [0-9.e+-]+ \([0-9.e+-]+\)$")
+AMPLITUDE = re.compile(r"^([01]+) => (\S+) \+ (\S+) i$")
 
 
-def run_bra(text):
+def run_bra(text, args=()):
     if not os.path.isfile(EXE):
         pytest.fail(f"{EXE} is not built (run __graft_entry__.build())")
-    print(f"gpu bra: running brk-bra on {text.splitlines()[0]!r} ...", flush=True)
+    print(f"gpu bra: running brk-bra {' '.join(args)} on {text.splitlines()[0]!r} ...", flush=True)
     sys.stdout.flush()
-    r = subprocess.run([EXE], input=text, capture_output=True, text=True, timeout=120)
+    r = subprocess.run([EXE, *args], input=text, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     return r.stdout.splitlines()
 
@@ -52,26 +55,128 @@ def table(lines, n):
     return lines[i + 1], [[float(x) for x in row.split()] for row in lines[i + 2:i + 2 + n]]
 
 
-def test_hadamards08_closed_form():
-    lines = run_bra(open(os.path.join(GOLDEN, "hadamards08.qcx")).read())
+def full_table(lines, n, which=0):
+    """The rows of the which-th "<Qx> <Qy> <Qz>" table (DO MEASUREMENT with a negative precision)."""
+    starts = [k for k, l in enumerate(lines) if l == "<Qx> <Qy> <Qz>"]
+    i = starts[which]
+    return np.array([[float(x) for x in lines[i + 1 + k].split()] for k in range(n)])
+
+
+def printed_amplitudes(lines, n):
+    """Every "bits => re + im i" line, as {index: complex} per DO AMPLITUDES block, in order."""
+    blocks, current = [], None
+    for line in lines:
+        m = AMPLITUDE.match(line)
+        if m:
+            if current is None:
+                current = []
+            assert len(m.group(1)) == n
+            current.append((int(m.group(1), 2), complex(float(m.group(2)), float(m.group(3)))))
+        elif current is not None:
+            blocks.append(current)
+            current = None
+    if current is not None:
+        blocks.append(current)
+    return blocks
+
+
+def test_h_wall_closed_form():
+    lines = run_bra(open(os.path.join(GOLDEN, "h_wall_10.qcx")).read(), ["--amplitude-precision", "17"])
     assert TIMING.match(lines[0]) and TIMING.match(lines[-1])
-    header, rows = table(lines, 8)
+    header, rows = table(lines, 10)
     # fmt "{:^8s}" columns for precision 3 (bra/src/state.cpp:1997-2006)
     assert header == " ".join(f"{h:^8}" for h in ("<Qx>", "<Qy>", "<Qz>"))
     # |+> on every qubit: <Qx> = 0, <Qy> = <Qz> = 1/2
-    assert all(abs(v - w) < 1e-12 for row in rows for v, w in zip(row, (0.0, 0.5, 0.5)))
     row = lines[lines.index(header) + 1]
     assert row.replace("-0.000", " 0.000") == " ".join(f"{v:^8}" for v in (" 0.000", " 0.500", " 0.500"))
+    assert np.max(np.abs(full_table(lines, 10) - np.array([[0.0, 0.5, 0.5]] * 10))) < 1e-15
+    blocks = printed_amplitudes(lines, 10)
+    assert [i for i, _ in blocks[0]] == [0, 5, 1023] and len(blocks[1]) == 1024
+    assert all(abs(a - 2.0 ** -5) < 1e-15 for b in blocks for _, a in b)
 
 
-@pytest.mark.parametrize("name", ["toffoli12.qcx", "adder6x2.qcx", "shor12.qcx"])
-def test_sample_programs_vs_oracle(oracle, name):
+@pytest.mark.parametrize("name", ["toffoli_ladder_12.qcx", "draper_adder_8.qcx", "shor_box_21.qcx"])
+def test_programs_vs_oracle(oracle, name):
     text = open(os.path.join(GOLDEN, name)).read()
-    lines = run_bra(text)
+    lines = run_bra(text, ["--amplitude-precision", "17"])
     q, s = oracle_state(oracle, text)
-    _, rows = table(lines, q.num_qubits)
-    ref = spins(oracle, s, q.num_qubits)
-    assert np.max(np.abs(np.array(rows) - np.array(ref))) <= 0.5e-3 + 1e-12
+    n = q.num_qubits
+    _, rows = table(lines, n)  # the 3-decimal table of DO MEASUREMENT's default precision
+    ref = np.array(spins(oracle, s, n))
+    assert np.max(np.abs(np.array(rows) - ref)) <= 0.5e-3 + 1e-12
+    assert np.max(np.abs(full_table(lines, n) - ref)) < 1e-12
+    (amps,) = printed_amplitudes(lines, n)
+    assert [i for i, _ in amps] == list(range(1 << n))
+    assert np.max(np.abs(np.array([a for _, a in amps]) - s)) < 1e-12
+
+
+# --- depolarizing noise ------------------------------------------------------------------------
+
+def replay_noisy(oracle, text, seed):
+    """The oracle runs the program as bra::state does with a depolarizing channel: after every
+    gate, one draw per noise qubit in bra's order (brk_qcx_gate), X / Y / Z by the thresholds
+    (bra/src/state.cpp:3331-3362), from the program's own generator when SEED > 0, else from
+    the state's generator (std::mt19937_64(--seed)), which M shares."""
+    q = brk.Qcx(text)
+    (px, py, pz), noise_seed = q.depolarizing
+    s = np.zeros(1 << q.num_qubits, dtype=np.complex128)
+    s[q.initial_state] = 1
+    g = oracle.mt64(seed)
+    gn = oracle.mt64(noise_seed) if noise_seed > 0 else g
+    inserted = 0
+
+    def noise(qubit):
+        nonlocal inserted
+        u = gn.canonical()
+        if u < px:
+            oracle.pauli_x(s, [qubit])
+        elif u < px + py:
+            oracle.pauli_y(s, [qubit])
+        elif u < px + py + pz:
+            oracle.pauli_z(s, [qubit])
+        else:
+            return
+        inserted += 1
+
+    amplitudes = []
+    for i, (kind, v) in enumerate(q.statements):
+        if kind == brk.QCX_GATES:
+            for first, count, qubits in q.gate_records[i]:
+                run_ops(oracle, s, v[first:first + count])
+                for qubit in qubits:
+                    noise(qubit)
+        elif kind == brk.QCX_PROJECTIVE_MEASUREMENT:
+            oracle.projective_measurement(s, v, g.canonical())
+        elif kind in (brk.QCX_CLEAR, brk.QCX_SET):
+            value = 0 if kind == brk.QCX_CLEAR else 1
+            oracle.project(s, v, value, oracle.zero_one_probabilities(s, v)[value] ** -0.5)
+            noise(v)
+        elif kind == brk.QCX_AMPLITUDES:
+            amplitudes.append(s.copy())
+    return q, amplitudes, s, inserted
+
+
+@pytest.mark.parametrize("name,args,seed", [("depolarizing_sz_9.qcx", [], 1),
+                                            ("depolarizing_m_7.qcx", [], 1),
+                                            ("depolarizing_m_7.qcx", ["--seed", "99"], 99)])
+def test_depolarizing_programs_vs_oracle_replay(oracle, name, args, seed):
+    text = open(os.path.join(GOLDEN, name)).read()
+    lines = run_bra(text, ["--amplitude-precision", "17", *args])
+    q, amplitudes, s, inserted = replay_noisy(oracle, text, seed)
+    assert inserted > 0  # the channel did insert gates
+    n = q.num_qubits
+    (amps,) = printed_amplitudes(lines, n)
+    assert np.max(np.abs(np.array([a for _, a in amps]) - amplitudes[0])) < 1e-12
+    assert np.max(np.abs(full_table(lines, n) - np.array(spins(oracle, amplitudes[0], n)))) < 1e-12
+
+
+def test_depolarizing_draws_differ_from_noise_free(oracle):
+    # the same program without its DEPOLARIZING CHANNEL line gives a different state
+    text = open(os.path.join(GOLDEN, "depolarizing_sz_9.qcx")).read()
+    clean = "\n".join(l for l in text.splitlines() if not l.startswith("DEPOLARIZING")) + "\n"
+    noisy = printed_amplitudes(run_bra(text, ["--amplitude-precision", "17"]), 9)[0]
+    plain = printed_amplitudes(run_bra(clean, ["--amplitude-precision", "17"]), 9)[0]
+    assert max(abs(a - b) for (_, a), (_, b) in zip(noisy, plain)) > 1e-3
 
 
 def test_full_precision_measurement_and_amplitudes(oracle):
@@ -93,6 +198,9 @@ def test_full_precision_measurement_and_amplitudes(oracle):
         assert index == (k if k < 32 else (3, 17)[k - 32])
         # default stream formatting: 6 significant digits
         assert abs(complex(float(m.group(2)), float(m.group(3))) - s[index]) < 1e-5
+    # the same run printing 17 digits
+    blocks = printed_amplitudes(run_bra(text, ["--amplitude-precision", "17"]), 5)
+    assert np.max(np.abs(np.array([a for _, a in blocks[0]]) - s)) < 1e-12
 
 
 def test_initial_state_and_errors():

@@ -1,10 +1,12 @@
 """CPU checks of the bra front end (quantum-assembler parser, no device).
 
-Pinned by: the reference's own sample programs (bra/qcx/*.qcx, copied under
-tests/golden/qcx/), the mnemonic semantics of docs/bra.md:109-188 checked
-through identities that hold independently of how the parser writes a gate
-down (e.g. +X = exp(i pi/4 X), SX^2 = X), and bra's operand order
-(controls first, interpreter.cpp:3341-3369)."""
+Pinned by: this repository's own test programs (tests/golden/qcx/*.qcx, written
+by tests/golden/qcx/make_programs.py) run on the CPU oracle against closed
+forms (an H wall, a QFT adder), the mnemonic semantics of docs/bra.md:109-188
+checked through identities that hold independently of how the parser writes a
+gate down (e.g. +X = exp(i pi/4 X), SX^2 = X, SZZ = sZ (x) sZ), bra's operand
+order (controls first, interpreter.cpp:3341-3369) and the order in which
+bra::state draws depolarizing noise (bra/src/state.cpp:1000-3330)."""
 import math
 import os
 
@@ -12,6 +14,7 @@ import numpy as np
 import pytest
 
 import brk
+import dense as D
 from oracle_circuits import run_ops
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -40,39 +43,56 @@ Y = np.array([[0, -1j], [1j, 0]], dtype=complex)
 I2 = np.eye(2)
 
 
-def test_hadamards08():
-    q = parse(sample("hadamards08.qcx"))
-    assert q.num_qubits == 8 and q.initial_state == 0
-    assert q.statements[-1] == (brk.QCX_MEASUREMENT, 3)
-    assert q.ops() == [("H", (i,), (), ()) for i in range(8)]
+def run_program(oracle, q):
+    s = np.zeros(1 << q.num_qubits, dtype=np.complex128)
+    s[q.initial_state] = 1
+    for kind, v in q.statements:
+        if kind == brk.QCX_GATES:
+            run_ops(oracle, s, v)
+        elif kind == brk.QCX_SHOR_BOX:
+            oracle.shor_box(s, *v)
+    return s
 
 
-def test_toffoli12():
-    q = parse(sample("toffoli12.qcx"))
+def test_h_wall_closed_form(oracle):
+    q = parse(sample("h_wall_10.qcx"))
+    assert q.num_qubits == 10 and q.initial_state == 0
+    assert q.ops() == [("H", (i,), (), ()) for i in range(10)]
+    assert q.statements[-1] == (brk.QCX_AMPLITUDES, []) and q.statements[-2] == (brk.QCX_MEASUREMENT, -17)
+    s = run_program(oracle, q)
+    assert np.max(np.abs(s - 2.0 ** -5)) < 1e-15
+
+
+def test_draper_adder_closed_form(oracle):
+    # |a>|5> -> |a>|a + 5 mod 16>, a uniform over 0..15 (qubits 0..3 hold a, 4..7 b)
+    q = parse(sample("draper_adder_8.qcx"))
+    kinds = [k for k, _ in q.statements]
+    assert brk.QCX_BEGIN_FUSION in kinds and brk.QCX_END_FUSION in kinds
+    s = run_program(oracle, q)
+    expected = np.zeros(256, dtype=np.complex128)
+    for a in range(16):
+        expected[a | (((a + 5) % 16) << 4)] = 0.25
+    assert np.max(np.abs(s - expected)) < 1e-14
+
+
+def test_toffoli_ladder_ops():
+    q = parse(sample("toffoli_ladder_12.qcx"))
     ops = q.ops()
-    assert q.num_qubits == 12 and len(ops) == 20
-    assert [o[0] for o in ops[:12]] == ["MATRIX"] * 12
-    # TOFFOLI c1 c2 t: X on t under two controls
-    assert ops[12] == ("X", (0,), (1, 2), ())
-    assert ops[-1] == ("X", (3,), (1, 10), ())
-    assert q.statements[-1] == (brk.QCX_MEASUREMENT, 3)  # BEGIN MEASUREMENT
-
-
-def test_adder_controlled_phase():
-    q = parse(sample("adder6x2.qcx"))
-    ops = q.ops()
-    # "U 7 6 2": phase 2 pi / 2^2 where both qubits are 1; "U 7 6 -2": its adjoint
-    assert ("U1", (7,), (6,), (math.pi / 2,)) in ops
-    assert ops[-2] == ("U1", (7,), (6,), (-math.pi / 2,))
-
-
-def test_shor12():
-    q = parse(sample("shor12.qcx"))
     assert q.num_qubits == 12
-    assert q.statements[0] == (brk.QCX_SHOR_BOX, (8, 15, 7))
-    assert q.statements[-1] == (brk.QCX_MEASUREMENT, 3)
+    assert [o[0] for o in ops[:12]] == ["MATRIX"] * 12
+    # TOFFOLI c1 c2 t: X on t under two controls; C3X: three controls
+    assert ops[12] == ("X", (2,), (0, 1), ())
+    assert ("X", (11,), (0, 4, 8), ()) in ops and ("Z", (3, 6, 9), (), ()) in ops
+    assert ("SWAP", (5, 6), (0, 11), ()) in ops
+
+
+def test_shor_box_program():
+    q = parse(sample("shor_box_21.qcx"))
+    assert q.num_qubits == 11
+    assert q.statements[0] == (brk.QCX_SHOR_BOX, (6, 21, 2))
     ops = q.ops()
-    assert ops[0] == ("H", (7,), (), ()) and ops[1] == ("U1", (7,), (6,), (-math.pi / 2,))
+    # "U c1 c2 -k": phase -2 pi / 2^k where both qubits are 1
+    assert ops[0] == ("H", (5,), (), ()) and ops[1] == ("U1", (6,), (5,), (-math.pi / 2,))
 
 
 def test_unsupported_statement_names_line():
@@ -97,15 +117,16 @@ def test_fixed_gates_match_their_definitions():
 
 def test_phase_gates():
     ops = only_ops("S 1\nS+ 1\nSZ 2\nT 3\nT+ 3\nU1 4 0.25\nU1+ 4 0.25\nR 5 3\nR+ 5 3\nR 5 -3\nCS 1 2\nCCR 0 1 2 1")
-    assert ops[0] == ("U1", (1,), (), (math.pi / 2,))
-    assert ops[1] == ("U1", (1,), (), (-math.pi / 2,))
-    assert ops[2] == ("U1", (2,), (), (math.pi / 2,))
+    # S = sqrt(Z): exactly i (ket::utility::imaginary_unit), S+ exactly -i
+    assert ops[0] == ("PHASE", (1,), (), (0.0, 1.0))
+    assert ops[1] == ("PHASE", (1,), (), (0.0, -1.0))
+    assert ops[2] == ("PHASE", (2,), (), (0.0, 1.0))
     assert ops[3] == ("U1", (3,), (), (math.pi / 4,)) and ops[4] == ("U1", (3,), (), (-math.pi / 4,))
     assert ops[5] == ("U1", (4,), (), (0.25,)) and ops[6] == ("U1", (4,), (), (-0.25,))
     assert ops[7] == ("U1", (5,), (), (2 * math.pi / 8,))
     assert ops[8] == ops[9] == ("U1", (5,), (), (-2 * math.pi / 8,))
     # controlled S on control qubits 1 and 2; CCR: three control qubits
-    assert ops[10] == ("U1", (2,), (1,), (math.pi / 2,))
+    assert ops[10] == ("PHASE", (2,), (1,), (0.0, 1.0))
     assert ops[11] == ("U1", (0,), (1, 2), (math.pi,))
 
 
@@ -169,7 +190,12 @@ def test_directives_and_comments():
     ("QUBITS 2\nU1 0 pi\n", 1, "not a number"),
     ("QUBITS 2\nH 0 1\n", 1, "takes 1 operands"),
     ("QUBITS 2\nBIT ASSIGNMENT 0 0\n", 1, "repeats"),
-    ("QUBITS 2\nDEPOLARIZING CHANNEL P = 0.1\n", 8, "DEPOLARIZING"),
+    ("QUBITS 2\nDEPOLARIZING CHANNEL P = 0.1\n", 1, "unknown key"),
+    ("QUBITS 2\nDEPOLARIZING CHANNEL P_X=0.6,P_Y=0.5\n", 1, "exceeds 1"),
+    ("QUBITS 2\nDEPOLARIZING CHANNEL P_X=0.1,P_X=0.2\n", 1, "twice"),
+    ("QUBITS 2\nDEPOLARIZING CHANNEL P_X=-0.1\n", 1, "probability"),
+    ("QUBITS 2\nDEPOLARIZING CHANNEL P_X=0.1,\n", 1, "KEY=value"),
+    ("QUBITS 2\nDEPOLARIZING NOISE P_X=0.1\n", 1, "CHANNEL"),
 ])
 def test_errors(text, status, fragment):
     with pytest.raises(brk.BrkError) as e:
@@ -194,3 +220,54 @@ def test_exponential_swap_forms(oracle):
     v = s.copy()
     oracle.swap(v, 0, 2)
     assert np.max(np.abs(u - 1j * v)) < 1e-15
+
+
+def test_sqrt_z_strings():
+    # SZZ / SZZZ.. / SZ<n> and adjoints: i (-i) per set target, under the controls; each a
+    # per-target phase of exactly i.  SZ<n> follows docs/bra.md:133 (C3SZ3 c1 c2 c3 t1 t2 t3).
+    ops = only_ops("SZZ 0 1\nSZZZ+ 2 3 4\nCSZZ 5 6 7\nSZ3 1 4 7\nC2SZ2 0 1 2 3\nSZ2+ 5 6\nCSZZ+ 0 1 2\nSZ1 3")
+    i, mi = (0.0, 1.0), (0.0, -1.0)
+    assert ops[0:2] == [("PHASE", (0,), (), i), ("PHASE", (1,), (), i)]
+    assert ops[2:5] == [("PHASE", (q,), (), mi) for q in (2, 3, 4)]
+    assert ops[5:7] == [("PHASE", (6,), (5,), i), ("PHASE", (7,), (5,), i)]
+    assert ops[7:10] == [("PHASE", (q,), (), i) for q in (1, 4, 7)]
+    assert ops[10:12] == [("PHASE", (2,), (0, 1), i), ("PHASE", (3,), (0, 1), i)]
+    assert ops[12:14] == [("PHASE", (5,), (), mi), ("PHASE", (6,), (), mi)]
+    assert ops[14:16] == [("PHASE", (1,), (0,), mi), ("PHASE", (2,), (0,), mi)]
+    assert ops[16] == ("PHASE", (3,), (), i)
+
+
+def test_sqrt_z_strings_match_their_definition(oracle):
+    # C(sZ (x) sZ (x) sZ) on a random state equals the dense kron of diag(1, i) under the control
+    rng = np.random.default_rng(3)
+    s = D.random_state(5, rng)
+    t = s.copy()
+    run_ops(oracle, t, only_ops("CSZZZ 4 0 2 3\nSZZ+ 1 3", 5))
+    sz = np.diag([1, 1j])
+    u = D.apply(s, D.kron_all([sz, sz, sz]), [0, 2, 3], [4])
+    u = D.apply(u, D.kron_all([sz.conj(), sz.conj()]), [1, 3])
+    assert np.max(np.abs(t - u)) < 1e-15
+
+
+def test_depolarizing_statement():
+    q = parse("QUBITS 3\nDEPOLARIZING CHANNEL p_x = 0.125 , P_Y=0.25,P_z =0.0625, seed=77\nH 0\n")
+    assert q.depolarizing == ([0.125, 0.25, 0.0625], 77)
+    q = parse("DEPOLARIZING CHANNEL P_Z=0.5\nQUBITS 3\nH 0\n")
+    assert q.depolarizing == ([0.0, 0.0, 0.5], -1)
+    assert parse("QUBITS 3\nH 0\n").depolarizing is None
+
+
+def test_noise_qubit_order():
+    """bra::state draws noise per operated qubit after each gate: targets then controls
+    (CNOT c t -> t, c; TOFFOLI c1 c2 t -> t, c1, c2), operand order for gates on "control"
+    qubits only (CZ, S, U1, U), and for identities too."""
+    q = parse("QUBITS 6\nCNOT 1 0\nTOFFOLI 3 4 2\nCZ 5 1\nCU1 2 3 0.5\nU 4 0 2\nCSZZ 5 0 1\nI 2\nIC 3\n"
+              "CI 1 4\nII 0 5\nEXX 3 4 0.2\nCH 2 0\nC2SWAP 1 2 3 4\nZZ 0 5\n")
+    (gi, _), = [(i, v) for i, (k, v) in enumerate(q.statements) if k == brk.QCX_GATES]
+    records = q.gate_records[gi]
+    noise = [r[2] for r in records]
+    assert noise == [[0, 1], [2, 3, 4], [5, 1], [2, 3], [4, 0], [0, 1, 5], [2], [3], [4, 1], [0, 5], [3, 4],
+                     [0, 2], [3, 4, 1, 2], [0, 5]]
+    counts = [r[1] for r in records]
+    assert counts == [1, 1, 1, 1, 1, 2, 0, 0, 0, 0, 1, 1, 1, 2]
+    assert [r[0] for r in records] == list(np.cumsum([0] + counts[:-1]))
