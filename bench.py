@@ -10,8 +10,13 @@ amplitudes (64 GiB) resident in HBM.  A step is one pass of the circuit.
   roofline   = the kernel instantiation with the most device time, its
                algorithmic bytes per launch over its mean launch time (HIP events
                recorded on the stream the engine launches on)
-  cpu_baseline = the CPU oracle (ket's algorithm, -Ofast -fopenmp, all host
-               cores) on the same circuit generator at 30 qubits (<= 25 s)
+  cpu_baseline = the reference CPU case -- 20 qubits complex128, H on every
+               qubit then 10 layers of RZ on each qubit and a CNOT brick (315
+               gates) -- run by the CPU oracle (ket's algorithm, -Ofast -fopenmp
+               like bra's nompi-release build, all host cores): gate loop only,
+               1 warm-up, median of 5; the GPU's per-gate and gate-block times
+               for the same circuit beside it.  cpu_baseline_30q: the oracle on
+               the benchmark circuit's generator at 30 qubits (bounded sample)
   fused_c64  = the precision form on the same circuit: greedy fusion into dense
                4/5/6-qubit blocks (width autotuned by timing each once), complex64
                state, FP32 MFMA GEMMs; circuit time, speedup over the complex128
@@ -65,9 +70,9 @@ def kernel_name(op, perm, n):
     if name in ("Z", "U1", "RZ", "EZ"):
         ctrl = pt + pc if name in ("Z", "U1") else pc
         high = [q for q in ctrl if q >= LINE_BITS]
-        parity = "true" if name in ("RZ", "EZ") else "false"
+        select = "1" if name in ("RZ", "EZ") else "0"  # coefficient by target parity / one coefficient
         cond = "true" if len(high) < len(ctrl) else "false"
-        return f"k_diag<double, {K(len(high))}, {parity}, true, {cond}>"
+        return f"k_diag<double, {K(len(high))}, {select}, true, {cond}>"
     mode = {"H": 0, "X": 1}.get(name, 2)
     if len(pt) == 1 and pt[0] < LINE_BITS and n >= LANE_BITS:
         high = [q for q in pc if q >= LANE_BITS]
@@ -114,7 +119,95 @@ def kind_and_band(ops, n, line_qubits=3):
     return out
 
 
-def cpu_baseline(n_cpu=30, budget_s=25.0, seed=2):
+def _host_description():
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return cores, model
+
+
+def cpu_baseline_20q(stream, runs=5):
+    """The reference's CPU case on the host: 20 qubits, H wall + 10 RZ / CNOT-brick layers (315
+    gates), the CPU oracle built like bra's nompi-release (-Ofast -fopenmp), all host cores.
+    Gate loop only, 1 warm-up, median of `runs`.  The same circuit on the GPU (per-gate and
+    gate blocks, median of `runs` after a warm-up) goes beside it."""
+    import numpy as np
+
+    import brk
+    import circuits as W
+    import hip
+    from devstate import DeviceState
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle import Oracle
+    from oracle_circuits import run_ops
+
+    n = 20
+    ops = W.layered_rz_cnot(n, 10, seed=1)
+    nbytes = sum(W.algorithmic_bytes(op, n) for op in ops)
+    o = Oracle(fast=True)
+    cores, model = _host_description()
+    o.set_num_threads(cores)
+    s = np.zeros(1 << n, dtype=np.complex128)
+    times = []
+    for k in range(runs + 1):
+        s[:] = 0
+        s[0] = 1
+        t0 = time.perf_counter()
+        run_ops(o, s, ops)
+        if k:  # the first run is the warm-up
+            times.append(time.perf_counter() - t0)
+    cpu_s = float(np.median(times))
+    circuit = brk.Circuit(n, ops)
+    plan = brk.BlockPlan(circuit, brk.C128)
+    st = DeviceState(n)
+    gpu = {"per_gate": [], "blocks": []}
+    for k in range(runs + 1):
+        for key, run in (("per_gate", lambda: circuit.run(st.ptr, stream=stream)),
+                         ("blocks", lambda: plan.run(st.ptr, stream))):
+            brk.set_basis_state(st.ptr, st.size, brk.C128, 0, stream)
+            hip.synchronize(stream)
+            e0, e1 = hip.Event(), hip.Event()
+            e0.record(stream)
+            run()
+            e1.record(stream)
+            e1.synchronize()
+            if k:
+                gpu[key].append(e0.elapsed_ms(e1))
+    got = st.download()
+    st.free()
+    err = float(np.max(np.abs(got - s)))
+    gpu_ms = {key: float(np.median(v)) for key, v in gpu.items()}
+    log(f"cpu baseline 20q: {cpu_s * 1e3:.1f} ms on {cores} threads; GPU per-gate {gpu_ms['per_gate']:.3f} ms, "
+        f"blocks {gpu_ms['blocks']:.3f} ms")
+    return {
+        "value": round(nbytes / cpu_s / 1e9, 2),
+        "unit": "GB/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"the reference CPU case, whole: 20 qubits complex128, H on every qubit then 10 layers of "
+                  f"RZ on each qubit + CNOT brick ({len(ops)} gates, seed 1), oracle -Ofast -fopenmp, "
+                  f"{cores} threads, gate loop only, median of {runs} after 1 warm-up; host {model}",
+        "seconds": round(cpu_s, 5),
+        "algorithmic_GB": round(nbytes / 1e9, 3),
+        "nproc": os.cpu_count(),
+        "cpu_model": model,
+        "gpu_same_circuit_ms": {k: round(v, 4) for k, v in gpu_ms.items()},
+        "gpu_speedup": {k: round(cpu_s * 1e3 / v, 1) for k, v in gpu_ms.items()},
+        "gpu_max_abs_diff_vs_cpu": err,
+    }
+
+
+def cpu_baseline_30q(n_cpu=30, budget_s=15.0, seed=2):
+    """The benchmark circuit's generator at 30 qubits (16 GiB; 32 qubits need more host
+    memory than a bounded sample should take) on the CPU oracle, first ops within budget_s."""
     import numpy as np
 
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -123,7 +216,7 @@ def cpu_baseline(n_cpu=30, budget_s=25.0, seed=2):
     from oracle_circuits import run_ops
 
     o = Oracle(fast=True)
-    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    cores, model = _host_description()
     o.set_num_threads(cores)
     ops = W.random_circuit(n_cpu, 200, 8, seed=seed)
     s = np.zeros(1 << n_cpu, dtype=np.complex128)
@@ -136,21 +229,12 @@ def cpu_baseline(n_cpu=30, budget_s=25.0, seed=2):
     _, done = run_ops(o, s, ops, time_budget=budget_s)
     dt = time.perf_counter() - t0
     nbytes = sum(W.algorithmic_bytes(op, n_cpu) for op in ops[:done])
-    model = ""
-    try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    model = line.split(":", 1)[1].strip()
-                    break
-    except OSError:
-        pass
     return {
         "value": round(nbytes / dt / 1e9, 2),
         "unit": "GB/s",
         "cores": cores,
         "kind": "port",
-        "sample": f"first {done} of {len(ops)} ops of the same generator at {n_cpu} qubits "
+        "sample": f"first {done} of {len(ops)} ops of the benchmark circuit's generator at {n_cpu} qubits "
                   f"(2^{n_cpu} c128 = {2 ** n_cpu * 16 / 2 ** 30:.0f} GiB), {dt:.1f} s, "
                   f"oracle -Ofast -fopenmp, {cores} threads, {model}",
     }
@@ -621,8 +705,9 @@ def main():
         result["roofline"]["traffic"] = traffic["bytes_per_launch"]
         result["roofline"]["traffic_source"] = traffic["source"]
     if not args.no_cpu:
-        log("cpu baseline (oracle, 30 qubits, <= 25 s)")
-        result["cpu_baseline"] = cpu_baseline(seed=args.seed)
+        log("cpu baseline (oracle: the 20-qubit reference case, then 30 qubits for <= 15 s)")
+        result["cpu_baseline"] = cpu_baseline_20q(stream)
+        result["cpu_baseline_30q"] = cpu_baseline_30q(seed=args.seed)
     print(json.dumps(result), flush=True)
     return 0
 

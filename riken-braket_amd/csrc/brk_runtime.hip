@@ -195,22 +195,13 @@ __global__ void k_convert(cpx<RD>* __restrict__ d, const cpx<RS>* __restrict__ s
   }
 }
 
-// Streaming copy: 4 elements per thread, block-contiguous.
-template <typename R>
+// Streaming copy with the gate kernels' access shape: one whole amplitude per thread,
+// non-temporal for large states, each XCD walking a contiguous range of blocks.
+template <typename R, bool NT>
 __global__ __launch_bounds__(256) void k_copy(cpx<R>* __restrict__ d, const cpx<R>* __restrict__ s, uint64_t size) {
-  constexpr int ITEMS = 4;
-  uint64_t base = (uint64_t)blockIdx.x * (256 * ITEMS) + threadIdx.x;
-  cpx<R> v[ITEMS];
-#pragma unroll
-  for (int u = 0; u < ITEMS; ++u) {
-    uint64_t i = base + (uint64_t)u * 256;
-    if (i < size) v[u] = s[i];
-  }
-#pragma unroll
-  for (int u = 0; u < ITEMS; ++u) {
-    uint64_t i = base + (uint64_t)u * 256;
-    if (i < size) d[i] = v[u];
-  }
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  for (uint64_t i = xcd_tile(blockIdx.x, gridDim.x, 1) * 256 + threadIdx.x; i < size; i += stride)
+    store_amp<NT>(d + i, load_amp<NT>(s + i));
 }
 
 static unsigned grid_for(uint64_t n, unsigned threads, unsigned cap = 65536u) {
@@ -506,14 +497,20 @@ int brk_copy(void* dst, const void* src, uint64_t size, int dtype, void* stream)
   unsigned nq;
   if (int st = check_state(dst, size, dtype, &nq)) return st;
   if (int st = check_state(src, size, dtype, &nq)) return st;
-  uint64_t blocks = (size + 1023) / 1024;
-  if (blocks > 0x7fffffffull) return fail(BRK_ERR_INVALID_ARGUMENT, "state too large for copy grid");
-  if (dtype == BRK_C128)
-    hipLaunchKernelGGL(k_copy<double>, dim3((unsigned)blocks), dim3(256), 0, as_stream(stream),
-                       (cpx<double>*)dst, (const cpx<double>*)src, size);
-  else
-    hipLaunchKernelGGL(k_copy<float>, dim3((unsigned)blocks), dim3(256), 0, as_stream(stream), (cpx<float>*)dst,
-                       (const cpx<float>*)src, size);
+  const unsigned blocks = (unsigned)std::min<uint64_t>((size + 255) / 256, (1ull << 24) - 1);
+  const bool nt = size * amp_bytes(dtype) >= get_tuning().nt_min_bytes;
+  hipStream_t st = as_stream(stream);
+  if (dtype == BRK_C128) {
+    auto* d = (cpx<double>*)dst;
+    auto* s = (const cpx<double>*)src;
+    if (nt) hipLaunchKernelGGL((k_copy<double, true>), dim3(blocks), dim3(256), 0, st, d, s, size);
+    else hipLaunchKernelGGL((k_copy<double, false>), dim3(blocks), dim3(256), 0, st, d, s, size);
+  } else {
+    auto* d = (cpx<float>*)dst;
+    auto* s = (const cpx<float>*)src;
+    if (nt) hipLaunchKernelGGL((k_copy<float, true>), dim3(blocks), dim3(256), 0, st, d, s, size);
+    else hipLaunchKernelGGL((k_copy<float, false>), dim3(blocks), dim3(256), 0, st, d, s, size);
+  }
   return check_launch("copy");
 }
 
