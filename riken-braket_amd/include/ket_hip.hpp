@@ -222,6 +222,13 @@ inline operands from_ranges(const Targets& targets, const Controls& controls) {
   add_all(o, controls);
   return o;
 }
+// iterator-pair forms: [first, last) of ket::qubit or ket::control values
+template <typename It>
+inline void add_all(operands& o, It first, It last) {
+  for (; first != last; ++first) collect(o, *first);
+}
+template <typename It>
+using if_iterator = decltype(*std::declval<It&>(), ++std::declval<It&>(), std::declval<It>() != std::declval<It>(), 0);
 
 }  // namespace detail
 }  // namespace hip
@@ -522,7 +529,13 @@ inline void adj_phase_shift3(hip::policy p, iter<Complex> first, iter<Complex> l
 //   runtime::qubit_ranges::NAME(policy, first, last, params..., targets[, controls])
 //   runtime::ranges::NAME(policy, state, params..., targets[, controls])
 // where targets is one ket::qubit or a range of them and controls a range of ket::control
-// (the runtime forms of the reference, e.g. hadamard.hpp:281-470, pauli_x.hpp:288-572).
+// (the runtime forms of the reference, e.g. hadamard.hpp:281-470, pauli_x.hpp:288-572), and
+// the iterator-pair runtime forms
+//   runtime::NAME(policy, first, last, params..., target)
+//   runtime::NAME(policy, first, last, params..., target, control_first, control_last)
+//   runtime::NAME(policy, first, last, params..., target_first, target_last)
+//   runtime::NAME(policy, first, last, params..., target_first, target_last, control_first, control_last)
+// (hadamard.hpp:407-437, pauli_x.hpp:395-440).
 #define KET_HIP_UNPAREN(...) __VA_ARGS__
 #define KET_HIP_GATE(NAME, PARAMS, ARGS)                                                                      \
   template <typename Complex, typename... Qubits>                                                             \
@@ -540,6 +553,41 @@ inline void adj_phase_shift3(hip::policy p, iter<Complex> first, iter<Complex> l
   }                                                                                                           \
   }                                                                                                           \
   namespace runtime {                                                                                         \
+  template <typename Complex, typename Target>                                                                \
+  inline void NAME(hip::policy p, hip::device_iterator<Complex> first, hip::device_iterator<Complex> last     \
+                       KET_HIP_UNPAREN PARAMS,                                                                \
+                   const Target& target) {                                                                    \
+    hip_core::NAME(p, first, last KET_HIP_UNPAREN ARGS, hip::detail::from_ranges(target));                    \
+  }                                                                                                           \
+  template <typename Complex, typename S, typename B, typename ControlIt,                                      \
+            hip::detail::if_iterator<ControlIt> = 0>                                                          \
+  inline void NAME(hip::policy p, hip::device_iterator<Complex> first, hip::device_iterator<Complex> last     \
+                       KET_HIP_UNPAREN PARAMS,                                                                \
+                   ::ket::qubit<S, B> target, ControlIt control_first, ControlIt control_last) {              \
+    hip::detail::operands o;                                                                                  \
+    hip::detail::collect(o, target);                                                                          \
+    hip::detail::add_all(o, control_first, control_last);                                                     \
+    hip_core::NAME(p, first, last KET_HIP_UNPAREN ARGS, o);                                                   \
+  }                                                                                                           \
+  template <typename Complex, typename TargetIt, hip::detail::if_iterator<TargetIt> = 0>                      \
+  inline void NAME(hip::policy p, hip::device_iterator<Complex> first, hip::device_iterator<Complex> last     \
+                       KET_HIP_UNPAREN PARAMS,                                                                \
+                   TargetIt target_first, TargetIt target_last) {                                             \
+    hip::detail::operands o;                                                                                  \
+    hip::detail::add_all(o, target_first, target_last);                                                       \
+    hip_core::NAME(p, first, last KET_HIP_UNPAREN ARGS, o);                                                   \
+  }                                                                                                           \
+  template <typename Complex, typename TargetIt, typename ControlIt, hip::detail::if_iterator<TargetIt> = 0,  \
+            hip::detail::if_iterator<ControlIt> = 0>                                                          \
+  inline void NAME(hip::policy p, hip::device_iterator<Complex> first, hip::device_iterator<Complex> last     \
+                       KET_HIP_UNPAREN PARAMS,                                                                \
+                   TargetIt target_first, TargetIt target_last, ControlIt control_first,                      \
+                   ControlIt control_last) {                                                                  \
+    hip::detail::operands o;                                                                                  \
+    hip::detail::add_all(o, target_first, target_last);                                                       \
+    hip::detail::add_all(o, control_first, control_last);                                                     \
+    hip_core::NAME(p, first, last KET_HIP_UNPAREN ARGS, o);                                                   \
+  }                                                                                                           \
   namespace qubit_ranges {                                                                                    \
   template <typename Complex, typename Targets, typename Controls>                                            \
   inline void NAME(hip::policy p, hip::device_iterator<Complex> first, hip::device_iterator<Complex> last     \

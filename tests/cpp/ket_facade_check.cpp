@@ -92,6 +92,15 @@ int run(const std::string& prefix, const char* tag) {
   ket::gate::runtime::ranges::pauli_x(p, s, targets, controls);
   ket::gate::runtime::qubit_ranges::hadamard(p, s.begin(), s.end(), 2_q, controls);
   ket::gate::runtime::ranges::exponential_pauli_z(p, s, 0.7, targets);
+  // iterator-pair runtime forms (ket::gate::runtime::NAME(policy, first, last, ...))
+  const std::vector<ket::qubit<>> zz = {4_q, 7_q};
+  const std::vector<ket::control<ket::qubit<>>> two_controls = {0_cq, 5_cq};
+  ket::gate::runtime::pauli_x(p, s.begin(), s.end(), targets.begin(), targets.end(), two_controls.begin(),
+                              two_controls.end());
+  ket::gate::runtime::hadamard(p, s.begin(), s.end(), 6_q, controls.begin(), controls.end());
+  ket::gate::runtime::exponential_pauli_y(p, s.begin(), s.end(), -0.45, zz.begin(), zz.end());
+  ket::gate::runtime::phase_shift3(p, s.begin(), s.end(), 0.3, -0.2, 0.9, 5_q);
+  ket::gate::runtime::sqrt_pauli_z(p, s.begin(), s.end(), zz.begin(), zz.end(), controls.begin(), controls.end());
 
   // local swap of physical qubits 1 and 6, then gates addressed through the permutation
   ket::mpi::qubit_permutation<> perm(8);

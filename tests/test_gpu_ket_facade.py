@@ -66,6 +66,11 @@ def oracle_sequence(o):
     o.pauli_x(s, [1, 6], [3])                             # runtime forms
     o.hadamard(s, 2, [3])
     o.exponential_pauli_z(s, 0.7, [1, 6])
+    o.pauli_x(s, [1, 6], [0, 5])                          # iterator-pair runtime forms
+    o.hadamard(s, 6, [3])
+    o.exponential_pauli_y(s, -0.45, [4, 7])
+    o.phase_shift3(s, 0.3, -0.2, 0.9, 5)
+    o.sqrt_pauli_zn(s, [4, 7], [3])
     perm, inv = list(range(8)), list(range(8))
     o.swap_permutated_local_qubits(s, 1, 6)
     o.permutate(perm, inv, inv[1], inv[6])
