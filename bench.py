@@ -276,8 +276,10 @@ def fused_leg(circuit, plain_state, args, stream):
         plan_ms = (_time.perf_counter() - t0) * 1e3
         brk.set_basis_state(st.ptr, st.size, brk.C64, 0, stream)
         ms = plans[w].apply_timed(st.ptr, stream=stream)
-        widths[w] = {"blocks": plans[w].num_blocks, "ms": round(sum(ms), 2), "plan_ms": round(plan_ms, 1)}
-        log(f"fused width {w}: {plans[w].num_blocks} blocks, {sum(ms):.1f} ms")
+        widths[w] = {"blocks": plans[w].num_blocks, "chunk_passes": plans[w].num_passes, "ms": round(sum(ms), 2),
+                     "plan_ms": round(plan_ms, 1)}
+        log(f"fused width {w}: {plans[w].num_blocks} blocks in {plans[w].num_passes or plans[w].num_blocks} "
+            f"launches, {sum(ms):.1f} ms")
     best = min(widths, key=lambda w: widths[w]["ms"])
     plan = plans[best]
     run_ms = []
@@ -296,11 +298,13 @@ def fused_leg(circuit, plain_state, args, stream):
     st.free()
     ms = sum(run_ms) / len(run_ms)
     blocks = plan.num_blocks
+    passes = plan.num_passes if brk.get_tune("fused_chunks") else 0
     flops = 8 * (1 << best) * (1 << n) * blocks
-    hbm = 2 * 8 * (1 << n) * blocks
+    hbm = 2 * 8 * (1 << n) * (passes or blocks)  # one read and one write of the c64 state per launch
     return {
         "width": best,
         "blocks": blocks,
+        "chunk_passes": passes,
         "ms_per_circuit": round(ms, 2),
         "infidelity": 1.0 - abs(ip) ** 2 / (n64 * n128),
         "widths": widths,

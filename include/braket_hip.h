@@ -75,7 +75,9 @@ int brk_device_name(char* buffer, size_t size);
    4), "fused_chunked" (0/1), "block_lds" (65536, 32768 or 16384: gate-block
    chunk bytes), "block_run_bytes" (power of two in [16, 4096]: the lowest
    qubits covering this many bytes stay local in every gate block),
-   "adjoint_blocks" (0/1: blocked or per-run adjoint sweep).  Each key can
+   "adjoint_blocks" (0/1: blocked or per-run adjoint sweep), "fused_chunks"
+   (0/1: width-4/5 fused plans run as chunk passes of several blocks, or one
+   launch per block).  Each key can
    also be set by the environment variable BRK_TUNE_<KEY> read at first use.
    Process-wide and not synchronised with launches: set before launching, not
    while another thread launches.  brk_get_tune reads a key back. */
@@ -487,9 +489,20 @@ int brk_fused_plan_block(const brk_fused_plan* plan, unsigned i, unsigned* qubit
                          double* matrix);
 /* Logical->physical permutation after the circuit (num_qubits entries). */
 int brk_fused_plan_final_permutation(const brk_fused_plan* plan, unsigned* permutation);
-/* Applies every block in order to a BRK_C64 state of 2^num_qubits amplitudes. */
+/* Number of chunk passes of a width-4 or width-5 plan on at least 13 qubits:
+   consecutive blocks (the plan's order puts blocks on few qubits next to each
+   other; blocks move only past blocks on disjoint qubits) whose qubits, with
+   the 4 lowest, fit in 13 local qubits run on one chunk of 2^13 amplitudes
+   held in LDS -- one HBM read and write per pass instead of per block --
+   with the per-block kernels' arithmetic (the same amplitudes, bit for bit).
+   0 when the plan has no passes. */
+int brk_fused_plan_num_passes(const brk_fused_plan* plan, unsigned* num_passes);
+/* Applies every block in order to a BRK_C64 state of 2^num_qubits amplitudes
+   (in chunk passes when the plan has them and tuning fused_chunks is 1). */
 int brk_fused_apply(const brk_fused_plan* plan, void* first, int dtype, void* stream);
-/* Same, with the device time of block i in block_ms[i] (synchronises). */
+/* Same, with the device time of launch i in block_ms[i] (synchronises): one
+   entry per pass when the plan runs in chunk passes, else one per block;
+   block_ms has room for num_blocks entries. */
 int brk_fused_apply_timed(const brk_fused_plan* plan, void* first, int dtype, float* block_ms, void* stream);
 
 /* ------------------------------------------------------------------------ */

@@ -348,6 +348,130 @@ static int launch_k(const fused_launch& L, void* state, void* stream) {
   return check_launch("fused block");
 }
 
+// ---------------------------------------------------------------------------
+// Chunk passes (k = 4, 5): one 512-thread workgroup per chunk of 2^13 complex64
+// amplitudes held in LDS (64 KiB, two workgroups per CU).  The chunk is loaded
+// once, every block of the pass runs over it, and it is stored once.  A block
+// on local bits row_bits[] sees the chunk as 2^k rows x 2^(13-k) columns (the
+// other local bits); each wave takes 32-column tiles and runs the per-block
+// kernels' MFMA chain on them -- B operands gathered from LDS, A fragments from
+// the plan's matrices (L2-resident), D written back over the wave's own columns --
+// so every amplitude gets the same products in the same order as with one launch
+// per block.  A barrier separates the blocks.
+// ---------------------------------------------------------------------------
+typedef const __attribute__((address_space(4))) fused_chunk_program* cchunk;
+
+template <int KB, bool NT>
+__global__ __launch_bounds__(FUSED_CHUNK_THREADS, 2) void k_fused_chunk(cpx<float>* __restrict__ s,
+                                                                         const fused_chunk_program* program,
+                                                                         const float* __restrict__ matrices) {
+  constexpr int R = 1 << KB, W = 2 * R, HALF = R / 2, DT = W / 32;
+  constexpr int PER = (1 << FUSED_CHUNK_LOCAL) / FUSED_CHUNK_THREADS;  // 16 amplitudes per thread
+  constexpr int NCOL = FUSED_CHUNK_LOCAL - KB;                         // column bits
+  constexpr int NTILES = 1 << (NCOL - 5);                              // 32-column tiles per block
+  constexpr int WAVES = FUSED_CHUNK_THREADS / 64;
+  __shared__ cpx<float> lds[1 << FUSED_CHUNK_LOCAL];
+  const cchunk prog = (cchunk)program;
+  const uint32_t t = threadIdx.x;
+
+  // chunk number -> the bits outside the local positions (XCD-contiguous chunk order)
+  const uint64_t chunk = xcd_tile(blockIdx.x, gridDim.x, 1);
+  uint64_t base = chunk & prog->outer_masks[0];
+  for (int j = 1; j <= FUSED_CHUNK_LOCAL; ++j) base |= (chunk & prog->outer_masks[j]) << j;
+  uint64_t low = 0;  // local index t + 512 k: the bits of t go to pos[0 .. 8]
+  for (int r = 0; r < 9; ++r) low |= (uint64_t)((t >> r) & 1u) << prog->pos[r];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) lds[t + FUSED_CHUNK_THREADS * k] = load_amp<NT>(s + (base | low | prog->deposit_high[k]));
+  __syncthreads();
+
+  const int lane = t & 63, wave = t >> 6;
+  const int i32 = lane & 31, kk = lane >> 5;
+  const int swz = (W >= 64 && kk) ? 32 : 0;
+  const uint32_t nops = prog->num_ops;
+  for (uint32_t o = 0; o < nops; ++o) {
+    const auto& op = prog->ops[o];
+    const float* A = matrices + (uint64_t)op.matrix * (uint64_t)(W * W);
+    // this lane's A fragments (the per-block kernels read the same elements from LDS)
+    float a[DT][R];
+#pragma unroll
+    for (int st = 0; st < R; ++st)
+#pragma unroll
+      for (int tt = 0; tt < DT; ++tt) a[tt][st] = A[(2 * st + kk) * W + ((32 * tt + i32) ^ swz)];
+    // row offsets: uniform parts (the row value without the lane's kk bit) plus kk at block-row
+    // bit 0 for the B gathers and at bit 2 for the D stores
+    auto rowoff = [&](int r) {
+      uint32_t d = 0;
+#pragma unroll
+      for (int b = 0; b < KB; ++b) d |= ((uint32_t)(r >> b) & 1u) << op.row_bits[b];
+      return d;
+    };
+    const uint32_t kk0 = kk ? 1u << op.row_bits[0] : 0u, kk2 = kk ? 1u << op.row_bits[2] : 0u;
+    uint32_t col_lane = 0;  // column bits 0..4 from i32
+#pragma unroll
+    for (int b = 0; b < 5; ++b) col_lane |= ((uint32_t)(i32 >> b) & 1u) << op.col_bits[b];
+    for (int tile = wave; tile < NTILES; tile += WAVES) {
+      uint32_t col = col_lane;  // column bits 5.. from the tile number
+#pragma unroll
+      for (int b = 5; b < NCOL; ++b) col |= ((uint32_t)(tile >> (b - 5)) & 1u) << op.col_bits[b];
+      cpx<float> x[HALF];
+#pragma unroll
+      for (int j = 0; j < HALF; ++j) x[j] = lds[col | rowoff(2 * j) | kk0];
+      f16v acc[DT];
+#pragma unroll
+      for (int tt = 0; tt < DT; ++tt) acc[tt] = f16v{};
+#pragma unroll
+      for (int st = 0; st < R; ++st) {
+        const float b = st < HALF ? x[st].re : x[st - HALF].im;
+#pragma unroll
+        for (int tt = 0; tt < DT; ++tt) acc[tt] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[tt][st], b, acc[tt], 0, 0, 0);
+      }
+      if constexpr (DT == 1) {
+#pragma unroll
+        for (int v = 0; v < 8; ++v) {
+          const int row = 8 * (v >> 2) + (v & 3);  // + 4 kk
+          lds[col | rowoff(row) | kk2] = cpx<float>{acc[0][v], acc[0][v + 8]};
+        }
+      } else {
+#pragma unroll
+        for (int tt = 0; tt < DT / 2; ++tt)
+#pragma unroll
+          for (int v = 0; v < 16; ++v) {
+            const int row = 32 * tt + 8 * (v >> 2) + (v & 3);  // + 4 kk
+            lds[col | rowoff(row) | kk2] = cpx<float>{acc[tt][v], acc[tt + DT / 2][v]};
+          }
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int k = 0; k < PER; ++k) store_amp<NT>(s + (base | low | prog->deposit_high[k]), lds[t + FUSED_CHUNK_THREADS * k]);
+}
+
+bool fused_chunks_enabled() { return get_tuning().fused_chunks != 0; }
+
+int launch_fused_chunk(const fused_chunk_program* device_program, const fused_chunk_program& program, int width,
+                       const float* matrices, void* state, uint64_t size, void* stream) {
+  unsigned nq;
+  if (int st = check_state(state, size, BRK_C64, &nq)) return st;
+  if (nq < (unsigned)FUSED_CHUNK_LOCAL || program.num_ops > (uint32_t)FUSED_CHUNK_MAX_OPS || (width != 4 && width != 5))
+    return fail(BRK_ERR_INVALID_ARGUMENT, "fused chunk pass does not fit (%u qubits, %u ops, width %d)", nq,
+                program.num_ops, width);
+  const uint64_t chunks = size >> FUSED_CHUNK_LOCAL;
+  if (chunks > 0x7fffffffull) return fail(BRK_ERR_INVALID_ARGUMENT, "too many chunks");
+  const bool nt = size * sizeof(cpx<float>) >= get_tuning().nt_min_bytes;
+  hipStream_t st = as_stream(stream);
+  auto* s = (cpx<float>*)state;
+  const dim3 grid((unsigned)chunks), block(FUSED_CHUNK_THREADS);
+  if (width == 4) {
+    if (nt) hipLaunchKernelGGL((k_fused_chunk<4, true>), grid, block, 0, st, s, device_program, matrices);
+    else hipLaunchKernelGGL((k_fused_chunk<4, false>), grid, block, 0, st, s, device_program, matrices);
+  } else {
+    if (nt) hipLaunchKernelGGL((k_fused_chunk<5, true>), grid, block, 0, st, s, device_program, matrices);
+    else hipLaunchKernelGGL((k_fused_chunk<5, false>), grid, block, 0, st, s, device_program, matrices);
+  }
+  return check_launch("fused chunk pass");
+}
+
 int launch_fused_block(const fused_launch& L, void* state, void* stream) {
   if (get_tuning().fused_staged && L.sntiles) {
     if (L.width == 4) return launch_staged<4>(L, state, stream);

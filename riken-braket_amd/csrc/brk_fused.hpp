@@ -43,6 +43,37 @@ constexpr int FUSED_STAGED_COLUMNS = 128;
 // Launches one block (state is complex64).  Defined in csrc/brk_fused.hip.
 int launch_fused_block(const fused_launch& L, void* state, void* stream);
 
+// Chunk passes: several blocks applied to one chunk of 2^13 complex64 amplitudes
+// (64 KiB) held in LDS, one HBM read and one write per pass.  Local bit r of a
+// chunk index is physical qubit pos[r] (ascending); the chunk number, expanded
+// with zeros at those positions (outer_masks), gives the other bits.  Op o of
+// the pass applies block matrix `matrix` (index into the plan's embeddings) on
+// the local bits row_bits[0..k) (block-row bit j: the block's j-th lowest
+// qubit); the remaining local bits, ascending, carry the column index.
+constexpr int FUSED_CHUNK_LOCAL = 13;
+constexpr int FUSED_CHUNK_THREADS = 512;
+constexpr int FUSED_CHUNK_MAX_OPS = 64;
+struct fused_chunk_op {
+  uint32_t matrix;
+  uint32_t row_bits[6];
+  uint32_t col_bits[9];  // 13 - k column bits (k = 4: 9, k = 5: 8)
+};
+struct fused_chunk_program {
+  uint32_t num_ops;
+  uint32_t pos[FUSED_CHUNK_LOCAL];
+  uint64_t outer_masks[17];           // expand masks inserting zeros at the local positions
+  uint64_t deposit_high[16];          // physical offset of local index k << 9 (k < 16)
+  fused_chunk_op ops[FUSED_CHUNK_MAX_OPS];
+};
+// Launches one pass of width-k blocks over a complex64 state of `size` amplitudes:
+// 2^(n - 13) workgroups.  `device_program` is the pass in device memory, `program`
+// its host copy, `matrices` the plan's device embeddings.
+int launch_fused_chunk(const fused_chunk_program* device_program, const fused_chunk_program& program, int width,
+                       const float* matrices, void* state, uint64_t size, void* stream);
+
+// Whether plans of width 4 and 5 run in chunk passes (tuning fused_chunks).
+bool fused_chunks_enabled();
+
 // Size in floats of a block's device matrix for width k.
 inline uint64_t fused_matrix_floats(int k) { return (uint64_t)(2 << k) * (uint64_t)(2 << k); }
 

@@ -129,6 +129,7 @@ struct tuning {
   int block_lds;          // gate blocks: LDS bytes per chunk (65536, 32768 or 16384) -> local qubits
   int block_run_bytes;    // gate blocks: the lowest qubits addressing this many bytes are always local
   int adjoint_blocks;     // adjoint gradient: 1 = forward pass and reverse sweep in gate blocks, 0 = per-run passes
+  int fused_chunks;       // precision form (k = 4, 5): 1 = several blocks per LDS-resident chunk pass, 0 = a launch per block
 };
 tuning& get_tuning();
 

@@ -62,7 +62,7 @@ static tuning g_tuning = {
     /*pair_lds=*/40960, /*diag_lds=*/0, /*pair_remap=*/1, /*diag_remap=*/1,
     /*nt_min_bytes=*/256ull << 20, /*max_blocks=*/(1ull << 24) - 1, /*line_bytes=*/64, /*fused_groups=*/8192,
     /*fused_occupancy=*/2, /*fused_staged=*/1, /*fused_staged_groups=*/768, /*fused_chunked=*/0,
-    /*block_lds=*/65536, /*block_run_bytes=*/128, /*adjoint_blocks=*/1,
+    /*block_lds=*/65536, /*block_run_bytes=*/128, /*adjoint_blocks=*/1, /*fused_chunks=*/0,
 };
 static std::once_flag g_tuning_once;
 
@@ -74,7 +74,7 @@ static void read_tuning_env() {
   static const char* const keys[] = {"pair_lds",     "diag_lds",          "pair_remap",          "diag_remap",
                                      "nt_min_bytes", "max_blocks",        "line_bytes",          "fused_groups",
                                      "fused_occupancy", "fused_staged",   "fused_staged_groups", "fused_chunked",
-                                     "block_lds",    "block_run_bytes",   "adjoint_blocks"};
+                                     "block_lds",    "block_run_bytes",   "adjoint_blocks",      "fused_chunks"};
   for (const char* key : keys) {
     std::string env = "BRK_TUNE_";
     for (const char* c = key; *c; ++c) env.push_back((char)std::toupper((unsigned char)*c));
@@ -238,6 +238,8 @@ static int set_tuning(tuning& t, const std::string& k, int64_t value) {
     t.block_lds = (int)value;
   } else if (k == "adjoint_blocks") {
     t.adjoint_blocks = value != 0;
+  } else if (k == "fused_chunks") {
+    t.fused_chunks = value != 0;
   } else if (k == "block_run_bytes") {
     if (value < 16 || value > 4096 || (value & (value - 1)))
       return fail(BRK_ERR_INVALID_ARGUMENT, "block_run_bytes must be a power of two in [16, 4096]");
@@ -304,6 +306,7 @@ int brk_get_tune(const char* key, int64_t* value) {
   else if (k == "block_lds") *value = t.block_lds;
   else if (k == "block_run_bytes") *value = t.block_run_bytes;
   else if (k == "adjoint_blocks") *value = t.adjoint_blocks;
+  else if (k == "fused_chunks") *value = t.fused_chunks;
   else if (k == "fused_occupancy") *value = t.fused_occupancy;
   else if (k == "fused_staged") *value = t.fused_staged;
   else if (k == "fused_staged_groups") *value = (int64_t)t.fused_staged_groups;

@@ -19,6 +19,12 @@
 //  3. The block matrix is the in-order product of its gates (each gate
 //     applied to every column), widened with identity qubits to exactly
 //     `width` qubits, then rounded once to FP32 for the device.
+//  4. Chunk passes (width 4 and 5, at least 13 qubits): the blocks are put in
+//     block order themselves (a dependence-preserving order that packs blocks
+//     into few groups of 13 local qubits; blocks on disjoint qubits commute),
+//     and consecutive blocks whose qubits, with the 4 lowest (128-byte runs of
+//     complex64), fit in 13 local qubits form one pass over LDS-resident
+//     chunks of 2^13 amplitudes.
 //
 // The reference's fusion (bra BEGIN/END FUSION, nompi_state.cpp:1285-1357)
 // applies the recorded gates one by one per 2^k-amplitude group; a dense
@@ -53,8 +59,11 @@ struct brk_fused_plan {
   std::vector<unsigned> final_perm;
   std::vector<float> host_matrices;   // FP32 embeddings, fused_matrix_floats(width) per block
   mutable float* device_matrices = nullptr;
+  std::vector<brk::fused_chunk_program> passes;  // chunk passes over the blocks in order (may be empty)
+  mutable brk::fused_chunk_program* device_passes = nullptr;
   ~brk_fused_plan() {
     if (device_matrices) (void)hipFree(device_matrices);
+    if (device_passes) (void)hipFree(device_passes);
   }
 };
 
@@ -251,6 +260,67 @@ static std::vector<fused_group> group_gates(const std::vector<physical_gate>& ga
   return groups;
 }
 
+// The 4 lowest qubits (128-byte runs of complex64) are local in every chunk pass.
+constexpr unsigned CHUNK_RUN_QUBITS = 4;
+
+// One pass program: local qubits = the run qubits, the blocks' qubits, then the lowest free
+// ones up to 13; each block's rows on the local bits of its qubits, its columns on the rest.
+static brk::fused_chunk_program make_pass(const brk_fused_plan& plan, size_t first, size_t count, uint64_t local) {
+  using namespace brk;
+  fused_chunk_program p{};
+  for (unsigned q = 0; __builtin_popcountll(local) < FUSED_CHUNK_LOCAL && q < plan.num_qubits; ++q) local |= 1ull << q;
+  unsigned pos[FUSED_CHUNK_LOCAL], k = 0;
+  for (unsigned q = 0; q < 64; ++q)
+    if ((local >> q) & 1ull) pos[k++] = q;
+  std::copy(pos, pos + FUSED_CHUNK_LOCAL, p.pos);
+  const expand_masks e = make_expand_masks(pos, FUSED_CHUNK_LOCAL);
+  std::copy(e.m, e.m + 17, p.outer_masks);
+  for (unsigned h = 0; h < 16; ++h) {
+    uint64_t d = 0;
+    for (unsigned b = 0; b < 4; ++b)
+      if ((h >> b) & 1u) d |= 1ull << pos[9 + b];
+    p.deposit_high[h] = d;
+  }
+  p.num_ops = (uint32_t)count;
+  for (size_t i = 0; i < count; ++i) {
+    const auto& blk = plan.blocks[first + i];
+    fused_chunk_op& op = p.ops[i];
+    op.matrix = (uint32_t)(first + i);
+    uint32_t rows = 0;
+    for (size_t j = 0; j < blk.qubits.size(); ++j) {
+      const unsigned r = (unsigned)(std::find(pos, pos + FUSED_CHUNK_LOCAL, blk.qubits[j]) - pos);
+      op.row_bits[j] = r;
+      rows |= 1u << r;
+    }
+    unsigned c = 0;
+    for (unsigned r = 0; r < (unsigned)FUSED_CHUNK_LOCAL; ++r)
+      if (!((rows >> r) & 1u)) op.col_bits[c++] = r;
+  }
+  return p;
+}
+
+// Consecutive blocks whose qubits, with the run qubits, fit in a chunk form one pass.
+static void build_passes(brk_fused_plan& plan) {
+  using namespace brk;
+  const uint64_t run = (1ull << CHUNK_RUN_QUBITS) - 1ull;
+  size_t first = 0;
+  uint64_t local = run;
+  for (size_t b = 0; b <= plan.blocks.size(); ++b) {
+    uint64_t m = 0;
+    if (b < plan.blocks.size())
+      for (unsigned q : plan.blocks[b].qubits) m |= 1ull << q;
+    const bool fits = b < plan.blocks.size() && __builtin_popcountll(local | m) <= FUSED_CHUNK_LOCAL &&
+                      b - first < (size_t)FUSED_CHUNK_MAX_OPS;
+    if (!fits) {
+      if (b > first) plan.passes.push_back(make_pass(plan, first, b - first, local));
+      first = b;
+      local = run | m;
+    } else {
+      local |= m;
+    }
+  }
+}
+
 // Block orders tried per plan: the block order of the given sequence, then that of random
 // topological orders (fixed seeds: the plan is reproducible); the fewest blocks win.  Long
 // circuits try fewer (planning time grows with ops x tries).
@@ -296,6 +366,20 @@ int brk_fused_plan_create(const brk_circuit* circuit, unsigned width, const unsi
     }
   }
 
+  // 4. blocks in block order for chunks of 13 local qubits (the 4 lowest always among them)
+  const bool chunked = width <= 5 && n >= (unsigned)FUSED_CHUNK_LOCAL;
+  if (chunked) {
+    std::vector<brk_circuit::op> pseudo(groups.size());
+    for (size_t g = 0; g < groups.size(); ++g) {
+      pseudo[g].code = BRK_OP_X;
+      pseudo[g].targets.assign(groups[g].qubits.begin(), groups[g].qubits.end());
+    }
+    std::vector<fused_group> ordered;
+    ordered.reserve(groups.size());
+    for (size_t g : brk::block_order(pseudo, FUSED_CHUNK_LOCAL, CHUNK_RUN_QUBITS)) ordered.push_back(std::move(groups[g]));
+    groups = std::move(ordered);
+  }
+
   auto* plan = new (std::nothrow) brk_fused_plan;
   if (!plan) return fail(BRK_ERR_OUT_OF_MEMORY, "cannot allocate plan");
   plan->num_qubits = n;
@@ -332,11 +416,12 @@ int brk_fused_plan_create(const brk_circuit* circuit, unsigned width, const unsi
     plan->blocks.push_back(std::move(blk));
   }
 
-  // 4. FP32 real embeddings for the device (uploaded on the first apply)
+  // FP32 real embeddings for the device (uploaded on the first apply)
   const uint64_t per = fused_matrix_floats((int)width);
   plan->host_matrices.resize(per * plan->blocks.size());
   for (size_t b = 0; b < plan->blocks.size(); ++b)
     embed(plan->blocks[b].u, width, plan->host_matrices.data() + b * per);
+  if (chunked) build_passes(*plan);
   *out = plan;
   return BRK_OK;
 }
@@ -354,6 +439,15 @@ static int ensure_device_matrices(const brk_fused_plan* plan) {
   const uint64_t per = fused_matrix_floats((int)plan->width);
   for (size_t b = 0; b < plan->blocks.size(); ++b) plan->blocks[b].launch.matrix = d + b * per;
   plan->device_matrices = d;
+  if (!plan->passes.empty()) {
+    const size_t pbytes = plan->passes.size() * sizeof(fused_chunk_program);
+    if (hipMalloc((void**)&plan->device_passes, pbytes) != hipSuccess) {
+      plan->device_passes = nullptr;
+      return fail(BRK_ERR_OUT_OF_MEMORY, "cannot allocate fused chunk passes");
+    }
+    if (hipMemcpy(plan->device_passes, plan->passes.data(), pbytes, hipMemcpyHostToDevice) != hipSuccess)
+      return fail(BRK_ERR_HIP, "cannot upload fused chunk passes");
+  }
   return BRK_OK;
 }
 
@@ -383,6 +477,12 @@ int brk_fused_plan_block(const brk_fused_plan* plan, unsigned i, unsigned* qubit
   return BRK_OK;
 }
 
+int brk_fused_plan_num_passes(const brk_fused_plan* plan, unsigned* num_passes) {
+  if (!plan || !num_passes) return fail(BRK_ERR_INVALID_ARGUMENT, "null argument");
+  *num_passes = (unsigned)plan->passes.size();
+  return BRK_OK;
+}
+
 int brk_fused_plan_final_permutation(const brk_fused_plan* plan, unsigned* permutation) {
   if (!plan || !permutation) return fail(BRK_ERR_INVALID_ARGUMENT, "null argument");
   std::copy(plan->final_perm.begin(), plan->final_perm.end(), permutation);
@@ -402,14 +502,20 @@ static int apply_impl(const brk_fused_plan* plan, void* first, int dtype, float*
     (void)hipEventRecord(ev[0], st);
   }
   int status = BRK_OK;
-  for (size_t b = 0; b < plan->blocks.size() && status == BRK_OK; ++b) {
-    status = launch_fused_block(plan->blocks[b].launch, first, stream);
+  const bool passes = !plan->passes.empty() && fused_chunks_enabled();
+  const size_t launches = passes ? plan->passes.size() : plan->blocks.size();
+  const uint64_t size = 1ull << plan->num_qubits;
+  for (size_t b = 0; b < launches && status == BRK_OK; ++b) {
+    status = passes ? launch_fused_chunk(plan->device_passes + b, plan->passes[b], (int)plan->width,
+                                         plan->device_matrices, first, size, stream)
+                    : launch_fused_block(plan->blocks[b].launch, first, stream);
     if (block_ms && status == BRK_OK) (void)hipEventRecord(ev[b + 1], st);
   }
   if (block_ms) {
-    if (status == BRK_OK && hipEventSynchronize(ev.back()) != hipSuccess)
+    if (status == BRK_OK && hipEventSynchronize(ev[launches]) != hipSuccess)
       status = fail(BRK_ERR_HIP, "hipEventSynchronize failed");
-    for (size_t b = 0; status == BRK_OK && b < plan->blocks.size(); ++b)
+    for (size_t b = 0; b < plan->blocks.size(); ++b) block_ms[b] = 0.0f;
+    for (size_t b = 0; status == BRK_OK && b < launches; ++b)
       (void)hipEventElapsedTime(&block_ms[b], ev[b], ev[b + 1]);
     for (auto& e : ev) (void)hipEventDestroy(e);
   }

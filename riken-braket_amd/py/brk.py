@@ -117,6 +117,7 @@ _SIGS = {
     "brk_fused_plan_create": [_p, _u, _up, C.POINTER(C.c_void_p)],
     "brk_fused_plan_destroy": [_p],
     "brk_fused_plan_num_blocks": [_p, _up],
+    "brk_fused_plan_num_passes": [_p, _up],
     "brk_fused_plan_block": [_p, _u, _up, _up, _dp],
     "brk_fused_plan_final_permutation": [_p, _up],
     "brk_fused_apply": [_p, _p, _i, _p],
@@ -522,6 +523,8 @@ class FusedPlan:
         n = C.c_uint()
         call("brk_fused_plan_num_blocks", self.h, C.byref(n))
         self.num_blocks = n.value
+        call("brk_fused_plan_num_passes", self.h, C.byref(n))
+        self.num_passes = n.value
 
     def block(self, i, with_matrix=True):
         import numpy as np

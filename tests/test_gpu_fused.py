@@ -79,10 +79,40 @@ def test_staged_and_register_direct_kernels_agree(n, width):
     plan = brk.FusedPlan(brk.Circuit(n, ops), width)
     s = D.random_state(n, np.random.default_rng(n))
     note(f"gpu fused staged vs register-direct n={n} width={width}")
-    brk.tune("fused_staged", 0)
+    old = brk.get_tune("fused_chunks")
+    brk.tune("fused_chunks", 0)  # one launch per block
     try:
-        direct = _run_fused(plan, s)
+        brk.tune("fused_staged", 0)
+        try:
+            direct = _run_fused(plan, s)
+        finally:
+            brk.tune("fused_staged", 1)
+        staged = _run_fused(plan, s)
     finally:
-        brk.tune("fused_staged", 1)
-    staged = _run_fused(plan, s)
+        brk.tune("fused_chunks", old)
     assert np.array_equal(direct, staged)
+
+
+@pytest.mark.parametrize("n,width,seed", [(13, 4, 1), (13, 5, 2), (16, 4, 3), (17, 5, 4), (20, 4, 5), (20, 5, 6)])
+def test_chunk_passes_equal_block_launches(oracle, n, width, seed):
+    """Chunk passes (several blocks per LDS-resident chunk) against one launch per block: the
+    same MFMA products in the same order, so the same amplitudes bit for bit; and the oracle."""
+    ops = W.random_circuit(n, 160, 6, seed=seed)
+    plan = brk.FusedPlan(brk.Circuit(n, ops), width)
+    assert plan.num_passes > 0 and plan.num_passes < plan.num_blocks
+    s = D.random_state(n, np.random.default_rng(seed))
+    note(f"gpu fused chunk passes n={n} width={width}: {plan.num_blocks} blocks in {plan.num_passes} passes")
+    old = brk.get_tune("fused_chunks")
+    try:
+        brk.tune("fused_chunks", 1)
+        chunked = _run_fused(plan, s)
+        brk.tune("fused_chunks", 0)
+        per_block = _run_fused(plan, s)
+    finally:
+        brk.tune("fused_chunks", old)
+    assert np.array_equal(chunked.astype(np.complex64).view(np.uint32), per_block.astype(np.complex64).view(np.uint32))
+    expected = s.copy()
+    run_ops(oracle, expected, ops)
+    infidelity = 1 - abs(np.vdot(expected, chunked)) ** 2 / np.vdot(chunked, chunked).real
+    assert infidelity < 1e-6, infidelity
+    assert np.max(np.abs(chunked - expected)) < 2e-6
