@@ -1,6 +1,6 @@
 // brk-bra: bra's nompi front end on the MI355X engine.
 //
-//   brk-bra [--file <path>] [--threads <n>] [--seed <n>] [--amplitude-precision <p>]
+//   brk-bra [--file <path>] [--threads <n>] [--seed <n>] [--amplitude-precision <p>] [--fusion-c64 <width>]
 //
 // Reads a quantum-assembler program (from --file or standard input, as
 // bra/src/bra.cpp does), runs it on one GPU in complex<double> and prints what
@@ -16,8 +16,11 @@
 // statement inserts X, Y or Z after every gate (and CLEAR / SET) with bra's draws
 // (bra::state::apply_noise, bra/src/state.cpp:3331-3362).  --seed seeds the state's
 // std::mt19937_64 (default 1, bra/src/bra.cpp:73); --threads is accepted for
-// command-line compatibility.  --amplitude-precision (not a bra option) sets the
-// significant digits DO AMPLITUDES prints (default: the stream's 6, as bra).
+// command-line compatibility.  Two options bra does not have: --amplitude-precision
+// sets the significant digits DO AMPLITUDES prints (default: the stream's 6, as bra);
+// --fusion-c64 <width> (4, 5 or 6) runs the gates between BEGIN FUSION and END FUSION
+// through the precision form (dense width-qubit blocks on MFMA in complex64) instead of
+// the exact gate blocks.
 #include <chrono>
 #include <cstdlib>
 #include <cstdio>
@@ -114,7 +117,8 @@ void exit_statement(brk::bra::hip_state& st, timer& t) {
 }
 
 int usage() {
-  std::cerr << "usage: brk-bra [--file <path>] [--threads <n>] [--seed <n>] [--amplitude-precision <p>]\n";
+  std::cerr << "usage: brk-bra [--file <path>] [--threads <n>] [--seed <n>] [--amplitude-precision <p>] "
+               "[--fusion-c64 <width>]\n";
   return 2;
 }
 
@@ -124,11 +128,15 @@ int main(int argc, char** argv) {
   std::string file;
   brk::bra::hip_state::seed_type seed = 1;
   int amplitude_precision = 0;
+  unsigned fusion_width = 0;  // 0: exact gate blocks inside BEGIN / END FUSION
   for (int i = 1; i < argc; ++i) {
     const std::string a = argv[i];
     if ((a == "--file" || a == "-f") && i + 1 < argc) file = argv[++i];
     else if (a == "--threads" && i + 1 < argc) ++i;
-    else if (a == "--amplitude-precision" && i + 1 < argc) {
+    else if (a == "--fusion-c64" && i + 1 < argc) {
+      fusion_width = (unsigned)std::atoi(argv[++i]);
+      if (fusion_width < 4 || fusion_width > 6) return usage();
+    } else if (a == "--amplitude-precision" && i + 1 < argc) {
       amplitude_precision = std::atoi(argv[++i]);
       if (amplitude_precision < 1 || amplitude_precision > 17) return usage();
     }
@@ -169,6 +177,7 @@ int main(int argc, char** argv) {
     timer t;
     brk::bra::hip_state st(n, initial, seed);
     st.set_depolarizing_channel(prog);
+    bool in_fusion = false;
     for (unsigned i = 0; i < count; ++i) {
       int kind = 0, precision = 3;
       const brk_circuit* gates = nullptr;
@@ -176,7 +185,12 @@ int main(int argc, char** argv) {
       const std::uint64_t* idx = nullptr;
       brk::bra::check(brk_qcx_statement(prog, i, &kind, &gates, &precision, &nidx, &idx));
       switch (kind) {
-        case BRK_QCX_GATES: st.apply_blocked(prog, i); break;
+        case BRK_QCX_GATES:
+          if (in_fusion && fusion_width) st.apply_fused(prog, i, fusion_width);
+          else st.apply_blocked(prog, i);
+          break;
+        case BRK_QCX_BEGIN_FUSION: in_fusion = true; break;
+        case BRK_QCX_END_FUSION: in_fusion = false; break;
         case BRK_QCX_MEASUREMENT: measurement(st, precision, t); break;
         case BRK_QCX_AMPLITUDES:
           amplitudes(st, std::vector<std::uint64_t>(idx, idx + nidx), amplitude_precision, t);

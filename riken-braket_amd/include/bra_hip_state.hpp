@@ -175,20 +175,50 @@ class hip_state {
   // the noise gates bra would insert after each gate are drawn now, in bra's order, and join
   // the circuit, so the run keeps its gate blocks.
   void apply_blocked(const brk_qcx* program, unsigned statement) {
+    owned_circuit noisy;
+    apply_blocked(statement_circuit(program, statement, noisy));
+  }
+  // The same statement through the precision form, for a BEGIN/END FUSION bracket run that
+  // way on request: its gates (with their noise) fused into dense width-qubit blocks applied
+  // on MFMA to a complex64 copy of the state, which then replaces it (the amplitudes carry
+  // complex64 rounding).  States with fewer than width + 5 qubits run the exact gate blocks.
+  void apply_fused(const brk_qcx* program, unsigned statement, unsigned width) {
+    owned_circuit noisy;
+    const brk_circuit* c = statement_circuit(program, statement, noisy);
+    if (n_ < width + 5) return apply_blocked(c);
+    brk_fused_plan* plan = nullptr;
+    check(brk_fused_plan_create(c, width, nullptr, &plan));
+    void* c64 = nullptr;
+    int status = brk_malloc(&c64, size_ * 8);
+    if (status == BRK_OK) status = brk_convert(c64, BRK_C64, data_, BRK_C128, size_, stream_);
+    if (status == BRK_OK) status = brk_fused_apply(plan, c64, BRK_C64, stream_);
+    if (status == BRK_OK) status = brk_convert(data_, BRK_C128, c64, BRK_C64, size_, stream_);
+    if (status == BRK_OK) status = brk_stream_synchronize(stream_);
+    const std::string message = status == BRK_OK ? "" : brk_last_error();
+    if (c64) (void)brk_free(c64);
+    (void)brk_fused_plan_destroy(plan);
+    if (status != BRK_OK) throw error(status, std::string(brk_status_string(status)) + ": " + message);
+  }
+
+ private:
+  struct owned_circuit {
+    brk_circuit* c = nullptr;
+    ~owned_circuit() {
+      if (c) brk_circuit_destroy(c);
+    }
+  };
+  // The statement's circuit, or, with a depolarizing channel, a copy with the noise gates
+  // drawn now (held by `noisy`).
+  const brk_circuit* statement_circuit(const brk_qcx* program, unsigned statement, owned_circuit& noisy) {
     int kind = 0;
     const brk_circuit* gates = nullptr;
     check(brk_qcx_statement(program, statement, &kind, &gates, nullptr, nullptr, nullptr));
     if (kind != BRK_QCX_GATES) throw error(BRK_ERR_INVALID_ARGUMENT, "statement is not a run of gates");
-    if (!noisy_) return apply_blocked(gates);
+    if (!noisy_) return gates;
     unsigned num_gates = 0, nq = 0;
     check(brk_qcx_num_gates(program, statement, &num_gates));
     check(brk_circuit_num_qubits(gates, &nq));
-    brk_circuit* noisy = nullptr;
-    check(brk_circuit_create(nq, &noisy));
-    struct destroy {
-      brk_circuit* c;
-      ~destroy() { brk_circuit_destroy(c); }
-    } guard{noisy};
+    check(brk_circuit_create(nq, &noisy.c));
     for (unsigned g = 0; g < num_gates; ++g) {
       unsigned first = 0, count = 0, noise[BRK_MAX_OPERATED_QUBITS], num_noise = 0;
       check(brk_qcx_gate(program, statement, g, &first, &count, noise, &num_noise));
@@ -197,15 +227,17 @@ class hip_state {
         unsigned t[BRK_MAX_OPERATED_QUBITS], c[BRK_MAX_OPERATED_QUBITS], nt = 0, nc = 0, np = 0;
         double p[8];
         check(brk_circuit_op(gates, i, &op, t, &nt, c, &nc, p, &np));
-        check(brk_circuit_add(noisy, op, t, nt, nc ? c : nullptr, nc, np ? p : nullptr, np));
+        check(brk_circuit_add(noisy.c, op, t, nt, nc ? c : nullptr, nc, np ? p : nullptr, np));
       }
       for (unsigned k = 0; k < num_noise; ++k) {
         const int op = draw_noise();
-        if (op >= 0) check(brk_circuit_add(noisy, op, &noise[k], 1, nullptr, 0, nullptr, 0));
+        if (op >= 0) check(brk_circuit_add(noisy.c, op, &noise[k], 1, nullptr, 0, nullptr, 0));
       }
     }
-    apply_blocked(noisy);
+    return noisy.c;
   }
+
+ public:
 
   // bra::state::shor_box: exponent qubits are the top m, the rest hold base^x mod divisor
   // (bra/src/state.cpp:2173-2191, bra/src/nompi_state.cpp:1276-1280)

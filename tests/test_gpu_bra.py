@@ -292,3 +292,25 @@ def test_generate_events_fresh_seed_and_exit(oracle):
     i = [k for k, l in enumerate(lines) if l.startswith("Measurement result: ")]
     assert len(i) == 1 and int(lines[i[0]].split(": ")[1]) == ref["exit"]
     assert lines[i[0] - 1].startswith("Operations finished") and lines[i[0] + 1].startswith("Measurement finished")
+
+
+def test_fusion_bracket_on_the_precision_form(oracle):
+    """--fusion-c64 5: the gates between BEGIN FUSION and END FUSION run as fused complex64
+    blocks on MFMA; outside the bracket the exact gate blocks.  Against the oracle (FP64):
+    infidelity <= 1e-6, amplitudes within 2e-6; without the option the same program is exact."""
+    lines = ["QUBITS 12"] + [f"H {q}" for q in range(12)] + ["BEGIN FUSION"]
+    rng = np.random.default_rng(8)
+    for k in range(60):
+        a, b = (int(x) for x in rng.choice(12, 2, replace=False))
+        lines += [f"U3 {a} {rng.uniform(-3, 3)!r} {rng.uniform(-3, 3)!r} {rng.uniform(-3, 3)!r}", f"CNOT {a} {b}",
+                  f"EZ {b} {rng.uniform(-3, 3)!r}", f"SZZ {a} {b}" if k % 3 == 0 else f"CZ {a} {b}"]
+    lines += ["END FUSION", "T 3", "DO AMPLITUDES"]
+    text = "\n".join(lines) + "\n"
+    q, s = oracle_state(oracle, text)
+    fused = np.array([a for _, a in printed_amplitudes(
+        run_bra(text, ["--amplitude-precision", "17", "--fusion-c64", "5"]), 12)[0]])
+    infidelity = 1 - abs(np.vdot(s, fused)) ** 2 / (np.vdot(fused, fused).real * np.vdot(s, s).real)
+    assert infidelity < 1e-6 and np.max(np.abs(fused - s)) < 2e-6
+    assert np.max(np.abs(fused - s)) > 1e-12  # complex64 rounding shows: the precision form ran
+    exact = np.array([a for _, a in printed_amplitudes(run_bra(text, ["--amplitude-precision", "17"]), 12)[0]])
+    assert np.max(np.abs(exact - s)) < 1e-12
