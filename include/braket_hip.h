@@ -356,6 +356,18 @@ int brk_project_qubit(void* first, uint64_t size, int dtype, unsigned qubit, int
    of a cumulative boundary.  Synchronises the stream. */
 int brk_generate_events(const void* first, uint64_t size, int dtype, const double* canonical, unsigned num_events,
                         uint64_t* events, void* stream);
+/* brk_generate_events without any device allocation: no counterpart in the
+   reference.  For callers that cannot allocate device memory (tight serving
+   loops, code that forbids hipMalloc).  Same arguments, checks and events
+   (bit for bit) as brk_generate_events; its buffers live in a static device
+   array of 6.5 MiB that each device holds until the process ends.  Limits:
+   states of at most 2^32 amplitudes (32 qubits), at most 65536 events per
+   call (BRK_ERR_INVALID_ARGUMENT beyond either; the draws of a larger sample
+   can be passed in consecutive calls).  Calls on one device share that array
+   and take turns under a per-device lock held until the events are on the
+   host.  Synchronises the stream; not for use under stream capture. */
+int brk_generate_events_noalloc(const void* first, uint64_t size, int dtype, const double* canonical,
+                                unsigned num_events, uint64_t* events, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Circuits: an ordered list of operations on logical qubits, run with the  */

@@ -411,6 +411,84 @@ __global__ __launch_bounds__(256) void k_locate_events(const double* __restrict_
   target[e] = r;
 }
 
+// Running sums over whole chunks for the no-allocation sampling entry point:
+// sums[c] becomes S[c] = S[c - 1] + sums[c] (S[-1] = 0), added left to right
+// exactly as brk_generate_events adds them on the host, so both give the same
+// S bit for bit.  One wavefront: every lane adds the same values in the same
+// order (run is uniform), lane l keeps the value after the l-th add of its
+// 64-chunk window.  meta[0] = S[nchunks - 1] (the total), meta[1] = the last
+// chunk whose sum is positive as an integer (-1: none, a zero-norm state).
+__global__ __launch_bounds__(64) void k_running_sums(double* __restrict__ sums, uint64_t nchunks,
+                                                     double* __restrict__ meta) {
+  const int lane = threadIdx.x;
+  double run = 0.0;
+  int64_t last = -1;
+  for (uint64_t base = 0; base < nchunks; base += 64) {
+    const uint64_t c = base + (uint64_t)lane;
+    const double v = c < nchunks ? sums[c] : 0.0;
+    if (c < nchunks && v > 0.0) last = (int64_t)c;
+    double mine = 0.0;
+    const uint64_t left = nchunks - base;
+    if (left >= 64) {
+#pragma unroll
+      for (int l = 0; l < 64; ++l) {
+        run = run + readlane_f64(v, l);
+        if (l == lane) mine = run;
+      }
+    } else {
+      for (int l = 0; l < (int)left; ++l) {
+        run = run + readlane_f64(v, l);
+        if (l == lane) mine = run;
+      }
+    }
+    if (c < nchunks) sums[c] = mine;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const int64_t o = __shfl_xor(last, off, 64);
+    last = o > last ? o : last;
+  }
+  if (lane == 0) {
+    meta[0] = run;
+    meta[1] = (double)last;
+  }
+}
+
+// Device scratch of brk_generate_events_noalloc.  Each device holds its own
+// copy for the life of the process: the chunk sums (then their running sums)
+// of a state of up to NOALLOC_MAX_CHUNKS chunks of SAMPLE_CHUNK amplitudes
+// (2^32 amplitudes, 32 qubits in either dtype), per event its draw, chunk,
+// chunk start, target and result for up to NOALLOC_MAX_EVENTS events, then
+// the two values k_running_sums writes -- 6.5 MiB in all.  Every element a
+// call reads is written earlier in the same call, so nothing is cleared.
+// Host code never passes `scratch` itself to a kernel or a copy (on the host
+// it names a stand-in, not device memory): it uses the per-device address
+// noalloc_scratch() obtains from hipGetSymbolAddress.
+constexpr uint64_t NOALLOC_MAX_CHUNKS = 1ull << 19;
+constexpr unsigned NOALLOC_MAX_EVENTS = 1u << 16;
+constexpr uint64_t NOALLOC_DOUBLES = NOALLOC_MAX_CHUNKS + 5ull * NOALLOC_MAX_EVENTS + 2;
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+// One lock per device, held by a call from its first launch until its events
+// are back on the host: calls on one device (from any host thread) take turns.
+static std::mutex g_noalloc_busy[NOALLOC_MAX_DEVICES];
+static double* g_noalloc_ptr[NOALLOC_MAX_DEVICES];  // written under g_noalloc_busy[device]
+
+// The current device's address of `scratch`, looked up on the device's first
+// call (its lock held by the caller).
+static int noalloc_scratch(int device, double** out) {
+  if (!g_noalloc_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(BRK_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+    g_noalloc_ptr[device] = static_cast<double*>(p);
+  }
+  *out = g_noalloc_ptr[device];
+  return BRK_OK;
+}
+
 constexpr int ROW_THREADS = 256;
 
 __global__ __launch_bounds__(ROW_THREADS) void k_row_sums(const double* __restrict__ in, uint64_t cols,
@@ -676,9 +754,9 @@ int brk_generate_events(const void* first, uint64_t size, int dtype, const doubl
   // device buffers: chunk sums, then their running sums | per event: draw, chunk, chunk start,
   // target, result
   const size_t bytes = sizeof(double) * nchunks + (sizeof(uint64_t) * 2 + sizeof(double) * 3) * num_events;
-  void* scratch = nullptr;
-  BRK_HIP_CHECK(hipMalloc(&scratch, bytes));
-  double* d_sums = static_cast<double*>(scratch);
+  void* buffers = nullptr;
+  BRK_HIP_CHECK(hipMalloc(&buffers, bytes));
+  double* d_sums = static_cast<double*>(buffers);
   double* d_canonical = d_sums + nchunks;
   uint64_t* d_chunk = reinterpret_cast<uint64_t*>(d_canonical + num_events);
   double* d_start = reinterpret_cast<double*>(d_chunk + num_events);
@@ -731,7 +809,80 @@ int brk_generate_events(const void* first, uint64_t size, int dtype, const doubl
                   hipStreamSynchronize(s) != hipSuccess))
     status = fail(BRK_ERR_HIP, "copying the events failed");
   (void)hipStreamSynchronize(s);
-  (void)hipFree(scratch);
+  (void)hipFree(buffers);
+  return status;
+}
+
+int brk_generate_events_noalloc(const void* first, uint64_t size, int dtype, const double* canonical,
+                                unsigned num_events, uint64_t* events, void* stream) {
+  unsigned nq;
+  if (int st = check_state(first, size, dtype, &nq)) return st;
+  if (num_events == 0) return BRK_OK;
+  if (!canonical || !events) return fail(BRK_ERR_INVALID_ARGUMENT, "null argument");
+  for (unsigned e = 0; e < num_events; ++e)
+    if (!(canonical[e] >= 0.0 && canonical[e] < 1.0))
+      return fail(BRK_ERR_INVALID_ARGUMENT, "canonical[%u] = %g is not in [0, 1)", e, canonical[e]);
+  const uint32_t chunk = (uint32_t)std::min<uint64_t>(size, SAMPLE_CHUNK);
+  const uint64_t nchunks = size / chunk;
+  if (nchunks > NOALLOC_MAX_CHUNKS)
+    return fail(BRK_ERR_INVALID_ARGUMENT,
+                "brk_generate_events_noalloc takes states of at most %llu amplitudes (%llu given)",
+                (unsigned long long)(NOALLOC_MAX_CHUNKS * SAMPLE_CHUNK), (unsigned long long)size);
+  if (num_events > NOALLOC_MAX_EVENTS)
+    return fail(BRK_ERR_INVALID_ARGUMENT, "brk_generate_events_noalloc takes at most %u events per call (%u given)",
+                NOALLOC_MAX_EVENTS, num_events);
+  int dev = 0;
+  BRK_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+    return fail(BRK_ERR_INVALID_ARGUMENT, "device %d: no-allocation scratch exists for devices 0..%d", dev,
+                NOALLOC_MAX_DEVICES - 1);
+  std::lock_guard<std::mutex> hold(g_noalloc_busy[dev]);
+  double* d_sums = nullptr;
+  if (int st = noalloc_scratch(dev, &d_sums)) return st;
+  double* d_canonical = d_sums + NOALLOC_MAX_CHUNKS;
+  uint64_t* d_chunk = reinterpret_cast<uint64_t*>(d_canonical + NOALLOC_MAX_EVENTS);
+  double* d_start = reinterpret_cast<double*>(d_chunk + NOALLOC_MAX_EVENTS);
+  double* d_target = d_start + NOALLOC_MAX_EVENTS;
+  uint64_t* d_events = reinterpret_cast<uint64_t*>(d_target + NOALLOC_MAX_EVENTS);
+  double* d_meta = reinterpret_cast<double*>(d_events + NOALLOC_MAX_EVENTS);
+  hipStream_t s = as_stream(stream);
+  // every exit after the first launch waits for the stream, so the next call
+  // (which takes the lock) never overlaps this call's kernels on the scratch
+  auto run = [&]() -> int {
+    if (dtype == BRK_C128)
+      hipLaunchKernelGGL(k_chunk_sums<double>, dim3((unsigned)nchunks), dim3(SAMPLE_THREADS), 0, s,
+                         (const cpx<double>*)first, chunk, d_sums);
+    else
+      hipLaunchKernelGGL(k_chunk_sums<float>, dim3((unsigned)nchunks), dim3(SAMPLE_THREADS), 0, s,
+                         (const cpx<float>*)first, chunk, d_sums);
+    if (int st = check_launch("chunk sums")) return st;
+    hipLaunchKernelGGL(k_running_sums, dim3(1), dim3(64), 0, s, d_sums, nchunks, d_meta);
+    if (int st = check_launch("running sums")) return st;
+    double meta[2] = {0.0, -1.0};  // total, last chunk holding probability
+    if (hipMemcpyAsync(meta, d_meta, sizeof(meta), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return fail(BRK_ERR_HIP, "copying the running-sum total failed");
+    if (meta[1] < 0.0) return fail(BRK_ERR_INVALID_ARGUMENT, "the state has zero norm");
+    if (hipMemcpyAsync(d_canonical, canonical, sizeof(double) * num_events, hipMemcpyHostToDevice, s) !=
+        hipSuccess)
+      return fail(BRK_ERR_HIP, "uploading the draws failed");
+    hipLaunchKernelGGL(k_locate_events, dim3((num_events + 255) / 256), dim3(256), 0, s, d_sums, nchunks, meta[0],
+                       (uint64_t)meta[1], d_canonical, num_events, d_chunk, d_start, d_target);
+    if (int st = check_launch("locate events")) return st;
+    if (dtype == BRK_C128)
+      hipLaunchKernelGGL(k_chunk_search<double>, dim3(num_events), dim3(SAMPLE_THREADS), 0, s,
+                         (const cpx<double>*)first, chunk, d_chunk, d_start, d_target, d_events);
+    else
+      hipLaunchKernelGGL(k_chunk_search<float>, dim3(num_events), dim3(SAMPLE_THREADS), 0, s,
+                         (const cpx<float>*)first, chunk, d_chunk, d_start, d_target, d_events);
+    if (int st = check_launch("chunk search")) return st;
+    if (hipMemcpyAsync(events, d_events, sizeof(uint64_t) * num_events, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return fail(BRK_ERR_HIP, "copying the events failed");
+    return BRK_OK;
+  };
+  const int status = run();
+  (void)hipStreamSynchronize(s);
   return status;
 }
 

@@ -99,6 +99,7 @@ _SIGS = {
     "brk_shor_box": [_p, _u64, _i, _u, _u64, _u64, _p],
     "brk_project_qubit": [_p, _u64, _i, _u, _i, _d, _p],
     "brk_generate_events": [_p, _u64, _i, _dp, _u, C.POINTER(C.c_uint64), _p],
+    "brk_generate_events_noalloc": [_p, _u64, _i, _dp, _u, C.POINTER(C.c_uint64), _p],
     "brk_circuit_create": [_u, C.POINTER(C.c_void_p)],
     "brk_circuit_destroy": [_p],
     "brk_circuit_add": [_p, _i, _up, _u, _up, _u, _dp, _u],
@@ -368,16 +369,22 @@ def project_qubit(ptr, size, dtype, qubit, value, multiplier, stream=0):
     call("brk_project_qubit", ptr, size, dtype, qubit, value, multiplier, stream)
 
 
-def generate_events(ptr, size, dtype, canonical, stream=0):
-    """canonical: sequence of numbers in [0, 1); returns a list of basis indices."""
+def generate_events(ptr, size, dtype, canonical, stream=0, entry="brk_generate_events"):
+    """canonical: sequence of numbers in [0, 1); returns a list of basis indices.
+    entry="brk_generate_events_noalloc" takes the entry point that allocates
+    no device memory."""
     import numpy as np
 
     c = np.ascontiguousarray(canonical, dtype=np.float64).reshape(-1)
     n = int(c.size)
     out = np.zeros(max(n, 1), dtype=np.uint64)
-    call("brk_generate_events", ptr, size, dtype, c.ctypes.data_as(_dp) if n else None, n,
+    call(entry, ptr, size, dtype, c.ctypes.data_as(_dp) if n else None, n,
          out.ctypes.data_as(C.POINTER(C.c_uint64)), stream)
     return out[:n].tolist()
+
+
+def generate_events_noalloc(ptr, size, dtype, canonical, stream=0):
+    return generate_events(ptr, size, dtype, canonical, stream, entry="brk_generate_events_noalloc")
 
 
 def expectation_pauli_string(ptr, size, dtype, pauli, qubits, stream=0):
