@@ -76,7 +76,7 @@ int brk_device_name(char* buffer, size_t size);
    chunk bytes), "block_run_bytes" (power of two in [16, 4096]: the lowest
    qubits covering this many bytes stay local in every gate block),
    "adjoint_blocks" (0/1: blocked or per-run adjoint sweep), "fused_chunks"
-   (0/1: width-4/5 fused plans run as chunk passes of several blocks, or one
+   (1/0: width-4/5 fused plans run as chunk passes of several blocks, or one
    launch per block).  Each key can
    also be set by the environment variable BRK_TUNE_<KEY> read at first use.
    Process-wide and not synchronised with launches: set before launching, not
@@ -504,10 +504,11 @@ int brk_fused_plan_final_permutation(const brk_fused_plan* plan, unsigned* permu
 /* Number of chunk passes of a width-4 or width-5 plan on at least 13 qubits:
    consecutive blocks (the plan's order puts blocks on few qubits next to each
    other; blocks move only past blocks on disjoint qubits) whose qubits, with
-   the 4 lowest, fit in 13 local qubits run on one chunk of 2^13 amplitudes
-   held in LDS -- one HBM read and write per pass instead of per block --
-   with the per-block kernels' arithmetic (the same amplitudes, bit for bit).
-   0 when the plan has no passes. */
+   the 4 lowest, fit in 13 local qubits -- at most 8 blocks at width 4, 2 at
+   width 5 -- run on one chunk of 2^13 amplitudes held in LDS, one HBM read
+   and write per pass instead of per block, with the per-block kernels'
+   arithmetic (the same amplitudes, bit for bit).  0 when the plan has no
+   passes. */
 int brk_fused_plan_num_passes(const brk_fused_plan* plan, unsigned* num_passes);
 /* Applies every block in order to a BRK_C64 state of 2^num_qubits amplitudes
    (in chunk passes when the plan has them and tuning fused_chunks is 1). */

@@ -349,102 +349,197 @@ static int launch_k(const fused_launch& L, void* state, void* stream) {
 }
 
 // ---------------------------------------------------------------------------
-// Chunk passes (k = 4, 5): one 512-thread workgroup per chunk of 2^13 complex64
-// amplitudes held in LDS (64 KiB, two workgroups per CU).  The chunk is loaded
-// once, every block of the pass runs over it, and it is stored once.  A block
-// on local bits row_bits[] sees the chunk as 2^k rows x 2^(13-k) columns (the
-// other local bits); each wave takes 32-column tiles and runs the per-block
-// kernels' MFMA chain on them -- B operands gathered from LDS, A fragments from
-// the plan's matrices (L2-resident), D written back over the wave's own columns --
-// so every amplitude gets the same products in the same order as with one launch
-// per block.  A barrier separates the blocks.
+// Chunk passes (k = 4, 5): several blocks applied to one chunk of 2^13 complex64
+// amplitudes held in LDS, one HBM read and one write per pass.  A persistent
+// 1024-thread workgroup per CU walks a contiguous run of chunks; its 160 KiB of LDS
+// hold two chunk buffers (2 x 64 KiB) and the pass's embeddings (up to 8 at k = 4,
+// 2 at k = 5, loaded once).  For chunk i every thread first stores its 8 amplitudes
+// of chunk i - 1 from the other buffer and issues the loads of its 8 amplitudes of
+// chunk i + 1 into registers; then the 16 waves run the pass's blocks on chunk i
+// (one barrier per block) and, before the last block's barrier, each thread writes
+// its loaded amplitudes into the other buffer.  A barrier waits only for LDS
+// operations and the blocks issue no global memory accesses, so the chunk transfers
+// stay in flight while the blocks run.  A thread touches the same 8 slots of either
+// buffer for the transfers, so its reads (for the stores) and later writes (of the
+// next chunk) of one buffer are ordered by program order.
+//
+// A block on local bits row_bits[] sees the chunk as 2^k rows x 2^(13-k) columns
+// (the other local bits), 2^(8-k) tiles of 32 columns.  Each wave runs the per-block
+// kernels' MFMA chain for one 32-row output tile of one column tile -- B operands
+// gathered from LDS, A fragments from the block's embedding in LDS, D written back
+// over the tile's elements: at k = 4 wave w takes column tile w (the single output
+// tile), at k = 5 column tile w % 8 and output tile w / 8 (the real or the imaginary
+// parts) -- so every amplitude gets the same products in the same order as with one
+// launch per block.
 // ---------------------------------------------------------------------------
 typedef const __attribute__((address_space(4))) fused_chunk_program* cchunk;
 
+constexpr int CHUNK_THREADS = 1024;
+constexpr size_t CHUNK_LDS_BYTES = 160 * 1024;  // the whole LDS of a CU
+constexpr size_t CHUNK_BUFFER_BYTES = sizeof(cpx<float>) << FUSED_CHUNK_LOCAL;
+
+template <int KB>
+struct chunk_shape {
+  static constexpr int R = 1 << KB, W = 2 * R, HALF = R / 2, DT = W / 32;
+  static constexpr int NCOL = FUSED_CHUNK_LOCAL - KB;  // column bits
+  static constexpr int NTILES = 1 << (NCOL - 5);       // 32-column tiles per block
+  static constexpr int WAVES = CHUNK_THREADS / 64;
+  static constexpr int AFLOATS = W * W;                // one embedding
+  static constexpr int SLOTS = fused_chunk_max_blocks(KB);
+  static constexpr int SB = 8;                         // MFMA steps per batch of A-fragment reads
+  static_assert(NTILES * DT == WAVES, "one (column tile, output tile) per wave");
+  static_assert(2 * CHUNK_BUFFER_BYTES + sizeof(float) * AFLOATS * SLOTS <= CHUNK_LDS_BYTES, "LDS");
+};
+
 template <int KB, bool NT>
-__global__ __launch_bounds__(FUSED_CHUNK_THREADS, 2) void k_fused_chunk(cpx<float>* __restrict__ s,
-                                                                         const fused_chunk_program* program,
-                                                                         const float* __restrict__ matrices) {
-  constexpr int R = 1 << KB, W = 2 * R, HALF = R / 2, DT = W / 32;
-  constexpr int PER = (1 << FUSED_CHUNK_LOCAL) / FUSED_CHUNK_THREADS;  // 16 amplitudes per thread
-  constexpr int NCOL = FUSED_CHUNK_LOCAL - KB;                         // column bits
-  constexpr int NTILES = 1 << (NCOL - 5);                              // 32-column tiles per block
-  constexpr int WAVES = FUSED_CHUNK_THREADS / 64;
-  __shared__ cpx<float> lds[1 << FUSED_CHUNK_LOCAL];
+__global__ __launch_bounds__(CHUNK_THREADS, 1) void k_fused_chunk(cpx<float>* __restrict__ s,
+                                                                   const fused_chunk_program* program,
+                                                                   const float* __restrict__ matrices,
+                                                                   uint64_t nchunks) {
+  using S = chunk_shape<KB>;
+  constexpr int R = S::R, W = S::W, HALF = S::HALF;
+  constexpr int CHUNK = 1 << FUSED_CHUNK_LOCAL;
+  constexpr int PER = CHUNK / CHUNK_THREADS;  // 8 amplitudes per thread in the transfers
+  extern __shared__ float smem[];
+  cpx<float>* bufs = reinterpret_cast<cpx<float>*>(smem);  // two chunk buffers
+  float* slots = smem + 4 * CHUNK;                          // the pass's embeddings (layout of the per-block kernels)
   const cchunk prog = (cchunk)program;
   const uint32_t t = threadIdx.x;
+  const uint32_t nops = prog->num_ops;  // 1 .. SLOTS (checked by the host)
 
-  // chunk number -> the bits outside the local positions (XCD-contiguous chunk order)
-  const uint64_t chunk = xcd_tile(blockIdx.x, gridDim.x, 1);
-  uint64_t base = chunk & prog->outer_masks[0];
-  for (int j = 1; j <= FUSED_CHUNK_LOCAL; ++j) base |= (chunk & prog->outer_masks[j]) << j;
-  uint64_t low = 0;  // local index t + 512 k: the bits of t go to pos[0 .. 8]
-  for (int r = 0; r < 9; ++r) low |= (uint64_t)((t >> r) & 1u) << prog->pos[r];
+  // This workgroup's chunks.  Workgroup b runs on XCD b % 8; with the grid a multiple of 8,
+  // XCD x walks the contiguous chunk range [x N/8, (x+1) N/8) with its G/8 workgroups, else
+  // the workgroups interleave over all chunks.
+  const uint64_t G = gridDim.x;
+  const bool by_xcd = (G % 8) == 0 && (nchunks % 8) == 0 && nchunks >= G;
+  const uint64_t g8 = G / 8, per_xcd = nchunks / 8, xcd = blockIdx.x % 8, lw = blockIdx.x / 8;
+  const uint64_t iters = by_xcd ? (per_xcd + g8 - 1 - lw) / g8 : (nchunks + G - 1 - blockIdx.x) / G;
+  auto chunk_of = [&](uint64_t it) { return by_xcd ? xcd * per_xcd + lw + it * g8 : blockIdx.x + it * G; };
+  // chunk number -> the bits outside the local positions
+  auto base_of = [&](uint64_t c) {
+    uint64_t b = c & prog->outer_masks[0];
+    for (int j = 1; j <= FUSED_CHUNK_LOCAL; ++j) b |= (c & prog->outer_masks[j]) << j;
+    return b;
+  };
+  // transfers: local index t + 1024 k, the bits of t at pos[0 .. 9], k at pos[10 .. 12]
+  // (deposit_high[2k] places local index 1024 k = (2k) << 9)
+  uint64_t low = 0;
+  for (int r = 0; r < 10; ++r) low |= (uint64_t)((t >> r) & 1u) << prog->pos[r];
+
+  // prologue: the pass's embeddings into their slots, chunk 0 into buffer 0
+  for (uint32_t m = 0; m < nops; ++m) {
+    const v4f* A4 = reinterpret_cast<const v4f*>(matrices + (uint64_t)prog->ops[m].matrix * (uint64_t)S::AFLOATS);
+    for (uint32_t i = t; i < (uint32_t)(S::AFLOATS / 4); i += CHUNK_THREADS)
+      reinterpret_cast<v4f*>(slots + m * S::AFLOATS)[i] = A4[i];
+  }
+  cpx<float> pre[PER];
+  if (iters > 0) {
+    const uint64_t b = base_of(chunk_of(0));
 #pragma unroll
-  for (int k = 0; k < PER; ++k) lds[t + FUSED_CHUNK_THREADS * k] = load_amp<NT>(s + (base | low | prog->deposit_high[k]));
+    for (int k = 0; k < PER; ++k) pre[k] = load_amp<NT>(s + (b | low | prog->deposit_high[2 * k]));
+#pragma unroll
+    for (int k = 0; k < PER; ++k) bufs[t + CHUNK_THREADS * k] = pre[k];
+  }
   __syncthreads();
 
   const int lane = t & 63, wave = t >> 6;
   const int i32 = lane & 31, kk = lane >> 5;
   const int swz = (W >= 64 && kk) ? 32 : 0;
-  const uint32_t nops = prog->num_ops;
-  for (uint32_t o = 0; o < nops; ++o) {
-    const auto& op = prog->ops[o];
-    const float* A = matrices + (uint64_t)op.matrix * (uint64_t)(W * W);
-    // this lane's A fragments (the per-block kernels read the same elements from LDS)
-    float a[DT][R];
+  const int tile = wave % S::NTILES;  // column tile
+  const int half = wave / S::NTILES;  // output tile: real rows (0) or imaginary rows (1) at k = 5
+  for (uint64_t it = 0; it < iters; ++it) {
+    cpx<float>* lds = bufs + (it & 1) * CHUNK;
+    cpx<float>* other = bufs + ((it + 1) & 1) * CHUNK;
+    if (it > 0) {  // chunk it - 1 back to HBM
+      const uint64_t b = base_of(chunk_of(it - 1));
 #pragma unroll
-    for (int st = 0; st < R; ++st)
+      for (int k = 0; k < PER; ++k)
+        store_amp<NT>(s + (b | low | prog->deposit_high[2 * k]), other[t + CHUNK_THREADS * k]);
+    }
+    const bool more = it + 1 < iters;
+    if (more) {  // chunk it + 1 into registers
+      const uint64_t b = base_of(chunk_of(it + 1));
 #pragma unroll
-      for (int tt = 0; tt < DT; ++tt) a[tt][st] = A[(2 * st + kk) * W + ((32 * tt + i32) ^ swz)];
-    // row offsets: uniform parts (the row value without the lane's kk bit) plus kk at block-row
-    // bit 0 for the B gathers and at bit 2 for the D stores
-    auto rowoff = [&](int r) {
-      uint32_t d = 0;
+      for (int k = 0; k < PER; ++k) pre[k] = load_amp<NT>(s + (b | low | prog->deposit_high[2 * k]));
+    }
+    for (uint32_t o = 0; o < nops; ++o) {
+      const auto& op = prog->ops[o];
+      const float* Al = slots + o * S::AFLOATS;
+      // row offsets: uniform parts (the row value without the lane's kk bit) plus kk at block-row
+      // bit 0 for the B gathers and at bit 2 for the D stores
+      auto rowoff = [&](int r) {
+        uint32_t d = 0;
 #pragma unroll
-      for (int b = 0; b < KB; ++b) d |= ((uint32_t)(r >> b) & 1u) << op.row_bits[b];
-      return d;
-    };
-    const uint32_t kk0 = kk ? 1u << op.row_bits[0] : 0u, kk2 = kk ? 1u << op.row_bits[2] : 0u;
-    uint32_t col_lane = 0;  // column bits 0..4 from i32
+        for (int b = 0; b < KB; ++b) d |= ((uint32_t)(r >> b) & 1u) << op.row_bits[b];
+        return d;
+      };
+      const uint32_t kk0 = kk ? 1u << op.row_bits[0] : 0u, kk2 = kk ? 1u << op.row_bits[2] : 0u;
+      uint32_t col = 0;  // column bits 0..4 from i32, 5.. from the tile number
 #pragma unroll
-    for (int b = 0; b < 5; ++b) col_lane |= ((uint32_t)(i32 >> b) & 1u) << op.col_bits[b];
-    for (int tile = wave; tile < NTILES; tile += WAVES) {
-      uint32_t col = col_lane;  // column bits 5.. from the tile number
+      for (int b = 0; b < 5; ++b) col |= ((uint32_t)(i32 >> b) & 1u) << op.col_bits[b];
 #pragma unroll
-      for (int b = 5; b < NCOL; ++b) col |= ((uint32_t)(tile >> (b - 5)) & 1u) << op.col_bits[b];
+      for (int b = 5; b < S::NCOL; ++b) col |= ((uint32_t)(tile >> (b - 5)) & 1u) << op.col_bits[b];
       cpx<float> x[HALF];
 #pragma unroll
       for (int j = 0; j < HALF; ++j) x[j] = lds[col | rowoff(2 * j) | kk0];
-      f16v acc[DT];
+      f16v acc = f16v{};
 #pragma unroll
-      for (int tt = 0; tt < DT; ++tt) acc[tt] = f16v{};
+      for (int sb = 0; sb < R; sb += S::SB) {
+        float a[S::SB];
 #pragma unroll
-      for (int st = 0; st < R; ++st) {
-        const float b = st < HALF ? x[st].re : x[st - HALF].im;
+        for (int st = 0; st < S::SB; ++st) a[st] = Al[(2 * (sb + st) + kk) * W + ((32 * half + i32) ^ swz)];
 #pragma unroll
-        for (int tt = 0; tt < DT; ++tt) acc[tt] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[tt][st], b, acc[tt], 0, 0, 0);
+        for (int st = 0; st < S::SB; ++st) {
+          const float b = sb + st < HALF ? x[sb + st].re : x[sb + st - HALF].im;
+          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[st], b, acc, 0, 0, 0);
+        }
       }
-      if constexpr (DT == 1) {
+      // every wave has read its B operands before any wave writes D (the tiles' elements overlap
+      // only within a column tile, which the two waves of a tile at k = 5 both gather)
+      if constexpr (S::DT == 2) __syncthreads();
+      if constexpr (S::DT == 1) {
+        // rows 0..15 real parts (v < 8), rows 16..31 imaginary parts (v + 8)
 #pragma unroll
         for (int v = 0; v < 8; ++v) {
           const int row = 8 * (v >> 2) + (v & 3);  // + 4 kk
-          lds[col | rowoff(row) | kk2] = cpx<float>{acc[0][v], acc[0][v + 8]};
+          lds[col | rowoff(row) | kk2] = cpx<float>{acc[v], acc[v + 8]};
         }
       } else {
+        // this wave's 32 real rows: the real (half 0) or imaginary (half 1) parts of complex rows 0..31
+        float* f = reinterpret_cast<float*>(lds);
 #pragma unroll
-        for (int tt = 0; tt < DT / 2; ++tt)
-#pragma unroll
-          for (int v = 0; v < 16; ++v) {
-            const int row = 32 * tt + 8 * (v >> 2) + (v & 3);  // + 4 kk
-            lds[col | rowoff(row) | kk2] = cpx<float>{acc[tt][v], acc[tt + DT / 2][v]};
-          }
+        for (int v = 0; v < 16; ++v) {
+          const int row = 8 * (v >> 2) + (v & 3);  // + 4 kk
+          f[2 * (col | rowoff(row) | kk2) + half] = acc[v];
+        }
       }
-    }
-    __syncthreads();
-  }
+      if (more && o + 1 == nops) {
+        // chunk it + 1 into the other buffer (this thread's slots), before the chunk's last barrier
 #pragma unroll
-  for (int k = 0; k < PER; ++k) store_amp<NT>(s + (base | low | prog->deposit_high[k]), lds[t + FUSED_CHUNK_THREADS * k]);
+        for (int k = 0; k < PER; ++k) other[t + CHUNK_THREADS * k] = pre[k];
+      }
+      __syncthreads();  // the block done: D rows (and, after the last block, the next chunk) in LDS
+    }
+  }
+  if (iters > 0) {  // the last chunk back to HBM
+    const cpx<float>* last = bufs + ((iters - 1) & 1) * CHUNK;
+    const uint64_t b = base_of(chunk_of(iters - 1));
+#pragma unroll
+    for (int k = 0; k < PER; ++k)
+      store_amp<NT>(s + (b | low | prog->deposit_high[2 * k]), last[t + CHUNK_THREADS * k]);
+  }
+}
+
+template <int KB, bool NT>
+static int launch_chunk(const fused_chunk_program* device_program, const float* matrices, cpx<float>* s,
+                        uint64_t chunks, hipStream_t st) {
+  constexpr size_t lds = CHUNK_LDS_BYTES;
+  if (const int rc = allow_dynamic_lds((const void*)k_fused_chunk<KB, NT>, lds)) return rc;
+  // one persistent workgroup per CU (a multiple of 8 when the chunks allow: XCD-contiguous runs)
+  const unsigned grid = (unsigned)std::min<uint64_t>(chunks, (uint64_t)device_cus());
+  hipLaunchKernelGGL((k_fused_chunk<KB, NT>), dim3(grid), dim3(CHUNK_THREADS), lds, st, s, device_program, matrices,
+                     chunks);
+  return check_launch("fused chunk pass");
 }
 
 bool fused_chunks_enabled() { return get_tuning().fused_chunks != 0; }
@@ -453,7 +548,9 @@ int launch_fused_chunk(const fused_chunk_program* device_program, const fused_ch
                        const float* matrices, void* state, uint64_t size, void* stream) {
   unsigned nq;
   if (int st = check_state(state, size, BRK_C64, &nq)) return st;
-  if (nq < (unsigned)FUSED_CHUNK_LOCAL || program.num_ops > (uint32_t)FUSED_CHUNK_MAX_OPS || (width != 4 && width != 5))
+  // (a pass holds 1 .. fused_chunk_max_blocks(width) blocks: the embeddings it keeps in LDS)
+  if (nq < (unsigned)FUSED_CHUNK_LOCAL || (width != 4 && width != 5) || program.num_ops == 0 ||
+      program.num_ops > (uint32_t)fused_chunk_max_blocks(width))
     return fail(BRK_ERR_INVALID_ARGUMENT, "fused chunk pass does not fit (%u qubits, %u ops, width %d)", nq,
                 program.num_ops, width);
   const uint64_t chunks = size >> FUSED_CHUNK_LOCAL;
@@ -461,15 +558,11 @@ int launch_fused_chunk(const fused_chunk_program* device_program, const fused_ch
   const bool nt = size * sizeof(cpx<float>) >= get_tuning().nt_min_bytes;
   hipStream_t st = as_stream(stream);
   auto* s = (cpx<float>*)state;
-  const dim3 grid((unsigned)chunks), block(FUSED_CHUNK_THREADS);
-  if (width == 4) {
-    if (nt) hipLaunchKernelGGL((k_fused_chunk<4, true>), grid, block, 0, st, s, device_program, matrices);
-    else hipLaunchKernelGGL((k_fused_chunk<4, false>), grid, block, 0, st, s, device_program, matrices);
-  } else {
-    if (nt) hipLaunchKernelGGL((k_fused_chunk<5, true>), grid, block, 0, st, s, device_program, matrices);
-    else hipLaunchKernelGGL((k_fused_chunk<5, false>), grid, block, 0, st, s, device_program, matrices);
-  }
-  return check_launch("fused chunk pass");
+  if (width == 4)
+    return nt ? launch_chunk<4, true>(device_program, matrices, s, chunks, st)
+              : launch_chunk<4, false>(device_program, matrices, s, chunks, st);
+  return nt ? launch_chunk<5, true>(device_program, matrices, s, chunks, st)
+            : launch_chunk<5, false>(device_program, matrices, s, chunks, st);
 }
 
 int launch_fused_block(const fused_launch& L, void* state, void* stream) {

@@ -65,9 +65,12 @@ struct fused_chunk_program {
   uint64_t deposit_high[16];          // physical offset of local index k << 9 (k < 16)
   fused_chunk_op ops[FUSED_CHUNK_MAX_OPS];
 };
+// The blocks of one pass at width k: the pass's embeddings (2^(k+1) x 2^(k+1) FP32) stay
+// in 32 KiB of LDS beside two chunk buffers: 8 at k = 4, 2 at k = 5.
+constexpr int fused_chunk_max_blocks(int k) { return k == 4 ? 8 : k == 5 ? 2 : 0; }
 // Launches one pass of width-k blocks over a complex64 state of `size` amplitudes:
-// 2^(n - 13) workgroups.  `device_program` is the pass in device memory, `program`
-// its host copy, `matrices` the plan's device embeddings.
+// one persistent workgroup per CU.  `device_program` is the pass in device memory,
+// `program` its host copy, `matrices` the plan's device embeddings.
 int launch_fused_chunk(const fused_chunk_program* device_program, const fused_chunk_program& program, int width,
                        const float* matrices, void* state, uint64_t size, void* stream);
 

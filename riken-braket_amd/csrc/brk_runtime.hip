@@ -62,7 +62,7 @@ static tuning g_tuning = {
     /*pair_lds=*/40960, /*diag_lds=*/0, /*pair_remap=*/1, /*diag_remap=*/1,
     /*nt_min_bytes=*/256ull << 20, /*max_blocks=*/(1ull << 24) - 1, /*line_bytes=*/64, /*fused_groups=*/8192,
     /*fused_occupancy=*/2, /*fused_staged=*/1, /*fused_staged_groups=*/768, /*fused_chunked=*/0,
-    /*block_lds=*/65536, /*block_run_bytes=*/128, /*adjoint_blocks=*/1, /*fused_chunks=*/0,
+    /*block_lds=*/65536, /*block_run_bytes=*/128, /*adjoint_blocks=*/1, /*fused_chunks=*/1,
 };
 static std::once_flag g_tuning_once;
 

@@ -299,7 +299,8 @@ static brk::fused_chunk_program make_pass(const brk_fused_plan& plan, size_t fir
   return p;
 }
 
-// Consecutive blocks whose qubits, with the run qubits, fit in a chunk form one pass.
+// Consecutive blocks whose qubits, with the run qubits, fit in a chunk form one pass, up to
+// the number of embeddings a pass keeps in LDS.
 static void build_passes(brk_fused_plan& plan) {
   using namespace brk;
   const uint64_t run = (1ull << CHUNK_RUN_QUBITS) - 1ull;
@@ -310,7 +311,7 @@ static void build_passes(brk_fused_plan& plan) {
     if (b < plan.blocks.size())
       for (unsigned q : plan.blocks[b].qubits) m |= 1ull << q;
     const bool fits = b < plan.blocks.size() && __builtin_popcountll(local | m) <= FUSED_CHUNK_LOCAL &&
-                      b - first < (size_t)FUSED_CHUNK_MAX_OPS;
+                      b - first < (size_t)fused_chunk_max_blocks((int)plan.width);
     if (!fits) {
       if (b > first) plan.passes.push_back(make_pass(plan, first, b - first, local));
       first = b;
@@ -321,10 +322,83 @@ static void build_passes(brk_fused_plan& plan) {
   }
 }
 
+// The block's qubits widened to `width` with the lowest others (the block acts as the
+// identity on them), as a mask.
+static uint64_t widened_mask(const std::set<unsigned>& S, unsigned width) {
+  uint64_t m = 0;
+  for (unsigned q : S) m |= 1ull << q;
+  for (unsigned q = 0; (unsigned)__builtin_popcountll(m) < width; ++q) m |= 1ull << q;
+  return m;
+}
+
+// The number of passes build_passes makes of these blocks, in this order.
+static size_t count_passes(const std::vector<fused_group>& groups, unsigned width) {
+  const uint64_t run = (1ull << CHUNK_RUN_QUBITS) - 1ull;
+  const size_t cap = (size_t)brk::fused_chunk_max_blocks((int)width);
+  size_t passes = 0, in_pass = 0;
+  uint64_t local = run;
+  for (const auto& g : groups) {
+    const uint64_t m = widened_mask(g.qubits, width);
+    if (in_pass == 0 || __builtin_popcountll(local | m) > brk::FUSED_CHUNK_LOCAL || in_pass >= cap) {
+      ++passes;
+      local = run | m;
+      in_pass = 1;
+    } else {
+      local |= m;
+      ++in_pass;
+    }
+  }
+  return passes;
+}
+
+// Orders the blocks for chunk passes: the given order, the block order for chunks of 13 local
+// qubits (the 4 lowest always among them) of it and of random dependence-respecting orders of
+// it (blocks sharing a qubit keep their order; fixed seeds, so plans are reproducible); the
+// order giving the fewest passes wins (the given one on a tie).  Returns that number.
+static size_t order_for_chunks(std::vector<fused_group>& groups, unsigned width, int tries) {
+  std::vector<brk_circuit::op> pseudo(groups.size());
+  for (size_t g = 0; g < groups.size(); ++g) {
+    pseudo[g].code = BRK_OP_X;
+    pseudo[g].targets.assign(groups[g].qubits.begin(), groups[g].qubits.end());
+  }
+  std::vector<size_t> best(groups.size());
+  for (size_t g = 0; g < best.size(); ++g) best[g] = g;
+  size_t best_passes = count_passes(groups, width);
+  for (int t = 0; t < tries; ++t) {
+    std::vector<size_t> seq(groups.size());
+    if (t == 0) {
+      for (size_t g = 0; g < seq.size(); ++g) seq[g] = g;
+    } else {
+      std::mt19937_64 gen((uint64_t)t);
+      seq = random_topological_order(pseudo, gen);
+    }
+    std::vector<brk_circuit::op> ops;
+    ops.reserve(seq.size());
+    for (size_t g : seq) ops.push_back(pseudo[g]);
+    std::vector<size_t> order;
+    order.reserve(seq.size());
+    for (size_t i : brk::block_order(ops, brk::FUSED_CHUNK_LOCAL, CHUNK_RUN_QUBITS)) order.push_back(seq[i]);
+    std::vector<fused_group> trial;
+    trial.reserve(order.size());
+    for (size_t g : order) trial.push_back(groups[g]);
+    const size_t p = count_passes(trial, width);
+    if (p < best_passes) {
+      best_passes = p;
+      best = std::move(order);
+    }
+  }
+  std::vector<fused_group> ordered;
+  ordered.reserve(best.size());
+  for (size_t g : best) ordered.push_back(std::move(groups[g]));
+  groups = std::move(ordered);
+  return best_passes;
+}
+
 // Block orders tried per plan: the block order of the given sequence, then that of random
 // topological orders (fixed seeds: the plan is reproducible); the fewest blocks win.  Long
 // circuits try fewer (planning time grows with ops x tries).
 constexpr int FUSION_ORDER_TRIES = 32;
+constexpr int CHUNK_ORDER_TRIES = 8;  // chunk orders tried per candidate gate order
 static int fusion_order_tries(size_t num_ops) {
   return num_ops <= 4096 ? FUSION_ORDER_TRIES : num_ops <= 65536 ? 4 : 1;
 }
@@ -341,6 +415,9 @@ int brk_fused_plan_create(const brk_circuit* circuit, unsigned width, const unsi
   if (permutation && !perm.load(permutation, n)) return fail(BRK_ERR_INVALID_ARGUMENT, "bad permutation");
 
   // 1-2. physical gate lists of candidate orders, greedy fusion of each; the fewest blocks win
+  // (for chunk passes: the fewest passes, then the fewest blocks)
+  const bool chunked = width <= 5 && n >= (unsigned)FUSED_CHUNK_LOCAL;
+  size_t best_passes = 0;
   std::vector<physical_gate> gates, trial;
   std::vector<fused_group> groups;
   std::vector<unsigned> final_perm, trial_perm;
@@ -359,26 +436,18 @@ int brk_fused_plan_create(const brk_circuit* circuit, unsigned width, const unsi
     }
     if (int st = physical_gates(*circuit, order, perm, width, trial, trial_perm)) return st;
     std::vector<fused_group> g = group_gates(trial, width);
-    if (t == 0 || g.size() < groups.size()) {
+    // chunk passes: the fewest passes (each reads and writes the state once), then the fewest blocks
+    const size_t passes = chunked ? order_for_chunks(g, width, CHUNK_ORDER_TRIES) : 0;
+    if (t == 0 || passes < best_passes || (passes == best_passes && g.size() < groups.size())) {
       groups = std::move(g);
       gates.swap(trial);
       final_perm = trial_perm;
+      best_passes = passes;
     }
   }
 
-  // 4. blocks in block order for chunks of 13 local qubits (the 4 lowest always among them)
-  const bool chunked = width <= 5 && n >= (unsigned)FUSED_CHUNK_LOCAL;
-  if (chunked) {
-    std::vector<brk_circuit::op> pseudo(groups.size());
-    for (size_t g = 0; g < groups.size(); ++g) {
-      pseudo[g].code = BRK_OP_X;
-      pseudo[g].targets.assign(groups[g].qubits.begin(), groups[g].qubits.end());
-    }
-    std::vector<fused_group> ordered;
-    ordered.reserve(groups.size());
-    for (size_t g : brk::block_order(pseudo, FUSED_CHUNK_LOCAL, CHUNK_RUN_QUBITS)) ordered.push_back(std::move(groups[g]));
-    groups = std::move(ordered);
-  }
+  // 4. blocks in an order for chunks of 13 local qubits that packs them into few passes
+  if (chunked) (void)order_for_chunks(groups, width, fusion_order_tries(groups.size()));
 
   auto* plan = new (std::nothrow) brk_fused_plan;
   if (!plan) return fail(BRK_ERR_OUT_OF_MEMORY, "cannot allocate plan");
