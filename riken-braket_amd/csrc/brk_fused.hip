@@ -309,21 +309,6 @@ static size_t staged_lds_bytes() {
          128 * sizeof(uint32_t) + 128 * sizeof(uint64_t);
 }
 
-// Raises a kernel's dynamic-LDS limit (above 64 KiB a kernel has to opt in) once per device:
-// the attribute belongs to the device the call is made on, and launches may come from several
-// host threads.
-static int allow_dynamic_lds(const void* kernel, size_t bytes) {
-  static std::mutex m;
-  static std::set<std::pair<const void*, int>> done;
-  int dev = 0;
-  BRK_HIP_CHECK(hipGetDevice(&dev));
-  std::lock_guard<std::mutex> lock(m);
-  if (done.count({kernel, dev})) return BRK_OK;
-  BRK_HIP_CHECK(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
-  done.insert({kernel, dev});
-  return BRK_OK;
-}
-
 template <int KB>
 static int launch_staged(const fused_launch& L, void* state, void* stream) {
   const size_t lds = staged_lds_bytes<KB>();

@@ -937,21 +937,22 @@ unsigned block_run_qubits(int dtype) {
 }
 
 template <typename R, bool NT, int PER>
-static void launch_block_inst(uint64_t chunks, hipStream_t st, cpx<R>* s, const block_program* p) {
+static int launch_block_inst(uint64_t chunks, hipStream_t st, cpx<R>* s, const block_program* p) {
   hipLaunchKernelGGL((k_block<R, NT, PER>), dim3((unsigned)chunks), dim3(BLOCK_THREADS), 0, st, s, p);
+  return BRK_OK;
 }
 
 // Picks the instantiation whose chunk is exactly 2^L amplitudes (L = 9 .. 13).
 template <typename R, bool NT>
-static void launch_block_per(uint64_t chunks, hipStream_t st, cpx<R>* s, const block_program* p, unsigned L) {
+static int launch_block_per(uint64_t chunks, hipStream_t st, cpx<R>* s, const block_program* p, unsigned L) {
   switch (L - BLOCK_THREAD_BITS) {
-    case 0: launch_block_inst<R, NT, 1>(chunks, st, s, p); break;
-    case 1: launch_block_inst<R, NT, 2>(chunks, st, s, p); break;
-    case 2: launch_block_inst<R, NT, 4>(chunks, st, s, p); break;
-    case 3: launch_block_inst<R, NT, 8>(chunks, st, s, p); break;
+    case 0: return launch_block_inst<R, NT, 1>(chunks, st, s, p);
+    case 1: return launch_block_inst<R, NT, 2>(chunks, st, s, p);
+    case 2: return launch_block_inst<R, NT, 4>(chunks, st, s, p);
+    case 3: return launch_block_inst<R, NT, 8>(chunks, st, s, p);
     default:
-      if constexpr (sizeof(R) == 4) launch_block_inst<R, NT, 16>(chunks, st, s, p);
-      break;
+      if constexpr (sizeof(R) == 4) return launch_block_inst<R, NT, 16>(chunks, st, s, p);
+      return fail(BRK_ERR_INVALID_ARGUMENT, "no gate-block kernel for %u local qubits", L);
   }
 }
 
@@ -961,21 +962,22 @@ int launch_block(const block_program* device_program, const block_program& progr
   if (int st = check_state(first, size, dtype, &nq)) return st;
   const uint32_t cap = dtype == BRK_C128 ? BLOCK_MAX_LOCAL_C128 : BLOCK_MAX_LOCAL_C64;
   if (program.num_local > cap || program.num_local < (uint32_t)BLOCK_THREAD_BITS || program.num_local > nq ||
-      program.num_ops > (uint32_t)BLOCK_MAX_OPS)
-    return fail(BRK_ERR_INVALID_ARGUMENT, "block program does not fit (%u local qubits, %u ops)", program.num_local,
-                program.num_ops);
+      program.num_ops > (uint32_t)BLOCK_MAX_OPS || program.num_passes > (uint32_t)BLOCK_MAX_OPS)
+    return fail(BRK_ERR_INVALID_ARGUMENT, "block program does not fit (%u local qubits, %u ops, %u passes)",
+                program.num_local, program.num_ops, program.num_passes);
   const uint64_t chunks = size >> program.num_local;
   if (chunks > 0x7fffffffull) return fail(BRK_ERR_INVALID_ARGUMENT, "too many chunks");
   const bool nt = size * amp_bytes(dtype) >= get_tuning().nt_min_bytes;
   hipStream_t st = as_stream(stream);
   const unsigned L = program.num_local;
-  if (dtype == BRK_C128) {
-    if (nt) launch_block_per<double, true>(chunks, st, (cpx<double>*)first, device_program, L);
-    else launch_block_per<double, false>(chunks, st, (cpx<double>*)first, device_program, L);
-  } else {
-    if (nt) launch_block_per<float, true>(chunks, st, (cpx<float>*)first, device_program, L);
-    else launch_block_per<float, false>(chunks, st, (cpx<float>*)first, device_program, L);
-  }
+  int rc;
+  if (dtype == BRK_C128)
+    rc = nt ? launch_block_per<double, true>(chunks, st, (cpx<double>*)first, device_program, L)
+            : launch_block_per<double, false>(chunks, st, (cpx<double>*)first, device_program, L);
+  else
+    rc = nt ? launch_block_per<float, true>(chunks, st, (cpx<float>*)first, device_program, L)
+            : launch_block_per<float, false>(chunks, st, (cpx<float>*)first, device_program, L);
+  if (rc) return rc;
   return check_launch("block");
 }
 

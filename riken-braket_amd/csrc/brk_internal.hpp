@@ -136,6 +136,11 @@ tuning& get_tuning();
 // Compute units of the current device (cached per device).
 unsigned device_cus();
 
+// Raises a kernel's dynamic-LDS limit to `bytes` (above 64 KiB a kernel has to opt in) once per
+// device: the attribute belongs to the device the call is made on, and launches may come from
+// several host threads.
+int allow_dynamic_lds(const void* kernel, size_t bytes);
+
 // Bit insertion: see brk_bits.hpp.
 
 template <int K>

@@ -9,6 +9,8 @@
 
 #include <memory>
 #include <mutex>
+#include <set>
+#include <utility>
 #include <cstdlib>
 #include <string>
 #include <type_traits>
@@ -107,6 +109,18 @@ unsigned device_cus() {
     cus[dev] = (unsigned)n;
   }
   return cus[dev];
+}
+
+int allow_dynamic_lds(const void* kernel, size_t bytes) {
+  static std::mutex m;
+  static std::set<std::pair<const void*, int>> done;
+  int dev = 0;
+  BRK_HIP_CHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(m);
+  if (done.count({kernel, dev})) return BRK_OK;
+  BRK_HIP_CHECK(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  done.insert({kernel, dev});
+  return BRK_OK;
 }
 
 static std::mutex g_scratch_mutex;
