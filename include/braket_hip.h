@@ -506,9 +506,10 @@ int brk_fused_plan_final_permutation(const brk_fused_plan* plan, unsigned* permu
    other; blocks move only past blocks on disjoint qubits) whose qubits, with
    the 4 lowest, fit in 13 local qubits -- at most 8 blocks at width 4, 2 at
    width 5 -- run on one chunk of 2^13 amplitudes held in LDS, one HBM read
-   and write per pass instead of per block, with the per-block kernels'
-   arithmetic (the same amplitudes, bit for bit).  0 when the plan has no
-   passes. */
+   and write per pass instead of per block.  A block there is Gauss's
+   three-product complex GEMM in FP32 (the per-block launches use the real
+   embedding): the same amplitudes within FP32 rounding.  0 when the plan has
+   no passes. */
 int brk_fused_plan_num_passes(const brk_fused_plan* plan, unsigned* num_passes);
 /* Applies every block in order to a BRK_C64 state of 2^num_qubits amplitudes
    (in chunk passes when the plan has them and tuning fused_chunks is 1). */

@@ -337,25 +337,27 @@ static int launch_k(const fused_launch& L, void* state, void* stream) {
 // Chunk passes (k = 4, 5): several blocks applied to one chunk of 2^13 complex64
 // amplitudes held in LDS, one HBM read and one write per pass.  A persistent
 // 1024-thread workgroup per CU walks a contiguous run of chunks; its 160 KiB of LDS
-// hold two chunk buffers (2 x 64 KiB) and the pass's embeddings (up to 8 at k = 4,
-// 2 at k = 5, loaded once).  For chunk i every thread first stores its 8 amplitudes
-// of chunk i - 1 from the other buffer and issues the loads of its 8 amplitudes of
-// chunk i + 1 into registers; then the 16 waves run the pass's blocks on chunk i
-// (one barrier per block) and, before the last block's barrier, each thread writes
-// its loaded amplitudes into the other buffer.  A barrier waits only for LDS
-// operations and the blocks issue no global memory accesses, so the chunk transfers
-// stay in flight while the blocks run.  A thread touches the same 8 slots of either
-// buffer for the transfers, so its reads (for the stores) and later writes (of the
-// next chunk) of one buffer are ordered by program order.
+// hold two chunk buffers (2 x 64 KiB) and the pass's blocks in three-product form
+// (up to 8 at k = 4, 2 at k = 5, loaded once).  For chunk i every thread first stores
+// its 8 amplitudes of chunk i - 1 from the other buffer and issues the loads of its 8
+// amplitudes of chunk i + 1 into registers; then the 16 waves run the pass's blocks
+// on chunk i (one barrier per block) and, before the last block's barrier, each
+// thread writes its loaded amplitudes into the other buffer.  A barrier waits only
+// for LDS operations and the blocks issue no global memory accesses, so the chunk
+// transfers stay in flight while the blocks run.  A thread touches the same 8 slots
+// of either buffer for the transfers, so its reads (for the stores) and later writes
+// (of the next chunk) of one buffer are ordered by program order.
 //
 // A block on local bits row_bits[] sees the chunk as 2^k rows x 2^(13-k) columns
-// (the other local bits), 2^(8-k) tiles of 32 columns.  Each wave runs the per-block
-// kernels' MFMA chain for one 32-row output tile of one column tile -- B operands
-// gathered from LDS, A fragments from the block's embedding in LDS, D written back
-// over the tile's elements: at k = 4 wave w takes column tile w (the single output
-// tile), at k = 5 column tile w % 8 and output tile w / 8 (the real or the imaginary
-// parts) -- so every amplitude gets the same products in the same order as with one
-// launch per block.
+// (the other local bits), 2^(8-k) tiles of 32 columns.  Gauss's complex product
+//   T1 = Ur Xr, T2 = Ui Xi, T3 = (Ur + Ui)(Xr + Xi);  Yr = T1 - T2, Yi = (T3 - T1) - T2
+// runs on v_mfma_f32_16x16x4_f32 (FP32 products and accumulation): three real
+// products of 2^k x 2^k by 2^k x 32 instead of the 2^(k+1)-square embedding, 3/4 of the
+// MFMA cycles, and six independent accumulation chains per wave.  Wave w takes column
+// tile w % 2^(8-k) and 16-row output tile w / 2^(8-k) (k = 5 has two).  MFMA 16x16x4
+// operand layout (lane l, accumulator register v):
+//   A[i][kk] = lane (i + 16 kk), B[kk][j] = lane (j + 16 kk),
+//   D[4 (l / 16) + v][l % 16] = acc[v] of lane l.
 // ---------------------------------------------------------------------------
 typedef const __attribute__((address_space(4))) fused_chunk_program* cchunk;
 
@@ -363,16 +365,19 @@ constexpr int CHUNK_THREADS = 1024;
 constexpr size_t CHUNK_LDS_BYTES = 160 * 1024;  // the whole LDS of a CU
 constexpr size_t CHUNK_BUFFER_BYTES = sizeof(cpx<float>) << FUSED_CHUNK_LOCAL;
 
+typedef float f4v __attribute__((ext_vector_type(4)));
+
 template <int KB>
 struct chunk_shape {
-  static constexpr int R = 1 << KB, W = 2 * R, HALF = R / 2, DT = W / 32;
+  static constexpr int R = 1 << KB;
   static constexpr int NCOL = FUSED_CHUNK_LOCAL - KB;  // column bits
   static constexpr int NTILES = 1 << (NCOL - 5);       // 32-column tiles per block
+  static constexpr int MT = R / 16;                    // 16-row output tiles per column tile
+  static constexpr int STEPS = R / 4;                  // MFMA steps of K = 4
   static constexpr int WAVES = CHUNK_THREADS / 64;
-  static constexpr int AFLOATS = W * W;                // one embedding
+  static constexpr int AFLOATS = 3 * R * R;            // one block's three-product form
   static constexpr int SLOTS = fused_chunk_max_blocks(KB);
-  static constexpr int SB = 8;                         // MFMA steps per batch of A-fragment reads
-  static_assert(NTILES * DT == WAVES, "one (column tile, output tile) per wave");
+  static_assert(NTILES * MT == WAVES, "one (column tile, row tile) per wave");
   static_assert(2 * CHUNK_BUFFER_BYTES + sizeof(float) * AFLOATS * SLOTS <= CHUNK_LDS_BYTES, "LDS");
 };
 
@@ -382,12 +387,12 @@ __global__ __launch_bounds__(CHUNK_THREADS, 1) void k_fused_chunk(cpx<float>* __
                                                                    const float* __restrict__ matrices,
                                                                    uint64_t nchunks) {
   using S = chunk_shape<KB>;
-  constexpr int R = S::R, W = S::W, HALF = S::HALF;
+  constexpr int R = S::R, STEPS = S::STEPS;
   constexpr int CHUNK = 1 << FUSED_CHUNK_LOCAL;
   constexpr int PER = CHUNK / CHUNK_THREADS;  // 8 amplitudes per thread in the transfers
   extern __shared__ float smem[];
   cpx<float>* bufs = reinterpret_cast<cpx<float>*>(smem);  // two chunk buffers
-  float* slots = smem + 4 * CHUNK;                          // the pass's embeddings (layout of the per-block kernels)
+  float* slots = smem + 4 * CHUNK;                          // the pass's three-product forms
   const cchunk prog = (cchunk)program;
   const uint32_t t = threadIdx.x;
   const uint32_t nops = prog->num_ops;  // 1 .. SLOTS (checked by the host)
@@ -411,7 +416,7 @@ __global__ __launch_bounds__(CHUNK_THREADS, 1) void k_fused_chunk(cpx<float>* __
   uint64_t low = 0;
   for (int r = 0; r < 10; ++r) low |= (uint64_t)((t >> r) & 1u) << prog->pos[r];
 
-  // prologue: the pass's embeddings into their slots, chunk 0 into buffer 0
+  // prologue: the pass's three-product forms into their slots, chunk 0 into buffer 0
   for (uint32_t m = 0; m < nops; ++m) {
     const v4f* A4 = reinterpret_cast<const v4f*>(matrices + (uint64_t)prog->ops[m].matrix * (uint64_t)S::AFLOATS);
     for (uint32_t i = t; i < (uint32_t)(S::AFLOATS / 4); i += CHUNK_THREADS)
@@ -428,10 +433,9 @@ __global__ __launch_bounds__(CHUNK_THREADS, 1) void k_fused_chunk(cpx<float>* __
   __syncthreads();
 
   const int lane = t & 63, wave = t >> 6;
-  const int i32 = lane & 31, kk = lane >> 5;
-  const int swz = (W >= 64 && kk) ? 32 : 0;
-  const int tile = wave % S::NTILES;  // column tile
-  const int half = wave / S::NTILES;  // output tile: real rows (0) or imaginary rows (1) at k = 5
+  const int li = lane & 15, lk = lane >> 4;  // A row / B and D column within a 16-tile; K row within a step
+  const int tile = wave % S::NTILES;         // column tile
+  const int mt = wave / S::NTILES;           // 16-row output tile
   for (uint64_t it = 0; it < iters; ++it) {
     cpx<float>* lds = bufs + (it & 1) * CHUNK;
     cpx<float>* other = bufs + ((it + 1) & 1) * CHUNK;
@@ -449,55 +453,57 @@ __global__ __launch_bounds__(CHUNK_THREADS, 1) void k_fused_chunk(cpx<float>* __
     }
     for (uint32_t o = 0; o < nops; ++o) {
       const auto& op = prog->ops[o];
-      const float* Al = slots + o * S::AFLOATS;
-      // row offsets: uniform parts (the row value without the lane's kk bit) plus kk at block-row
-      // bit 0 for the B gathers and at bit 2 for the D stores
+      const float* Ar = slots + o * S::AFLOATS;  // Ur, Ui, Ur + Ui, each transposed
+      // local offset of block row r (bit b of r at local bit row_bits[b])
       auto rowoff = [&](int r) {
         uint32_t d = 0;
 #pragma unroll
         for (int b = 0; b < KB; ++b) d |= ((uint32_t)(r >> b) & 1u) << op.row_bits[b];
         return d;
       };
-      const uint32_t kk0 = kk ? 1u << op.row_bits[0] : 0u, kk2 = kk ? 1u << op.row_bits[2] : 0u;
-      uint32_t col = 0;  // column bits 0..4 from i32, 5.. from the tile number
+      // column 16 nt + li of this wave's tile: bits 0..3 from li, bit 4 from nt, bits 5.. from the tile
+      uint32_t colbase = 0;
 #pragma unroll
-      for (int b = 0; b < 5; ++b) col |= ((uint32_t)(i32 >> b) & 1u) << op.col_bits[b];
+      for (int b = 0; b < 4; ++b) colbase |= ((uint32_t)(li >> b) & 1u) << op.col_bits[b];
 #pragma unroll
-      for (int b = 5; b < S::NCOL; ++b) col |= ((uint32_t)(tile >> (b - 5)) & 1u) << op.col_bits[b];
-      cpx<float> x[HALF];
+      for (int b = 5; b < S::NCOL; ++b) colbase |= ((uint32_t)(tile >> (b - 5)) & 1u) << op.col_bits[b];
+      const uint32_t col[2] = {colbase, colbase | (1u << op.col_bits[4])};
+      const uint32_t krow = rowoff(lk);  // the lane's K row within a step
+      // B operands: X[4 st + lk][16 nt + li]
+      cpx<float> x[STEPS][2];
 #pragma unroll
-      for (int j = 0; j < HALF; ++j) x[j] = lds[col | rowoff(2 * j) | kk0];
-      f16v acc = f16v{};
+      for (int st = 0; st < STEPS; ++st)
 #pragma unroll
-      for (int sb = 0; sb < R; sb += S::SB) {
-        float a[S::SB];
+        for (int nt = 0; nt < 2; ++nt) x[st][nt] = lds[col[nt] | rowoff(4 * st) | krow];
+      f4v acc[3][2];
 #pragma unroll
-        for (int st = 0; st < S::SB; ++st) a[st] = Al[(2 * (sb + st) + kk) * W + ((32 * half + i32) ^ swz)];
+      for (int p = 0; p < 3; ++p)
 #pragma unroll
-        for (int st = 0; st < S::SB; ++st) {
-          const float b = sb + st < HALF ? x[sb + st].re : x[sb + st - HALF].im;
-          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[st], b, acc, 0, 0, 0);
+        for (int nt = 0; nt < 2; ++nt) acc[p][nt] = f4v{};
+#pragma unroll
+      for (int st = 0; st < STEPS; ++st) {
+        const int ai = (4 * st + lk) * R + 16 * mt + li;  // A[16 mt + li][4 st + lk], transposed
+        const float a0 = Ar[ai], a1 = Ar[R * R + ai], a2 = Ar[2 * R * R + ai];
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt) {
+          const cpx<float> v = x[st][nt];
+          acc[0][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, v.re, acc[0][nt], 0, 0, 0);
+          acc[1][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, v.im, acc[1][nt], 0, 0, 0);
+          acc[2][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a2, v.re + v.im, acc[2][nt], 0, 0, 0);
         }
       }
-      // every wave has read its B operands before any wave writes D (the tiles' elements overlap
-      // only within a column tile, which the two waves of a tile at k = 5 both gather)
-      if constexpr (S::DT == 2) __syncthreads();
-      if constexpr (S::DT == 1) {
-        // rows 0..15 real parts (v < 8), rows 16..31 imaginary parts (v + 8)
+      // k = 5: both row tiles of a column tile gather all its rows; every gather ends before a
+      // wave writes its rows
+      if constexpr (S::MT > 1) __syncthreads();
+      // D: rows 16 mt + 4 lk + v, column 16 nt + li
+      const uint32_t drow = rowoff(16 * mt + 4 * lk);
 #pragma unroll
-        for (int v = 0; v < 8; ++v) {
-          const int row = 8 * (v >> 2) + (v & 3);  // + 4 kk
-          lds[col | rowoff(row) | kk2] = cpx<float>{acc[v], acc[v + 8]};
-        }
-      } else {
-        // this wave's 32 real rows: the real (half 0) or imaginary (half 1) parts of complex rows 0..31
-        float* f = reinterpret_cast<float*>(lds);
+      for (int nt = 0; nt < 2; ++nt)
 #pragma unroll
-        for (int v = 0; v < 16; ++v) {
-          const int row = 8 * (v >> 2) + (v & 3);  // + 4 kk
-          f[2 * (col | rowoff(row) | kk2) + half] = acc[v];
+        for (int v = 0; v < 4; ++v) {
+          const float t1 = acc[0][nt][v], t2 = acc[1][nt][v], t3 = acc[2][nt][v];
+          lds[col[nt] | drow | rowoff(v)] = cpx<float>{t1 - t2, (t3 - t1) - t2};
         }
-      }
       if (more && o + 1 == nops) {
         // chunk it + 1 into the other buffer (this thread's slots), before the chunk's last barrier
 #pragma unroll

@@ -65,12 +65,13 @@ struct fused_chunk_program {
   uint64_t deposit_high[16];          // physical offset of local index k << 9 (k < 16)
   fused_chunk_op ops[FUSED_CHUNK_MAX_OPS];
 };
-// The blocks of one pass at width k: the pass's embeddings (2^(k+1) x 2^(k+1) FP32) stay
+// The blocks of one pass at width k: the pass's three-product forms (3 x 2^k x 2^k FP32) stay
 // in 32 KiB of LDS beside two chunk buffers: 8 at k = 4, 2 at k = 5.
 constexpr int fused_chunk_max_blocks(int k) { return k == 4 ? 8 : k == 5 ? 2 : 0; }
 // Launches one pass of width-k blocks over a complex64 state of `size` amplitudes:
 // one persistent workgroup per CU.  `device_program` is the pass in device memory,
-// `program` its host copy, `matrices` the plan's device embeddings.
+// `program` its host copy, `matrices` the plan's device three-product forms
+// (fused_split_floats(width) per block).
 int launch_fused_chunk(const fused_chunk_program* device_program, const fused_chunk_program& program, int width,
                        const float* matrices, void* state, uint64_t size, void* stream);
 
@@ -79,5 +80,8 @@ bool fused_chunks_enabled();
 
 // Size in floats of a block's device matrix for width k.
 inline uint64_t fused_matrix_floats(int k) { return (uint64_t)(2 << k) * (uint64_t)(2 << k); }
+// Size in floats of a block's three-product form for chunk passes: Ur, Ui and Ur + Ui, each
+// 2^k x 2^k, stored transposed (element (i, c) at c * 2^k + i).
+inline uint64_t fused_split_floats(int k) { return 3ull << (2 * k); }
 
 }  // namespace brk

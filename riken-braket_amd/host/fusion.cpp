@@ -59,10 +59,13 @@ struct brk_fused_plan {
   std::vector<unsigned> final_perm;
   std::vector<float> host_matrices;   // FP32 embeddings, fused_matrix_floats(width) per block
   mutable float* device_matrices = nullptr;
+  std::vector<float> host_split;      // chunk passes: FP32 three-product forms, fused_split_floats(width) per block
+  mutable float* device_split = nullptr;
   std::vector<brk::fused_chunk_program> passes;  // chunk passes over the blocks in order (may be empty)
   mutable brk::fused_chunk_program* device_passes = nullptr;
   ~brk_fused_plan() {
     if (device_matrices) (void)hipFree(device_matrices);
+    if (device_split) (void)hipFree(device_split);
     if (device_passes) (void)hipFree(device_passes);
   }
 };
@@ -163,6 +166,21 @@ static void embed(const std::vector<cd>& u, unsigned w, float* out) {
       else v = z.imag();              // bottom-left: Ui
       const unsigned swz = (W >= 64 && (c & 1u)) ? 32u : 0u;
       out[(size_t)c * W + (r ^ swz)] = (float)v;
+    }
+}
+
+// Three-product form of a 2^w block for Gauss's complex product Y = U X:
+//   T1 = Ur Xr, T2 = Ui Xi, T3 = (Ur + Ui)(Xr + Xi);  Yr = T1 - T2, Yi = T3 - T1 - T2.
+// Ur, Ui and Ur + Ui (the sum formed in FP64), each stored transposed: element (i, c) at c*R + i.
+static void split3(const std::vector<cd>& u, unsigned w, float* out) {
+  const unsigned R = 1u << w;
+  for (unsigned i = 0; i < R; ++i)
+    for (unsigned c = 0; c < R; ++c) {
+      const cd z = u[(size_t)i * R + c];
+      const size_t at = (size_t)c * R + i;
+      out[at] = (float)z.real();
+      out[(size_t)R * R + at] = (float)z.imag();
+      out[2 * (size_t)R * R + at] = (float)(z.real() + z.imag());
     }
 }
 
@@ -497,6 +515,12 @@ int brk_fused_plan_create(const brk_circuit* circuit, unsigned width, const unsi
   for (size_t b = 0; b < plan->blocks.size(); ++b)
     embed(plan->blocks[b].u, width, plan->host_matrices.data() + b * per);
   if (chunked) build_passes(*plan);
+  if (!plan->passes.empty()) {
+    const uint64_t per3 = fused_split_floats((int)width);
+    plan->host_split.resize(per3 * plan->blocks.size());
+    for (size_t b = 0; b < plan->blocks.size(); ++b)
+      split3(plan->blocks[b].u, width, plan->host_split.data() + b * per3);
+  }
   *out = plan;
   return BRK_OK;
 }
@@ -515,6 +539,13 @@ static int ensure_device_matrices(const brk_fused_plan* plan) {
   for (size_t b = 0; b < plan->blocks.size(); ++b) plan->blocks[b].launch.matrix = d + b * per;
   plan->device_matrices = d;
   if (!plan->passes.empty()) {
+    const size_t sbytes = plan->host_split.size() * sizeof(float);
+    if (hipMalloc((void**)&plan->device_split, sbytes) != hipSuccess) {
+      plan->device_split = nullptr;
+      return fail(BRK_ERR_OUT_OF_MEMORY, "cannot allocate fused three-product forms");
+    }
+    if (hipMemcpy(plan->device_split, plan->host_split.data(), sbytes, hipMemcpyHostToDevice) != hipSuccess)
+      return fail(BRK_ERR_HIP, "cannot upload fused three-product forms");
     const size_t pbytes = plan->passes.size() * sizeof(fused_chunk_program);
     if (hipMalloc((void**)&plan->device_passes, pbytes) != hipSuccess) {
       plan->device_passes = nullptr;
@@ -582,7 +613,7 @@ static int apply_impl(const brk_fused_plan* plan, void* first, int dtype, float*
   const uint64_t size = 1ull << plan->num_qubits;
   for (size_t b = 0; b < launches && status == BRK_OK; ++b) {
     status = passes ? launch_fused_chunk(plan->device_passes + b, plan->passes[b], (int)plan->width,
-                                         plan->device_matrices, first, size, stream)
+                                         plan->device_split, first, size, stream)
                     : launch_fused_block(plan->blocks[b].launch, first, stream);
     if (block_ms && status == BRK_OK) (void)hipEventRecord(ev[b + 1], st);
   }

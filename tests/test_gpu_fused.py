@@ -94,9 +94,11 @@ def test_staged_and_register_direct_kernels_agree(n, width):
 
 
 @pytest.mark.parametrize("n,width,seed", [(13, 4, 1), (13, 5, 2), (16, 4, 3), (17, 5, 4), (20, 4, 5), (20, 5, 6)])
-def test_chunk_passes_equal_block_launches(oracle, n, width, seed):
-    """Chunk passes (several blocks per LDS-resident chunk) against one launch per block: the
-    same MFMA products in the same order, so the same amplitudes bit for bit; and the oracle."""
+def test_chunk_passes_match_block_launches(oracle, n, width, seed):
+    """Chunk passes (several blocks per LDS-resident chunk, Gauss's three-product complex GEMM)
+    against one launch per block (the real-embedding GEMM): the same blocks in FP32 with another
+    product, equal within FP32 rounding (2e-6 max-abs on the unit-norm state); both against the
+    oracle: infidelity <= 1e-6 and max-abs <= 2e-6."""
     ops = W.random_circuit(n, 160, 6, seed=seed)
     plan = brk.FusedPlan(brk.Circuit(n, ops), width)
     assert plan.num_passes > 0 and plan.num_passes < plan.num_blocks
@@ -110,9 +112,10 @@ def test_chunk_passes_equal_block_launches(oracle, n, width, seed):
         per_block = _run_fused(plan, s)
     finally:
         brk.tune("fused_chunks", old)
-    assert np.array_equal(chunked.astype(np.complex64).view(np.uint32), per_block.astype(np.complex64).view(np.uint32))
+    assert np.max(np.abs(chunked - per_block)) < 2e-6
     expected = s.copy()
     run_ops(oracle, expected, ops)
-    infidelity = 1 - abs(np.vdot(expected, chunked)) ** 2 / np.vdot(chunked, chunked).real
-    assert infidelity < 1e-6, infidelity
-    assert np.max(np.abs(chunked - expected)) < 2e-6
+    for got in (chunked, per_block):
+        infidelity = 1 - abs(np.vdot(expected, got)) ** 2 / np.vdot(got, got).real
+        assert infidelity < 1e-6, infidelity
+        assert np.max(np.abs(got - expected)) < 2e-6
