@@ -445,9 +445,12 @@ __global__ __launch_bounds__(CHUNK_THREADS, 1) void k_fused_chunk(cpx<float>* __
       for (int k = 0; k < PER; ++k)
         store_amp<NT>(s + (b | low | prog->deposit_high[2 * k]), other[t + CHUNK_THREADS * k]);
     }
-    const bool more = it + 1 < iters;
-    if (more) {  // chunk it + 1 into registers
-      const uint64_t b = base_of(chunk_of(it + 1));
+    // chunk it + 1 into registers -- on the last iteration the current chunk again: its loads and
+    // the write into the free buffer below are then harmless, and on every path the loads are
+    // waited for before the chunk's last barrier (so the compiler does not wait for them, and for
+    // the stores issued after them, when the next iteration reuses their registers)
+    {
+      const uint64_t b = base_of(chunk_of(it + 1 < iters ? it + 1 : it));
 #pragma unroll
       for (int k = 0; k < PER; ++k) pre[k] = load_amp<NT>(s + (b | low | prog->deposit_high[2 * k]));
     }
@@ -504,13 +507,12 @@ __global__ __launch_bounds__(CHUNK_THREADS, 1) void k_fused_chunk(cpx<float>* __
           const float t1 = acc[0][nt][v], t2 = acc[1][nt][v], t3 = acc[2][nt][v];
           lds[col[nt] | drow | rowoff(v)] = cpx<float>{t1 - t2, (t3 - t1) - t2};
         }
-      if (more && o + 1 == nops) {
-        // chunk it + 1 into the other buffer (this thread's slots), before the chunk's last barrier
-#pragma unroll
-        for (int k = 0; k < PER; ++k) other[t + CHUNK_THREADS * k] = pre[k];
-      }
-      __syncthreads();  // the block done: D rows (and, after the last block, the next chunk) in LDS
+      if (o + 1 < nops) __syncthreads();  // the block done: its D rows in LDS
     }
+    // chunk it + 1 into the other buffer (this thread's slots), then the chunk's last barrier
+#pragma unroll
+    for (int k = 0; k < PER; ++k) other[t + CHUNK_THREADS * k] = pre[k];
+    __syncthreads();
   }
   if (iters > 0) {  // the last chunk back to HBM
     const cpx<float>* last = bufs + ((iters - 1) & 1) * CHUNK;
