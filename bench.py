@@ -300,8 +300,15 @@ def fused_leg(circuit, plain_state, args, stream):
     ms = sum(run_ms) / len(run_ms)
     blocks = plan.num_blocks
     passes = plan.num_passes if brk.get_tune("fused_chunks") else 0
-    flops = 8 * (1 << best) * (1 << n) * blocks
+    flops = 8 * (1 << best) * (1 << n) * blocks  # complex GEMM count: 8 real FLOPs per complex MAC
     hbm = 2 * 8 * (1 << n) * (passes or blocks)  # one read and one write of the c64 state per launch
+    # per launch: the larger of its HBM time at 8 TB/s and its executed MFMA FLOPs at 157.3 TFLOP/s
+    # (chunk passes run Gauss's three-product form, 6 real FLOPs per complex MAC; one launch per
+    # block runs the real embedding, 8)
+    launches = passes or blocks
+    executed = (6 if passes else 8) * (1 << best) * (1 << n) * blocks
+    bound_ms = max(hbm / launches / (HBM_PEAK_GBS * 1e9), executed / launches / 157.3e12) * 1e3
+    launch_ms = ms / launches
     return {
         "width": best,
         "blocks": blocks,
@@ -311,7 +318,16 @@ def fused_leg(circuit, plain_state, args, stream):
         "widths": widths,
         "mfma_tflops": round(flops / (ms * 1e-3) / 1e12, 1),
         "mfma_fraction_fp32": round(flops / (ms * 1e-3) / 157.3e12, 4),
+        "flop_count": "8 * 2^width * 2^n per block (complex GEMM); chunk passes execute 6 * 2^width * 2^n "
+                      "on MFMA (Gauss's three-product form)",
         "hbm_GBps": round(hbm / (ms * 1e-3) / 1e9, 1),
+        "roofline_per_launch": {
+            "kernel": "k_fused_chunk" if passes else "k_fused_staged / k_fused",
+            "launch_ms": round(launch_ms, 3),
+            "bound_ms": round(bound_ms, 3),
+            "bound": "mfma" if executed / 157.3e12 > hbm / (HBM_PEAK_GBS * 1e9) else "hbm",
+            "frac": round(bound_ms / launch_ms, 4),
+        },
         "dtype": "c64 state, FP32 MFMA (v_mfma_f32_32x32x2_f32), FP32 accumulate; FP64 fidelity",
     }
 
