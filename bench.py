@@ -573,7 +573,7 @@ def adjoint_leg(args, stream):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--qubits", type=int, default=32)
     ap.add_argument("--seed", type=int, default=2)
@@ -694,6 +694,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(ms_per_step, 2),
+        "ms_per_step_median": round(sorted(step_ms)[len(step_ms) // 2], 2),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
