@@ -438,7 +438,6 @@ int brk_fused_plan_create(const brk_circuit* circuit, unsigned width, const unsi
   // 1-2. physical gate lists of candidate orders, greedy fusion of each; the fewest blocks win
   // (for chunk passes: the fewest passes, then the fewest blocks)
   const bool chunked = width <= 5 && n >= (unsigned)FUSED_CHUNK_LOCAL;
-  size_t best_passes = 0;
   double best_cost = 0.0;
   std::vector<physical_gate> gates, trial;
   std::vector<fused_group> groups;
@@ -466,7 +465,6 @@ int brk_fused_plan_create(const brk_circuit* circuit, unsigned width, const unsi
       groups = std::move(g);
       gates.swap(trial);
       final_perm = trial_perm;
-      best_passes = passes;
     }
   }
 
@@ -525,36 +523,38 @@ int brk_fused_plan_create(const brk_circuit* circuit, unsigned width, const unsi
   return BRK_OK;
 }
 
-// Uploads the block matrices to the current device once.
-static int ensure_device_matrices(const brk_fused_plan* plan) {
-  if (plan->device_matrices || plan->host_matrices.empty()) return BRK_OK;
-  const size_t bytes = plan->host_matrices.size() * sizeof(float);
-  float* d = nullptr;
-  if (hipMalloc(&d, bytes) != hipSuccess) return fail(BRK_ERR_OUT_OF_MEMORY, "cannot allocate fused matrices");
-  if (hipMemcpy(d, plan->host_matrices.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) {
+// Copies `bytes` from host to a new device buffer in *out (nothing when it is already there).
+static int upload_once(void** out, const void* host, size_t bytes, const char* what) {
+  if (*out || bytes == 0) return BRK_OK;
+  void* d = nullptr;
+  if (hipMalloc(&d, bytes) != hipSuccess) return fail(BRK_ERR_OUT_OF_MEMORY, "cannot allocate %s", what);
+  if (hipMemcpy(d, host, bytes, hipMemcpyHostToDevice) != hipSuccess) {
     (void)hipFree(d);
-    return fail(BRK_ERR_HIP, "cannot upload fused matrices");
+    return fail(BRK_ERR_HIP, "cannot upload %s", what);
   }
-  const uint64_t per = fused_matrix_floats((int)plan->width);
-  for (size_t b = 0; b < plan->blocks.size(); ++b) plan->blocks[b].launch.matrix = d + b * per;
-  plan->device_matrices = d;
-  if (!plan->passes.empty()) {
-    const size_t sbytes = plan->host_split.size() * sizeof(float);
-    if (hipMalloc((void**)&plan->device_split, sbytes) != hipSuccess) {
-      plan->device_split = nullptr;
-      return fail(BRK_ERR_OUT_OF_MEMORY, "cannot allocate fused three-product forms");
-    }
-    if (hipMemcpy(plan->device_split, plan->host_split.data(), sbytes, hipMemcpyHostToDevice) != hipSuccess)
-      return fail(BRK_ERR_HIP, "cannot upload fused three-product forms");
-    const size_t pbytes = plan->passes.size() * sizeof(fused_chunk_program);
-    if (hipMalloc((void**)&plan->device_passes, pbytes) != hipSuccess) {
-      plan->device_passes = nullptr;
-      return fail(BRK_ERR_OUT_OF_MEMORY, "cannot allocate fused chunk passes");
-    }
-    if (hipMemcpy(plan->device_passes, plan->passes.data(), pbytes, hipMemcpyHostToDevice) != hipSuccess)
-      return fail(BRK_ERR_HIP, "cannot upload fused chunk passes");
-  }
+  *out = d;
   return BRK_OK;
+}
+
+// Uploads the plan's device data to the current device once: the block embeddings and, for
+// plans with chunk passes, the three-product forms and the pass programs.  Each piece is
+// uploaded independently, so a call after a failed one retries what is missing.
+static int ensure_device_matrices(const brk_fused_plan* plan) {
+  void* d = plan->device_matrices;
+  int st = upload_once(&d, plan->host_matrices.data(), plan->host_matrices.size() * sizeof(float), "fused matrices");
+  plan->device_matrices = static_cast<float*>(d);
+  if (st) return st;
+  const uint64_t per = fused_matrix_floats((int)plan->width);
+  for (size_t b = 0; b < plan->blocks.size(); ++b) plan->blocks[b].launch.matrix = plan->device_matrices + b * per;
+  if (plan->passes.empty()) return BRK_OK;
+  d = plan->device_split;
+  st = upload_once(&d, plan->host_split.data(), plan->host_split.size() * sizeof(float), "fused three-product forms");
+  plan->device_split = static_cast<float*>(d);
+  if (st) return st;
+  d = plan->device_passes;
+  st = upload_once(&d, plan->passes.data(), plan->passes.size() * sizeof(fused_chunk_program), "fused chunk passes");
+  plan->device_passes = static_cast<fused_chunk_program*>(d);
+  return st;
 }
 
 int brk_fused_plan_destroy(brk_fused_plan* plan) {
