@@ -95,6 +95,32 @@ __device__ __forceinline__ void apply_slot(f2 (&a)[1 << NP], cu32 op) {
   }
 }
 
+// Real 2x2 [[r00, r01], [r10, r11]] on the pairs of slot S: one multiply and one fused
+// multiply-add per component.
+template <int NP, int S>
+__device__ __forceinline__ void apply_real_slot(f2 (&a)[1 << NP], cu32 op) {
+  const float r00 = __uint_as_float(op[0]), r01 = __uint_as_float(op[1]);
+  const float r10 = __uint_as_float(op[2]), r11 = __uint_as_float(op[3]);
+#pragma unroll
+  for (int e = 0; e < (1 << NP); ++e) {
+    if (e & (1 << S)) continue;
+    const f2 a0 = a[e], a1 = a[e | (1 << S)];
+    a[e] = f2{__builtin_fmaf(r00, a0.re, r01 * a1.re), __builtin_fmaf(r00, a0.im, r01 * a1.im)};
+    a[e | (1 << S)] = f2{__builtin_fmaf(r10, a0.re, r11 * a1.re), __builtin_fmaf(r10, a0.im, r11 * a1.im)};
+  }
+}
+
+// a[e] *= table[e] (complex64 words, the same for every thread of the pass).
+template <int NP>
+__device__ __forceinline__ void apply_phases(f2 (&a)[1 << NP], cu32 table) {
+#pragma unroll
+  for (int e = 0; e < (1 << NP); ++e) {
+    const float zr = __uint_as_float(table[2 * e]), zi = __uint_as_float(table[2 * e + 1]);
+    const f2 v = a[e];
+    a[e] = f2{__builtin_fmaf(zr, v.re, -zi * v.im), __builtin_fmaf(zr, v.im, zi * v.re)};
+  }
+}
+
 struct chunk_ctx {
   f2* chunk;
   const uint32_t* tlo;
@@ -108,7 +134,8 @@ __device__ __forceinline__ uint32_t global_index(const chunk_ctx& c, uint32_t j)
   return c.cbase | c.tlo[j & 127u] | c.thi[j >> 7];
 }
 
-template <int NP>
+// FORM: BI_GATES (complex 2x2 per position), BI_GATES_REAL, BI_GATES_REAL_PRE (brk_batch.hpp).
+template <int NP, uint32_t FORM>
 __device__ void gate_pass(const chunk_ctx& c, cu32 w, uint32_t tid) {
   const uint32_t w0 = w[0], pw = w[1], qw = w[2];
   const uint32_t npre = (w0 >> 16) & 0xffu, npost = (w0 >> 24) & 0xffu;
@@ -145,10 +172,20 @@ __device__ void gate_pass(const chunk_ctx& c, cu32 w, uint32_t tid) {
     uint32_t xb = 0;
     if (npre | npost) xb = global_index(c, base);
     if (npre) apply_terms<NP>(a, xb, xoff, pre, npre);
-    apply_slot<NP, 0>(a, ops);
-    if constexpr (NP > 1) apply_slot<NP, 1>(a, ops + BATCH_GATE_WORDS);
-    if constexpr (NP > 2) apply_slot<NP, 2>(a, ops + 2 * BATCH_GATE_WORDS);
-    if constexpr (NP > 3) apply_slot<NP, 3>(a, ops + 3 * BATCH_GATE_WORDS);
+    if constexpr (FORM == BI_GATES) {
+      apply_slot<NP, 0>(a, ops);
+      if constexpr (NP > 1) apply_slot<NP, 1>(a, ops + BATCH_GATE_WORDS);
+      if constexpr (NP > 2) apply_slot<NP, 2>(a, ops + 2 * BATCH_GATE_WORDS);
+      if constexpr (NP > 3) apply_slot<NP, 3>(a, ops + 3 * BATCH_GATE_WORDS);
+    } else {
+      const cu32 post_table = ops + NP * BATCH_REAL_WORDS;
+      if constexpr (FORM == BI_GATES_REAL_PRE) apply_phases<NP>(a, post_table + 2 * (1 << NP));
+      apply_real_slot<NP, 0>(a, ops);
+      if constexpr (NP > 1) apply_real_slot<NP, 1>(a, ops + BATCH_REAL_WORDS);
+      if constexpr (NP > 2) apply_real_slot<NP, 2>(a, ops + 2 * BATCH_REAL_WORDS);
+      if constexpr (NP > 3) apply_real_slot<NP, 3>(a, ops + 3 * BATCH_REAL_WORDS);
+      apply_phases<NP>(a, post_table);
+    }
     if (npost) apply_terms<NP>(a, xb, xoff, post, npost);
     // recompute the store addresses instead of keeping 2^NP of them live across the gates
     uint32_t pbs = pb;
@@ -251,16 +288,34 @@ __global__ __launch_bounds__(BATCH_THREADS, BATCH_WG_PER_CU) void k_batch(batch_
         __syncthreads();
         cu32 w = first_word;
         for (uint32_t i = 0; i < num_instr; ++i) {
-          const uint32_t w0 = w[0];
-          if ((w0 & 0xffu) == BI_GATES) {
+          const uint32_t w0 = w[0], op = w0 & 0xffu;
+          if (op == BI_GATES) {
             const uint32_t npos = (w0 >> 8) & 0xffu;
             switch (npos) {
-              case 1: gate_pass<1>(c, w, tid); break;
-              case 2: gate_pass<2>(c, w, tid); break;
-              case 3: gate_pass<3>(c, w, tid); break;
-              default: gate_pass<4>(c, w, tid); break;
+              case 1: gate_pass<1, BI_GATES>(c, w, tid); break;
+              case 2: gate_pass<2, BI_GATES>(c, w, tid); break;
+              case 3: gate_pass<3, BI_GATES>(c, w, tid); break;
+              default: gate_pass<4, BI_GATES>(c, w, tid); break;
             }
             w += 3 + 2 * (((w0 >> 16) & 0xffu) + (w0 >> 24)) + BATCH_GATE_WORDS * npos;
+          } else if (op == BI_GATES_REAL || op == BI_GATES_REAL_PRE) {
+            // the planner emits these for 2..4 positions (for one position the complex form is cheaper)
+            const uint32_t npos = (w0 >> 8) & 0xffu;
+            if (op == BI_GATES_REAL) {
+              switch (npos) {
+                case 2: gate_pass<2, BI_GATES_REAL>(c, w, tid); break;
+                case 3: gate_pass<3, BI_GATES_REAL>(c, w, tid); break;
+                default: gate_pass<4, BI_GATES_REAL>(c, w, tid); break;
+              }
+            } else {
+              switch (npos) {
+                case 2: gate_pass<2, BI_GATES_REAL_PRE>(c, w, tid); break;
+                case 3: gate_pass<3, BI_GATES_REAL_PRE>(c, w, tid); break;
+                default: gate_pass<4, BI_GATES_REAL_PRE>(c, w, tid); break;
+              }
+            }
+            w += 3 + 2 * (((w0 >> 16) & 0xffu) + (w0 >> 24)) + BATCH_REAL_WORDS * npos +
+                 2 * (1u << npos) * (op == BI_GATES_REAL_PRE ? 2u : 1u);
           } else {
             sign_pass(c, w, tid);
             w += 1 + 2 * ((w0 >> 16) & 0xffffu);

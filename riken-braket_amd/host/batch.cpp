@@ -26,6 +26,8 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <array>
+#include <cmath>
 #include <complex>
 #include <cstring>
 #include <vector>
@@ -174,6 +176,47 @@ std::vector<std::pair<uint32_t, uint32_t>> sign_terms(const std::vector<uint32_t
   return terms;
 }
 
+// U = diag(p0, p1) R diag(1, q) with R real, for a 2x2 unitary U (row-major).  Within a row of a
+// unitary the two entries' phases differ by the same angle in both rows (mod pi), so q removes
+// that difference and each row is then its own phase times a real row.  q is taken in the right
+// half-plane (a factor -1 moves into R), and exactly 1 when it is within rounding of 1, so the
+// RZ RY products of an ansatz layer need no pre-phase.  ok is false when a row is not real
+// after the phases come out (U not unitary to FP64 rounding).
+struct real_form {
+  double r[4];
+  cd p[2];
+  cd q;
+  bool ok;
+};
+
+real_form real_decomposition(const std::array<cd, 4>& u) {
+  real_form f{};
+  const double w0 = std::abs(u[0]) * std::abs(u[1]), w1 = std::abs(u[2]) * std::abs(u[3]);
+  cd q(1.0, 0.0);
+  if (std::max(w0, w1) > 0.0) {
+    const int r = w0 >= w1 ? 0 : 2;
+    q = (u[r + 1] / std::abs(u[r + 1])) / (u[r] / std::abs(u[r]));
+    if (q.real() < 0.0) q = -q;
+    if (std::abs(q - cd(1.0, 0.0)) <= 1e-14) q = cd(1.0, 0.0);
+  }
+  const cd v[4] = {u[0], u[1] * std::conj(q), u[2], u[3] * std::conj(q)};
+  double scale = 0.0;
+  for (const cd& x : u) scale = std::max(scale, std::abs(x));
+  f.ok = scale > 0.0;
+  for (int r = 0; r < 2; ++r) {
+    const cd big = std::abs(v[2 * r]) >= std::abs(v[2 * r + 1]) ? v[2 * r] : v[2 * r + 1];
+    const double m = std::abs(big);
+    f.p[r] = m > 0.0 ? big / m : cd(1.0, 0.0);
+    for (int c = 0; c < 2; ++c) {
+      const cd x = v[2 * r + c] * std::conj(f.p[r]);
+      f.r[2 * r + c] = x.real();
+      if (std::abs(x.imag()) > 1e-12 * scale) f.ok = false;
+    }
+  }
+  f.q = q;
+  return f;
+}
+
 // Appends the instruction stream of one stage; returns the instruction count.
 uint32_t emit_stage(const planned_stage& st, std::vector<uint32_t>& words, uint32_t& measure_x, uint32_t& flags) {
   int pos_of[32];
@@ -253,17 +296,45 @@ uint32_t emit_stage(const planned_stage& st, std::vector<uint32_t>& words, uint3
         mats[slot] = {g->m[0] * a[0] + g->m[1] * a[2], g->m[0] * a[1] + g->m[1] * a[3],
                       g->m[2] * a[0] + g->m[3] * a[2], g->m[2] * a[1] + g->m[3] * a[3]};
       }
-      words.push_back(brk::BI_GATES | (uint32_t)pos.size() << 8 | (uint32_t)sg.pre.size() << 16 |
-                      (uint32_t)sg.post.size() << 24);
+      // real form (real 2x2 per position between two uniform phase tables) when it takes fewer
+      // lane operations than the complex products: per thread 4 np 2^np + 4 2^np per table
+      // against 8 np 2^np
+      const size_t np = pos.size(), ne = (size_t)1 << np;
+      std::vector<real_form> forms;
+      bool real_ok = true, pre_phase = false;
+      for (const auto& m : mats) {
+        forms.push_back(real_decomposition(m));
+        real_ok &= forms.back().ok;
+        pre_phase |= forms.back().q != cd(1.0, 0.0);
+      }
+      const bool use_real = real_ok && 4 * np * ne + 4 * ne * (pre_phase ? 2 : 1) < 8 * np * ne;
+      const uint32_t op = use_real ? (pre_phase ? brk::BI_GATES_REAL_PRE : brk::BI_GATES_REAL) : brk::BI_GATES;
+      words.push_back(op | (uint32_t)np << 8 | (uint32_t)sg.pre.size() << 16 | (uint32_t)sg.post.size() << 24);
       words.push_back(pw);
       words.push_back(qw);
       for (const auto& t : sg.pre) words.insert(words.end(), {t.first, t.second});
       for (const auto& t : sg.post) words.insert(words.end(), {t.first, t.second});
-      for (const auto& m : mats)
-        for (int k = 0; k < 4; ++k) {
-          words.push_back(float_bits(m[k].real()));
-          words.push_back(float_bits(m[k].imag()));
-        }
+      if (!use_real) {
+        for (const auto& m : mats)
+          for (int k = 0; k < 4; ++k) {
+            words.push_back(float_bits(m[k].real()));
+            words.push_back(float_bits(m[k].imag()));
+          }
+      } else {
+        for (const auto& f : forms)
+          for (int k = 0; k < 4; ++k) words.push_back(float_bits(f.r[k]));
+        // tables over the group's elements e: bit j of e is the value at position j
+        for (int table = 0; table < (pre_phase ? 2 : 1); ++table)
+          for (size_t e = 0; e < ne; ++e) {
+            cd z(1.0, 0.0);
+            for (size_t j = 0; j < np; ++j) {
+              const unsigned bit = (unsigned)((e >> j) & 1u);
+              z *= table == 0 ? forms[j].p[bit] : (bit ? forms[j].q : cd(1.0, 0.0));
+            }
+            words.push_back(float_bits(z.real()));
+            words.push_back(float_bits(z.imag()));
+          }
+      }
     } else {
       if (sg.attached) continue;
       const auto terms = sign_terms(sg.masks);
