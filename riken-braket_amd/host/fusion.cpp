@@ -369,11 +369,55 @@ static size_t count_passes(const std::vector<fused_group>& groups, unsigned widt
   return passes;
 }
 
+// Passes built one at a time: a pass starts with the run qubits and repeatedly takes, among the
+// blocks whose earlier blocks on shared qubits are all placed, one that still fits in the chunk,
+// preferring the block that adds the fewest qubits (ties and near-ties broken by the seeded noise:
+// the score is added qubits + noise * u, u uniform in [0, 1)); the pass ends when no such block
+// fits or it holds the blocks a pass can keep.  Returns the blocks in pass order; build_passes and
+// count_passes cut this order at the same places, since every pass ends at a block that does not
+// fit or at the block limit.
+static std::vector<size_t> greedy_chunk_order(const std::vector<uint64_t>& masks, const std::vector<std::vector<size_t>>& succ,
+                                              const std::vector<unsigned>& npred0, size_t cap, double noise,
+                                              std::mt19937_64& gen) {
+  const uint64_t run = (1ull << CHUNK_RUN_QUBITS) - 1ull;
+  const size_t B = masks.size();
+  std::vector<unsigned> npred = npred0;
+  std::vector<char> placed(B, 0);
+  std::vector<size_t> order;
+  order.reserve(B);
+  std::uniform_real_distribution<double> u(0.0, 1.0);
+  while (order.size() < B) {
+    uint64_t local = run;
+    size_t in_pass = 0;
+    while (in_pass < cap) {
+      size_t pick = B;
+      double best = 0.0;
+      for (size_t i = 0; i < B; ++i) {
+        if (placed[i] || npred[i]) continue;
+        const uint64_t m = local | masks[i];
+        if (__builtin_popcountll(m) > brk::FUSED_CHUNK_LOCAL) continue;
+        const double score = (double)__builtin_popcountll(m & ~local) + noise * u(gen);
+        if (pick == B || score < best) {
+          pick = i;
+          best = score;
+        }
+      }
+      if (pick == B) break;
+      placed[pick] = 1;
+      local |= masks[pick];
+      order.push_back(pick);
+      ++in_pass;
+      for (size_t j : succ[pick]) --npred[j];
+    }
+  }
+  return order;
+}
+
 // Orders the blocks for chunk passes: the given order, the block order for chunks of 13 local
 // qubits (the 4 lowest always among them) of it and of random dependence-respecting orders of
 // it (blocks sharing a qubit keep their order; fixed seeds, so plans are reproducible); the
 // order giving the fewest passes wins (the given one on a tie).  Returns that number.
-static size_t order_for_chunks(std::vector<fused_group>& groups, unsigned width, int tries) {
+static size_t order_for_chunks(std::vector<fused_group>& groups, unsigned width, int tries, int greedy_tries) {
   std::vector<brk_circuit::op> pseudo(groups.size());
   for (size_t g = 0; g < groups.size(); ++g) {
     pseudo[g].code = BRK_OP_X;
@@ -405,6 +449,35 @@ static size_t order_for_chunks(std::vector<fused_group>& groups, unsigned width,
       best = std::move(order);
     }
   }
+  // randomized greedy passes (fixed seeds), kept when they need fewer passes
+  if (greedy_tries > 0) {
+    const size_t B = groups.size();
+    std::vector<uint64_t> masks(B);
+    for (size_t g = 0; g < B; ++g) masks[g] = widened_mask(groups[g].qubits, width);
+    std::vector<std::vector<size_t>> succ(B);
+    std::vector<unsigned> npred(B, 0);
+    for (size_t j = 0; j < B; ++j)
+      for (size_t i = 0; i < j; ++i)
+        if (!groups[i].qubits.empty() && !groups[j].qubits.empty() &&
+            std::any_of(groups[j].qubits.begin(), groups[j].qubits.end(),
+                        [&](unsigned q) { return groups[i].qubits.count(q) != 0; })) {
+          succ[i].push_back(j);
+          ++npred[j];
+        }
+    const size_t cap = (size_t)brk::fused_chunk_max_blocks((int)width);
+    for (int t = 0; t < greedy_tries; ++t) {
+      std::mt19937_64 gen(0x9e3779b97f4a7c15ull + (uint64_t)t);
+      const std::vector<size_t> order = greedy_chunk_order(masks, succ, npred, cap, 2.0, gen);
+      std::vector<fused_group> trial;
+      trial.reserve(order.size());
+      for (size_t g : order) trial.push_back(groups[g]);
+      const size_t p = count_passes(trial, width);
+      if (p < best_passes) {
+        best_passes = p;
+        best = order;
+      }
+    }
+  }
   std::vector<fused_group> ordered;
   ordered.reserve(best.size());
   for (size_t g : best) ordered.push_back(std::move(groups[g]));
@@ -422,6 +495,15 @@ constexpr int CHUNK_ORDER_TRIES = 8;  // chunk orders tried per candidate gate o
 constexpr double CHUNK_PASS_COST = 1.25;
 static int fusion_order_tries(size_t num_ops) {
   return num_ops <= 4096 ? FUSION_ORDER_TRIES : num_ops <= 65536 ? 4 : 1;
+}
+// Randomized greedy pass orders (greedy_chunk_order) tried per candidate and for the final plan;
+// one try costs ~blocks^2 steps, so long plans try fewer.
+constexpr int CHUNK_GREEDY_TRIES = 16;
+constexpr int CHUNK_FINAL_GREEDY_TRIES = 1024;
+static int chunk_greedy_tries(size_t blocks, int tries) {
+  const size_t budget = (size_t)1 << 24;  // block-pair steps
+  const size_t per = std::max<size_t>(1, blocks * blocks);
+  return (int)std::min<size_t>((size_t)tries, budget / per);
 }
 
 extern "C" {
@@ -458,7 +540,7 @@ int brk_fused_plan_create(const brk_circuit* circuit, unsigned width, const unsi
     if (int st = physical_gates(*circuit, order, perm, width, trial, trial_perm)) return st;
     std::vector<fused_group> g = group_gates(trial, width);
     // chunk passes: the fewest passes (each reads and writes the state once), then the fewest blocks
-    const size_t passes = chunked ? order_for_chunks(g, width, CHUNK_ORDER_TRIES) : 0;
+    const size_t passes = chunked ? order_for_chunks(g, width, CHUNK_ORDER_TRIES, chunk_greedy_tries(g.size(), CHUNK_GREEDY_TRIES)) : 0;
     const double cost = chunked ? CHUNK_PASS_COST * (double)passes + (double)g.size() : (double)g.size();
     if (t == 0 || cost < best_cost) {
       best_cost = cost;
@@ -469,7 +551,9 @@ int brk_fused_plan_create(const brk_circuit* circuit, unsigned width, const unsi
   }
 
   // 4. blocks in an order for chunks of 13 local qubits that packs them into few passes
-  if (chunked) (void)order_for_chunks(groups, width, fusion_order_tries(groups.size()));
+  if (chunked)
+    (void)order_for_chunks(groups, width, fusion_order_tries(groups.size()),
+                           chunk_greedy_tries(groups.size(), CHUNK_FINAL_GREEDY_TRIES));
 
   auto* plan = new (std::nothrow) brk_fused_plan;
   if (!plan) return fail(BRK_ERR_OUT_OF_MEMORY, "cannot allocate plan");
