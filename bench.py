@@ -271,12 +271,14 @@ def fused_leg(circuit, plain_state, args, stream):
     st = DeviceState(n, brk.C64)
     widths = {}
     plans = {}
+    per_launch = {}
     for w in (4, 5, 6):
         t0 = _time.perf_counter()
         plans[w] = brk.FusedPlan(circuit, w)
         plan_ms = (_time.perf_counter() - t0) * 1e3
         brk.set_basis_state(st.ptr, st.size, brk.C64, 0, stream)
         ms = plans[w].apply_timed(st.ptr, stream=stream)
+        per_launch[w] = [round(x, 2) for x in ms]
         widths[w] = {"blocks": plans[w].num_blocks, "chunk_passes": plans[w].num_passes, "ms": round(sum(ms), 2),
                      "plan_ms": round(plan_ms, 1)}
         log(f"fused width {w}: {plans[w].num_blocks} blocks in {plans[w].num_passes or plans[w].num_blocks} "
@@ -327,8 +329,9 @@ def fused_leg(circuit, plain_state, args, stream):
             "bound_ms": round(bound_ms, 3),
             "bound": "mfma" if executed / 157.3e12 > hbm / (HBM_PEAK_GBS * 1e9) else "hbm",
             "frac": round(bound_ms / launch_ms, 4),
+            "each_ms_autotune_run": per_launch[best],
         },
-        "dtype": "c64 state, FP32 MFMA (v_mfma_f32_32x32x2_f32), FP32 accumulate; FP64 fidelity",
+        "dtype": "c64 state, FP32 MFMA (v_mfma_f32_16x16x4_f32 in chunk passes, v_mfma_f32_32x32x2_f32 in per-block launches), FP32 accumulate; FP64 fidelity",
     }
 
 
