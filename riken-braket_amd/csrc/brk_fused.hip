@@ -174,6 +174,18 @@ __device__ __forceinline__ void store_f4(cpx<float>* p, f4 a) {
   __builtin_nontemporal_store(v, reinterpret_cast<v4f*>(p));
 }
 
+// Two consecutive complex64 amplitudes as one 16-byte access (non-temporal for large states).
+template <bool NT>
+__device__ __forceinline__ v4f load_pair(const cpx<float>* p) {
+  if constexpr (NT) return __builtin_nontemporal_load(reinterpret_cast<const v4f*>(p));
+  else return *reinterpret_cast<const v4f*>(p);
+}
+template <bool NT>
+__device__ __forceinline__ void store_pair(cpx<float>* p, v4f v) {
+  if constexpr (NT) __builtin_nontemporal_store(v, reinterpret_cast<v4f*>(p));
+  else *reinterpret_cast<v4f*>(p) = v;
+}
+
 template <int KB>
 __global__ __launch_bounds__(STAGED_THREADS, 3) void k_fused_staged(cpx<float>* __restrict__ s, fused_launch L) {
   constexpr int R = 1 << KB, W = 2 * R, HALF = R / 2, DT = W / 32;
@@ -339,10 +351,10 @@ static int launch_k(const fused_launch& L, void* state, void* stream) {
 // 1024-thread workgroup per CU walks a contiguous run of chunks; its 160 KiB of LDS
 // hold two chunk buffers (2 x 64 KiB) and the pass's blocks in three-product form
 // (up to 8 at k = 4, 2 at k = 5, loaded once).  For chunk i every thread first stores
-// its 8 amplitudes of chunk i - 1 from the other buffer and issues the loads of its 8
-// amplitudes of chunk i + 1 into registers; then the 16 waves run the pass's blocks
-// on chunk i (one barrier per block) and, before the last block's barrier, each
-// thread writes its loaded amplitudes into the other buffer.  A barrier waits only
+// its 4 amplitude pairs of chunk i - 1 from the other buffer and issues the loads of its
+// 4 pairs of chunk i + 1 into registers (16-byte accesses); then the 16 waves run the
+// pass's blocks on chunk i (one barrier per block) and, before the last block's barrier,
+// each thread writes its loaded pairs into the other buffer.  A barrier waits only
 // for LDS operations and the blocks issue no global memory accesses, so the chunk
 // transfers stay in flight while the blocks run.  A thread touches the same 8 slots
 // of either buffer for the transfers, so its reads (for the stores) and later writes
@@ -389,7 +401,7 @@ __global__ __launch_bounds__(CHUNK_THREADS, 1) void k_fused_chunk(cpx<float>* __
   using S = chunk_shape<KB>;
   constexpr int R = S::R, STEPS = S::STEPS;
   constexpr int CHUNK = 1 << FUSED_CHUNK_LOCAL;
-  constexpr int PER = CHUNK / CHUNK_THREADS;  // 8 amplitudes per thread in the transfers
+  constexpr int PAIRS = CHUNK / (2 * CHUNK_THREADS);  // 4 amplitude pairs (16 bytes each) per thread
   extern __shared__ float smem[];
   cpx<float>* bufs = reinterpret_cast<cpx<float>*>(smem);  // two chunk buffers
   float* slots = smem + 4 * CHUNK;                          // the pass's three-product forms
@@ -411,10 +423,14 @@ __global__ __launch_bounds__(CHUNK_THREADS, 1) void k_fused_chunk(cpx<float>* __
     for (int j = 1; j <= FUSED_CHUNK_LOCAL; ++j) b |= (c & prog->outer_masks[j]) << j;
     return b;
   };
-  // transfers: local index t + 1024 k, the bits of t at pos[0 .. 9], k at pos[10 .. 12]
-  // (deposit_high[2k] places local index 1024 k = (2k) << 9)
+  // transfers in 16-byte pairs (local bit 0 is qubit 0, so a pair is two consecutive amplitudes):
+  // pair k of thread t holds local indices 2 (t + 1024 k) and the one after, the bits of t at
+  // pos[1 .. 10], k at pos[11 .. 12] (deposit_high[4k] places local index 2048 k = (4k) << 9).
+  // Half as many memory instructions as 8-byte amplitudes, which shortens the transfer issue at the
+  // start of every chunk.
   uint64_t low = 0;
-  for (int r = 0; r < 10; ++r) low |= (uint64_t)((t >> r) & 1u) << prog->pos[r];
+  for (int r = 0; r < 10; ++r) low |= (uint64_t)((t >> r) & 1u) << prog->pos[r + 1];
+  auto pair_at = [&](uint64_t b, int k) { return s + (b | low | prog->deposit_high[4 * k]); };
 
   // prologue: the pass's three-product forms into their slots, chunk 0 into buffer 0
   for (uint32_t m = 0; m < nops; ++m) {
@@ -422,13 +438,13 @@ __global__ __launch_bounds__(CHUNK_THREADS, 1) void k_fused_chunk(cpx<float>* __
     for (uint32_t i = t; i < (uint32_t)(S::AFLOATS / 4); i += CHUNK_THREADS)
       reinterpret_cast<v4f*>(slots + m * S::AFLOATS)[i] = A4[i];
   }
-  cpx<float> pre[PER];
+  v4f pre[PAIRS];
   if (iters > 0) {
     const uint64_t b = base_of(chunk_of(0));
 #pragma unroll
-    for (int k = 0; k < PER; ++k) pre[k] = load_amp<NT>(s + (b | low | prog->deposit_high[2 * k]));
+    for (int k = 0; k < PAIRS; ++k) pre[k] = load_pair<NT>(pair_at(b, k));
 #pragma unroll
-    for (int k = 0; k < PER; ++k) bufs[t + CHUNK_THREADS * k] = pre[k];
+    for (int k = 0; k < PAIRS; ++k) reinterpret_cast<v4f*>(bufs)[t + CHUNK_THREADS * k] = pre[k];
   }
   __syncthreads();
 
@@ -442,8 +458,8 @@ __global__ __launch_bounds__(CHUNK_THREADS, 1) void k_fused_chunk(cpx<float>* __
     if (it > 0) {  // chunk it - 1 back to HBM
       const uint64_t b = base_of(chunk_of(it - 1));
 #pragma unroll
-      for (int k = 0; k < PER; ++k)
-        store_amp<NT>(s + (b | low | prog->deposit_high[2 * k]), other[t + CHUNK_THREADS * k]);
+      for (int k = 0; k < PAIRS; ++k)
+        store_pair<NT>(pair_at(b, k), reinterpret_cast<const v4f*>(other)[t + CHUNK_THREADS * k]);
     }
     // chunk it + 1 into registers -- on the last iteration the current chunk again: its loads and
     // the write into the free buffer below are then harmless, and on every path the loads are
@@ -452,7 +468,7 @@ __global__ __launch_bounds__(CHUNK_THREADS, 1) void k_fused_chunk(cpx<float>* __
     {
       const uint64_t b = base_of(chunk_of(it + 1 < iters ? it + 1 : it));
 #pragma unroll
-      for (int k = 0; k < PER; ++k) pre[k] = load_amp<NT>(s + (b | low | prog->deposit_high[2 * k]));
+      for (int k = 0; k < PAIRS; ++k) pre[k] = load_pair<NT>(pair_at(b, k));
     }
     for (uint32_t o = 0; o < nops; ++o) {
       const auto& op = prog->ops[o];
@@ -511,15 +527,15 @@ __global__ __launch_bounds__(CHUNK_THREADS, 1) void k_fused_chunk(cpx<float>* __
     }
     // chunk it + 1 into the other buffer (this thread's slots), then the chunk's last barrier
 #pragma unroll
-    for (int k = 0; k < PER; ++k) other[t + CHUNK_THREADS * k] = pre[k];
+    for (int k = 0; k < PAIRS; ++k) reinterpret_cast<v4f*>(other)[t + CHUNK_THREADS * k] = pre[k];
     __syncthreads();
   }
   if (iters > 0) {  // the last chunk back to HBM
     const cpx<float>* last = bufs + ((iters - 1) & 1) * CHUNK;
     const uint64_t b = base_of(chunk_of(iters - 1));
 #pragma unroll
-    for (int k = 0; k < PER; ++k)
-      store_amp<NT>(s + (b | low | prog->deposit_high[2 * k]), last[t + CHUNK_THREADS * k]);
+    for (int k = 0; k < PAIRS; ++k)
+      store_pair<NT>(pair_at(b, k), reinterpret_cast<const v4f*>(last)[t + CHUNK_THREADS * k]);
   }
 }
 
