@@ -95,18 +95,24 @@ __device__ __forceinline__ void apply_slot(f2 (&a)[1 << NP], cu32 op) {
   }
 }
 
-// Real 2x2 [[r00, r01], [r10, r11]] on the pairs of slot S: one multiply and one fused
-// multiply-add per component.
+// Rotation [[c, -s], [s, c]], |theta| <= pi/2, on the pairs of slot S as three shears
+// (t = tan(theta/2), |t| <= 1): a0 -= t a1; a1 += s a0; a0 -= t a1, which leaves
+// a0 = c a0 - s a1 and a1 = s a0 + c a1.  Three fused multiply-adds per component.
 template <int NP, int S>
 __device__ __forceinline__ void apply_real_slot(f2 (&a)[1 << NP], cu32 op) {
-  const float r00 = __uint_as_float(op[0]), r01 = __uint_as_float(op[1]);
-  const float r10 = __uint_as_float(op[2]), r11 = __uint_as_float(op[3]);
+  const float t = __uint_as_float(op[0]), s = __uint_as_float(op[1]);
 #pragma unroll
   for (int e = 0; e < (1 << NP); ++e) {
     if (e & (1 << S)) continue;
-    const f2 a0 = a[e], a1 = a[e | (1 << S)];
-    a[e] = f2{__builtin_fmaf(r00, a0.re, r01 * a1.re), __builtin_fmaf(r00, a0.im, r01 * a1.im)};
-    a[e | (1 << S)] = f2{__builtin_fmaf(r10, a0.re, r11 * a1.re), __builtin_fmaf(r10, a0.im, r11 * a1.im)};
+    f2 x = a[e], y = a[e | (1 << S)];
+    x.re = __builtin_fmaf(-t, y.re, x.re);
+    x.im = __builtin_fmaf(-t, y.im, x.im);
+    y.re = __builtin_fmaf(s, x.re, y.re);
+    y.im = __builtin_fmaf(s, x.im, y.im);
+    x.re = __builtin_fmaf(-t, y.re, x.re);
+    x.im = __builtin_fmaf(-t, y.im, x.im);
+    a[e] = x;
+    a[e | (1 << S)] = y;
   }
 }
 
@@ -299,10 +305,12 @@ __global__ __launch_bounds__(BATCH_THREADS, BATCH_WG_PER_CU) void k_batch(batch_
             }
             w += 3 + 2 * (((w0 >> 16) & 0xffu) + (w0 >> 24)) + BATCH_GATE_WORDS * npos;
           } else if (op == BI_GATES_REAL || op == BI_GATES_REAL_PRE) {
-            // the planner emits these for 2..4 positions (for one position the complex form is cheaper)
+            // the planner emits BI_GATES_REAL for 1..4 positions, BI_GATES_REAL_PRE for 2..4 (for one
+            // position with both tables the complex form is cheaper)
             const uint32_t npos = (w0 >> 8) & 0xffu;
             if (op == BI_GATES_REAL) {
               switch (npos) {
+                case 1: gate_pass<1, BI_GATES_REAL>(c, w, tid); break;
                 case 2: gate_pass<2, BI_GATES_REAL>(c, w, tid); break;
                 case 3: gate_pass<3, BI_GATES_REAL>(c, w, tid); break;
                 default: gate_pass<4, BI_GATES_REAL>(c, w, tid); break;

@@ -33,16 +33,17 @@ constexpr int BATCH_CHUNK_PADDED = BATCH_CHUNK;  // LDS slots of a chunk (swizzl
 //                host): gates on different positions commute.  Pre-signs are applied to the group's
 //                amplitudes before its gates, post-signs after them; attached terms have d < 32.
 //   real group : w0 = BI_GATES_REAL or BI_GATES_REAL_PRE, then w1, w2 and the sign terms as for a gate
-//                group, then one real 2x2 matrix per position (4 words: r00, r01, r10, r11, FP32), then
-//                the post-phase table (2^npos complex64, 2 words each), and for BI_GATES_REAL_PRE the
-//                pre-phase table (2^npos complex64).  The product matrix U of position j is written as
-//                U = diag(p0, p1) R diag(1, q) with R real (every unitary has this form: within a row
-//                the phase difference of the two entries is the same for both rows, mod pi), so the
+//                group, then one real rotation per position (2 words: t = tan(theta/2), s = sin(theta),
+//                FP32, |theta| <= pi/2), then the post-phase table (2^npos complex64, 2 words each), and
+//                for BI_GATES_REAL_PRE the pre-phase table (2^npos complex64).  The product matrix U of
+//                position j is written as U = diag(p0, p1) R diag(1, q) with R the rotation
+//                [[c, -s], [s, c]] (every unitary has this form: within a row the phase difference of
+//                the two entries is the same for both rows, mod pi; signs move between R and p), so the
 //                group applies, to its element e (bit j of e: the value at position j):
-//                a[e] *= prod_j q_j^(e_j) (pre table), the real R_j on position j's pairs,
-//                a[e] *= prod_j p_j[e_j] (post table).  The diagonal factors of different positions
-//                commute with each other and with the sign terms, and the tables are the same for
-//                every thread of the pass.
+//                a[e] *= prod_j q_j^(e_j) (pre table), R_j on position j's pairs (a0, a1) as three
+//                shears a0 -= t a1, a1 += s a0, a0 -= t a1, then a[e] *= prod_j p_j[e_j] (post table).
+//                The diagonal factors of different positions commute with each other and with the
+//                sign terms, and the tables are the same for every thread of the pass.
 //   signs      : w0 = BI_SIGN | nterms << 16, then nterms sign terms (a pass of its own).
 // A sign term (d, A) negates an amplitude with index x when its bit count is odd:
 //   d = 0      : x & A                  (Z on each qubit of A)
@@ -51,7 +52,7 @@ constexpr int BATCH_CHUNK_PADDED = BATCH_CHUNK;  // LDS slots of a chunk (swizzl
 // Terms are merged by XOR, so repeated masks cancel exactly as the gates do.
 enum : uint32_t { BI_GATES = 1, BI_SIGN = 2, BI_GATES_REAL = 3, BI_GATES_REAL_PRE = 4 };
 constexpr int BATCH_GATE_WORDS = 8;
-constexpr int BATCH_REAL_WORDS = 4;
+constexpr int BATCH_REAL_WORDS = 2;
 constexpr int BATCH_MAX_ATTACHED_TERMS = 15;  // per pre/post list of a gate group
 constexpr uint32_t BATCH_TERM_ALL = 32;
 

@@ -176,14 +176,18 @@ std::vector<std::pair<uint32_t, uint32_t>> sign_terms(const std::vector<uint32_t
   return terms;
 }
 
-// U = diag(p0, p1) R diag(1, q) with R real, for a 2x2 unitary U (row-major).  Within a row of a
-// unitary the two entries' phases differ by the same angle in both rows (mod pi), so q removes
-// that difference and each row is then its own phase times a real row.  q is taken in the right
-// half-plane (a factor -1 moves into R), and exactly 1 when it is within rounding of 1, so the
-// RZ RY products of an ansatz layer need no pre-phase.  ok is false when a row is not real
-// after the phases come out (U not unitary to FP64 rounding).
+// U = diag(p0, p1) R diag(1, q) with R a real rotation by |theta| <= pi/2, for a 2x2 unitary U
+// (row-major).  Within a row of a unitary the two entries' phases differ by the same angle in both
+// rows (mod pi), so q removes that difference and each row is then its own phase times a real row.
+// q is taken in the right half-plane (a factor -1 moves into R), and exactly 1 when it is within
+// rounding of 1, so the RZ RY products of an ansatz layer need no pre-phase.  The real R is
+// orthogonal: a reflection becomes a rotation by negating its second row (and p1), and a rotation
+// with cos theta < 0 by negating it (and both p), so |theta| <= pi/2 and the kernel can apply it as
+// three shears with t = tan(theta / 2) = s / (1 + c), |t| <= 1:
+//   x -= t y;  y += s x;  x -= t y      (x' = c x - s y, y' = s x + c y).
+// ok is false when a row is not real after the phases come out (U not unitary to FP64 rounding).
 struct real_form {
-  double r[4];
+  double t, s;  // tan(theta / 2), sin(theta)
   cd p[2];
   cd q;
   bool ok;
@@ -203,16 +207,39 @@ real_form real_decomposition(const std::array<cd, 4>& u) {
   double scale = 0.0;
   for (const cd& x : u) scale = std::max(scale, std::abs(x));
   f.ok = scale > 0.0;
+  double R[4];
   for (int r = 0; r < 2; ++r) {
     const cd big = std::abs(v[2 * r]) >= std::abs(v[2 * r + 1]) ? v[2 * r] : v[2 * r + 1];
     const double m = std::abs(big);
     f.p[r] = m > 0.0 ? big / m : cd(1.0, 0.0);
     for (int c = 0; c < 2; ++c) {
       const cd x = v[2 * r + c] * std::conj(f.p[r]);
-      f.r[2 * r + c] = x.real();
+      R[2 * r + c] = x.real();
       if (std::abs(x.imag()) > 1e-12 * scale) f.ok = false;
     }
   }
+  if (R[0] * R[3] - R[1] * R[2] < 0.0) {  // reflection: diag(1, -1) times a rotation
+    R[2] = -R[2];
+    R[3] = -R[3];
+    f.p[1] = -f.p[1];
+  }
+  // rotation [[c, -s], [s, c]] (the two estimates of c and of s agree to rounding)
+  double c = 0.5 * (R[0] + R[3]), sn = 0.5 * (R[2] - R[1]);
+  if (c < 0.0) {
+    c = -c;
+    sn = -sn;
+    f.p[0] = -f.p[0];
+    f.p[1] = -f.p[1];
+  }
+  const double h = std::hypot(c, sn);
+  if (!(h > 0.0) || std::abs(h - 1.0) > 1e-12 || std::abs(R[0] - R[3]) > 1e-12 || std::abs(R[1] + R[2]) > 1e-12)
+    f.ok = false;
+  if (f.ok) {
+    c /= h;
+    sn /= h;
+  }
+  f.t = sn / (1.0 + c);
+  f.s = sn;
   f.q = q;
   return f;
 }
@@ -296,8 +323,8 @@ uint32_t emit_stage(const planned_stage& st, std::vector<uint32_t>& words, uint3
         mats[slot] = {g->m[0] * a[0] + g->m[1] * a[2], g->m[0] * a[1] + g->m[1] * a[3],
                       g->m[2] * a[0] + g->m[3] * a[2], g->m[2] * a[1] + g->m[3] * a[3]};
       }
-      // real form (real 2x2 per position between two uniform phase tables) when it takes fewer
-      // lane operations than the complex products: per thread 4 np 2^np + 4 2^np per table
+      // real form (a rotation per position between two uniform phase tables) when it takes fewer
+      // lane operations than the complex products: per thread 3 np 2^np + 4 2^np per table
       // against 8 np 2^np
       const size_t np = pos.size(), ne = (size_t)1 << np;
       std::vector<real_form> forms;
@@ -307,7 +334,7 @@ uint32_t emit_stage(const planned_stage& st, std::vector<uint32_t>& words, uint3
         real_ok &= forms.back().ok;
         pre_phase |= forms.back().q != cd(1.0, 0.0);
       }
-      const bool use_real = real_ok && 4 * np * ne + 4 * ne * (pre_phase ? 2 : 1) < 8 * np * ne;
+      const bool use_real = real_ok && 3 * np * ne + 4 * ne * (pre_phase ? 2 : 1) < 8 * np * ne;
       const uint32_t op = use_real ? (pre_phase ? brk::BI_GATES_REAL_PRE : brk::BI_GATES_REAL) : brk::BI_GATES;
       words.push_back(op | (uint32_t)np << 8 | (uint32_t)sg.pre.size() << 16 | (uint32_t)sg.post.size() << 24);
       words.push_back(pw);
@@ -321,8 +348,10 @@ uint32_t emit_stage(const planned_stage& st, std::vector<uint32_t>& words, uint3
             words.push_back(float_bits(m[k].imag()));
           }
       } else {
-        for (const auto& f : forms)
-          for (int k = 0; k < 4; ++k) words.push_back(float_bits(f.r[k]));
+        for (const auto& f : forms) {
+          words.push_back(float_bits(f.t));
+          words.push_back(float_bits(f.s));
+        }
         // tables over the group's elements e: bit j of e is the value at position j
         for (int table = 0; table < (pre_phase ? 2 : 1); ++table)
           for (size_t e = 0; e < ne; ++e) {
