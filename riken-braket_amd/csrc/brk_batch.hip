@@ -51,24 +51,46 @@ __device__ __forceinline__ f2 flip(f2 a, uint32_t odd) {
   return f2{__uint_as_float(__float_as_uint(a.re) ^ s), __uint_as_float(__float_as_uint(a.im) ^ s)};
 }
 
-// Sign terms attached to a gate group are of the Z / CZ kinds only (d < 32): the
-// bit count of x & (x >> d) & A is branch-free and accumulates in one instruction.
+// Sign terms attached to a gate group are of the Z / CZ kinds only (d < 32).  Element e of a
+// thread's group has the index x = xb | xoff[e]: the thread's bits xb and the element's bits
+// xoff[e] (one qubit q_j per position j) are disjoint, so x & (x >> d) & A is the disjoint union
+// of four parts and its bit count, mod 2, is
+//   popcount(xb & (xb >> d) & A)                                  the thread's own sign
+// + popcount(xoff[e] & M),  M = ((xb & A) << d) ^ ((xb >> d) & A)  linear in e: bit q_j of M per position
+// + popcount(xoff[e] & (xoff[e] >> d) & A)                        the same for every thread.
+// The planner sums the last part over the terms (bit e of `uniform`: its parity for element e);
+// the others take a few instructions per thread and term, and the element signs follow from the
+// positions' bits of M by one XOR each.
+
 template <int NP>
-__device__ __forceinline__ void apply_terms(f2 (&a)[1 << NP], uint32_t xb, const uint32_t (&xoff)[1 << NP], cu32 t,
-                                            uint32_t nterms) {
-  uint32_t x[1 << NP], par[1 << NP];
-#pragma unroll
-  for (int e = 0; e < (1 << NP); ++e) {
-    x[e] = xb | xoff[e];
-    par[e] = 0;
-  }
+__device__ __forceinline__ void apply_terms(f2 (&a)[1 << NP], uint32_t xb, const uint32_t (&q)[NP], cu32 t,
+                                            uint32_t nterms, uint32_t uniform) {
+  uint32_t own = 0, M = 0;
   for (uint32_t k = 0; k < nterms; ++k) {
     const uint32_t d = t[2 * k], A = t[2 * k + 1];
+    own += __builtin_popcount(xb & (xb >> d) & A);
+    M ^= ((xb & A) << d) ^ ((xb >> d) & A);
+  }
+  // sign bit (bit 31) of element e, without the uniform part: own + sum over the positions j set in e
+  // of bit q_j of M
+  uint32_t sg[1 << NP];
+  sg[0] = own << 31;
 #pragma unroll
-    for (int e = 0; e < (1 << NP); ++e) par[e] += __builtin_popcount(x[e] & (x[e] >> d) & A);
+  for (int j = 0; j < NP; ++j) {
+    const uint32_t mj = (M >> q[j]) << 31;
+#pragma unroll
+    for (int e = 0; e < (1 << j); ++e) sg[e | (1 << j)] = sg[e] ^ mj;
+  }
+  if (uniform) {  // (the planner moves them into the real form's phase tables)
+    // the element masks are formed here, not kept live across the pass's loop
+    uint32_t u = uniform;
+    asm volatile("" : "+s"(u));
+#pragma unroll
+    for (int e = 0; e < (1 << NP); ++e) sg[e] ^= ((u >> e) & 1u) << 31;
   }
 #pragma unroll
-  for (int e = 0; e < (1 << NP); ++e) a[e] = flip(a[e], par[e] & 1u);
+  for (int e = 0; e < (1 << NP); ++e)
+    a[e] = f2{__uint_as_float(__float_as_uint(a[e].re) ^ sg[e]), __uint_as_float(__float_as_uint(a[e].im) ^ sg[e])};
 }
 
 // m0 a0 + m1 a1 as one multiply and three fused multiply-adds per component
@@ -145,23 +167,23 @@ template <int NP, uint32_t FORM>
 __device__ void gate_pass(const chunk_ctx& c, cu32 w, uint32_t tid) {
   const uint32_t w0 = w[0], pw = w[1], qw = w[2];
   const uint32_t npre = (w0 >> 16) & 0xffu, npost = (w0 >> 24) & 0xffu;
-  const cu32 pre = w + 3;
+  const uint32_t pre_uniform = w[3] & 0xffffu, post_uniform = w[3] >> 16;
+  const cu32 pre = w + 4;
   const cu32 post = pre + 2 * npre;
   const cu32 ops = post + 2 * npost;
-  uint32_t p[NP];
+  uint32_t p[NP], q[NP];
 #pragma unroll
-  for (int k = 0; k < NP; ++k) p[k] = (pw >> (8 * k)) & 0xffu;
-  // per group element (uniform): global index bits and swizzled LDS byte offset
-  uint32_t xoff[1 << NP], boff[1 << NP];
+  for (int k = 0; k < NP; ++k) {
+    p[k] = (pw >> (8 * k)) & 0xffu;
+    q[k] = (qw >> (8 * k)) & 0xffu;
+  }
+  // per group element (uniform): swizzled LDS byte offset
+  uint32_t boff[1 << NP];
 #pragma unroll
   for (int e = 0; e < (1 << NP); ++e) {
-    uint32_t x = 0, off = 0;
+    uint32_t off = 0;
 #pragma unroll
-    for (int k = 0; k < NP; ++k) {
-      x |= ((uint32_t)(e >> k) & 1u) << ((qw >> (8 * k)) & 0xffu);
-      off |= ((uint32_t)(e >> k) & 1u) << p[k];
-    }
-    xoff[e] = x;
+    for (int k = 0; k < NP; ++k) off |= ((uint32_t)(e >> k) & 1u) << p[k];
     boff[e] = lpad(off) * 8u;
   }
   const uint32_t ngroups = 1u << (c.L - NP);
@@ -177,7 +199,7 @@ __device__ void gate_pass(const chunk_ctx& c, cu32 w, uint32_t tid) {
     for (int e = 0; e < (1 << NP); ++e) a[e] = *reinterpret_cast<const f2*>(bytes + (pb ^ boff[e]));
     uint32_t xb = 0;
     if (npre | npost) xb = global_index(c, base);
-    if (npre) apply_terms<NP>(a, xb, xoff, pre, npre);
+    if (npre) apply_terms<NP>(a, xb, q, pre, npre, pre_uniform);
     if constexpr (FORM == BI_GATES) {
       apply_slot<NP, 0>(a, ops);
       if constexpr (NP > 1) apply_slot<NP, 1>(a, ops + BATCH_GATE_WORDS);
@@ -192,7 +214,7 @@ __device__ void gate_pass(const chunk_ctx& c, cu32 w, uint32_t tid) {
       if constexpr (NP > 3) apply_real_slot<NP, 3>(a, ops + 3 * BATCH_REAL_WORDS);
       apply_phases<NP>(a, post_table);
     }
-    if (npost) apply_terms<NP>(a, xb, xoff, post, npost);
+    if (npost) apply_terms<NP>(a, xb, q, post, npost, post_uniform);
     // recompute the store addresses instead of keeping 2^NP of them live across the gates
     uint32_t pbs = pb;
     asm volatile("" : "+v"(pbs));
@@ -227,7 +249,12 @@ __device__ void sign_pass(const chunk_ctx& c, cu32 w, uint32_t tid) {
   }
 }
 
-__global__ __launch_bounds__(BATCH_THREADS, BATCH_WG_PER_CU) void k_batch(batch_launch P) {
+constexpr int CHUNK_PER_THREAD = BATCH_CHUNK / BATCH_THREADS;
+// (the second bound is waves per SIMD: BATCH_WG_PER_CU workgroups of BATCH_THREADS / 64 waves over
+// 4 SIMDs, which caps the registers at 512 / 4 = 128 VGPRs)
+constexpr int BATCH_WAVES_PER_SIMD = BATCH_WG_PER_CU * BATCH_THREADS / 64 / 4;
+
+__global__ __launch_bounds__(BATCH_THREADS, BATCH_WAVES_PER_SIMD) void k_batch(batch_launch P) {
   // static LDS: addresses are compile-time constants, so group offsets fold into the instructions
   __shared__ __align__(16) f2 chunk[BATCH_CHUNK_PADDED];
   __shared__ uint32_t tlo[128], thi[128];
@@ -289,7 +316,14 @@ __global__ __launch_bounds__(BATCH_THREADS, BATCH_WG_PER_CU) void k_batch(batch_
           for (uint32_t j = tid; j < csize; j += BATCH_THREADS)
             chunk[lpad(j)] = f2{global_index(c, j) == 0 ? 1.0f : 0.0f, 0.0f};
         } else {
-          for (uint32_t j = tid; j < csize; j += BATCH_THREADS) chunk[lpad(j)] = slot[global_index(c, j)];
+          // a circuit with stages after the first has n > BATCH_MAX_LOCAL, so its chunks are full
+          // (csize = BATCH_CHUNK); all of the thread's loads go out at once (a loop would wait for
+          // each in turn)
+          f2 v[CHUNK_PER_THREAD];
+#pragma unroll
+          for (int r = 0; r < CHUNK_PER_THREAD; ++r) v[r] = slot[global_index(c, tid + BATCH_THREADS * r)];
+#pragma unroll
+          for (int r = 0; r < CHUNK_PER_THREAD; ++r) chunk[lpad(tid + BATCH_THREADS * r)] = v[r];
         }
         __syncthreads();
         cu32 w = first_word;
@@ -303,7 +337,7 @@ __global__ __launch_bounds__(BATCH_THREADS, BATCH_WG_PER_CU) void k_batch(batch_
               case 3: gate_pass<3, BI_GATES>(c, w, tid); break;
               default: gate_pass<4, BI_GATES>(c, w, tid); break;
             }
-            w += 3 + 2 * (((w0 >> 16) & 0xffu) + (w0 >> 24)) + BATCH_GATE_WORDS * npos;
+            w += 4 + 2 * (((w0 >> 16) & 0xffu) + (w0 >> 24)) + BATCH_GATE_WORDS * npos;
           } else if (op == BI_GATES_REAL || op == BI_GATES_REAL_PRE) {
             // the planner emits BI_GATES_REAL for 1..4 positions, BI_GATES_REAL_PRE for 2..4 (for one
             // position with both tables the complex form is cheaper)
@@ -322,7 +356,7 @@ __global__ __launch_bounds__(BATCH_THREADS, BATCH_WG_PER_CU) void k_batch(batch_
                 default: gate_pass<4, BI_GATES_REAL_PRE>(c, w, tid); break;
               }
             }
-            w += 3 + 2 * (((w0 >> 16) & 0xffu) + (w0 >> 24)) + BATCH_REAL_WORDS * npos +
+            w += 4 + 2 * (((w0 >> 16) & 0xffu) + (w0 >> 24)) + BATCH_REAL_WORDS * npos +
                  2 * (1u << npos) * (op == BI_GATES_REAL_PRE ? 2u : 1u);
           } else {
             sign_pass(c, w, tid);
@@ -350,7 +384,11 @@ __global__ __launch_bounds__(BATCH_THREADS, BATCH_WG_PER_CU) void k_batch(batch_
           }
         }
         if (flags & BS_STORE) {
-          for (uint32_t j = tid; j < csize; j += BATCH_THREADS) slot[global_index(c, j)] = chunk[lpad(j)];
+          f2 v[CHUNK_PER_THREAD];  // (full chunks, as for the loads)
+#pragma unroll
+          for (int r = 0; r < CHUNK_PER_THREAD; ++r) v[r] = chunk[lpad(tid + BATCH_THREADS * r)];
+#pragma unroll
+          for (int r = 0; r < CHUNK_PER_THREAD; ++r) slot[global_index(c, tid + BATCH_THREADS * r)] = v[r];
         }
         __syncthreads();
       }

@@ -25,14 +25,18 @@ constexpr int BATCH_CHUNK_PADDED = BATCH_CHUNK;  // LDS slots of a chunk (swizzl
 
 // Instruction stream (32-bit words):
 //   gate group : w0 = BI_GATES | npos << 8 | npre << 16 | npost << 24
-//                w1 = local positions (bytes, ascending), w2 = the qubits at those positions (bytes)
+//                w1 = local positions (bytes, ascending), w2 = the qubits at those positions (bytes),
+//                w3 = the thread-independent parts of the sign terms' parities: bit e (pre terms) and
+//                bit 16 + e (post terms) is the parity of the sum over the terms (d, A) of
+//                popcount(xe & (xe >> d) & A), xe the index bits of group element e (bit j of e at
+//                the qubit of position j),
 //                then npre sign terms, npost sign terms (2 words each), then one 2x2 matrix per position,
 //                8 words each: m00.re, m00.im, m01.re, m01.im, m10.re, m10.im, m11.re, m11.im (FP32 bit
 //                patterns; a0' = m00 a0 + m01 a1, a1' = m10 a0 + m11 a1 on the pairs of that position).
 //                The gates of a group on one position are multiplied into that matrix (in FP64 on the
 //                host): gates on different positions commute.  Pre-signs are applied to the group's
 //                amplitudes before its gates, post-signs after them; attached terms have d < 32.
-//   real group : w0 = BI_GATES_REAL or BI_GATES_REAL_PRE, then w1, w2 and the sign terms as for a gate
+//   real group : w0 = BI_GATES_REAL or BI_GATES_REAL_PRE, then w1, w2, w3 and the sign terms as for a gate
 //                group, then one real rotation per position (2 words: t = tan(theta/2), s = sin(theta),
 //                FP32, |theta| <= pi/2), then the post-phase table (2^npos complex64, 2 words each), and
 //                for BI_GATES_REAL_PRE the pre-phase table (2^npos complex64).  The product matrix U of

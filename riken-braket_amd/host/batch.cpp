@@ -285,7 +285,9 @@ uint32_t emit_stage(const planned_stage& st, std::vector<uint32_t>& words, uint3
       flags |= brk::BS_ZZ;
     }
   }
-  // fold each sign run into the next group (before its gates) or the previous one (after its gates)
+  // fold each sign run into the previous group (after its gates) or the next one (before its gates);
+  // after the gates first: there the real form's post-phase table takes the run's thread-independent
+  // signs
   for (size_t i = 0; i < segs.size(); ++i) {
     if (segs[i].gates) continue;
     auto terms = sign_terms(segs[i].masks);
@@ -297,11 +299,11 @@ uint32_t emit_stage(const planned_stage& st, std::vector<uint32_t>& words, uint3
     bool pairs_only = terms.size() <= (size_t)brk::BATCH_MAX_ATTACHED_TERMS;
     for (const auto& t : terms) pairs_only &= t.first < brk::BATCH_TERM_ALL;
     if (!pairs_only) continue;
-    if (i + 1 < segs.size() && segs[i + 1].gates && segs[i + 1].pre.empty()) {
-      segs[i + 1].pre = terms;
-      segs[i].attached = true;
-    } else if (i > 0 && segs[i - 1].gates && segs[i - 1].post.empty()) {
+    if (i > 0 && segs[i - 1].gates && segs[i - 1].post.empty()) {
       segs[i - 1].post = terms;
+      segs[i].attached = true;
+    } else if (i + 1 < segs.size() && segs[i + 1].gates && segs[i + 1].pre.empty()) {
+      segs[i + 1].pre = terms;
       segs[i].attached = true;
     }
   }
@@ -339,6 +341,23 @@ uint32_t emit_stage(const planned_stage& st, std::vector<uint32_t>& words, uint3
       words.push_back(op | (uint32_t)np << 8 | (uint32_t)sg.pre.size() << 16 | (uint32_t)sg.post.size() << 24);
       words.push_back(pw);
       words.push_back(qw);
+      // the sign terms' parity over the group's own index bits, per element (the kernel adds the
+      // parts that depend on the thread's bits); a phase table of the real form takes them instead
+      // (the signs and the table are diagonal and commute), so the kernel then has none to apply
+      auto uniform_parities = [&](const std::vector<std::pair<uint32_t, uint32_t>>& terms) {
+        uint32_t bits = 0;
+        for (uint32_t e = 0; e < (1u << pos.size()); ++e) {
+          uint32_t xe = 0;
+          for (size_t k = 0; k < pos.size(); ++k) xe |= ((e >> k) & 1u) << st.local[pos[k]];
+          uint32_t par = 0;
+          for (const auto& t : terms) par += (uint32_t)__builtin_popcount(xe & (xe >> t.first) & t.second);
+          bits |= (par & 1u) << e;
+        }
+        return bits;
+      };
+      const uint32_t pre_uniform = uniform_parities(sg.pre), post_uniform = uniform_parities(sg.post);
+      const bool post_in_table = use_real, pre_in_table = use_real && pre_phase;
+      words.push_back((pre_in_table ? 0u : pre_uniform) | (post_in_table ? 0u : post_uniform) << 16);
       for (const auto& t : sg.pre) words.insert(words.end(), {t.first, t.second});
       for (const auto& t : sg.post) words.insert(words.end(), {t.first, t.second});
       if (!use_real) {
@@ -360,6 +379,7 @@ uint32_t emit_stage(const planned_stage& st, std::vector<uint32_t>& words, uint3
               const unsigned bit = (unsigned)((e >> j) & 1u);
               z *= table == 0 ? forms[j].p[bit] : (bit ? forms[j].q : cd(1.0, 0.0));
             }
+            if ((((table == 0 ? post_uniform : pre_uniform) >> e) & 1u) != 0) z = -z;
             words.push_back(float_bits(z.real()));
             words.push_back(float_bits(z.imag()));
           }
@@ -587,6 +607,12 @@ int brk_batch_create(const brk_circuit* const* circuits, unsigned num_circuits, 
     double work = 0.0;
     for (size_t s = 0; s < pc.stages.size(); ++s) {
       const planned_stage& st = pc.stages[s];
+      // the kernel moves the chunks of a multi-stage circuit as full chunks
+      if (pc.stages.size() > 1 && st.local.size() != (size_t)brk::BATCH_MAX_LOCAL) {
+        delete b;
+        return fail(BRK_ERR_UNSUPPORTED, "circuit %u: stage %zu has %zu local qubits, not %d", i, s, st.local.size(),
+                    brk::BATCH_MAX_LOCAL);
+      }
       brk::batch_stage bs{};
       bs.L = (uint32_t)st.local.size();
       uint32_t local_mask = 0;
