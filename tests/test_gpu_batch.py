@@ -74,6 +74,33 @@ def test_mixed_ops_and_sizes():
         assert abs(e[i] - plain_energy(n, ops, J, h)) < E_TOL, (n, e[i])
 
 
+def test_cz_terms_of_every_reach():
+    """CZ between qubits at every distance d (sign terms x & (x >> d) & A), between and inside the
+    gate groups, with U3 / H / RY around them so that every sign reaches the measured energy; one-
+    and multi-stage circuits, complex and real gate forms, signs before and after the gates."""
+    rng = np.random.default_rng(17)
+    items = []
+    for n in (3, 6, 11, 13, 14, 16, 19):
+        ops = []
+        for layer in range(6):
+            for q in range(n):
+                k = int(rng.integers(3))
+                if k == 0:
+                    ops.append(("U3", (q,), (), tuple(float(x) for x in rng.uniform(-3, 3, 3))))
+                elif k == 1:
+                    ops.append(("H", (q,), (), ()))
+                else:
+                    ops.append(("RY", (q,), (), (float(rng.uniform(-3, 3)),)))
+            for _ in range(n):
+                a, c = (int(x) for x in rng.choice(n, 2, replace=False))
+                ops.append(("Z", (a,), (c,), ()))
+        items.append((n, ops, rng.uniform(-1, 1, n - 1), rng.uniform(-1, 1, n)))
+    note("gpu batch: CZ terms, n = 3..19")
+    e = brk.Batch(items).run()
+    for i, (n, ops, J, h) in enumerate(items):
+        assert abs(e[i] - plain_energy(n, ops, J, h)) < E_TOL, (n, e[i])
+
+
 def test_one_qubit_closed_form():
     # RY(t)|0> = cos(t/2)|0> + sin(t/2)|1>: <X> = sin t
     t, hx = 0.8, -0.6
