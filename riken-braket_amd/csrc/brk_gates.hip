@@ -839,7 +839,12 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_adjoint_block(cpx<R>* __restr
                                                                 double* __restrict__ partials) {
   constexpr uint32_t CAP = PER * BLOCK_THREADS;
   __shared__ cpx<R> lp[CAP], ll[CAP];
-  __shared__ double wave_sums[ADJ_SEGMENT_GRADIENTS][BLOCK_THREADS / 64];
+  // Per-wave gradient sums, two sets used in turn by the passes that make gradient terms: a pass
+  // writes its set, waits at a barrier and a few threads add it up; the next such pass writes the
+  // other set, so those threads finish while the workgroup goes on, and the pass after that starts
+  // writing the first set only after the next pass's barrier, which they have passed.
+  __shared__ double wave_sums[2][ADJ_SEGMENT_GRADIENTS][BLOCK_THREADS / 64];
+  uint32_t wbuf = 0;
   const cadjoint aprog = (cadjoint)program;
   const cprogram prog = (cprogram)&program->base;
   const uint32_t L = prog->num_local;
@@ -859,8 +864,9 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_adjoint_block(cpx<R>* __restr
   for (uint32_t pi = 0; pi < npasses; ++pi) {
     const auto& pass = prog->passes[pi];
     if (pass.kind == BLOCK_PASS_SEGMENT) {
-      adjoint_segment<R, PER>(lp, ll, aprog, pass.first_op, pass.num_ops, pass.group, t, wave_sums, partials);
-      __syncthreads();
+      // (ends with its barrier between the LDS writes and the gradient sums)
+      adjoint_segment<R, PER>(lp, ll, aprog, pass.first_op, pass.num_ops, pass.group, t, wave_sums[wbuf], partials);
+      wbuf ^= 1u;
       continue;
     }
     const uint32_t o = pass.first_op;
@@ -881,14 +887,14 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_adjoint_block(cpx<R>* __restr
       default: acc = param_pass<ADJ_GRAD_GENERAL, R, PER>(lp, ll, op, aprog->d[o], t); break;
     }
     acc = wave_sum_dpp(acc);
-    if ((t & 63) == 0) wave_sums[0][t >> 6] = acc;
+    if ((t & 63) == 0) wave_sums[wbuf][0][t >> 6] = acc;
     __syncthreads();
     if (t == 0) {
       double sum = 0.0;
-      for (int w = 0; w < BLOCK_THREADS / 64; ++w) sum += wave_sums[0][w];
+      for (int w = 0; w < BLOCK_THREADS / 64; ++w) sum += wave_sums[wbuf][0][w];
       partials[(uint64_t)slot * gridDim.x + blockIdx.x] = sum;
     }
-    __syncthreads();
+    wbuf ^= 1u;
   }
 #pragma unroll
   for (uint32_t k = 0; k < PER; ++k) {
