@@ -368,23 +368,13 @@ static int blocked_gradient(const brk_circuit* given, const std::vector<int>& gi
   forward_order.num_qubits = ordered.num_qubits = n;
   for (size_t i : block_order(given->ops, block_max_local(dtype), line_bits))
     forward_order.ops.push_back(given->ops[i]);
-  std::vector<int> pidx;
-  for (size_t i : block_order(given->ops, L, line_bits)) {
-    ordered.ops.push_back(given->ops[i]);
-    pidx.push_back(given_pidx[i]);
-  }
-  const auto& ops = ordered.ops;
 
-  // forward pass in gate blocks, lam = H psi, E = <psi|lam>
+  // forward pass in gate blocks and lam = H psi on the device, while the host builds the reverse
+  // sweep's programs; then E = <psi|lam> (which synchronises the stream)
   brk_block_plan* plan = nullptr;
   if (int s = brk_block_plan_create(&forward_order, dtype, nullptr, &plan)) return s;
   int status = brk_block_plan_run(plan, psi, stream);
   if (!status) status = brk_apply_ising(lam, psi, size, dtype, J, h, stream);
-  double ip[2] = {0.0, 0.0};
-  if (!status) status = brk_inner_product(lam, dtype, psi, dtype, size, ip, stream);
-  (void)brk_block_plan_destroy(plan);  // brk_inner_product synchronised the stream
-  if (status) return status;
-  *energy = ip[0];
 
   // reverse sweep: blocks of inverse ops, last op first
   std::vector<adjoint_block_program> programs;
@@ -415,41 +405,58 @@ static int blocked_gradient(const brk_circuit* given, const std::vector<int>& gi
     targets.clear();
     qubits = 0;
   };
-  for (size_t i = ops.size(); i-- > 0;) {
-    const brk_circuit::op inv = inverse_op(ops[i]);
-    gate_desc d;
-    if (int s = describe_physical_op(inv.code, inv.targets.data(), (unsigned)inv.targets.size(),
-                                     inv.controls.empty() ? nullptr : inv.controls.data(),
-                                     (unsigned)inv.controls.size(), inv.params.data(), &d))
-      return s;
-    if (__builtin_popcountll(qubits | d.qubits | line) > (int)L || descs.size() == (size_t)BLOCK_MAX_OPS) flush();
-    std::array<double, 8> dm{};
-    if (pidx[i] >= 0) {
-      adj_op a;
-      if (!make_adj_op(ops[i], a)) return fail(BRK_ERR_UNSUPPORTED, "op %zu has no 2x2 form", i);
-      std::copy(a.d, a.d + 8, dm.begin());
+  std::vector<int> pidx;
+  auto build_programs = [&]() -> int {
+    for (size_t i : block_order(given->ops, L, line_bits)) {
+      ordered.ops.push_back(given->ops[i]);
+      pidx.push_back(given_pidx[i]);
     }
-    descs.push_back(d);
-    slots.push_back(pidx[i]);
-    dmats.push_back(dm);
-    targets.push_back(ops[i].targets[0]);
-    qubits |= d.qubits;
-  }
-  flush();
+    const auto& ops = ordered.ops;
+    for (size_t i = ops.size(); i-- > 0;) {
+      const brk_circuit::op inv = inverse_op(ops[i]);
+      gate_desc d;
+      if (int s = describe_physical_op(inv.code, inv.targets.data(), (unsigned)inv.targets.size(),
+                                       inv.controls.empty() ? nullptr : inv.controls.data(),
+                                       (unsigned)inv.controls.size(), inv.params.data(), &d))
+        return s;
+      if (__builtin_popcountll(qubits | d.qubits | line) > (int)L || descs.size() == (size_t)BLOCK_MAX_OPS) flush();
+      std::array<double, 8> dm{};
+      if (pidx[i] >= 0) {
+        adj_op a;
+        if (!make_adj_op(ops[i], a)) return fail(BRK_ERR_UNSUPPORTED, "op %zu has no 2x2 form", i);
+        std::copy(a.d, a.d + 8, dm.begin());
+      }
+      descs.push_back(d);
+      slots.push_back(pidx[i]);
+      dmats.push_back(dm);
+      targets.push_back(ops[i].targets[0]);
+      qubits |= d.qubits;
+    }
+    flush();
+    return BRK_OK;
+  };
+
+  if (!status) status = build_programs();
+  double ip[2] = {0.0, 0.0};
+  if (!status) status = brk_inner_product(lam, dtype, psi, dtype, size, ip, stream);
+  else (void)hipStreamSynchronize(st);  // the plan's device programs are idle before they are freed
+  (void)brk_block_plan_destroy(plan);   // (brk_inner_product synchronised the stream)
+  if (status) return status;
+  *energy = ip[0];
 
   const uint64_t chunks = size >> L;
-  // nparams x chunks partials, then the nparams gradients
+  // one leased buffer: nparams x chunks partials, the nparams gradients, then the programs
+  static_assert(alignof(adjoint_block_program) <= alignof(double), "programs follow doubles");
+  const size_t nsums = std::max(1u, nparams);
+  const size_t bytes = sizeof(adjoint_block_program) * programs.size();
   partials_lease lease;
-  if (int s = get_partials((size_t)nparams * chunks + std::max(1u, nparams), st, lease)) return s;
+  if (int s = get_partials((size_t)nparams * chunks + nsums + (bytes + sizeof(double) - 1) / sizeof(double), st, lease))
+    return s;
   double* partials = lease.ptr;
   double* sums = partials + (size_t)nparams * chunks;
-  adjoint_block_program* device = nullptr;
-  const size_t bytes = sizeof(adjoint_block_program) * programs.size();
-  if (!programs.empty()) {
-    if (hipMalloc((void**)&device, bytes) != hipSuccess) return fail(BRK_ERR_OUT_OF_MEMORY, "adjoint programs");
-    if (hipMemcpy(device, programs.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) status = BRK_ERR_HIP;
-  }
-  if (status) status = fail(BRK_ERR_HIP, "uploading the adjoint programs failed");
+  auto* device = reinterpret_cast<adjoint_block_program*>(sums + nsums);
+  if (!programs.empty() && hipMemcpyAsync(device, programs.data(), bytes, hipMemcpyHostToDevice, st) != hipSuccess)
+    status = fail(BRK_ERR_HIP, "uploading the adjoint programs failed");
   for (size_t b = 0; !status && b < programs.size(); ++b)
     status = launch_adjoint_block(device + b, programs[b], psi, lam, size, dtype, partials, stream);
   if (!status) status = sum_gradient_partials(partials, nparams, chunks, 2.0, sums, stream);
@@ -457,7 +464,6 @@ static int blocked_gradient(const brk_circuit* given, const std::vector<int>& gi
       hipMemcpyAsync(gradient, sums, nparams * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess)
     status = fail(BRK_ERR_HIP, "copying the gradient failed");
   if (hipStreamSynchronize(st) != hipSuccess && !status) status = fail(BRK_ERR_HIP, "hipStreamSynchronize failed");
-  if (device) (void)hipFree(device);
   return status;
 }
 
