@@ -16,8 +16,8 @@ Pinned by:
   that no event lands on an amplitude of probability zero;
 - for the Shor box, Python's exact modular power pow(base, x, divisor) and an
   explicit bit reversal (v_0 = 1 unreduced, only the low n - m bits kept).
-No reference test or fixture covers these paths; the reference's sample
-program shor12.qcx is run end to end by the GPU bra tests."""
+No reference test or fixture covers these paths; the test program
+tests/golden/qcx/shor_box_21.qcx is run end to end by the GPU bra tests."""
 import math
 import os
 import shutil
