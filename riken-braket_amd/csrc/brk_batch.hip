@@ -177,14 +177,15 @@ __device__ void gate_pass(const chunk_ctx& c, cu32 w, uint32_t tid) {
     p[k] = (pw >> (8 * k)) & 0xffu;
     q[k] = (qw >> (8 * k)) & 0xffu;
   }
-  // per group element (uniform): swizzled LDS byte offset
+  // per group element (uniform): swizzled LDS byte offset, the XOR of its positions' offsets (the
+  // swizzle is linear over GF(2))
   uint32_t boff[1 << NP];
+  boff[0] = 0;
 #pragma unroll
-  for (int e = 0; e < (1 << NP); ++e) {
-    uint32_t off = 0;
+  for (int k = 0; k < NP; ++k) {
+    const uint32_t bk = lpad(1u << p[k]) * 8u;
 #pragma unroll
-    for (int k = 0; k < NP; ++k) off |= ((uint32_t)(e >> k) & 1u) << p[k];
-    boff[e] = lpad(off) * 8u;
+    for (int e = 0; e < (1 << k); ++e) boff[e | (1 << k)] = boff[e] ^ bk;
   }
   const uint32_t ngroups = 1u << (c.L - NP);
   for (uint32_t g = tid; g < ngroups; g += BATCH_THREADS) {
