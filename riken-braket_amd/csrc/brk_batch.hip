@@ -216,11 +216,8 @@ __device__ void gate_pass(const chunk_ctx& c, cu32 w, uint32_t tid) {
       apply_phases<NP>(a, post_table);
     }
     if (npost) apply_terms<NP>(a, xb, q, post, npost, post_uniform);
-    // recompute the store addresses instead of keeping 2^NP of them live across the gates
-    uint32_t pbs = pb;
-    asm volatile("" : "+v"(pbs));
 #pragma unroll
-    for (int e = 0; e < (1 << NP); ++e) *reinterpret_cast<f2*>(bytes + (pbs ^ boff[e])) = a[e];
+    for (int e = 0; e < (1 << NP); ++e) *reinterpret_cast<f2*>(bytes + (pb ^ boff[e])) = a[e];
   }
 }
 
